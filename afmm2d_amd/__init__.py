@@ -1,0 +1,331 @@
+"""MI355X AFMM2D matvec engine: Python mirror of the reference's AFMM API.
+
+The reference interface (include/AFMM2D.hpp:14-33) is
+
+    AFMM(N, MinParticlesInLeaf, TOL_POW, loc)   # KD tree + quad tree
+    afmm.assemble()                              # ACA skeletons, L2P, M2L
+    y = afmm.computeMatVecProduct(b)             # y = K b, caller order
+    afmm.evaluateError()                         # one random leaf vs direct sum
+
+with the kernel supplied by ``userkernel::getMatrixEntry``.  Here the same
+flow calls libafmm2d.so (hand-written HIP kernels for gfx950) through its C
+ABI (include/afmm2d.h); the kernel is chosen from the built-in registry.
+There is no CPU path: if the library is missing the import of a handle fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libafmm2d.so")
+
+KERNELS = {"log": 0, "inv_r": 1, "gaussian": 2, "reg_inv_r": 3}
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_LEVELS = 2
+ERR_STATE = 3
+ERR_OUT_OF_MEMORY = 4
+ERR_HIP = 5
+ERR_UNSUPPORTED = 6
+ERR_NOT_FOUND = 7
+
+# Every symbol include/afmm2d.h declares.
+EXPORTS = [
+    "afmm2d_last_error", "afmm2d_status_string", "afmm2d_create", "afmm2d_destroy",
+    "afmm2d_assemble", "afmm2d_matvec", "afmm2d_matvec_device", "afmm2d_levels",
+    "afmm2d_get_permutation", "afmm2d_get_sorted_locations", "afmm2d_get_leaf_counts",
+    "afmm2d_get_leaf_range", "afmm2d_get_interactions", "afmm2d_get_ranks",
+    "afmm2d_get_skeleton", "afmm2d_get_l2p", "afmm2d_get_m2l", "afmm2d_direct",
+    "afmm2d_direct_rows", "afmm2d_compute_error", "afmm2d_evaluate_error",
+    "afmm2d_get_matrix_entry", "afmm2d_kernel_value", "afmm2d_get_stats",
+    "afmm2d_set_profiling", "afmm2d_get_kernel_time", "afmm2d_reset_kernel_times",
+]
+
+
+class AFMM2DError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("afmm2d status %d: %s" % (status, msg))
+        self.status = status
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("levels", ctypes.c_int), ("tol_pow", ctypes.c_int),
+        ("kernel_id", ctypes.c_int), ("max_rank", ctypes.c_int),
+        ("skeleton_total", ctypes.c_int64), ("il_pairs", ctypes.c_int64),
+        ("near_pairs", ctypes.c_int64), ("m2l_bytes", ctypes.c_int64),
+        ("l2p_bytes", ctypes.c_int64), ("matvec_bytes", ctypes.c_int64),
+        ("create_seconds", ctypes.c_double), ("assemble_seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+
+def library_path():
+    return LIB_PATH
+
+
+def lib():
+    """Loads libafmm2d.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libafmm2d.so is not built (run __graft_entry__.build() or "
+                          "python -m afmm2d_amd.build): the engine has no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    i, d, c = ctypes.c_int, ctypes.c_double, ctypes.c_char_p
+    sig = {
+        "afmm2d_last_error": (c, []),
+        "afmm2d_status_string": (c, [i]),
+        "afmm2d_create": (i, [i, i, i, _dp, i, i, _dp, i, ctypes.POINTER(_vp)]),
+        "afmm2d_destroy": (i, [_vp]),
+        "afmm2d_assemble": (i, [_vp]),
+        "afmm2d_matvec": (i, [_vp, _dp, _dp, i]),
+        "afmm2d_matvec_device": (i, [_vp, _vp, _vp, _vp]),
+        "afmm2d_levels": (i, [_vp, _ip]),
+        "afmm2d_get_permutation": (i, [_vp, _lp]),
+        "afmm2d_get_sorted_locations": (i, [_vp, _dp]),
+        "afmm2d_get_leaf_counts": (i, [_vp, _ip]),
+        "afmm2d_get_leaf_range": (i, [_vp, i, _ip, _ip]),
+        "afmm2d_get_interactions": (i, [_vp, i, _ip, _ip, _ip]),
+        "afmm2d_get_ranks": (i, [_vp, i, _ip]),
+        "afmm2d_get_skeleton": (i, [_vp, i, i, _ip, i, _ip]),
+        "afmm2d_get_l2p": (i, [_vp, i, i, _dp, ctypes.c_int64, _ip, _ip]),
+        "afmm2d_get_m2l": (i, [_vp, i, i, i, _dp, ctypes.c_int64, _ip, _ip]),
+        "afmm2d_direct": (i, [_vp, _dp, _dp]),
+        "afmm2d_direct_rows": (i, [_vp, i, _lp, _dp, _dp]),
+        "afmm2d_compute_error": (i, [_vp, i, _dp]),
+        "afmm2d_evaluate_error": (i, [_vp, ctypes.c_uint, _ip, _dp]),
+        "afmm2d_get_matrix_entry": (i, [_vp, i, i, _dp]),
+        "afmm2d_kernel_value": (i, [i, _dp, d, d, d, d, d, _dp]),
+        "afmm2d_get_stats": (i, [_vp, ctypes.POINTER(Stats)]),
+        "afmm2d_set_profiling": (i, [_vp, i]),
+        "afmm2d_get_kernel_time": (i, [_vp, c, _dp, _lp]),
+        "afmm2d_reset_kernel_times": (i, [_vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != OK:
+        msg = lib().afmm2d_last_error().decode(errors="replace")
+        raise AFMM2DError(status, msg)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def kernel_value(kernel, xi, yi, xj, yj, a=0.0, kparams=None):
+    """The built-in kernel on explicit coordinates (host evaluation)."""
+    kid = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+    kp = np.zeros(4) if kparams is None else np.asarray(kparams, dtype=np.float64)
+    out = ctypes.c_double()
+    _check(lib().afmm2d_kernel_value(kid, _d(kp), a, xi, yi, xj, yj, ctypes.byref(out)))
+    return out.value
+
+
+def n_levels(N, MinParticlesInLeaf):
+    """nLevels = (unsigned)(log(N/M)/log(4)) with integer N/M (AFMM2D.cpp:7)."""
+    q = int(N) // int(MinParticlesInLeaf)
+    return int(np.log(float(q)) / np.log(4.0)) if q > 0 else 0
+
+
+class AFMM:
+    """The operator: AFMM(N, MinParticlesInLeaf, TOL_POW, loc) (AFMM2D.hpp:14-33).
+
+    loc: (N, 2) array of point coordinates.  kernel: "log", "inv_r",
+    "gaussian" or "reg_inv_r" (the examples' a/R kernel).
+    """
+
+    def __init__(self, N, MinParticlesInLeaf, TOL_POW, loc, kernel="log", kparams=None,
+                 device=-1):
+        loc = np.ascontiguousarray(loc, dtype=np.float64)
+        if loc.shape != (int(N), 2):
+            raise ValueError("loc must have shape (N, 2)")
+        self.N = int(N)
+        self.MinParticlesInLeaf = int(MinParticlesInLeaf)
+        self.TOL_POW = int(TOL_POW)
+        self.kernel = kernel
+        kid = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+        kp = np.zeros(4) if kparams is None else np.asarray(kparams, dtype=np.float64)
+        h = _vp()
+        _check(lib().afmm2d_create(self.N, self.MinParticlesInLeaf, self.TOL_POW, _d(loc), 0, kid,
+                                   _d(kp), int(device), ctypes.byref(h)))
+        self._h = h
+        lv = ctypes.c_int()
+        _check(lib().afmm2d_levels(self._h, ctypes.byref(lv)))
+        self.nLevels = lv.value
+        self._perm = None
+
+    # lifecycle --------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().afmm2d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # the reference flow -----------------------------------------------------
+    def assemble(self):
+        _check(lib().afmm2d_assemble(self._h))
+
+    def computeMatVecProduct(self, b):
+        """y = K b for b of shape (N,) or (N, R), in the caller's point order."""
+        b = np.asarray(b, dtype=np.float64)
+        one = b.ndim == 1
+        B = np.asfortranarray(b.reshape(self.N, -1))
+        Y = np.empty_like(B, order="F")
+        _check(lib().afmm2d_matvec(self._h, _d(B), _d(Y), B.shape[1]))
+        return Y[:, 0].copy() if one else Y
+
+    def matvec_device(self, d_b, d_y, stream=None):
+        """y = K b on device-resident vectors (device pointers as ints)."""
+        _check(lib().afmm2d_matvec_device(self._h, _vp(d_b), _vp(d_y), _vp(stream or 0)))
+
+    def evaluateError(self, seed=0):
+        """One leaf's relative error vs the direct sum (AFMM2D.cpp:67-74)."""
+        box = ctypes.c_int()
+        err = ctypes.c_double()
+        _check(lib().afmm2d_evaluate_error(self._h, seed, ctypes.byref(box), ctypes.byref(err)))
+        return box.value, err.value
+
+    def compute_error(self, box):
+        err = ctypes.c_double()
+        _check(lib().afmm2d_compute_error(self._h, int(box), ctypes.byref(err)))
+        return err.value
+
+    # tree data ----------------------------------------------------------------
+    @property
+    def sorted_Properties(self):
+        """perm[i] = caller index of KD-sorted point i."""
+        if self._perm is None:
+            p = np.empty(self.N, dtype=np.int64)
+            _check(lib().afmm2d_get_permutation(self._h, p.ctypes.data_as(_lp)))
+            self._perm = p
+        return self._perm
+
+    @property
+    def sorted_Locations(self):
+        xy = np.empty((self.N, 2), dtype=np.float64)
+        _check(lib().afmm2d_get_sorted_locations(self._h, _d(xy)))
+        return xy
+
+    @property
+    def NumberOfParticlesInLeaves(self):
+        c = np.empty(4 ** self.nLevels, dtype=np.int32)
+        _check(lib().afmm2d_get_leaf_counts(self._h, _i(c)))
+        return c
+
+    def leaf_range(self, box):
+        s, c = ctypes.c_int(), ctypes.c_int()
+        _check(lib().afmm2d_get_leaf_range(self._h, int(box), ctypes.byref(s), ctypes.byref(c)))
+        return s.value, c.value
+
+    def interactions(self, level):
+        nb = 4 ** level
+        nbr = np.empty((nb, 8), dtype=np.int32)
+        inner = np.empty((nb, 16), dtype=np.int32)
+        outer = np.empty((nb, 24), dtype=np.int32)
+        _check(lib().afmm2d_get_interactions(self._h, level, _i(nbr), _i(inner), _i(outer)))
+        return nbr, inner, outer
+
+    def ranks(self, level):
+        r = np.empty(4 ** level, dtype=np.int32)
+        _check(lib().afmm2d_get_ranks(self._h, level, _i(r)))
+        return r
+
+    def skeleton(self, level, box):
+        cap = 4096
+        idx = np.empty(cap, dtype=np.int32)
+        cnt = ctypes.c_int()
+        _check(lib().afmm2d_get_skeleton(self._h, level, box, _i(idx), cap, ctypes.byref(cnt)))
+        return idx[: cnt.value].copy()
+
+    def l2p(self, level, box):
+        rows, cols = ctypes.c_int(), ctypes.c_int()
+        cap = 1 << 20
+        out = np.empty(cap, dtype=np.float64)
+        _check(lib().afmm2d_get_l2p(self._h, level, box, _d(out), cap, ctypes.byref(rows),
+                                    ctypes.byref(cols)))
+        return out[: rows.value * cols.value].reshape(rows.value, cols.value).copy()
+
+    def m2l(self, level, box, slot):
+        rows, cols = ctypes.c_int(), ctypes.c_int()
+        cap = 1 << 20
+        out = np.empty(cap, dtype=np.float64)
+        st = lib().afmm2d_get_m2l(self._h, level, box, slot, _d(out), cap, ctypes.byref(rows),
+                                  ctypes.byref(cols))
+        if st == ERR_NOT_FOUND:
+            return None
+        _check(st)
+        return out[: rows.value * cols.value].reshape(rows.value, cols.value).copy()
+
+    def getMatrixEntry(self, i, j):
+        """kernel::getMatrixEntry on KD-sorted indices (kernel.hpp:23-26)."""
+        v = ctypes.c_double()
+        _check(lib().afmm2d_get_matrix_entry(self._h, int(i), int(j), ctypes.byref(v)))
+        return v.value
+
+    # checks and measurement -------------------------------------------------
+    def direct(self, b):
+        """Dense direct sum y = K b on the device (tutorial.cpp:97-131)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        y = np.empty(self.N, dtype=np.float64)
+        _check(lib().afmm2d_direct(self._h, _d(b), _d(y)))
+        return y
+
+    def direct_rows(self, rows, b):
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.empty(rows.shape[0], dtype=np.float64)
+        _check(lib().afmm2d_direct_rows(self._h, rows.shape[0], rows.ctypes.data_as(_lp), _d(b),
+                                        _d(out)))
+        return out
+
+    def stats(self):
+        st = Stats()
+        _check(lib().afmm2d_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def set_profiling(self, on):
+        _check(lib().afmm2d_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_time(self, name):
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        _check(lib().afmm2d_get_kernel_time(self._h, name.encode(), ctypes.byref(ms),
+                                            ctypes.byref(n)))
+        return ms.value, n.value
+
+    def reset_kernel_times(self):
+        _check(lib().afmm2d_reset_kernel_times(self._h))

@@ -1,0 +1,604 @@
+// api.cpp -- the C ABI of libafmm2d.so: problem set-up, the quad-tree
+// interaction lists, and the host side of assemble/matvec.  The numerical
+// work runs in the HIP kernels of kd_sort.hip, assemble.hip and matvec.hip;
+// this file only moves data and builds index tables.
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <random>
+
+#include "../../include/afmm2d.h"
+#include "engine.hpp"
+
+using namespace afmm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const std::string& msg) {
+  g_last_error = msg;
+  return status;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    f();
+    return AFMM2D_OK;
+  } catch (const Error& e) {
+    return fail(e.status, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(AFMM2D_ERR_OUT_OF_MEMORY, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, e.what());
+  }
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// nLevels = (unsigned)(log((double)(N/M))/log(4))  (AFMM2D.cpp:7)
+int levels_of(int n, int leaf) {
+  if (n <= 0 || leaf <= 0) return -1;
+  int q = n / leaf;
+  if (q <= 0) return 0;
+  return (int)(unsigned)(std::log((double)q) / std::log(4.0));
+}
+
+// Quad-tree position of box k at level j: base-4 digits, child 0 = SW
+// (0,0), 1 = SE (1,0), 2 = NE (1,1), 3 = NW (0,1) (AFMM2DTree.cpp:969-989).
+void box_xy(int j, int k, int& x, int& y) {
+  x = y = 0;
+  for (int d = j - 1; d >= 0; d--) {
+    int c = (k >> (2 * d)) & 3;
+    x = 2 * x + (c == 1 || c == 2);
+    y = 2 * y + (c >= 2);
+  }
+}
+
+int box_id(int j, int x, int y) {
+  int k = 0;
+  for (int d = j - 1; d >= 0; d--) {
+    int xb = (x >> d) & 1, yb = (y >> d) & 1;
+    int c = yb ? (xb ? 2 : 3) : (xb ? 1 : 0);
+    k = 4 * k + c;
+  }
+  return k;
+}
+
+// Offset of slot s on the square ring of Chebyshev radius d, counter-
+// clockwise from the ring's SW corner (the slot numbering of
+// neighborNumbers / innerNumbers / outerNumbers drawn in
+// AFMM2DTree.cpp:65-943).
+void ring_offset(int d, int s, int& dx, int& dy) {
+  if (s <= 2 * d) { dx = -d + s; dy = -d; return; }
+  s -= 2 * d;
+  if (s <= 2 * d) { dx = d; dy = -d + s; return; }
+  s -= 2 * d;
+  if (s <= 2 * d) { dx = d - s; dy = d; return; }
+  s -= 2 * d;
+  dx = -d;
+  dy = d - s;
+}
+
+// Interaction lists of every box of level j.  Ring 1 holds the neighbours;
+// rings 2 and 3 hold the interaction list: the children of the parent's
+// neighbours that are not adjacent.  Ring 2 always lies inside the parent's
+// neighbourhood; ring 3 does on the two sides away from the box's own
+// parent-relative position.
+void topology_level(int j, std::vector<int>& nbr, std::vector<int>& inner,
+                    std::vector<int>& outer) {
+  const int nb = 1 << (2 * j), side = 1 << j;
+  nbr.assign(8 * (size_t)nb, -1);
+  inner.assign(16 * (size_t)nb, -1);
+  outer.assign(24 * (size_t)nb, -1);
+  if (j == 0) return;
+  for (int k = 0; k < nb; k++) {
+    int ix, iy;
+    box_xy(j, k, ix, iy);
+    for (int d = 1; d <= 3; d++) {
+      for (int s = 0; s < 8 * d; s++) {
+        int dx, dy;
+        ring_offset(d, s, dx, dy);
+        int bx = ix + dx, by = iy + dy;
+        if (bx < 0 || by < 0 || bx >= side || by >= side) continue;
+        if (std::abs((bx >> 1) - (ix >> 1)) > 1 || std::abs((by >> 1) - (iy >> 1)) > 1) continue;
+        int b = box_id(j, bx, by);
+        if (d == 1) nbr[8 * (size_t)k + s] = b;
+        else if (d == 2) inner[16 * (size_t)k + s] = b;
+        else outer[24 * (size_t)k + s] = b;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+namespace afmm {
+
+Engine::~Engine() {
+  if (device >= 0) (void)hipSetDevice(device);
+  for (auto& p : ev_pool) {
+    (void)hipEventDestroy(p[0]);
+    (void)hipEventDestroy(p[1]);
+  }
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (s_main) (void)hipStreamDestroy(s_main);
+  if (s_aux) (void)hipStreamDestroy(s_aux);
+}
+
+void Engine::prof_begin(const char* name, hipStream_t s, hipEvent_t* a, hipEvent_t* b) {
+  *a = *b = nullptr;
+  if (!profiling) return;
+  if (ev_used == ev_pool.size()) {
+    std::array<hipEvent_t, 2> p;
+    AFMM_HIP(hipEventCreate(&p[0]));
+    AFMM_HIP(hipEventCreate(&p[1]));
+    ev_pool.push_back(p);
+  }
+  *a = ev_pool[ev_used][0];
+  *b = ev_pool[ev_used][1];
+  ev_used++;
+  AFMM_HIP(hipEventRecord(*a, s));
+  pending.push_back(Pending{name, *a, *b});
+}
+
+void Engine::prof_end(hipStream_t s, hipEvent_t b) {
+  if (!profiling || !b) return;
+  AFMM_HIP(hipEventRecord(b, s));
+}
+
+void Engine::prof_collect() {
+  for (auto& p : pending) {
+    AFMM_HIP(hipEventSynchronize(p.b));
+    float ms = 0.f;
+    AFMM_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    KernelTimer& t = timers[p.name];
+    t.total_ms += ms;
+    t.launches += 1;
+  }
+  pending.clear();
+  ev_used = 0;
+}
+
+}  // namespace afmm
+
+struct afmm2d_handle_s {
+  std::unique_ptr<Engine> e;
+};
+
+extern "C" {
+
+const char* afmm2d_last_error(void) { return g_last_error.c_str(); }
+
+const char* afmm2d_status_string(int s) {
+  switch (s) {
+    case AFMM2D_OK: return "ok";
+    case AFMM2D_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case AFMM2D_ERR_LEVELS: return "fewer than 2 tree levels";
+    case AFMM2D_ERR_STATE: return "invalid state";
+    case AFMM2D_ERR_OUT_OF_MEMORY: return "out of memory";
+    case AFMM2D_ERR_HIP: return "HIP error";
+    case AFMM2D_ERR_UNSUPPORTED: return "unsupported size";
+    case AFMM2D_ERR_NOT_FOUND: return "not found";
+  }
+  return "unknown status";
+}
+
+int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, int kernel_id,
+                  const double* kparams, int device, afmm2d_handle* out) {
+  if (!out) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  if (!loc) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "loc is NULL");
+  if (n <= 1 || leaf <= 0) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "need N > 1 and leaf > 0");
+  if (kernel_id < 0 || kernel_id > 3) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown kernel id");
+  if (layout != AFMM2D_LOC_INTERLEAVED && layout != AFMM2D_LOC_COLUMN_MAJOR)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown location layout");
+  if (tol_pow < 0 || tol_pow > 300) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "tol_pow out of range");
+  const int L = levels_of(n, leaf);
+  if (L < 2)
+    return fail(AFMM2D_ERR_LEVELS, "nLevels = log4(N/M) = " + std::to_string(L) +
+                                       " < 2: no interaction lists exist");
+  if (L > 13) return fail(AFMM2D_ERR_UNSUPPORTED, "more than 13 levels");
+  return guarded([&] {
+    auto e = std::make_unique<Engine>();
+    if (device < 0) AFMM_HIP(hipGetDevice(&device));
+    AFMM_HIP(hipSetDevice(device));
+    e->device = device;
+    e->n = n;
+    e->leaf = leaf;
+    e->tol_pow = tol_pow;
+    e->tol = std::pow(10.0, -1.0 * tol_pow);  // ACA.cpp:7
+    e->L = L;
+    e->kid = kernel_id;
+    e->kp.p0 = (kparams && kparams[0] > 0.0) ? kparams[0] : 0.01;
+    e->kp.a = std::ldexp(1.0, -L);  // a = 0.5^L (AFMM2DTree.cpp:13-19)
+    if (kparams && kernel_id == AFMM2D_KERNEL_REG_INV_R && kparams[1] > 0.0) e->kp.a = kparams[1];
+    AFMM_HIP(hipStreamCreateWithFlags(&e->s_main, hipStreamNonBlocking));
+    AFMM_HIP(hipStreamCreateWithFlags(&e->s_aux, hipStreamNonBlocking));
+    AFMM_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+    AFMM_HIP(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
+    double t0 = now_s();
+    std::vector<double> hx(n), hy(n);
+    for (int i = 0; i < n; i++) {
+      hx[i] = layout == AFMM2D_LOC_INTERLEAVED ? loc[2 * (size_t)i] : loc[i];
+      hy[i] = layout == AFMM2D_LOC_INTERLEAVED ? loc[2 * (size_t)i + 1] : loc[(size_t)n + i];
+      if (!std::isfinite(hx[i]) || !std::isfinite(hy[i]))
+        throw Error(AFMM2D_ERR_INVALID_ARGUMENT, "non-finite coordinate at index " + std::to_string(i));
+    }
+    kd_partition(*e, hx.data(), hy.data());
+    // Leaf ranges (AFMM2DTree.cpp:1099-1118) and KD-order box numbers
+    // (KDTree.cpp:211-223).
+    const int nleaf = 1 << (2 * L);
+    e->kd_boxnum.resize(nleaf);
+    static const int map[4] = {0, 3, 1, 2};
+    for (int t = 0; t < nleaf; t++) {
+      int b = 0;
+      for (int d = L - 1; d >= 0; d--) b = 4 * b + map[(t >> (2 * d)) & 3];
+      e->kd_boxnum[t] = b;
+    }
+    e->leaf_start.assign(nleaf, 0);
+    e->leaf_count.assign(nleaf, 0);
+    int st = 0;
+    for (int t = 0; t < nleaf; t++) {
+      int b = e->kd_boxnum[t];
+      e->leaf_start[b] = st;
+      e->leaf_count[b] = e->kd_counts[t];
+      e->max_leaf = std::max(e->max_leaf, e->kd_counts[t]);
+      st += e->kd_counts[t];
+    }
+    e->nbr.resize(L + 1);
+    e->inner.resize(L + 1);
+    e->outer.resize(L + 1);
+    for (int j = 0; j <= L; j++) topology_level(j, e->nbr[j], e->inner[j], e->outer[j]);
+    e->near_pairs = nleaf;
+    for (int v : e->nbr[L]) e->near_pairs += v >= 0;
+    hipStream_t s = e->s_main;
+    e->d_leaf_start.alloc(sizeof(int) * nleaf);
+    e->d_leaf_count.alloc(sizeof(int) * nleaf);
+    e->d_leaf_nbr.alloc(sizeof(int) * 8 * (size_t)nleaf);
+    AFMM_HIP(hipMemcpyAsync(e->d_leaf_start.p, e->leaf_start.data(), sizeof(int) * nleaf,
+                            hipMemcpyHostToDevice, s));
+    AFMM_HIP(hipMemcpyAsync(e->d_leaf_count.p, e->leaf_count.data(), sizeof(int) * nleaf,
+                            hipMemcpyHostToDevice, s));
+    AFMM_HIP(hipMemcpyAsync(e->d_leaf_nbr.p, e->nbr[L].data(), sizeof(int) * 8 * (size_t)nleaf,
+                            hipMemcpyHostToDevice, s));
+    AFMM_HIP(hipStreamSynchronize(s));
+    e->create_seconds = now_s() - t0;
+    *out = new afmm2d_handle_s{std::move(e)};
+  });
+}
+
+int afmm2d_destroy(afmm2d_handle h) {
+  if (!h) return AFMM2D_OK;
+  return guarded([&] {
+    if (h->e) {
+      (void)hipSetDevice(h->e->device);
+      (void)hipStreamSynchronize(h->e->s_main);
+      (void)hipStreamSynchronize(h->e->s_aux);
+    }
+    delete h;
+  });
+}
+
+int afmm2d_assemble(afmm2d_handle h) {
+  if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "handle is NULL");
+  return guarded([&] {
+    Engine& e = *h->e;
+    AFMM_HIP(hipSetDevice(e.device));
+    if (e.assembled) return;
+    double t0 = now_s();
+    assemble_tree(e);
+    e.assembled = true;
+    e.assemble_seconds = now_s() - t0;
+  });
+}
+
+int afmm2d_matvec_device(afmm2d_handle h, const double* d_b, double* d_y, void* stream) {
+  if (!h || !d_b || !d_y) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!h->e->assembled) return fail(AFMM2D_ERR_STATE, "matvec before assemble");
+  return guarded([&] {
+    Engine& e = *h->e;
+    AFMM_HIP(hipSetDevice(e.device));
+    hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)0;
+    matvec_device(e, d_b, d_y, s);
+    if (e.profiling) e.prof_collect();
+  });
+}
+
+int afmm2d_matvec(afmm2d_handle h, const double* b, double* y, int nrhs) {
+  if (!h || !b || !y || nrhs <= 0) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "bad argument");
+  if (!h->e->assembled) return fail(AFMM2D_ERR_STATE, "matvec before assemble");
+  return guarded([&] {
+    Engine& e = *h->e;
+    AFMM_HIP(hipSetDevice(e.device));
+    const size_t nb = sizeof(double) * e.n;
+    e.io_b.alloc(nb);
+    e.io_y.alloc(nb);
+    hipStream_t s = e.s_main;
+    for (int r = 0; r < nrhs; r++) {
+      AFMM_HIP(hipMemcpyAsync(e.io_b.p, b + (size_t)r * e.n, nb, hipMemcpyHostToDevice, s));
+      matvec_device(e, e.io_b.as<double>(), e.io_y.as<double>(), s);
+      AFMM_HIP(hipMemcpyAsync(y + (size_t)r * e.n, e.io_y.p, nb, hipMemcpyDeviceToHost, s));
+      AFMM_HIP(hipStreamSynchronize(s));
+      if (e.profiling) e.prof_collect();
+      if (r == 0) {
+        e.last_qs.resize(e.n);
+        e.last_ps.resize(e.n);
+        for (int i = 0; i < e.n; i++) {
+          e.last_qs[i] = b[e.h_perm[i]];
+          e.last_ps[i] = y[e.h_perm[i]];
+        }
+        e.have_matvec = true;
+      }
+    }
+  });
+}
+
+int afmm2d_levels(afmm2d_handle h, int* levels) {
+  if (!h || !levels) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  *levels = h->e->L;
+  return AFMM2D_OK;
+}
+
+int afmm2d_get_permutation(afmm2d_handle h, int64_t* perm) {
+  if (!h || !perm) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::memcpy(perm, h->e->h_perm.data(), sizeof(int64_t) * h->e->n);
+  return AFMM2D_OK;
+}
+
+int afmm2d_get_sorted_locations(afmm2d_handle h, double* xy) {
+  if (!h || !xy) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::memcpy(xy, h->e->h_xy.data(), sizeof(double) * 2 * (size_t)h->e->n);
+  return AFMM2D_OK;
+}
+
+int afmm2d_get_leaf_counts(afmm2d_handle h, int* counts) {
+  if (!h || !counts) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::memcpy(counts, h->e->kd_counts.data(), sizeof(int) * h->e->kd_counts.size());
+  return AFMM2D_OK;
+}
+
+int afmm2d_get_leaf_range(afmm2d_handle h, int box, int* start, int* count) {
+  if (!h || !start || !count) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (box < 0 || box >= (int)h->e->leaf_start.size())
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "box out of range");
+  *start = h->e->leaf_start[box];
+  *count = h->e->leaf_count[box];
+  return AFMM2D_OK;
+}
+
+int afmm2d_get_interactions(afmm2d_handle h, int level, int* nbr, int* inner, int* outer) {
+  if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL handle");
+  Engine& e = *h->e;
+  if (level < 0 || level > e.L) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "level out of range");
+  if (nbr) std::memcpy(nbr, e.nbr[level].data(), sizeof(int) * e.nbr[level].size());
+  if (inner) std::memcpy(inner, e.inner[level].data(), sizeof(int) * e.inner[level].size());
+  if (outer) std::memcpy(outer, e.outer[level].data(), sizeof(int) * e.outer[level].size());
+  return AFMM2D_OK;
+}
+
+int afmm2d_get_ranks(afmm2d_handle h, int level, int* ranks) {
+  if (!h || !ranks) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  Engine& e = *h->e;
+  if (!e.assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
+  if (level < 2 || level > e.L) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "level out of range");
+  std::memcpy(ranks, e.lv[level].rank.data(), sizeof(int) * e.lv[level].nb);
+  return AFMM2D_OK;
+}
+
+int afmm2d_get_skeleton(afmm2d_handle h, int level, int box, int* idx, int capacity, int* count) {
+  if (!h || !count) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  Engine& e = *h->e;
+  if (!e.assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
+  if (level < 2 || level > e.L || box < 0 || box >= e.lv[level].nb)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "level/box out of range");
+  return guarded([&] {
+    const LevelInfo& li = e.lv[level];
+    int r = li.rank[box];
+    *count = r;
+    if (r > capacity || !idx) throw Error(AFMM2D_ERR_INVALID_ARGUMENT, "capacity too small");
+    AFMM_HIP(hipMemcpy(idx, e.skidx.as<int>() + li.soff[box], sizeof(int) * r, hipMemcpyDeviceToHost));
+  });
+}
+
+int afmm2d_get_l2p(afmm2d_handle h, int level, int box, double* out, int64_t capacity, int* rows,
+                   int* cols) {
+  if (!h || !rows || !cols) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  Engine& e = *h->e;
+  if (!e.assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
+  if (level < 2 || level > e.L || box < 0 || box >= e.lv[level].nb)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "level/box out of range");
+  return guarded([&] {
+    const LevelInfo& li = e.lv[level];
+    *rows = li.nin[box];
+    *cols = li.rank[box];
+    int64_t m = (int64_t)*rows * *cols;
+    if (m > capacity || (!out && m > 0)) throw Error(AFMM2D_ERR_INVALID_ARGUMENT, "capacity too small");
+    if (m > 0)
+      AFMM_HIP(hipMemcpy(out, e.l2p_rm.as<double>() + li.l2poff[box], sizeof(double) * m,
+                         hipMemcpyDeviceToHost));
+  });
+}
+
+int afmm2d_get_m2l(afmm2d_handle h, int level, int box, int slot, double* out, int64_t capacity,
+                   int* rows, int* cols) {
+  if (!h || !rows || !cols) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  Engine& e = *h->e;
+  if (!e.assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
+  if (level < 2 || level > e.L || box < 0 || box >= e.lv[level].nb || slot < 0 || slot >= 40)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "level/box/slot out of range");
+  const LevelInfo& li = e.lv[level];
+  auto slot_box = [&](int s) {
+    return s < 16 ? e.inner[level][16 * (size_t)box + s] : e.outer[level][24 * (size_t)box + s - 16];
+  };
+  if (slot_box(slot) < 0) return fail(AFMM2D_ERR_NOT_FOUND, "empty interaction slot");
+  return guarded([&] {
+    int col0 = 0;
+    for (int s = 0; s < slot; s++) {
+      int kb = slot_box(s);
+      if (kb >= 0) col0 += li.rank[kb];
+    }
+    const int r = li.rank[box], rc = li.rank[slot_box(slot)];
+    const int C = (int)(li.m2lcoff[box + 1] - li.m2lcoff[box]);
+    *rows = r;
+    *cols = rc;
+    if ((int64_t)r * rc > capacity) throw Error(AFMM2D_ERR_INVALID_ARGUMENT, "capacity too small");
+    std::vector<double> blk((size_t)r * C);
+    if (!blk.empty())
+      AFMM_HIP(hipMemcpy(blk.data(), e.m2l.as<double>() + li.m2loff[box], sizeof(double) * blk.size(),
+                         hipMemcpyDeviceToHost));
+    for (int i = 0; i < r; i++)
+      for (int c = 0; c < rc; c++) out[(size_t)i * rc + c] = blk[(size_t)(col0 + c) * r + i];
+  });
+}
+
+int afmm2d_direct_rows(afmm2d_handle h, int nrows, const int64_t* rows, const double* b,
+                       double* out) {
+  if (!h || !rows || !b || !out || nrows < 0) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "bad argument");
+  return guarded([&] {
+    Engine& e = *h->e;
+    AFMM_HIP(hipSetDevice(e.device));
+    // caller index -> KD-sorted index
+    std::vector<int> inv(e.n);
+    for (int i = 0; i < e.n; i++) inv[e.h_perm[i]] = i;
+    std::vector<int> srows(nrows);
+    for (int r = 0; r < nrows; r++) {
+      if (rows[r] < 0 || rows[r] >= e.n) throw Error(AFMM2D_ERR_INVALID_ARGUMENT, "row out of range");
+      srows[r] = inv[rows[r]];
+    }
+    hipStream_t s = e.s_main;
+    DevBuf d_b, d_qs, d_rows, d_out;
+    d_b.alloc(sizeof(double) * e.n);
+    d_qs.alloc(sizeof(double) * e.n);
+    d_rows.alloc(sizeof(int) * std::max(nrows, 1));
+    d_out.alloc(sizeof(double) * std::max(nrows, 1));
+    AFMM_HIP(hipMemcpyAsync(d_b.p, b, sizeof(double) * e.n, hipMemcpyHostToDevice, s));
+    if (nrows > 0)
+      AFMM_HIP(hipMemcpyAsync(d_rows.p, srows.data(), sizeof(int) * nrows, hipMemcpyHostToDevice, s));
+    gather_sorted(e, d_b.as<double>(), d_qs.as<double>(), s);
+    direct_rows_device(e, nrows, d_rows.as<int>(), d_qs.as<double>(), d_out.as<double>(), s);
+    if (nrows > 0)
+      AFMM_HIP(hipMemcpyAsync(out, d_out.p, sizeof(double) * nrows, hipMemcpyDeviceToHost, s));
+    AFMM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int afmm2d_direct(afmm2d_handle h, const double* b, double* y) {
+  if (!h || !b || !y) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::vector<int64_t> rows(h->e->n);
+  for (int i = 0; i < h->e->n; i++) rows[i] = i;
+  return afmm2d_direct_rows(h, h->e->n, rows.data(), b, y);
+}
+
+int afmm2d_compute_error(afmm2d_handle h, int box, double* err) {
+  if (!h || !err) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  Engine& e = *h->e;
+  if (!e.have_matvec) return fail(AFMM2D_ERR_STATE, "no host matvec has run yet");
+  if (box < 0 || box >= (int)e.leaf_start.size()) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "box out of range");
+  return guarded([&] {
+    AFMM_HIP(hipSetDevice(e.device));
+    const int st = e.leaf_start[box], n = e.leaf_count[box];
+    hipStream_t s = e.s_main;
+    DevBuf d_qs, d_rows, d_out;
+    d_qs.alloc(sizeof(double) * e.n);
+    d_rows.alloc(sizeof(int) * std::max(n, 1));
+    d_out.alloc(sizeof(double) * std::max(n, 1));
+    std::vector<int> rows(n);
+    for (int i = 0; i < n; i++) rows[i] = st + i;
+    AFMM_HIP(hipMemcpyAsync(d_qs.p, e.last_qs.data(), sizeof(double) * e.n, hipMemcpyHostToDevice, s));
+    if (n > 0) AFMM_HIP(hipMemcpyAsync(d_rows.p, rows.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    direct_rows_device(e, n, d_rows.as<int>(), d_qs.as<double>(), d_out.as<double>(), s);
+    std::vector<double> tr(n);
+    if (n > 0) AFMM_HIP(hipMemcpy(tr.data(), d_out.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < n; i++) {
+      double d = tr[i] - e.last_ps[st + i];
+      num += d * d;
+      den += tr[i] * tr[i];
+    }
+    *err = std::sqrt(num) / std::sqrt(den);
+  });
+}
+
+int afmm2d_evaluate_error(afmm2d_handle h, unsigned seed, int* box, double* err) {
+  if (!h || !box || !err) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::mt19937 rng(seed);
+  *box = (int)(rng() % (unsigned)h->e->leaf_start.size());
+  return afmm2d_compute_error(h, *box, err);
+}
+
+int afmm2d_get_matrix_entry(afmm2d_handle h, int i, int j, double* value) {
+  if (!h || !value) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  Engine& e = *h->e;
+  if (i < 0 || j < 0 || i >= e.n || j >= e.n) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "index out of range");
+  *value = kernel_host(e.kid, e.kp, e.h_xy[2 * (size_t)i], e.h_xy[2 * (size_t)i + 1],
+                       e.h_xy[2 * (size_t)j], e.h_xy[2 * (size_t)j + 1]);
+  return AFMM2D_OK;
+}
+
+int afmm2d_kernel_value(int kernel_id, const double* kparams, double a, double xi, double yi,
+                        double xj, double yj, double* value) {
+  if (!value) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (kernel_id < 0 || kernel_id > 3) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown kernel id");
+  KParams p{(kparams && kparams[0] > 0.0) ? kparams[0] : 0.01, a};
+  *value = kernel_host(kernel_id, p, xi, yi, xj, yj);
+  return AFMM2D_OK;
+}
+
+int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st) {
+  if (!h || !st) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  Engine& e = *h->e;
+  std::memset(st, 0, sizeof(*st));
+  st->n = e.n;
+  st->levels = e.L;
+  st->tol_pow = e.tol_pow;
+  st->kernel_id = e.kid;
+  st->max_rank = e.max_rank;
+  st->skeleton_total = e.skel_total;
+  st->il_pairs = e.il_pairs;
+  st->near_pairs = e.near_pairs;
+  st->m2l_bytes = 8 * e.m2l_total;
+  st->l2p_bytes = 8 * e.l2p_total;
+  // One matvec reads q (gather) and the coordinates (near field), L2P twice
+  // (M2M and L2L), every stored M2L coefficient and its column index once,
+  // the multipole/local vectors, and writes y.
+  st->matvec_bytes = e.assembled
+                         ? (int64_t)e.n * (8 + 4 + 8 + 16 + 8 + 4 + 8 + 8) + 2 * 8 * e.l2p_total +
+                               8 * e.m2l_total + 4 * e.m2l_cols + 4 * 8 * e.skel_total
+                         : 0;
+  st->create_seconds = e.create_seconds;
+  st->assemble_seconds = e.assemble_seconds;
+  return AFMM2D_OK;
+}
+
+int afmm2d_set_profiling(afmm2d_handle h, int enable) {
+  if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL handle");
+  h->e->profiling = enable != 0;
+  return AFMM2D_OK;
+}
+
+int afmm2d_get_kernel_time(afmm2d_handle h, const char* name, double* total_ms, int64_t* launches) {
+  if (!h || !name || !total_ms || !launches) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  auto it = h->e->timers.find(name);
+  if (it == h->e->timers.end()) {
+    *total_ms = 0.0;
+    *launches = 0;
+    return AFMM2D_OK;
+  }
+  *total_ms = it->second.total_ms;
+  *launches = it->second.launches;
+  return AFMM2D_OK;
+}
+
+int afmm2d_reset_kernel_times(afmm2d_handle h) {
+  if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL handle");
+  h->e->timers.clear();
+  return AFMM2D_OK;
+}
+
+}  // extern "C"

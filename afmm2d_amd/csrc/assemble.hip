@@ -1,0 +1,741 @@
+// assemble.hip -- operator assembly on the GPU: per-box ACA skeletons, the
+// interpolative L2P basis and the M2L blocks.
+//
+// Levels run L -> 2 (AFMM2DTree.cpp:991-995).  At level j the rows of box k
+// are its leaf points (j = L) or the concatenated skeletons of its children
+// in child order 0..3; the columns are the rows of its interaction-list boxes,
+// inner slots 0..15 then outer slots 0..23 (AFMM2DTree.cpp:997-1034).  Since
+// children 4k..4k+3 are consecutive and skeletons are stored box after box,
+// every box's rows are one contiguous range of the level's source points.
+//
+// ACA (ACA.cpp:27-133): one workgroup per box, the rank loop is sequential,
+// each step's row/column evaluation, residual update, argmax and norm
+// reductions are parallel over the workgroup.  Pivot choice and stopping rule
+// follow the reference exactly (first maximum, |prev - normS| >= tol*prev,
+// normS^2 += (|u||v|)^2 + sum_l 2|U_l.u||V_l.v| including the new pair,
+// rank-2 quirk).  Products and differences are not contracted into FMAs so
+// residuals round as the CPU oracle's do.
+//
+// ID (AFMM2DTree.cpp:1042-1060): L2P = Ac D^{-1} with D = K(skeleton rows,
+// skeleton columns), through LU with partial pivoting of D^T.
+//
+// M2L (AFMM2DTree.cpp:1070-1097): K(skeleton_k, skeleton_kIL) for every
+// ordered pair, written per target box as one column-major matrix.
+#include <algorithm>
+#include <cfloat>
+#include <numeric>
+
+#include "engine.hpp"
+
+namespace afmm {
+
+namespace {
+
+constexpr int ACA_THREADS = 256;
+constexpr int MAXR = 1024;      // largest ACA rank a workgroup can hold
+constexpr int MAXOWN = 64;      // owned rows/cols per thread (mask bits)
+constexpr int RLDS = 84;        // ID matrices up to 84x84 live in LDS (< 64 KiB)
+
+struct Cand {
+  double a;  // |value|
+  double v;  // value
+  int i;     // index, -1 = none
+};
+
+__device__ __forceinline__ Cand cbetter(const Cand& x, const Cand& y) {
+  if (y.i < 0) return x;
+  if (x.i < 0) return y;
+  return (y.a > x.a || (y.a == x.a && y.i < x.i)) ? y : x;
+}
+
+__device__ __forceinline__ Cand wave_argmax(Cand c) {
+  for (int off = 32; off > 0; off >>= 1) {
+    Cand o;
+    o.a = __shfl_xor(c.a, off, 64);
+    o.v = __shfl_xor(c.v, off, 64);
+    o.i = __shfl_xor(c.i, off, 64);
+    c = cbetter(c, o);
+  }
+  return c;
+}
+
+__device__ __forceinline__ double wave_sum(double s) {
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  return s;
+}
+
+// Workgroup argmax; every thread gets the result.
+__device__ Cand block_argmax(Cand c, Cand* s4) {
+  c = wave_argmax(c);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) s4[w] = c;
+  __syncthreads();
+  Cand r = s4[0];
+  for (int q = 1; q < (int)(blockDim.x >> 6); q++) r = cbetter(r, s4[q]);
+  __syncthreads();
+  return r;
+}
+
+struct AcaArgs {
+  const double* src_x;
+  const double* src_y;
+  const int* rstart;   // per box: first row in the level's source points
+  const int* n1;       // per box: rows
+  const int* n2;       // per box: columns
+  const int* il;       // per box: 40 interaction slots (-1 absent)
+  const int* boxes;    // boxes of this batch
+  const int64_t* scr_off;  // per batch entry: scratch offset (doubles)
+  double* scr;
+  const int64_t* piv_off;  // per box: pivot output offset
+  int* rank;
+  int* prow;           // local row index of each pivot
+  double* pcx;         // coordinates of each pivot column's point
+  double* pcy;
+  double tol;
+  KParams kp;
+};
+
+template <int KID>
+__global__ __launch_bounds__(ACA_THREADS) void k_aca(AcaArgs a) {
+#pragma clang fp contract(off)
+  __shared__ double coefU[MAXR];
+  __shared__ double coefV[MAXR];
+  __shared__ double dU[MAXR + 1];
+  __shared__ double dV[MAXR + 1];
+  __shared__ int il_start[40], il_pref[41];
+  __shared__ int nslot;
+  __shared__ Cand s4[4];
+  __shared__ double s_pt[4];
+  __shared__ double s_norm[2];
+
+  const int tid = threadIdx.x;
+  const int b = a.boxes[blockIdx.x];
+  const int n1 = a.n1[b], n2 = a.n2[b];
+  const int mn = min(n1, n2);
+  const int64_t po = a.piv_off[b];
+  if (mn <= 0) {
+    if (tid == 0) a.rank[b] = 0;
+    return;
+  }
+  double* U = a.scr + a.scr_off[blockIdx.x];
+  double* V = U + (int64_t)n1 * mn;
+  double* cx = V + (int64_t)n2 * mn;
+  double* cy = cx + n2;
+  const int r0 = a.rstart[b];
+  if (tid == 0) {
+    int p = 0, c = 0;
+    for (int s = 0; s < 40; s++) {
+      int kb = a.il[40 * b + s];
+      if (kb < 0) continue;
+      il_start[c] = a.rstart[kb];
+      il_pref[c] = p;
+      p += a.n1[kb];
+      c++;
+    }
+    il_pref[c] = p;
+    nslot = c;
+  }
+  __syncthreads();
+  // column points, each owned by thread (j mod 256)
+  for (int j = tid; j < n2; j += ACA_THREADS) {
+    int s = 0;
+    while (s + 1 < nslot && il_pref[s + 1] <= j) s++;
+    int p = il_start[s] + (j - il_pref[s]);
+    cx[j] = a.src_x[p];
+    cy[j] = a.src_y[p];
+  }
+  uint64_t cmask = 0, rmask = 0;  // used columns / rows owned by this thread
+
+  // First pivot: first row whose largest entry exceeds tol (ACA.cpp:56-63).
+  int istar = -1, jstar = -1;
+  double piv = 0.0;
+  for (int l = 0; l < n1; l++) {
+    const double px = a.src_x[r0 + l], py = a.src_y[r0 + l];
+    Cand c{0.0, 0.0, -1};
+    for (int j = tid, m = 0; j < n2; j += ACA_THREADS, m++) {
+      double v = kernel_xy<KID>(px, py, cx[j], cy[j], a.kp);
+      V[j] = v;
+      c = cbetter(c, Cand{fabs(v), v, j});
+    }
+    Cand r = block_argmax(c, s4);
+    if (fabs(r.v) > a.tol) {
+      istar = l;
+      jstar = r.i;
+      piv = r.v;
+      break;
+    }
+  }
+  if (istar < 0) {
+    if (tid == 0) a.rank[b] = 0;
+    return;
+  }
+  if (tid == (jstar & (ACA_THREADS - 1))) {
+    s_pt[0] = cx[jstar];
+    s_pt[1] = cy[jstar];
+    cmask |= 1ull << (jstar / ACA_THREADS);
+  }
+  if (tid == (istar & (ACA_THREADS - 1))) rmask |= 1ull << (istar / ACA_THREADS);
+  for (int j = tid; j < n2; j += ACA_THREADS) V[j] = V[j] / piv;
+  __syncthreads();
+  double cjx = s_pt[0], cjy = s_pt[1];
+  if (tid == 0) {
+    a.prow[po] = istar;
+    a.pcx[po] = cjx;
+    a.pcy[po] = cjy;
+  }
+  Cand cr{0.0, 0.0, -1};
+  for (int i = tid, m = 0; i < n1; i += ACA_THREADS, m++) {
+    double u = kernel_xy<KID>(a.src_x[r0 + i], a.src_y[r0 + i], cjx, cjy, a.kp);
+    U[i] = u;
+    if (!((rmask >> m) & 1)) cr = cbetter(cr, Cand{fabs(u), u, i});
+  }
+  int rank = 1;
+  Cand nr = block_argmax(cr, s4);
+  istar = nr.i;
+  double normS = 0.0, prev = DBL_MAX;
+
+  while (fabs(prev - normS) >= a.tol * prev && rank < mn && istar >= 0) {
+    const int t = rank;
+    for (int l = tid; l < t; l += ACA_THREADS) coefU[l] = U[(int64_t)l * n1 + istar];
+    __syncthreads();
+    // Phase A: residual row of pivot row istar.
+    const double px = a.src_x[r0 + istar], py = a.src_y[r0 + istar];
+    double* Vt = V + (int64_t)t * n2;
+    Cand c{0.0, 0.0, -1};
+    for (int j = tid, m = 0; j < n2; j += ACA_THREADS, m++) {
+      double r = kernel_xy<KID>(px, py, cx[j], cy[j], a.kp);
+      for (int l = 0; l < t; l++) r -= coefU[l] * V[(int64_t)l * n2 + j];
+      Vt[j] = r;
+      if (!((cmask >> m) & 1)) c = cbetter(c, Cand{fabs(r), r, j});
+    }
+    Cand rc = block_argmax(c, s4);
+    jstar = rc.i;
+    piv = rc.v;
+    if (jstar < 0 || piv == 0.0) break;  // a zero pivot ends the ACA
+    for (int l = tid; l < t; l += ACA_THREADS) coefV[l] = V[(int64_t)l * n2 + jstar];
+    if (tid == (jstar & (ACA_THREADS - 1))) {
+      s_pt[0] = cx[jstar];
+      s_pt[1] = cy[jstar];
+      cmask |= 1ull << (jstar / ACA_THREADS);
+    }
+    if (tid == (istar & (ACA_THREADS - 1))) rmask |= 1ull << (istar / ACA_THREADS);
+    __syncthreads();
+    cjx = s_pt[0];
+    cjy = s_pt[1];
+    if (tid == 0) {
+      a.prow[po + t] = istar;
+      a.pcx[po + t] = cjx;
+      a.pcy[po + t] = cjy;
+    }
+    // Phase B: v = row / pivot; residual column u.
+    for (int j = tid; j < n2; j += ACA_THREADS) Vt[j] = Vt[j] / piv;
+    double* Ut = U + (int64_t)t * n1;
+    Cand cu{0.0, 0.0, -1};
+    for (int i = tid, m = 0; i < n1; i += ACA_THREADS, m++) {
+      double u = kernel_xy<KID>(a.src_x[r0 + i], a.src_y[r0 + i], cjx, cjy, a.kp);
+      for (int l = 0; l < t; l++) u -= coefV[l] * U[(int64_t)l * n1 + i];
+      Ut[i] = u;
+      if (!((rmask >> m) & 1)) cu = cbetter(cu, Cand{fabs(u), u, i});
+    }
+    rank = t + 1;
+    nr = block_argmax(cu, s4);  // includes a barrier: U_t, V_t complete
+    // Phase C: U_l.u and V_l.v for l = 0..t.
+    {
+      const int w = tid >> 6, lane = tid & 63;
+      for (int l = w; l <= t; l += ACA_THREADS / 64) {
+        double su = 0.0, sv = 0.0;
+        const double* Ul = U + (int64_t)l * n1;
+        const double* Vl = V + (int64_t)l * n2;
+        for (int i = lane; i < n1; i += 64) su += Ul[i] * Ut[i];
+        for (int j = lane; j < n2; j += 64) sv += Vl[j] * Vt[j];
+        su = wave_sum(su);
+        sv = wave_sum(sv);
+        if (lane == 0) {
+          dU[l] = su;
+          dV[l] = sv;
+        }
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double pv = prev;
+      if (rank != 2) pv = normS;
+      double p = sqrt(dU[t]) * sqrt(dV[t]);
+      double s = normS * normS + p * p;
+      for (int l = 0; l <= t; l++) s += 2.0 * fabs(dU[l] * dV[l]);
+      s_norm[0] = pv;
+      s_norm[1] = sqrt(s);
+    }
+    __syncthreads();
+    prev = s_norm[0];
+    normS = s_norm[1];
+    istar = nr.i;
+    __syncthreads();
+  }
+  if (tid == 0) a.rank[b] = rank;
+}
+
+struct IdArgs {
+  const double* src_x;
+  const double* src_y;
+  const int* rstart;
+  const int* n1;
+  const int* rank;
+  const int64_t* piv_off;
+  const int* prow;
+  const double* pcx;
+  const double* pcy;
+  const int64_t* l2p_off;  // per box, local level offset
+  double* l2p_rm;
+  double* l2p_cm;
+  double* mscr;            // global LU scratch when rank > RLDS
+  const int64_t* m_off;
+  int use_lds;
+  KParams kp;
+};
+
+template <int KID>
+__global__ __launch_bounds__(256) void k_id(IdArgs a) {
+#pragma clang fp contract(off)
+  extern __shared__ double lds[];
+  __shared__ int perm[MAXR];
+  __shared__ int s_p;
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int r = a.rank[b];
+  if (r <= 0) return;
+  const int n1 = a.n1[b];
+  const int r0 = a.rstart[b];
+  const int64_t po = a.piv_off[b];
+  double* M = a.use_lds ? lds : a.mscr + a.m_off[b];
+  // M = D^T: M[i][q] = K(skeleton row q, skeleton column i)
+  for (int e = tid; e < r * r; e += blockDim.x) {
+    int i = e / r, q = e % r;
+    int rq = r0 + a.prow[po + q];
+    M[e] = kernel_xy<KID>(a.src_x[rq], a.src_y[rq], a.pcx[po + i], a.pcy[po + i], a.kp);
+  }
+  __syncthreads();
+  for (int k = 0; k < r; k++) {
+    if (tid < 64) {
+      Cand c{0.0, 0.0, -1};
+      for (int i = k + tid; i < r; i += 64) {
+        double v = M[i * r + k];
+        c = cbetter(c, Cand{fabs(v), v, i});
+      }
+      c = wave_argmax(c);
+      if (tid == 0) s_p = c.i;
+    }
+    __syncthreads();
+    const int p = s_p;
+    if (tid == 0) perm[k] = p;
+    if (p != k)
+      for (int q = tid; q < r; q += blockDim.x) {
+        double t0 = M[k * r + q];
+        M[k * r + q] = M[p * r + q];
+        M[p * r + q] = t0;
+      }
+    __syncthreads();
+    const double d = M[k * r + k];
+    if (d != 0.0)
+      for (int i = k + 1 + tid; i < r; i += blockDim.x) M[i * r + k] = M[i * r + k] / d;
+    __syncthreads();
+    const int w = r - k - 1;
+    for (int e = tid; e < w * w; e += blockDim.x) {
+      int i = k + 1 + e / w, q = k + 1 + e % w;
+      M[i * r + q] = M[i * r + q] - M[i * r + k] * M[k * r + q];
+    }
+    __syncthreads();
+  }
+  double* rm = a.l2p_rm + a.l2p_off[b];
+  double* cm = a.l2p_cm + a.l2p_off[b];
+  for (int i = tid; i < n1; i += blockDim.x) {
+    double* x = rm + (int64_t)i * r;
+    const double xi = a.src_x[r0 + i], yi = a.src_y[r0 + i];
+    for (int q = 0; q < r; q++) x[q] = kernel_xy<KID>(xi, yi, a.pcx[po + q], a.pcy[po + q], a.kp);
+    for (int k = 0; k < r; k++) {
+      int p = perm[k];
+      if (p != k) {
+        double t0 = x[k];
+        x[k] = x[p];
+        x[p] = t0;
+      }
+    }
+    for (int k = 0; k < r; k++) {
+      double s = x[k];
+      for (int q = 0; q < k; q++) s = s - M[k * r + q] * x[q];
+      x[k] = s;
+    }
+    for (int k = r - 1; k >= 0; k--) {
+      double s = x[k];
+      for (int q = k + 1; q < r; q++) s = s - M[k * r + q] * x[q];
+      x[k] = s / M[k * r + k];
+    }
+    for (int q = 0; q < r; q++) cm[i + (int64_t)q * n1] = x[q];
+  }
+}
+
+// Skeleton points of a level: skx[sloc[k] + q] = src point prow[q] of box k.
+__global__ void k_skeleton(const double* __restrict__ src_x, const double* __restrict__ src_y,
+                           const int* __restrict__ src_g, const int* __restrict__ rstart,
+                           const int* __restrict__ rank, const int64_t* __restrict__ piv_off,
+                           const int* __restrict__ prow, const int64_t* __restrict__ sloc,
+                           double* __restrict__ skx, double* __restrict__ sky,
+                           int* __restrict__ skg, int nb) {
+  const int b = blockIdx.x;
+  if (b >= nb) return;
+  const int r = rank[b];
+  for (int q = threadIdx.x; q < r; q += blockDim.x) {
+    int p = rstart[b] + prow[piv_off[b] + q];
+    skx[sloc[b] + q] = src_x[p];
+    sky[sloc[b] + q] = src_y[p];
+    skg[sloc[b] + q] = src_g ? src_g[p] : p;
+  }
+}
+
+struct M2LArgs {
+  const double* skx;
+  const double* sky;
+  const int* rank;       // per box of the level
+  const int64_t* soff;   // per box: global skeleton offset
+  const int* il;         // per box: 40 slots
+  const int64_t* m2loff; // per box (global element offset)
+  const int64_t* m2lcoff;
+  double* m2l;
+  int* m2l_src;
+  KParams kp;
+};
+
+constexpr int M2L_LDS_COLS = 4096;
+
+template <int KID>
+__global__ __launch_bounds__(256) void k_m2l_build(M2LArgs a) {
+  __shared__ int s_soff[40], s_pref[41];
+  __shared__ int ns;
+  __shared__ int s_src[M2L_LDS_COLS];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int r = a.rank[b];
+  const int64_t c0 = a.m2lcoff[b];
+  const int C = (int)(a.m2lcoff[b + 1] - c0);
+  if (tid == 0) {
+    int c = 0, p = 0;
+    for (int s = 0; s < 40; s++) {
+      int kb = a.il[40 * b + s];
+      if (kb < 0) continue;
+      s_soff[c] = (int)a.soff[kb];
+      s_pref[c] = p;
+      p += a.rank[kb];
+      c++;
+    }
+    s_pref[c] = p;
+    ns = c;
+  }
+  __syncthreads();
+  for (int c = tid; c < C; c += blockDim.x) {
+    int s = 0;
+    while (s + 1 < ns && s_pref[s + 1] <= c) s++;
+    int g = s_soff[s] + (c - s_pref[s]);
+    a.m2l_src[c0 + c] = g;
+    if (c < M2L_LDS_COLS) s_src[c] = g;
+  }
+  __syncthreads();
+  if (r == 0) return;
+  const int64_t sb = a.soff[b];
+  double* out = a.m2l + a.m2loff[b];
+  const int64_t tot = (int64_t)r * C;
+  for (int64_t e = tid; e < tot; e += blockDim.x) {
+    int c = (int)(e / r), i = (int)(e % r);
+    int g;
+    if (c < M2L_LDS_COLS) {
+      g = s_src[c];
+    } else {
+      int s = 0;
+      while (s + 1 < ns && s_pref[s + 1] <= c) s++;
+      g = s_soff[s] + (c - s_pref[s]);
+    }
+    out[e] = kernel_xy<KID>(a.skx[sb + i], a.sky[sb + i], a.skx[g], a.sky[g], a.kp);
+  }
+}
+
+template <class T>
+void upload(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
+  d.alloc(sizeof(T) * std::max<size_t>(h.size(), 1));
+  if (!h.empty()) {
+    AFMM_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+    AFMM_HIP(hipStreamSynchronize(s));  // h may go out of scope after return
+  }
+}
+
+size_t free_device_bytes() {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)8 << 30;
+  return fr;
+}
+
+}  // namespace
+
+void assemble_tree(Engine& e) {
+  const int L = e.L;
+  hipStream_t s = e.s_main;
+  e.lv.assign(L + 1, LevelInfo{});
+  // per-level skeleton points, kept until the global arrays are built
+  std::vector<DevBuf> lskx(L + 2), lsky(L + 2), lskg(L + 2), lrm(L + 1), lcm(L + 1);
+  std::vector<std::vector<int64_t>> sloc(L + 2);
+  DevBuf d_rstart, d_n1, d_n2, d_il, d_boxes, d_scroff, d_scr, d_pivoff, d_rank, d_prow, d_pcx,
+      d_pcy, d_sloc, d_l2poff, d_moff, d_mscr;
+
+  for (int j = L; j >= 2; j--) {
+    LevelInfo& li = e.lv[j];
+    li.j = j;
+    li.nb = 1 << (2 * j);
+    const int nb = li.nb;
+    std::vector<int> rstart(nb), n1(nb), n2(nb, 0);
+    const double* src_x;
+    const double* src_y;
+    const int* src_g;
+    if (j == L) {
+      for (int k = 0; k < nb; k++) {
+        rstart[k] = e.leaf_start[k];
+        n1[k] = e.leaf_count[k];
+      }
+      src_x = e.x.as<double>();
+      src_y = e.y.as<double>();
+      src_g = nullptr;
+    } else {
+      const std::vector<int64_t>& cs = sloc[j + 1];
+      for (int k = 0; k < nb; k++) {
+        rstart[k] = (int)cs[4 * k];
+        n1[k] = (int)(cs[4 * k + 4] - cs[4 * k]);
+      }
+      src_x = lskx[j + 1].as<double>();
+      src_y = lsky[j + 1].as<double>();
+      src_g = lskg[j + 1].as<int>();
+    }
+    std::vector<int> il(40 * (size_t)nb);
+    for (int k = 0; k < nb; k++) {
+      for (int q = 0; q < 16; q++) il[40 * k + q] = e.inner[j][16 * k + q];
+      for (int q = 0; q < 24; q++) il[40 * k + 16 + q] = e.outer[j][24 * k + q];
+      int c = 0;
+      for (int q = 0; q < 40; q++)
+        if (il[40 * k + q] >= 0) c += n1[il[40 * k + q]];
+      n2[k] = c;
+    }
+    std::vector<int64_t> pivoff(nb + 1, 0);
+    int maxn = 0;
+    for (int k = 0; k < nb; k++) {
+      int mn = std::min(n1[k], n2[k]);
+      pivoff[k + 1] = pivoff[k] + std::max(mn, 0);
+      maxn = std::max(maxn, std::max(n1[k], n2[k]));
+      if (mn > MAXR)
+        throw Error(6, "ACA block " + std::to_string(n1[k]) + "x" + std::to_string(n2[k]) +
+                           " exceeds the supported rank capacity");
+      if ((n1[k] + ACA_THREADS - 1) / ACA_THREADS > MAXOWN ||
+          (n2[k] + ACA_THREADS - 1) / ACA_THREADS > MAXOWN)
+        throw Error(6, "ACA block too large for one workgroup");
+    }
+    upload(d_rstart, rstart, s);
+    upload(d_n1, n1, s);
+    upload(d_n2, n2, s);
+    upload(d_il, il, s);
+    upload(d_pivoff, pivoff, s);
+    const int64_t npiv = std::max<int64_t>(pivoff[nb], 1);
+    d_rank.alloc(sizeof(int) * nb);
+    d_prow.alloc(sizeof(int) * npiv);
+    d_pcx.alloc(sizeof(double) * npiv);
+    d_pcy.alloc(sizeof(double) * npiv);
+
+    // ACA in batches bounded by a scratch budget.
+    const size_t budget = std::max<size_t>((size_t)1 << 30, free_device_bytes() / 4);
+    int k0 = 0;
+    while (k0 < nb) {
+      std::vector<int> boxes;
+      std::vector<int64_t> soffs;
+      int64_t tot = 0;
+      int k = k0;
+      for (; k < nb; k++) {
+        int mn = std::max(0, std::min(n1[k], n2[k]));
+        int64_t need = (int64_t)(n1[k] + n2[k]) * mn + 2 * (int64_t)n2[k];
+        need = (need + 31) & ~(int64_t)31;
+        if (!boxes.empty() && (size_t)(tot + need) * sizeof(double) > budget) break;
+        boxes.push_back(k);
+        soffs.push_back(tot);
+        tot += need;
+      }
+      k0 = k;
+      upload(d_boxes, boxes, s);
+      upload(d_scroff, soffs, s);
+      d_scr.alloc(sizeof(double) * std::max<int64_t>(tot, 1));
+      AcaArgs a{src_x, src_y, d_rstart.as<int>(), d_n1.as<int>(), d_n2.as<int>(), d_il.as<int>(),
+                d_boxes.as<int>(), d_scroff.as<int64_t>(), d_scr.as<double>(),
+                d_pivoff.as<int64_t>(), d_rank.as<int>(), d_prow.as<int>(), d_pcx.as<double>(),
+                d_pcy.as<double>(), e.tol, e.kp};
+      AFMM_DISPATCH_KERNEL(e.kid, KID,
+                           k_aca<KID><<<(unsigned)boxes.size(), ACA_THREADS, 0, s>>>(a));
+      AFMM_LAUNCH_CHECK();
+    }
+    li.rank.resize(nb);
+    AFMM_HIP(hipMemcpyAsync(li.rank.data(), d_rank.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+    AFMM_HIP(hipStreamSynchronize(s));
+    li.nin = n1;
+    // local skeleton offsets and L2P offsets
+    sloc[j].assign(nb + 1, 0);
+    std::vector<int64_t> l2poff(nb + 1, 0), moff(nb + 1, 0);
+    int maxr = 0;
+    for (int k = 0; k < nb; k++) {
+      int r = li.rank[k];
+      maxr = std::max(maxr, r);
+      sloc[j][k + 1] = sloc[j][k] + r;
+      l2poff[k + 1] = l2poff[k] + (int64_t)r * n1[k];
+      moff[k + 1] = moff[k] + (int64_t)r * r;
+    }
+    e.max_rank = std::max(e.max_rank, maxr);
+    li.l2poff = l2poff;
+    const int64_t nsk = std::max<int64_t>(sloc[j][nb], 1);
+    lskx[j].alloc(sizeof(double) * nsk);
+    lsky[j].alloc(sizeof(double) * nsk);
+    lskg[j].alloc(sizeof(int) * nsk);
+    upload(d_sloc, sloc[j], s);
+    k_skeleton<<<nb, 64, 0, s>>>(src_x, src_y, src_g, d_rstart.as<int>(), d_rank.as<int>(),
+                                 d_pivoff.as<int64_t>(), d_prow.as<int>(), d_sloc.as<int64_t>(),
+                                 lskx[j].as<double>(), lsky[j].as<double>(), lskg[j].as<int>(), nb);
+    AFMM_LAUNCH_CHECK();
+    // ID
+    lrm[j].alloc(sizeof(double) * std::max<int64_t>(l2poff[nb], 1));
+    lcm[j].alloc(sizeof(double) * std::max<int64_t>(l2poff[nb], 1));
+    upload(d_l2poff, l2poff, s);
+    const int use_lds = maxr <= RLDS;
+    if (!use_lds) {
+      upload(d_moff, moff, s);
+      d_mscr.alloc(sizeof(double) * std::max<int64_t>(moff[nb], 1));
+    }
+    if (maxr > MAXR) throw Error(6, "rank exceeds the supported capacity");
+    IdArgs ia{src_x, src_y, d_rstart.as<int>(), d_n1.as<int>(), d_rank.as<int>(),
+              d_pivoff.as<int64_t>(), d_prow.as<int>(), d_pcx.as<double>(), d_pcy.as<double>(),
+              d_l2poff.as<int64_t>(), lrm[j].as<double>(), lcm[j].as<double>(),
+              use_lds ? nullptr : d_mscr.as<double>(), use_lds ? nullptr : d_moff.as<int64_t>(),
+              use_lds, e.kp};
+    const size_t shm = use_lds ? sizeof(double) * (size_t)maxr * maxr : 0;
+    AFMM_DISPATCH_KERNEL(e.kid, KID, k_id<KID><<<nb, 256, shm, s>>>(ia));
+    AFMM_LAUNCH_CHECK();
+    AFMM_HIP(hipStreamSynchronize(s));
+  }
+
+  // Global numbering: levels 2..L in order, boxes in quad-tree order.
+  int64_t gb = 0, sb = 0, lb = 0;
+  for (int j = 2; j <= L; j++) {
+    LevelInfo& li = e.lv[j];
+    li.gbase = gb;
+    li.soff.resize(li.nb + 1);
+    for (int k = 0; k <= li.nb; k++) li.soff[k] = sb + sloc[j][k];
+    for (int k = 0; k <= li.nb; k++) li.l2poff[k] += lb;
+    gb += li.nb;
+    sb += sloc[j][li.nb];
+    lb = li.l2poff[li.nb];
+  }
+  const int64_t nbox = gb;
+  e.skel_total = sb;
+  e.l2p_total = lb;
+  e.skx.alloc(sizeof(double) * std::max<int64_t>(sb, 1));
+  e.sky.alloc(sizeof(double) * std::max<int64_t>(sb, 1));
+  e.skidx.alloc(sizeof(int) * std::max<int64_t>(sb, 1));
+  e.l2p_rm.alloc(sizeof(double) * std::max<int64_t>(lb, 1));
+  e.l2p_cm.alloc(sizeof(double) * std::max<int64_t>(lb, 1));
+  for (int j = 2; j <= L; j++) {
+    LevelInfo& li = e.lv[j];
+    int64_t ns = li.soff[li.nb] - li.soff[0];
+    int64_t nl = li.l2poff[li.nb] - li.l2poff[0];
+    if (ns > 0) {
+      AFMM_HIP(hipMemcpyAsync(e.skx.as<double>() + li.soff[0], lskx[j].p, sizeof(double) * ns,
+                              hipMemcpyDeviceToDevice, s));
+      AFMM_HIP(hipMemcpyAsync(e.sky.as<double>() + li.soff[0], lsky[j].p, sizeof(double) * ns,
+                              hipMemcpyDeviceToDevice, s));
+      AFMM_HIP(hipMemcpyAsync(e.skidx.as<int>() + li.soff[0], lskg[j].p, sizeof(int) * ns,
+                              hipMemcpyDeviceToDevice, s));
+    }
+    if (nl > 0) {
+      AFMM_HIP(hipMemcpyAsync(e.l2p_rm.as<double>() + li.l2poff[0], lrm[j].p,
+                              sizeof(double) * nl, hipMemcpyDeviceToDevice, s));
+      AFMM_HIP(hipMemcpyAsync(e.l2p_cm.as<double>() + li.l2poff[0], lcm[j].p,
+                              sizeof(double) * nl, hipMemcpyDeviceToDevice, s));
+    }
+  }
+  AFMM_HIP(hipStreamSynchronize(s));
+  for (int j = 0; j <= L + 1; j++) {
+    lskx[j].release(); lsky[j].release(); lskg[j].release();
+  }
+  for (int j = 0; j <= L; j++) { lrm[j].release(); lcm[j].release(); }
+  d_scr.release();
+  d_mscr.release();
+
+  // Per-box arrays over all levels, and the M2L layout.
+  std::vector<int> rank_g(nbox), nin_g(nbox);
+  std::vector<int64_t> soff_g(nbox + 1), l2poff_g(nbox + 1), m2loff_g(nbox + 1),
+      m2lcoff_g(nbox + 1);
+  int64_t mo = 0, mc = 0;
+  e.il_pairs = 0;
+  for (int j = 2; j <= L; j++) {
+    LevelInfo& li = e.lv[j];
+    li.m2loff.assign(li.nb + 1, 0);
+    li.m2lcoff.assign(li.nb + 1, 0);
+    for (int k = 0; k < li.nb; k++) {
+      int64_t g = li.gbase + k;
+      rank_g[g] = li.rank[k];
+      nin_g[g] = li.nin[k];
+      soff_g[g] = li.soff[k];
+      l2poff_g[g] = li.l2poff[k];
+      int C = 0;
+      for (int q = 0; q < 16; q++) {
+        int kb = e.inner[j][16 * k + q];
+        if (kb >= 0) { C += li.rank[kb]; e.il_pairs++; }
+      }
+      for (int q = 0; q < 24; q++) {
+        int kb = e.outer[j][24 * k + q];
+        if (kb >= 0) { C += li.rank[kb]; e.il_pairs++; }
+      }
+      li.m2loff[k] = mo;
+      li.m2lcoff[k] = mc;
+      m2loff_g[g] = mo;
+      m2lcoff_g[g] = mc;
+      mo += (int64_t)li.rank[k] * C;
+      mc += C;
+    }
+    li.m2loff[li.nb] = mo;
+    li.m2lcoff[li.nb] = mc;
+  }
+  soff_g[nbox] = sb;
+  l2poff_g[nbox] = lb;
+  m2loff_g[nbox] = mo;
+  m2lcoff_g[nbox] = mc;
+  e.m2l_total = mo;
+  e.m2l_cols = mc;
+  upload(e.rank_g, rank_g, s);
+  upload(e.nin_g, nin_g, s);
+  upload(e.soff_g, soff_g, s);
+  upload(e.l2poff_g, l2poff_g, s);
+  upload(e.m2loff_g, m2loff_g, s);
+  upload(e.m2lcoff_g, m2lcoff_g, s);
+  e.m2l.alloc(sizeof(double) * std::max<int64_t>(mo, 1));
+  e.m2l_src.alloc(sizeof(int) * std::max<int64_t>(mc, 1));
+  DevBuf d_lil;
+  for (int j = 2; j <= L; j++) {
+    LevelInfo& li = e.lv[j];
+    std::vector<int> il(40 * (size_t)li.nb);
+    for (int k = 0; k < li.nb; k++) {
+      for (int q = 0; q < 16; q++) il[40 * k + q] = e.inner[j][16 * k + q];
+      for (int q = 0; q < 24; q++) il[40 * k + 16 + q] = e.outer[j][24 * k + q];
+    }
+    upload(d_lil, il, s);
+    M2LArgs ma{e.skx.as<double>(), e.sky.as<double>(), e.rank_g.as<int>() + li.gbase,
+               e.soff_g.as<int64_t>() + li.gbase, d_lil.as<int>(),
+               e.m2loff_g.as<int64_t>() + li.gbase, e.m2lcoff_g.as<int64_t>() + li.gbase,
+               e.m2l.as<double>(), e.m2l_src.as<int>(), e.kp};
+    AFMM_DISPATCH_KERNEL(e.kid, KID, k_m2l_build<KID><<<li.nb, 256, 0, s>>>(ma));
+    AFMM_LAUNCH_CHECK();
+    AFMM_HIP(hipStreamSynchronize(s));
+  }
+  e.outv.alloc(sizeof(double) * std::max<int64_t>(sb, 1));
+  e.inv.alloc(sizeof(double) * std::max<int64_t>(sb, 1));
+  AFMM_HIP(hipMemsetAsync(e.outv.p, 0, e.outv.bytes, s));
+  AFMM_HIP(hipMemsetAsync(e.inv.p, 0, e.inv.bytes, s));
+  AFMM_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace afmm

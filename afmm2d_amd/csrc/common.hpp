@@ -1,0 +1,95 @@
+// common.hpp -- shared definitions of the MI355X AFMM2D engine: HIP error
+// plumbing, the built-in kernel registry (device and host), launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace afmm {
+
+// Error carried from the implementation to the C ABI.
+struct Error : std::runtime_error {
+  int status;
+  Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define AFMM_HIP(call)                                                                 \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw ::afmm::Error(e_ == hipErrorOutOfMemory ? 4 : 5,                           \
+                          std::string(#call) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+#define AFMM_LAUNCH_CHECK() AFMM_HIP(hipGetLastError())
+
+// Kernel ids match afmm2d_kernel_id.
+enum KernelId { K_LOG = 0, K_INV_R = 1, K_GAUSS = 2, K_REG_INV_R = 3 };
+
+// Parameters every kernel needs: p0 = Gaussian width (s^2), a = the
+// regularisation radius of K_REG_INV_R (0.5^L).
+struct KParams {
+  double p0;
+  double a;
+};
+
+// Kernel value from the squared distance.  R2 == 0 (the diagonal) gives the
+// kernel's diagonal value: 0 for the singular kernels, exp(0) = 1 for the
+// Gaussian, 0/a for K_REG_INV_R.  Shared by host and device so the host
+// plug-in and the device kernels evaluate the same formula.
+template <int KID>
+__host__ __device__ __forceinline__ double kernel_r2(double r2, const KParams& p) {
+  if constexpr (KID == K_LOG) {
+    return r2 == 0.0 ? 0.0 : 0.5 * log(r2);
+  } else if constexpr (KID == K_INV_R) {
+    return r2 == 0.0 ? 0.0 : 1.0 / sqrt(r2);
+  } else if constexpr (KID == K_GAUSS) {
+    return exp(-r2 / p.p0);
+  } else {
+    double r = sqrt(r2);
+    return r < p.a ? r / p.a : p.a / r;
+  }
+}
+
+// Squared distance with separate multiplies and an add (no fused
+// multiply-add), matching the host evaluation bit for bit.
+__host__ __device__ __forceinline__ double dist2_exact(double xi, double yi, double xj, double yj) {
+#pragma clang fp contract(off)
+  double dx = xi - xj;
+  double dy = yi - yj;
+  double a = dx * dx;
+  double b = dy * dy;
+  return a + b;
+}
+
+template <int KID>
+__host__ __device__ __forceinline__ double kernel_xy(double xi, double yi, double xj, double yj,
+                                                     const KParams& p) {
+  return kernel_r2<KID>(dist2_exact(xi, yi, xj, yj), p);
+}
+
+inline double kernel_host(int kid, const KParams& p, double xi, double yi, double xj, double yj) {
+  switch (kid) {
+    case K_LOG: return kernel_xy<K_LOG>(xi, yi, xj, yj, p);
+    case K_INV_R: return kernel_xy<K_INV_R>(xi, yi, xj, yj, p);
+    case K_GAUSS: return kernel_xy<K_GAUSS>(xi, yi, xj, yj, p);
+    default: return kernel_xy<K_REG_INV_R>(xi, yi, xj, yj, p);
+  }
+}
+
+// Dispatch a templated launcher on the runtime kernel id.
+#define AFMM_DISPATCH_KERNEL(kid, KID, ...)                       \
+  switch (kid) {                                                  \
+    case ::afmm::K_LOG: { constexpr int KID = 0; __VA_ARGS__; break; }     \
+    case ::afmm::K_INV_R: { constexpr int KID = 1; __VA_ARGS__; break; }   \
+    case ::afmm::K_GAUSS: { constexpr int KID = 2; __VA_ARGS__; break; }   \
+    default: { constexpr int KID = 3; __VA_ARGS__; break; }                \
+  }
+
+inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace afmm

@@ -1,0 +1,170 @@
+/*
+ * afmm2d.h -- C ABI of the MI355X AFMM2D matvec engine (libafmm2d.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns an afmm2d_status;
+ * afmm2d_last_error() gives a message for the calling thread's last failure.
+ * Handles are not thread-safe: one caller thread per handle.  Host-pointer
+ * entry points synchronise before returning; afmm2d_matvec_device is ordered
+ * on the stream it is given.
+ *
+ * Each declaration names the reference interface it replaces (file:line in
+ * the vaishna77/AFMM2D repository).
+ */
+#ifndef AFMM2D_H
+#define AFMM2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  AFMM2D_OK = 0,
+  AFMM2D_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad size, bad kernel id, non-finite input */
+  AFMM2D_ERR_LEVELS = 2,           /* nLevels = log4(N/M) < 2: no far field (AFMM2D.cpp:7) */
+  AFMM2D_ERR_STATE = 3,            /* e.g. matvec before assemble */
+  AFMM2D_ERR_OUT_OF_MEMORY = 4,
+  AFMM2D_ERR_HIP = 5,              /* a HIP runtime call failed */
+  AFMM2D_ERR_UNSUPPORTED = 6,      /* a size beyond what this build handles */
+  AFMM2D_ERR_NOT_FOUND = 7         /* missing device or symbol */
+} afmm2d_status;
+
+/* Built-in kernels.  The reference takes any user-defined
+ * userkernel::getMatrixEntry (kernel.hpp:58, defined in the user's file,
+ * e.g. examples/testAFMM2D.cpp:9-20); device code needs the kernel compiled
+ * in, so the engine ships this registry.  R2 = |x_i - x_j|^2. */
+typedef enum {
+  AFMM2D_KERNEL_LOG = 0,       /* 0.5*log(R2), 0 when R2 == 0 */
+  AFMM2D_KERNEL_INV_R = 1,     /* 1/sqrt(R2), 0 when R2 == 0 */
+  AFMM2D_KERNEL_GAUSSIAN = 2,  /* exp(-R2/kparams[0]), kparams[0] defaults to 0.01 */
+  AFMM2D_KERNEL_REG_INV_R = 3  /* R < a ? R/a : a/R, a = 0.5^nLevels (testAFMM2D.cpp:9-20) */
+} afmm2d_kernel_id;
+
+typedef enum {
+  AFMM2D_LOC_INTERLEAVED = 0,  /* x0,y0,x1,y1,... (N x 2 row-major) */
+  AFMM2D_LOC_COLUMN_MAJOR = 1  /* all x then all y (Eigen::MatrixXd N x 2) */
+} afmm2d_loc_layout;
+
+typedef struct afmm2d_handle_s* afmm2d_handle;
+
+typedef struct {
+  int n;                     /* points */
+  int levels;                /* nLevels */
+  int tol_pow;
+  int kernel_id;
+  int max_rank;              /* over all boxes */
+  int64_t skeleton_total;    /* sum of ranks over levels 2..L */
+  int64_t il_pairs;          /* ordered interaction-list pairs, levels 2..L */
+  int64_t near_pairs;        /* ordered leaf neighbour pairs incl. self */
+  int64_t m2l_bytes;         /* stored M2L coefficients */
+  int64_t l2p_bytes;         /* stored L2P coefficients (one layout) */
+  int64_t matvec_bytes;      /* algorithmic HBM bytes of one matvec */
+  double create_seconds;     /* KD partition + topology */
+  double assemble_seconds;   /* ACA + ID + M2L build */
+} afmm2d_stats;
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* afmm2d_last_error(void);
+const char* afmm2d_status_string(int status);
+
+/* Replaces AFMM::AFMM (AFMM2D.cpp:3-39): nLevels = log4(N/M) (line 7), KD
+ * partition (KDTree.cpp:200-243), tree and interaction lists
+ * (AFMM2DTree.cpp:3-62, 65-967), leaf point ranges (1099-1118).  loc is a
+ * host array of N points in the given layout.  kparams may be NULL (4
+ * doubles otherwise).  device < 0 uses the current HIP device. */
+int afmm2d_create(int n, int min_particles_in_leaf, int tol_pow, const double* loc,
+                  int loc_layout, int kernel_id, const double* kparams, int device,
+                  afmm2d_handle* out);
+
+/* Replaces AFMM::~AFMM (AFMM2D.cpp:76-82). */
+int afmm2d_destroy(afmm2d_handle h);
+
+/* Replaces AFMM::assemble (AFMM2D.cpp:41-44): per-box ACA skeletons and L2P
+ * (AFMM2DTree.cpp:991-1068), M2L blocks (1070-1097). */
+int afmm2d_assemble(afmm2d_handle h);
+
+/* Replaces AFMM::computeMatVecProduct (AFMM2D.cpp:46-65): y = K b for nrhs
+ * right-hand sides, host arrays in the caller's point order, column-major
+ * with leading dimension n. */
+int afmm2d_matvec(afmm2d_handle h, const double* b, double* y, int nrhs);
+
+/* Same on device-resident vectors (one RHS each call), ordered on 'stream'
+ * (a hipStream_t; NULL = the null stream).  Nothing is copied to the host. */
+int afmm2d_matvec_device(afmm2d_handle h, const double* d_b, double* d_y, void* stream);
+
+/* nLevels (AFMM2D.hpp:20). */
+int afmm2d_levels(afmm2d_handle h, int* levels);
+
+/* sorted_Properties (AFMM2D.hpp:24): perm[i] = caller index of KD-sorted point i. */
+int afmm2d_get_permutation(afmm2d_handle h, int64_t* perm);
+
+/* sorted_Locations (AFMM2D.hpp:23), interleaved x,y in KD order. */
+int afmm2d_get_sorted_locations(afmm2d_handle h, double* xy);
+
+/* NumberOfParticlesInLeaves in KD order (KDTree.cpp:122-132), 4^L entries. */
+int afmm2d_get_leaf_counts(afmm2d_handle h, int* counts);
+
+/* Leaf box 'box' owns KD-sorted points [start, start+count)
+ * (AFMM2DTree.cpp:1099-1118). */
+int afmm2d_get_leaf_range(afmm2d_handle h, int box, int* start, int* count);
+
+/* Interaction lists of every box of 'level' (FMM2DBox::neighborNumbers,
+ * innerNumbers, outerNumbers; AFMM2DBox.hpp:17-19): arrays of 8, 16, 24 ints
+ * per box, -1 = absent.  Any pointer may be NULL. */
+int afmm2d_get_interactions(afmm2d_handle h, int level, int* nbr, int* inner, int* outer);
+
+/* ACA ranks of every box of 'level' (levels 2..L), after assemble. */
+int afmm2d_get_ranks(afmm2d_handle h, int level, int* ranks);
+
+/* incoming_checkPoints of a box (AFMM2DBox.hpp:35): skeleton as KD-sorted
+ * point indices in pivot order.  *count receives the rank. */
+int afmm2d_get_skeleton(afmm2d_handle h, int level, int box, int* idx, int capacity, int* count);
+
+/* L2P of a box (AFMM2DBox.hpp:25), rows x rank, row-major. */
+int afmm2d_get_l2p(afmm2d_handle h, int level, int box, double* out, int64_t capacity, int* rows,
+                   int* cols);
+
+/* M2L[kIL] of a box (AFMM2DBox.hpp:24) for interaction slot 'slot'
+ * (0-15 innerNumbers, 16-39 outerNumbers), rank x rank(kIL), row-major.
+ * Returns AFMM2D_ERR_NOT_FOUND if the slot is empty. */
+int afmm2d_get_m2l(afmm2d_handle h, int level, int box, int slot, double* out, int64_t capacity,
+                   int* rows, int* cols);
+
+/* Dense direct sum y = K b on the device (the check of tutorial.cpp:97-131),
+ * host arrays in caller order. */
+int afmm2d_direct(afmm2d_handle h, const double* b, double* y);
+
+/* Rows of the dense direct sum: out[r] = sum_j K(rows[r], j) b[j], rows given
+ * in caller order. */
+int afmm2d_direct_rows(afmm2d_handle h, int nrows, const int64_t* rows, const double* b,
+                       double* out);
+
+/* FMM2DTree::compute_error (AFMM2DTree.cpp:1227-1236): relative 2-norm error
+ * of leaf 'box' against the direct sum, for the last matvec's first RHS. */
+int afmm2d_compute_error(afmm2d_handle h, int box, double* err);
+
+/* AFMM::evaluateError (AFMM2D.cpp:67-74) with an explicit seed instead of
+ * srand(time(NULL)): picks one leaf box, reports it and its error. */
+int afmm2d_evaluate_error(afmm2d_handle h, unsigned seed, int* box, double* err);
+
+/* kernel::getMatrixEntry on the KD-sorted points (kernel.hpp:23-26), host side. */
+int afmm2d_get_matrix_entry(afmm2d_handle h, int i, int j, double* value);
+
+/* The built-in kernel on explicit coordinates, host side. */
+int afmm2d_kernel_value(int kernel_id, const double* kparams, double a, double xi, double yi,
+                        double xj, double yj, double* value);
+
+int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st);
+
+/* Per-kernel device timing with HIP events recorded on the stream each
+ * kernel runs on.  names: "gather", "m2m", "m2l", "l2l", "p2p", "l2p_out". */
+int afmm2d_set_profiling(afmm2d_handle h, int enable);
+int afmm2d_get_kernel_time(afmm2d_handle h, const char* name, double* total_ms, int64_t* launches);
+int afmm2d_reset_kernel_times(afmm2d_handle h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

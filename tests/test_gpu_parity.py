@@ -1,0 +1,78 @@
+"""GPU parity: the HIP path against the CPU oracle on the same seeded inputs."""
+import numpy as np
+import pytest
+
+from conftest import uniform_points
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,leaf,seed,quant", [
+    (4096, 64, 1, 0), (5000, 64, 2, 0), (70001, 100, 3, 0), (20000, 64, 4, 64), (3000, 40, 5, 8)])
+def test_kd_partition_matches_oracle(engine, oracle_mod, n, leaf, seed, quant):
+    xy, _ = uniform_points(n, seed)
+    if quant:  # many equal coordinates: exercises the reference's tie order
+        xy = np.round(xy * quant) / quant
+    L = oracle_mod.nlevels(n, leaf)
+    sxy, perm, counts, boxnum = oracle_mod.kd_sort(xy, L)
+    a = engine.AFMM(n, leaf, 8, xy)
+    assert a.nLevels == L
+    np.testing.assert_array_equal(a.sorted_Properties, perm)
+    np.testing.assert_array_equal(a.sorted_Locations, sxy)
+    np.testing.assert_array_equal(a.NumberOfParticlesInLeaves, counts)
+    a.close()
+
+
+def test_topology_matches_oracle(engine, oracle_mod):
+    xy, _ = uniform_points(4 ** 5 * 4, 7)
+    a = engine.AFMM(xy.shape[0], 4, 8, xy)
+    assert a.nLevels == 5
+    for level in range(0, 6):
+        got = a.interactions(level)
+        ref = oracle_mod.topology(level)
+        for g, r in zip(got, ref):
+            np.testing.assert_array_equal(g, r)
+    a.close()
+
+
+@pytest.mark.parametrize("n,leaf,tol_pow,kernel", [
+    (4096, 64, 10, "log"), (4096, 64, 8, "log"), (16384, 64, 12, "log"),
+    (8192, 64, 10, "inv_r"), (8192, 64, 10, "reg_inv_r"), (8192, 64, 6, "gaussian")])
+def test_assembly_and_matvec_match_oracle(engine, oracle_mod, n, leaf, tol_pow, kernel):
+    kid = engine.KERNELS[kernel]
+    xy, q = uniform_points(n, 11 + tol_pow)
+    T = oracle_mod.Tree(xy, leaf, tol_pow, kid)
+    T.assemble()
+    a = engine.AFMM(n, leaf, tol_pow, xy, kernel=kernel)
+    a.assemble()
+    mism = 0
+    tot = 0
+    for j in range(2, T.L + 1):
+        rg, ro = a.ranks(j), T.ranks(j)
+        mism += int((rg != ro).sum())
+        tot += rg.size
+    assert mism == 0, "rank mismatches %d of %d" % (mism, tot)
+    for j in (2, T.L):
+        for box in (0, 4 ** j // 3, 4 ** j - 1):
+            np.testing.assert_array_equal(a.skeleton(j, box), T.skeleton(j, box))
+            P, Po = a.l2p(j, box), T.l2p(j, box)
+            assert P.shape == Po.shape
+            if P.size:
+                assert np.abs(P - Po).max() <= 1e-9 * max(1.0, np.abs(Po).max())
+    y = a.computeMatVecProduct(q)
+    yo = T.matvec(q)
+    rel = np.linalg.norm(y - yo) / np.linalg.norm(yo)
+    assert rel <= 1e-12, rel
+    T.close()
+    a.close()
+
+
+def test_direct_sum_matches_oracle(engine, oracle_mod):
+    xy, q = uniform_points(4096, 21)
+    T = oracle_mod.Tree(xy, 64, 10)
+    a = engine.AFMM(4096, 64, 10, xy)
+    yd = a.direct(q)
+    yo = T.direct(q)
+    assert np.linalg.norm(yd - yo) / np.linalg.norm(yo) <= 1e-13
+    T.close()
+    a.close()
