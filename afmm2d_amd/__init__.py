@@ -56,6 +56,7 @@ class Stats(ctypes.Structure):
         ("kernel_id", ctypes.c_int), ("max_rank", ctypes.c_int),
         ("skeleton_total", ctypes.c_int64), ("il_pairs", ctypes.c_int64),
         ("near_pairs", ctypes.c_int64), ("m2l_bytes", ctypes.c_int64),
+        ("m2l_columns", ctypes.c_int64),
         ("l2p_bytes", ctypes.c_int64), ("matvec_bytes", ctypes.c_int64),
         ("create_seconds", ctypes.c_double), ("assemble_seconds", ctypes.c_double),
     ]

@@ -306,7 +306,6 @@ int afmm2d_matvec_device(afmm2d_handle h, const double* d_b, double* d_y, void* 
     AFMM_HIP(hipSetDevice(e.device));
     hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)0;
     matvec_device(e, d_b, d_y, s);
-    if (e.profiling) e.prof_collect();
   });
 }
 
@@ -563,6 +562,7 @@ int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st) {
   st->il_pairs = e.il_pairs;
   st->near_pairs = e.near_pairs;
   st->m2l_bytes = 8 * e.m2l_total;
+  st->m2l_columns = e.m2l_cols;
   st->l2p_bytes = 8 * e.l2p_total;
   // One matvec reads q (gather) and the coordinates (near field), L2P twice
   // (M2M and L2L), every stored M2L coefficient and its column index once,
@@ -584,6 +584,11 @@ int afmm2d_set_profiling(afmm2d_handle h, int enable) {
 
 int afmm2d_get_kernel_time(afmm2d_handle h, const char* name, double* total_ms, int64_t* launches) {
   if (!h || !name || !total_ms || !launches) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  int st = guarded([&] {
+    AFMM_HIP(hipSetDevice(h->e->device));
+    h->e->prof_collect();
+  });
+  if (st != AFMM2D_OK) return st;
   auto it = h->e->timers.find(name);
   if (it == h->e->timers.end()) {
     *total_ms = 0.0;
@@ -597,8 +602,11 @@ int afmm2d_get_kernel_time(afmm2d_handle h, const char* name, double* total_ms, 
 
 int afmm2d_reset_kernel_times(afmm2d_handle h) {
   if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL handle");
-  h->e->timers.clear();
-  return AFMM2D_OK;
+  return guarded([&] {
+    AFMM_HIP(hipSetDevice(h->e->device));
+    h->e->prof_collect();
+    h->e->timers.clear();
+  });
 }
 
 }  // extern "C"

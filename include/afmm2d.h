@@ -58,6 +58,7 @@ typedef struct {
   int64_t il_pairs;          /* ordered interaction-list pairs, levels 2..L */
   int64_t near_pairs;        /* ordered leaf neighbour pairs incl. self */
   int64_t m2l_bytes;         /* stored M2L coefficients */
+  int64_t m2l_columns;       /* M2L columns over all target boxes (one index each) */
   int64_t l2p_bytes;         /* stored L2P coefficients (one layout) */
   int64_t matvec_bytes;      /* algorithmic HBM bytes of one matvec */
   double create_seconds;     /* KD partition + topology */
