@@ -32,7 +32,7 @@ def common_flags():
     return [
         "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-Wno-unused-command-line-argument",
+        "-munsafe-fp-atomics", "-Wno-unused-command-line-argument",
         "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
     ]
 

@@ -3,6 +3,7 @@
 // work runs in the HIP kernels of kd_sort.hip, assemble.hip and matvec.hip;
 // this file only moves data and builds index tables.
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -204,6 +205,8 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
     return fail(AFMM2D_ERR_LEVELS, "nLevels = log4(N/M) = " + std::to_string(L) +
                                        " < 2: no interaction lists exist");
   if (L > 13) return fail(AFMM2D_ERR_UNSUPPORTED, "more than 13 levels");
+  if (n / (1 << (2 * L)) + 1 > 512)
+    return fail(AFMM2D_ERR_UNSUPPORTED, "leaves above 512 points are not supported");
   return guarded([&] {
     auto e = std::make_unique<Engine>();
     if (device < 0) AFMM_HIP(hipGetDevice(&device));
@@ -218,6 +221,10 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
     e->kp.p0 = (kparams && kparams[0] > 0.0) ? kparams[0] : 0.01;
     e->kp.a = std::ldexp(1.0, -L);  // a = 0.5^L (AFMM2DTree.cpp:13-19)
     if (kparams && kernel_id == AFMM2D_KERNEL_REG_INV_R && kparams[1] > 0.0) e->kp.a = kparams[1];
+    AFMM_HIP(hipDeviceGetAttribute(&e->num_cus, hipDeviceAttributeMultiprocessorCount, device));
+    if (const char* v = std::getenv("AFMM2D_NEAR_OVERLAP")) e->near_overlap = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AFMM2D_M2L_BLOCKS_PER_CU"))
+      e->m2l_blocks_per_cu = std::max(1, std::min(8, std::atoi(v)));
     AFMM_HIP(hipStreamCreateWithFlags(&e->s_main, hipStreamNonBlocking));
     AFMM_HIP(hipStreamCreateWithFlags(&e->s_aux, hipStreamNonBlocking));
     AFMM_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
@@ -257,6 +264,69 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
     for (int j = 0; j <= L; j++) topology_level(j, e->nbr[j], e->inner[j], e->outer[j]);
     e->near_pairs = nleaf;
     for (int v : e->nbr[L]) e->near_pairs += v >= 0;
+    {
+      // unordered near-field pairs: self pairs and neighbour pairs A < B
+      std::vector<int> pair, ppart, slot(9 * (size_t)nleaf, -1);
+      std::vector<int64_t> self_off(nleaf, -1);
+      std::map<int64_t, int> pidx;
+      int64_t off = 0;
+      for (int a = 0; a < nleaf; a++) {
+        pair.push_back(a);
+        pair.push_back(a);
+        ppart.push_back((int)off);
+        slot[9 * (size_t)a] = (int)off;
+        off += e->leaf_count[a];
+        for (int q = 0; q < 8; q++) {
+          int b = e->nbr[L][8 * (size_t)a + q];
+          if (b <= a) continue;
+          pidx[((int64_t)a << 32) | b] = (int)ppart.size();
+          pair.push_back(a);
+          pair.push_back(b);
+          ppart.push_back((int)off);
+          off += e->leaf_count[a] + e->leaf_count[b];
+        }
+      }
+      for (int a = 0; a < nleaf; a++)
+        for (int q = 0; q < 8; q++) {
+          int b = e->nbr[L][8 * (size_t)a + q];
+          if (b < 0) continue;
+          if (a < b) slot[9 * (size_t)a + 1 + q] = ppart[pidx.at(((int64_t)a << 32) | b)];
+          else slot[9 * (size_t)a + 1 + q] = ppart[pidx.at(((int64_t)b << 32) | a)] + e->leaf_count[b];
+        }
+      if (off > INT32_MAX) throw Error(AFMM2D_ERR_UNSUPPORTED, "near-field partials exceed 2^31");
+      e->near_upairs = (int64_t)ppart.size();
+      e->near_part_total = off;
+      e->near_pair.alloc(sizeof(int) * pair.size());
+      e->near_pair_part.alloc(sizeof(int) * ppart.size());
+      e->near_slot_part.alloc(sizeof(int) * slot.size());
+      e->near_part.alloc(sizeof(double) * std::max<int64_t>(off, 1));
+      AFMM_HIP(hipMemcpyAsync(e->near_pair.p, pair.data(), sizeof(int) * pair.size(),
+                              hipMemcpyHostToDevice, e->s_main));
+      AFMM_HIP(hipMemcpyAsync(e->near_pair_part.p, ppart.data(), sizeof(int) * ppart.size(),
+                              hipMemcpyHostToDevice, e->s_main));
+      AFMM_HIP(hipMemcpyAsync(e->near_slot_part.p, slot.data(), sizeof(int) * slot.size(),
+                              hipMemcpyHostToDevice, e->s_main));
+      AFMM_HIP(hipStreamSynchronize(e->s_main));
+    }
+    // Table of the near-field log: (RN(1/c_j), -log RN(1/c_j)) for
+    // c_j = 1 + (j + 1/2)/128, evaluated in extended precision; ln2 split so
+    // that e * ln2_hi is exact.
+    {
+      std::vector<double> tab(256);
+      for (int j = 0; j < 128; j++) {
+        long double c = 1.0L + ((long double)j + 0.5L) / 128.0L;
+        double invc = (double)(1.0L / c);
+        tab[2 * j] = invc;
+        tab[2 * j + 1] = (double)(-logl((long double)invc));
+      }
+      const long double ln2 = logl(2.0L);
+      e->ln2_hi = (double)(float)ln2;
+      e->ln2_lo = (double)(ln2 - (long double)e->ln2_hi);
+      e->log_tab.alloc(sizeof(double) * tab.size());
+      AFMM_HIP(hipMemcpyAsync(e->log_tab.p, tab.data(), sizeof(double) * tab.size(),
+                              hipMemcpyHostToDevice, e->s_main));
+      AFMM_HIP(hipStreamSynchronize(e->s_main));
+    }
     hipStream_t s = e->s_main;
     e->d_leaf_start.alloc(sizeof(int) * nleaf);
     e->d_leaf_count.alloc(sizeof(int) * nleaf);
@@ -435,24 +505,27 @@ int afmm2d_get_m2l(afmm2d_handle h, int level, int box, int slot, double* out, i
   auto slot_box = [&](int s) {
     return s < 16 ? e.inner[level][16 * (size_t)box + s] : e.outer[level][24 * (size_t)box + s - 16];
   };
-  if (slot_box(slot) < 0) return fail(AFMM2D_ERR_NOT_FOUND, "empty interaction slot");
+  const int kb = slot_box(slot);
+  if (kb < 0) return fail(AFMM2D_ERR_NOT_FOUND, "empty interaction slot");
   return guarded([&] {
-    int col0 = 0;
-    for (int s = 0; s < slot; s++) {
-      int kb = slot_box(s);
-      if (kb >= 0) col0 += li.rank[kb];
-    }
-    const int r = li.rank[box], rc = li.rank[slot_box(slot)];
-    const int C = (int)(li.m2lcoff[box + 1] - li.m2lcoff[box]);
+    // position of this slot among the box's present slots
+    int t = e.h_red_start[li.gbase + box];
+    for (int q = 0; q < slot; q++) t += slot_box(q) >= 0;
+    const int p = e.h_red_pair[t];
+    const int r = li.rank[box], rc = li.rank[kb];
     *rows = r;
     *cols = rc;
-    if ((int64_t)r * rc > capacity) throw Error(AFMM2D_ERR_INVALID_ARGUMENT, "capacity too small");
-    std::vector<double> blk((size_t)r * C);
+    if ((int64_t)r * rc > capacity || (!out && r * rc > 0))
+      throw Error(AFMM2D_ERR_INVALID_ARGUMENT, "capacity too small");
+    std::vector<double> blk((size_t)r * rc);
     if (!blk.empty())
-      AFMM_HIP(hipMemcpy(blk.data(), e.m2l.as<double>() + li.m2loff[box], sizeof(double) * blk.size(),
-                         hipMemcpyDeviceToHost));
+      AFMM_HIP(hipMemcpy(blk.data(), e.m2l.as<double>() + e.h_pair_off[p],
+                         sizeof(double) * blk.size(), hipMemcpyDeviceToHost));
+    // the pair stores K(skeleton_lo, skeleton_hi) column-major
+    const bool first = e.h_pair_k[p] == (int)(li.gbase + box);
     for (int i = 0; i < r; i++)
-      for (int c = 0; c < rc; c++) out[(size_t)i * rc + c] = blk[(size_t)(col0 + c) * r + i];
+      for (int c = 0; c < rc; c++)
+        out[(size_t)i * rc + c] = first ? blk[(size_t)c * r + i] : blk[(size_t)i * rc + c];
   });
 }
 
@@ -567,9 +640,15 @@ int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st) {
   // One matvec reads q (gather) and the coordinates (near field), L2P twice
   // (M2M and L2L), every stored M2L coefficient and its column index once,
   // the multipole/local vectors, and writes y.
+  // One matvec reads b and perm (gather), writes and rereads q in KD order,
+  // reads the coordinates (near field), writes and rereads the near-field
+  // potentials, reads perm again and writes y; it reads L2P twice (M2M, L2L),
+  // every stored M2L block once, writes and rereads the M2L partials, and
+  // touches the multipole/local vectors a few times.
   st->matvec_bytes = e.assembled
-                         ? (int64_t)e.n * (8 + 4 + 8 + 16 + 8 + 4 + 8 + 8) + 2 * 8 * e.l2p_total +
-                               8 * e.m2l_total + 4 * e.m2l_cols + 4 * 8 * e.skel_total
+                         ? (int64_t)e.n * (8 + 4 + 8 + 8 + 16 + 8 + 8 + 4 + 8) +
+                               2 * 8 * e.l2p_total + 8 * e.m2l_total + 16 * e.m2l_cols +
+                               4 * e.il_pairs + 6 * 8 * e.skel_total
                          : 0;
   st->create_seconds = e.create_seconds;
   st->assemble_seconds = e.assemble_seconds;

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <numeric>
+#include <unordered_map>
 
 #include "engine.hpp"
 
@@ -391,67 +392,22 @@ __global__ void k_skeleton(const double* __restrict__ src_x, const double* __res
   }
 }
 
-struct M2LArgs {
-  const double* skx;
-  const double* sky;
-  const int* rank;       // per box of the level
-  const int64_t* soff;   // per box: global skeleton offset
-  const int* il;         // per box: 40 slots
-  const int64_t* m2loff; // per box (global element offset)
-  const int64_t* m2lcoff;
-  double* m2l;
-  int* m2l_src;
-  KParams kp;
-};
-
-constexpr int M2L_LDS_COLS = 4096;
-
+// M2L block of one unordered interaction pair (k, b), k < b:
+// B[i + c * rank_k] = K(skeleton_k[i], skeleton_b[c]).
 template <int KID>
-__global__ __launch_bounds__(256) void k_m2l_build(M2LArgs a) {
-  __shared__ int s_soff[40], s_pref[41];
-  __shared__ int ns;
-  __shared__ int s_src[M2L_LDS_COLS];
-  const int b = blockIdx.x, tid = threadIdx.x;
-  const int r = a.rank[b];
-  const int64_t c0 = a.m2lcoff[b];
-  const int C = (int)(a.m2lcoff[b + 1] - c0);
-  if (tid == 0) {
-    int c = 0, p = 0;
-    for (int s = 0; s < 40; s++) {
-      int kb = a.il[40 * b + s];
-      if (kb < 0) continue;
-      s_soff[c] = (int)a.soff[kb];
-      s_pref[c] = p;
-      p += a.rank[kb];
-      c++;
-    }
-    s_pref[c] = p;
-    ns = c;
-  }
-  __syncthreads();
-  for (int c = tid; c < C; c += blockDim.x) {
-    int s = 0;
-    while (s + 1 < ns && s_pref[s + 1] <= c) s++;
-    int g = s_soff[s] + (c - s_pref[s]);
-    a.m2l_src[c0 + c] = g;
-    if (c < M2L_LDS_COLS) s_src[c] = g;
-  }
-  __syncthreads();
-  if (r == 0) return;
-  const int64_t sb = a.soff[b];
-  double* out = a.m2l + a.m2loff[b];
-  const int64_t tot = (int64_t)r * C;
-  for (int64_t e = tid; e < tot; e += blockDim.x) {
-    int c = (int)(e / r), i = (int)(e % r);
-    int g;
-    if (c < M2L_LDS_COLS) {
-      g = s_src[c];
-    } else {
-      int s = 0;
-      while (s + 1 < ns && s_pref[s + 1] <= c) s++;
-      g = s_soff[s] + (c - s_pref[s]);
-    }
-    out[e] = kernel_xy<KID>(a.skx[sb + i], a.sky[sb + i], a.skx[g], a.sky[g], a.kp);
+__global__ __launch_bounds__(64) void k_m2l_pair_build(
+    const double* __restrict__ skx, const double* __restrict__ sky, const int* __restrict__ rank,
+    const int64_t* __restrict__ soff, const int* __restrict__ pk, const int* __restrict__ pb,
+    const int64_t* __restrict__ poff, double* __restrict__ m2l, KParams kp) {
+  const int64_t p = blockIdx.x;
+  const int k = pk[p], b = pb[p];
+  const int rk = rank[k], rb = rank[b];
+  const int64_t sk = soff[k], sb = soff[b];
+  double* out = m2l + poff[p];
+  const int tot = rk * rb;
+  for (int e = threadIdx.x; e < tot; e += 64) {
+    const int c = e / rk, i = e - c * rk;
+    out[e] = kernel_xy<KID>(skx[sk + i], sky[sk + i], skx[sb + c], sky[sb + c], kp);
   }
 }
 
@@ -665,72 +621,118 @@ void assemble_tree(Engine& e) {
   d_scr.release();
   d_mscr.release();
 
-  // Per-box arrays over all levels, and the M2L layout.
+  // Per-box arrays over all levels.
   std::vector<int> rank_g(nbox), nin_g(nbox);
-  std::vector<int64_t> soff_g(nbox + 1), l2poff_g(nbox + 1), m2loff_g(nbox + 1),
-      m2lcoff_g(nbox + 1);
-  int64_t mo = 0, mc = 0;
-  e.il_pairs = 0;
+  std::vector<int64_t> soff_g(nbox + 1), l2poff_g(nbox + 1);
   for (int j = 2; j <= L; j++) {
     LevelInfo& li = e.lv[j];
-    li.m2loff.assign(li.nb + 1, 0);
-    li.m2lcoff.assign(li.nb + 1, 0);
     for (int k = 0; k < li.nb; k++) {
       int64_t g = li.gbase + k;
       rank_g[g] = li.rank[k];
       nin_g[g] = li.nin[k];
       soff_g[g] = li.soff[k];
       l2poff_g[g] = li.l2poff[k];
-      int C = 0;
-      for (int q = 0; q < 16; q++) {
-        int kb = e.inner[j][16 * k + q];
-        if (kb >= 0) { C += li.rank[kb]; e.il_pairs++; }
-      }
-      for (int q = 0; q < 24; q++) {
-        int kb = e.outer[j][24 * k + q];
-        if (kb >= 0) { C += li.rank[kb]; e.il_pairs++; }
-      }
-      li.m2loff[k] = mo;
-      li.m2lcoff[k] = mc;
-      m2loff_g[g] = mo;
-      m2lcoff_g[g] = mc;
-      mo += (int64_t)li.rank[k] * C;
-      mc += C;
     }
-    li.m2loff[li.nb] = mo;
-    li.m2lcoff[li.nb] = mc;
   }
   soff_g[nbox] = sb;
   l2poff_g[nbox] = lb;
-  m2loff_g[nbox] = mo;
-  m2lcoff_g[nbox] = mc;
-  e.m2l_total = mo;
-  e.m2l_cols = mc;
   upload(e.rank_g, rank_g, s);
   upload(e.nin_g, nin_g, s);
   upload(e.soff_g, soff_g, s);
   upload(e.l2poff_g, l2poff_g, s);
-  upload(e.m2loff_g, m2loff_g, s);
-  upload(e.m2lcoff_g, m2lcoff_g, s);
-  e.m2l.alloc(sizeof(double) * std::max<int64_t>(mo, 1));
-  e.m2l_src.alloc(sizeof(int) * std::max<int64_t>(mc, 1));
-  DevBuf d_lil;
+
+  // M2L layout: every unordered interaction pair once (k < b), in the order
+  // the boxes' slots first meet them; each box then lists, in slot order,
+  // which pair partial feeds its local coefficients.
+  e.h_pair_k.clear();
+  e.h_pair_b.clear();
+  e.h_pair_off.clear();
+  e.h_pair_part.clear();
+  e.h_red_start.assign(nbox + 1, 0);
+  e.h_red_pair.clear();
+  std::vector<int> red_part;
+  std::unordered_map<int64_t, int> pair_of;
+  int64_t off = 0, part = 0;
+  int tile = 2;
+  e.il_pairs = 0;
+  e.m2l_total = 0;
+  auto slot_box = [&](int j, int k, int q) {
+    return q < 16 ? e.inner[j][16 * (size_t)k + q] : e.outer[j][24 * (size_t)k + q - 16];
+  };
   for (int j = 2; j <= L; j++) {
     LevelInfo& li = e.lv[j];
-    std::vector<int> il(40 * (size_t)li.nb);
     for (int k = 0; k < li.nb; k++) {
-      for (int q = 0; q < 16; q++) il[40 * k + q] = e.inner[j][16 * k + q];
-      for (int q = 0; q < 24; q++) il[40 * k + 16 + q] = e.outer[j][24 * k + q];
+      for (int q = 0; q < 40; q++) {
+        int b = slot_box(j, k, q);
+        if (b < 0 || b <= k) continue;
+        const int gk = (int)(li.gbase + k), gbx = (int)(li.gbase + b);
+        const int rk = li.rank[k], rb = li.rank[b];
+        pair_of[((int64_t)gk << 32) | gbx] = (int)e.h_pair_k.size();
+        e.h_pair_k.push_back(gk);
+        e.h_pair_b.push_back(gbx);
+        e.h_pair_off.push_back(off);
+        e.h_pair_part.push_back((int)part);
+        off += ((int64_t)rk * rb + 1) & ~(int64_t)1;  // 16-byte aligned blocks
+        part += rk + rb;
+        e.m2l_total += (int64_t)rk * rb;
+        tile = std::max(tile, (rk * rb + 1) & ~1);
+      }
     }
-    upload(d_lil, il, s);
-    M2LArgs ma{e.skx.as<double>(), e.sky.as<double>(), e.rank_g.as<int>() + li.gbase,
-               e.soff_g.as<int64_t>() + li.gbase, d_lil.as<int>(),
-               e.m2loff_g.as<int64_t>() + li.gbase, e.m2lcoff_g.as<int64_t>() + li.gbase,
-               e.m2l.as<double>(), e.m2l_src.as<int>(), e.kp};
-    AFMM_DISPATCH_KERNEL(e.kid, KID, k_m2l_build<KID><<<li.nb, 256, 0, s>>>(ma));
-    AFMM_LAUNCH_CHECK();
-    AFMM_HIP(hipStreamSynchronize(s));
   }
+  for (int j = 2; j <= L; j++) {
+    LevelInfo& li = e.lv[j];
+    for (int k = 0; k < li.nb; k++) {
+      const int gk = (int)(li.gbase + k);
+      e.h_red_start[gk] = (int)red_part.size();
+      for (int q = 0; q < 40; q++) {
+        int b = slot_box(j, k, q);
+        if (b < 0) continue;
+        const int gbx = (int)(li.gbase + b);
+        const int lo = std::min(gk, gbx), hi = std::max(gk, gbx);
+        const int pidx = pair_of.at(((int64_t)lo << 32) | hi);
+        e.h_red_pair.push_back(pidx);
+        // B out_b lands at part (k is the pair's first box), B^T out_k after
+        // the first box's rank_k values otherwise
+        red_part.push_back(gk < gbx ? e.h_pair_part[pidx]
+                                    : e.h_pair_part[pidx] + li.rank[b]);
+        e.il_pairs++;
+      }
+    }
+  }
+  e.h_red_start[nbox] = (int)red_part.size();
+  if (part > INT32_MAX) throw Error(6, "M2L partial buffer exceeds 2^31 entries");
+  e.npairs = (int64_t)e.h_pair_k.size();
+  e.m2l_cols = part;
+  // LDS for one staged block: whole blocks up to 48 KiB, larger ones in
+  // column chunks; the pair kernel holds up to 256 rows per lane set.
+  e.pair_tile = std::min(tile, 6144);
+  if (e.max_rank > 256) throw Error(6, "ranks above 256 are not supported by the M2L kernel");
+  upload(e.pair_k, e.h_pair_k, s);
+  upload(e.pair_b, e.h_pair_b, s);
+  upload(e.pair_off, e.h_pair_off, s);
+  upload(e.pair_part, e.h_pair_part, s);
+  upload(e.red_start, e.h_red_start, s);
+  upload(e.red_part, red_part, s);
+  {
+    std::vector<PairDesc> pd(e.npairs);
+    for (int64_t q = 0; q < e.npairs; q++) {
+      const int gk = e.h_pair_k[q], gbx = e.h_pair_b[q];
+      pd[q] = PairDesc{rank_g[gk], rank_g[gbx], e.h_pair_part[q], 0, e.h_pair_off[q],
+                       soff_g[gk], soff_g[gbx], 0};
+    }
+    upload(e.pair_desc, pd, s);
+  }
+  e.m2l.alloc(sizeof(double) * std::max<int64_t>(off, 2));
+  e.m2l_part.alloc(sizeof(double) * std::max<int64_t>(part, 2));
+  if (e.npairs > 0) {
+    AFMM_DISPATCH_KERNEL(e.kid, KID,
+                         k_m2l_pair_build<KID><<<(unsigned)e.npairs, 64, 0, s>>>(
+                             e.skx.as<double>(), e.sky.as<double>(), e.rank_g.as<int>(),
+                             e.soff_g.as<int64_t>(), e.pair_k.as<int>(), e.pair_b.as<int>(),
+                             e.pair_off.as<int64_t>(), e.m2l.as<double>(), e.kp));
+    AFMM_LAUNCH_CHECK();
+  }
+  AFMM_HIP(hipStreamSynchronize(s));
   e.outv.alloc(sizeof(double) * std::max<int64_t>(sb, 1));
   e.inv.alloc(sizeof(double) * std::max<int64_t>(sb, 1));
   AFMM_HIP(hipMemsetAsync(e.outv.p, 0, e.outv.bytes, s));

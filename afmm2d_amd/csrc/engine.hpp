@@ -8,10 +8,13 @@
 //                 order; box gid owns [soff[gid], soff[gid] + rank[gid])
 //   l2p_rm        L2P of every box, rows x rank row-major   (read by M2M)
 //   l2p_cm        the same matrices column-major             (read by L2L)
-//   m2l           per target box, the blocks of its interaction list
-//                 concatenated column-wise into one rank x C column-major
-//                 matrix (C = sum of the list's ranks); m2l_src[c] is the
-//                 skeleton offset of column c's source coefficient
+//   m2l           one block per unordered interaction pair (k, b), k < b:
+//                 B = K(skeleton_k, skeleton_b), rank_k x rank_b column-major,
+//                 16-byte aligned; K(skeleton_b, skeleton_k) = B^T is not
+//                 stored (the kernels are symmetric)
+//   m2l_part      per pair: B out_b (rank_k values) then B^T out_k (rank_b)
+//   red_part      per box, in interaction-slot order: the offset of the
+//                 partial vector that slot contributes to the box's locals
 //   outv, inv     multipole / local coefficients, indexed like skx
 #pragma once
 
@@ -55,8 +58,16 @@ struct LevelInfo {
   std::vector<int> rank;        // ACA rank
   std::vector<int64_t> soff;    // global skeleton offset per box (nb + 1 entries)
   std::vector<int64_t> l2poff;  // element offset into l2p_* (nb + 1)
-  std::vector<int64_t> m2loff;  // element offset into m2l (nb + 1)
-  std::vector<int64_t> m2lcoff; // column offset into m2l_src (nb + 1)
+};
+
+// Everything the M2L pair kernel needs about one pair, in one record.
+struct PairDesc {
+  int rk, rb;      // ranks of the pair's boxes
+  int part;        // partial-vector offset
+  int pad;
+  int64_t off;     // element offset of the block
+  int64_t sk, sb;  // skeleton offsets of the boxes
+  int64_t pad2;
 };
 
 struct KernelTimer {
@@ -86,18 +97,45 @@ struct Engine {
   std::vector<std::vector<int>> nbr, inner, outer;
   int64_t il_pairs = 0, near_pairs = 0;
 
+  // near field, symmetric: every unordered neighbour pair (A < B) and every
+  // self pair (A, A) once; a pair writes A's sums then (A != B) B's sums to
+  // its partial vector; near_slot_part[9 A + s] is the offset of the
+  // partial that slot s (0 = self, 1..8 = neighbours 0..7) adds to leaf A,
+  // -1 if absent
+  int64_t near_upairs = 0, near_part_total = 0;
+  DevBuf near_pair, near_pair_part, near_slot_part, near_part;
+
   // assembly
   bool assembled = false;
   std::vector<LevelInfo> lv;  // indexed by level (0..L), levels < 2 unused
   int64_t skel_total = 0, l2p_total = 0, m2l_total = 0, m2l_cols = 0;
   int max_rank = 0;
   DevBuf skx, sky, skidx;
-  DevBuf rank_g, nin_g, soff_g, l2poff_g, m2loff_g, m2lcoff_g;
-  DevBuf l2p_rm, l2p_cm, m2l, m2l_src;
+  DevBuf rank_g, nin_g, soff_g, l2poff_g;
+  DevBuf l2p_rm, l2p_cm;
+  // M2L, symmetric pair storage
+  int64_t npairs = 0;
+  int pair_tile = 0;                      // LDS doubles for one block
+  std::vector<int> h_pair_k, h_pair_b;    // box gids, k < b
+  std::vector<int64_t> h_pair_off;        // element offset of each block
+  std::vector<int> h_pair_part;           // partial-vector offset of each pair
+  std::vector<int> h_red_start;           // per box gid (nbox + 1)
+  std::vector<int> h_red_pair;            // per ordered slot entry: pair index
+  DevBuf pair_k, pair_b, pair_off, pair_part, red_start, red_part, m2l, m2l_part;
+  DevBuf pair_desc;                       // PairDesc per pair (one 32-byte load)
+  int num_cus = 256;
+  // schedule options (environment, read at create):
+  //   AFMM2D_NEAR_OVERLAP=0  near field on the main stream after M2L
+  //   AFMM2D_M2L_BLOCKS_PER_CU=n  persistent M2L workgroups per CU
+  bool near_overlap = true;
+  int m2l_blocks_per_cu = 4;
+  // FP64 log table for the near field: (1/c_j, -log(1/c_j)), c_j = 1 + (j + 1/2)/128
+  DevBuf log_tab;
+  double ln2_hi = 0.0, ln2_lo = 0.0;
   double create_seconds = 0.0, assemble_seconds = 0.0;
 
   // matvec state
-  DevBuf outv, inv, qs, pot_near, io_b, io_y;
+  DevBuf outv, inv, qs, io_b, io_y;
   bool have_matvec = false;
   std::vector<double> last_qs, last_ps;  // host copies for compute_error
 
