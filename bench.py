@@ -45,7 +45,7 @@ def parse():
     p.add_argument("--cpu-reps", type=int, default=3)
     p.add_argument("--no-side-run", action="store_true",
                    help="skip the run at the configured tolerance (profiling runs)")
-    p.add_argument("--calib-copy-mb", type=int, default=256,
+    p.add_argument("--calib-copy-mb", type=int, default=1024,
                    help="one device-to-device copy of this size before timing, so a "
                         "counter run sees a plain copy of known bytes")
     return p.parse_args()
@@ -59,14 +59,19 @@ def inputs(n, seed):
 
 
 def calib_copy(mb, torch):
-    """A plain copy of `mb` MiB (read mb, write mb): the reference point for
-    the HBM byte counters."""
+    """A plain copy of `mb` MiB (read mb, write mb) from a source that an
+    equally large fill has pushed out of the 256 MB last-level cache: the
+    reference point for the HBM byte counters."""
     if mb <= 0:
         return
-    src = torch.ones(mb * (1 << 20) // 8, dtype=torch.float64, device="cuda:0")
+    n = mb * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    flush = torch.empty(n, dtype=torch.float64, device="cuda:0")
+    flush.fill_(2.0)
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
+    del src, flush, dst
 
 
 def run_gpu(A, q, steps, warmup, torch):

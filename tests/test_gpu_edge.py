@@ -1,0 +1,139 @@
+"""GPU edge cases: ragged and large leaves, rank-0 far fields, coincident
+points, several right-hand sides, determinism and the error entry points --
+each against the CPU oracle on the same inputs."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import uniform_points
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("n,leaf,tol_pow", [
+    (5000, 64, 10),    # L=3, leaves of 78: two 64-point chunks in the near field
+    (70001, 100, 9),   # L=4, leaves of ~273: five chunks
+    (12345, 48, 11),   # odd sizes everywhere
+    (1 << 14, 16, 10),  # L=5, leaves of 16
+])
+def test_ragged_leaves_match_oracle(engine, oracle_mod, n, leaf, tol_pow):
+    xy, q = uniform_points(n, n)
+    T = oracle_mod.Tree(xy, leaf, tol_pow)
+    T.assemble()
+    A = engine.AFMM(n, leaf, tol_pow, xy)
+    A.assemble()
+    assert A.nLevels == T.L
+    np.testing.assert_array_equal(A.NumberOfParticlesInLeaves, T.kd_counts())
+    for j in range(2, T.L + 1):
+        np.testing.assert_array_equal(A.ranks(j), T.ranks(j))
+    y, yo = A.computeMatVecProduct(q), T.matvec(q)
+    assert rel(y, yo) <= 1e-11
+    A.close()
+    T.close()
+
+
+def test_rank_zero_far_field(engine, oracle_mod):
+    # a narrow Gaussian: every far-field block is below tol, every rank is 0,
+    # and the operator reduces to the near field
+    xy, q = uniform_points(4096, 3)
+    A = engine.AFMM(4096, 64, 10, xy, kernel="gaussian", kparams=[1e-5, 0, 0, 0])
+    A.assemble()
+    for j in range(2, A.nLevels + 1):
+        assert (A.ranks(j) == 0).all()
+    y = A.computeMatVecProduct(q)
+    yd = A.direct(q)
+    assert rel(y, yd) <= 1e-13
+    A.close()
+
+
+def test_coincident_points(engine, oracle_mod):
+    # Duplicated points make whole kernel rows identical: the reference's ACA
+    # stopping rule becomes borderline (a few ranks differ between any two
+    # FP64 implementations) and the far field near a duplicated pair that
+    # straddles two KD boxes is not low rank, so the operator is only accurate
+    # to ~1e-2 in the oracle itself.  The defined behaviour checked here: a
+    # finite result whose error against the direct sum is of the same size as
+    # the oracle's.
+    xy, q = uniform_points(4096, 8)
+    xy[100:200] = xy[0:100]
+    T = oracle_mod.Tree(xy, 64, 10)
+    T.assemble()
+    A = engine.AFMM(4096, 64, 10, xy)
+    A.assemble()
+    y, yo = A.computeMatVecProduct(q), T.matvec(q)
+    yd = A.direct(q)
+    e_gpu, e_cpu = rel(y, yd), rel(yo, T.direct(q))
+    print("coincident points: GPU vs direct %.3e, oracle vs direct %.3e" % (e_gpu, e_cpu))
+    assert np.isfinite(y).all()
+    assert e_gpu <= 3 * e_cpu + 1e-12
+
+
+def test_multiple_rhs_and_determinism(engine):
+    xy, _ = uniform_points(20000, 4)
+    rng = np.random.default_rng(5)
+    Q = rng.standard_normal((20000, 3))
+    A = engine.AFMM(20000, 64, 10, xy)
+    A.assemble()
+    Y = A.computeMatVecProduct(Q)
+    for r in range(3):
+        y = A.computeMatVecProduct(Q[:, r])
+        np.testing.assert_array_equal(Y[:, r], y)  # deterministic reductions
+    # linearity
+    y3 = A.computeMatVecProduct(2.0 * Q[:, 0] - Q[:, 1])
+    assert rel(y3, 2.0 * Y[:, 0] - Y[:, 1]) <= 1e-13
+    A.close()
+
+
+def test_error_entry_points_match_oracle(engine, oracle_mod):
+    xy, q = uniform_points(4096, 9)
+    T = oracle_mod.Tree(xy, 64, 10)
+    T.assemble()
+    T.matvec(q)
+    A = engine.AFMM(4096, 64, 10, xy)
+    A.assemble()
+    A.computeMatVecProduct(q)
+    for box in (0, 17, 63):
+        e, eo = A.compute_error(box), T.compute_error(box)
+        assert abs(e - eo) <= 1e-6 * eo + 1e-14
+    box, err = A.evaluateError(seed=3)
+    assert 0 <= box < 64 and err < 1e-6
+    # the host plug-in entry on KD-sorted indices equals the oracle's kernel
+    sxy = A.sorted_Locations
+    for i, j in ((0, 1), (5, 4000), (77, 77)):
+        ref = oracle_mod.kernel_value(0, sxy[i, 0], sxy[i, 1], sxy[j, 0], sxy[j, 1])
+        assert A.getMatrixEntry(i, j) == ref
+
+
+def test_m2l_blocks_are_kernel_blocks(engine, oracle_mod):
+    xy, _ = uniform_points(4096, 10)
+    T = oracle_mod.Tree(xy, 64, 10)
+    T.assemble()
+    A = engine.AFMM(4096, 64, 10, xy)
+    A.assemble()
+    _, inner, outer = A.interactions(3)
+    for box in (0, 9, 40):
+        for slot in range(40):
+            kb = inner[box, slot] if slot < 16 else outer[box, slot - 16]
+            B = A.m2l(3, box, slot)
+            if kb < 0:
+                assert B is None
+                continue
+            Bo = T.m2l(3, box, slot, T.rank(3, kb))
+            assert B.shape == Bo.shape
+            assert np.abs(B - Bo).max() <= 1e-14 * max(1.0, np.abs(Bo).max())
+
+
+def test_cpp_driver_runs():
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "afmm2d_amd", "examples", "test_afmm2d")
+    if not os.path.exists(exe):
+        pytest.skip("C++ driver not built")
+    out = subprocess.run([exe, "16384", "64", "10"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "err:" in out.stdout and "Assembly Time" in out.stdout
