@@ -223,8 +223,8 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
     if (kparams && kernel_id == AFMM2D_KERNEL_REG_INV_R && kparams[1] > 0.0) e->kp.a = kparams[1];
     AFMM_HIP(hipDeviceGetAttribute(&e->num_cus, hipDeviceAttributeMultiprocessorCount, device));
     if (const char* v = std::getenv("AFMM2D_NEAR_OVERLAP")) e->near_overlap = std::atoi(v) != 0;
-    if (const char* v = std::getenv("AFMM2D_M2L_BLOCKS_PER_CU"))
-      e->m2l_blocks_per_cu = std::max(1, std::min(8, std::atoi(v)));
+    if (const char* v = std::getenv("AFMM2D_P2P_WAVES"))
+      e->p2p_waves_per_simd = std::max(1, std::min(16, std::atoi(v)));
     AFMM_HIP(hipStreamCreateWithFlags(&e->s_main, hipStreamNonBlocking));
     AFMM_HIP(hipStreamCreateWithFlags(&e->s_aux, hipStreamNonBlocking));
     AFMM_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
@@ -517,15 +517,18 @@ int afmm2d_get_m2l(afmm2d_handle h, int level, int box, int slot, double* out, i
     *cols = rc;
     if ((int64_t)r * rc > capacity || (!out && r * rc > 0))
       throw Error(AFMM2D_ERR_INVALID_ARGUMENT, "capacity too small");
-    std::vector<double> blk((size_t)r * rc);
+    // the pair stores K(skeleton_lo, skeleton_hi) column-major with the
+    // first box's rank rounded up to even as leading dimension
+    const bool first = e.h_pair_k[p] == (int)(li.gbase + box);
+    const int rows_lo = first ? r : rc, cols_lo = first ? rc : r;
+    const int ld = rows_lo + (rows_lo & 1);
+    std::vector<double> blk((size_t)ld * cols_lo);
     if (!blk.empty())
       AFMM_HIP(hipMemcpy(blk.data(), e.m2l.as<double>() + e.h_pair_off[p],
                          sizeof(double) * blk.size(), hipMemcpyDeviceToHost));
-    // the pair stores K(skeleton_lo, skeleton_hi) column-major
-    const bool first = e.h_pair_k[p] == (int)(li.gbase + box);
     for (int i = 0; i < r; i++)
       for (int c = 0; c < rc; c++)
-        out[(size_t)i * rc + c] = first ? blk[(size_t)c * r + i] : blk[(size_t)i * rc + c];
+        out[(size_t)i * rc + c] = first ? blk[(size_t)c * ld + i] : blk[(size_t)i * ld + c];
   });
 }
 

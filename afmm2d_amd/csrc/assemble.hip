@@ -393,7 +393,8 @@ __global__ void k_skeleton(const double* __restrict__ src_x, const double* __res
 }
 
 // M2L block of one unordered interaction pair (k, b), k < b:
-// B[i + c * rank_k] = K(skeleton_k[i], skeleton_b[c]).
+// B[i + c * ld] = K(skeleton_k[i], skeleton_b[c]) for i < rank_k, 0 for the
+// padding row i = rank_k of an odd rank_k (ld = rank_k rounded up to even).
 template <int KID>
 __global__ __launch_bounds__(64) void k_m2l_pair_build(
     const double* __restrict__ skx, const double* __restrict__ sky, const int* __restrict__ rank,
@@ -404,10 +405,11 @@ __global__ __launch_bounds__(64) void k_m2l_pair_build(
   const int rk = rank[k], rb = rank[b];
   const int64_t sk = soff[k], sb = soff[b];
   double* out = m2l + poff[p];
-  const int tot = rk * rb;
+  const int ld = rk + (rk & 1);
+  const int tot = ld * rb;
   for (int e = threadIdx.x; e < tot; e += 64) {
-    const int c = e / rk, i = e - c * rk;
-    out[e] = kernel_xy<KID>(skx[sk + i], sky[sk + i], skx[sb + c], sky[sb + c], kp);
+    const int c = e / ld, i = e - c * ld;
+    out[e] = i < rk ? kernel_xy<KID>(skx[sk + i], sky[sk + i], skx[sb + c], sky[sb + c], kp) : 0.0;
   }
 }
 
@@ -646,6 +648,8 @@ void assemble_tree(Engine& e) {
   // which pair partial feeds its local coefficients.
   e.h_pair_k.clear();
   e.h_pair_b.clear();
+  e.pair_max_ld = 0;
+  e.pair_max_rb = 0;
   e.h_pair_off.clear();
   e.h_pair_part.clear();
   e.h_red_start.assign(nbox + 1, 0);
@@ -672,10 +676,13 @@ void assemble_tree(Engine& e) {
         e.h_pair_b.push_back(gbx);
         e.h_pair_off.push_back(off);
         e.h_pair_part.push_back((int)part);
-        off += ((int64_t)rk * rb + 1) & ~(int64_t)1;  // 16-byte aligned blocks
+        const int ld = rk + (rk & 1);
+        off += (int64_t)ld * rb;  // ld even: every block and column 16-byte aligned
         part += rk + rb;
         e.m2l_total += (int64_t)rk * rb;
-        tile = std::max(tile, (rk * rb + 1) & ~1);
+        tile = std::max(tile, ld * rb);
+        e.pair_max_ld = std::max(e.pair_max_ld, ld);
+        e.pair_max_rb = std::max(e.pair_max_rb, rb);
       }
     }
   }
@@ -717,7 +724,8 @@ void assemble_tree(Engine& e) {
     std::vector<PairDesc> pd(e.npairs);
     for (int64_t q = 0; q < e.npairs; q++) {
       const int gk = e.h_pair_k[q], gbx = e.h_pair_b[q];
-      pd[q] = PairDesc{rank_g[gk], rank_g[gbx], e.h_pair_part[q], 0, e.h_pair_off[q],
+      pd[q] = PairDesc{rank_g[gk], rank_g[gbx], e.h_pair_part[q], rank_g[gk] + (rank_g[gk] & 1),
+                       e.h_pair_off[q],
                        soff_g[gk], soff_g[gbx], 0};
     }
     upload(e.pair_desc, pd, s);

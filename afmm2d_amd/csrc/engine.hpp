@@ -9,9 +9,11 @@
 //   l2p_rm        L2P of every box, rows x rank row-major   (read by M2M)
 //   l2p_cm        the same matrices column-major             (read by L2L)
 //   m2l           one block per unordered interaction pair (k, b), k < b:
-//                 B = K(skeleton_k, skeleton_b), rank_k x rank_b column-major,
-//                 16-byte aligned; K(skeleton_b, skeleton_k) = B^T is not
-//                 stored (the kernels are symmetric)
+//                 B = K(skeleton_k, skeleton_b), rank_k x rank_b column-major
+//                 with leading dimension ld = rank_k rounded up to even (a
+//                 zero row when rank_k is odd), so every column starts 16-byte
+//                 aligned; K(skeleton_b, skeleton_k) = B^T is not stored (the
+//                 kernels are symmetric)
 //   m2l_part      per pair: B out_b (rank_k values) then B^T out_k (rank_b)
 //   red_part      per box, in interaction-slot order: the offset of the
 //                 partial vector that slot contributes to the box's locals
@@ -64,7 +66,7 @@ struct LevelInfo {
 struct PairDesc {
   int rk, rb;      // ranks of the pair's boxes
   int part;        // partial-vector offset
-  int pad;
+  int ld;          // leading dimension of the block: rk rounded up to even
   int64_t off;     // element offset of the block
   int64_t sk, sb;  // skeleton offsets of the boxes
   int64_t pad2;
@@ -115,7 +117,8 @@ struct Engine {
   DevBuf l2p_rm, l2p_cm;
   // M2L, symmetric pair storage
   int64_t npairs = 0;
-  int pair_tile = 0;                      // LDS doubles for one block
+  int pair_tile = 0;                      // LDS doubles for one block (fallback kernel)
+  int pair_max_ld = 0, pair_max_rb = 0;   // largest block shape
   std::vector<int> h_pair_k, h_pair_b;    // box gids, k < b
   std::vector<int64_t> h_pair_off;        // element offset of each block
   std::vector<int> h_pair_part;           // partial-vector offset of each pair
@@ -125,10 +128,10 @@ struct Engine {
   DevBuf pair_desc;                       // PairDesc per pair (one 32-byte load)
   int num_cus = 256;
   // schedule options (environment, read at create):
-  //   AFMM2D_NEAR_OVERLAP=0  near field on the main stream after M2L
-  //   AFMM2D_M2L_BLOCKS_PER_CU=n  persistent M2L workgroups per CU
+  //   AFMM2D_NEAR_OVERLAP=0   near field on the main stream after M2L
+  //   AFMM2D_P2P_WAVES=n      near-field waves per SIMD (persistent grid)
   bool near_overlap = true;
-  int m2l_blocks_per_cu = 4;
+  int p2p_waves_per_simd = 4;
   // FP64 log table for the near field: (1/c_j, -log(1/c_j)), c_j = 1 + (j + 1/2)/128
   DevBuf log_tab;
   double ln2_hi = 0.0, ln2_lo = 0.0;

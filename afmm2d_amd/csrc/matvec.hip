@@ -12,10 +12,11 @@
 // HBM-bound M2L and L2L; the last kernel joins both (leaf L2P + near field +
 // scatter, fused).
 //
-// M2L reads each symmetric block once: the pair kernel stages
-// B = K(skel_k, skel_b) in LDS with 16-byte loads and writes B out_b and
-// B^T out_k as partials; the reduction adds each box's partials in its
-// interaction-slot order, so results do not depend on scheduling.
+// M2L reads each symmetric block once: the pair kernel holds
+// B = K(skel_k, skel_b) in registers (16-byte loads, all issued up front) and
+// writes B out_b and B^T out_k as partials; the reduction adds each box's
+// partials in its interaction-slot order, so results do not depend on
+// scheduling.
 #include "engine.hpp"
 
 namespace afmm {
@@ -66,72 +67,146 @@ __global__ __launch_bounds__(256) void k_m2m(const double* __restrict__ l2p_rm,
   }
 }
 
-// M2L pair kernel: one workgroup of 4 waves per pair.  LDS holds the block
-// (tile doubles), out_k and out_b (rcap each) and the per-wave partial sums
-// (2 x 4 x rcap).  The 4 waves split the columns for B out_b and the rows
-// for B^T out_k; the partials are then summed in wave order.
-__global__ __launch_bounds__(256) void k_m2l_pairs(const double* __restrict__ m2l,
-                                                   const int* __restrict__ pk,
-                                                   const int* __restrict__ pb,
-                                                   const int64_t* __restrict__ poff,
-                                                   const int* __restrict__ ppart,
-                                                   const int* __restrict__ rank,
-                                                   const int64_t* __restrict__ soff,
-                                                   const double* __restrict__ outv,
-                                                   double* __restrict__ part, int tile, int rcap) {
+// Wave-level sums through DPP (data-parallel lane moves) on the two 32-bit
+// halves of a double.  Lane layout of the M2L tiles: lane = 8 ci + rp.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+// sum over the 8 consecutive lanes 8g..8g+7 (quad_perm xor 1, xor 2, then
+// row_half_mirror, which pairs each quad with the other half of its 8 lanes)
+__device__ __forceinline__ double sum_lanes8(double v) {
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x141>(v);
+  return v;
+}
+// sum over lanes l, l^8, l^16, ..., l^56 (row_ror 8 swaps the halves of each
+// 16-lane row; then two cross-row exchanges)
+__device__ __forceinline__ double sum_lanes_stride8(double v) {
+  v += dpp_move<0x128>(v);
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+// M2L pair kernel: one wave per unordered pair, no LDS.  The block (ld x rb,
+// column-major, ld even) is read in tiles of 16 rows x 8 columns, one
+// 16-byte load per lane: lane 8 ci + rp holds rows 2 rp, 2 rp + 1 of column
+// ci of the tile.  All of a pair's loads are issued before any use (tile
+// counts are compile-time: ld <= 16 MAXRT, rb <= 8 MAXCT).  B out_b: each
+// lane accumulates its two rows over its columns, then the 8 lanes sharing
+// rp are summed; B^T out_k: per column tile, the 8 lanes of a column are
+// summed.  Writes B out_b then B^T out_k to the pair's partial vector.
+template <int MAXRT, int MAXCT>
+__global__ __launch_bounds__(64) void k_m2l_reg(const double* __restrict__ m2l,
+                                                const PairDesc* __restrict__ pd,
+                                                const double* __restrict__ outv,
+                                                double* __restrict__ part, int npairs) {
+  const int p = blockIdx.x;
+  if (p >= npairs) return;
+  const PairDesc d = pd[p];
+  const int lane = threadIdx.x, rp = lane & 7, ci = lane >> 3;
+  const int rk = d.rk, rb = d.rb, ld = d.ld;
+  double* fw = part + d.part;
+  double* tw = fw + rk;
+  const double* xk = outv + d.sk;
+  const double* xb = outv + d.sb;
+  const double2* B2 = reinterpret_cast<const double2*>(m2l + d.off);
+  // staging registers as double arrays (arrays of double2 stay in scratch)
+  double ax[MAXCT][MAXRT], ay[MAXCT][MAXRT], xbv[MAXCT], k0[MAXRT], k1[MAXRT];
+#pragma unroll
+  for (int u = 0; u < MAXCT; u++) {
+    const int c = 8 * u + ci;
+    const bool vc = c < rb;
+    xbv[u] = xb[vc ? c : 0];
+#pragma unroll
+    for (int t = 0; t < MAXRT; t++) {
+      const int r = 16 * t + 2 * rp;
+      const bool v = vc && r < ld;
+      const double2 a = B2[v ? ((c * ld + r) >> 1) : 0];  // clamped: always issued
+      ax[u][t] = v ? a.x : 0.0;
+      ay[u][t] = v ? a.y : 0.0;
+    }
+    if (!vc) xbv[u] = 0.0;
+  }
+#pragma unroll
+  for (int t = 0; t < MAXRT; t++) {
+    const int r = 16 * t + 2 * rp;
+    k0[t] = xk[r < rk ? r : 0];
+    k1[t] = xk[r + 1 < rk ? r + 1 : 0];
+    if (!(r < rk)) k0[t] = 0.0;
+    if (!(r + 1 < rk)) k1[t] = 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < MAXCT; u++) {
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < MAXRT; t++) s += ax[u][t] * k0[t] + ay[u][t] * k1[t];
+    s = sum_lanes8(s);
+    const int c = 8 * u + ci;
+    if (rp == 0 && c < rb) tw[c] = s;
+  }
+#pragma unroll
+  for (int t = 0; t < MAXRT; t++) {
+    double f0 = 0.0, f1 = 0.0;
+#pragma unroll
+    for (int u = 0; u < MAXCT; u++) {
+      f0 += ax[u][t] * xbv[u];
+      f1 += ay[u][t] * xbv[u];
+    }
+    f0 = sum_lanes_stride8(f0);
+    f1 = sum_lanes_stride8(f1);
+    const int r = 16 * t + 2 * rp;
+    if (ci == 0) {
+      if (r < rk) fw[r] = f0;
+      if (r + 1 < rk) fw[r + 1] = f1;
+    }
+  }
+}
+
+// M2L pair kernel for blocks too large for the register tiles: one
+// workgroup of 4 waves per pair, the block staged through LDS in column
+// chunks of at most 'tile' doubles.  LDS: the chunk, out_k and out_b (rcap
+// each) and the per-wave partial sums (2 x 4 x rcap).
+__global__ __launch_bounds__(256) void k_m2l_lds(const double* __restrict__ m2l,
+                                                 const PairDesc* __restrict__ pd,
+                                                 const double* __restrict__ outv,
+                                                 double* __restrict__ part, int tile, int rcap) {
   extern __shared__ double lds[];
   double* T = lds;
   double* xk = lds + tile;
   double* xb = xk + rcap;
   double* rf = xb + rcap;      // [4][rcap]
   double* rt = rf + 4 * rcap;  // [4][rcap]
-  const int p = blockIdx.x;
+  const PairDesc d = pd[blockIdx.x];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int k = pk[p], b = pb[p];
-  const int rk = rank[k], rb = rank[b];
-  double* fw = part + ppart[p];
+  const int rk = d.rk, rb = d.rb, ld = d.ld;
+  double* fw = part + d.part;
   double* tw = fw + rk;
-  const double* B = m2l + poff[p];
-  for (int i = tid; i < rk; i += 256) xk[i] = outv[soff[k] + i];
-  for (int c = tid; c < rb; c += 256) xb[c] = outv[soff[b] + c];
+  const double* B = m2l + d.off;
+  for (int i = tid; i < rk; i += 256) xk[i] = outv[d.sk + i];
+  for (int c = tid; c < rb; c += 256) xb[c] = outv[d.sb + c];
   for (int u = tid; u < 4 * rcap; u += 256) rf[u] = 0.0;
-  const int tot = rk * rb;
-  // columns [c0, c1) of the block are resident in T, at T[i + (c - c0) rk]
-  const int chunk = tot <= tile ? rb : max(1, tile / rk);
+  const int chunk = max(1, tile / ld);
   for (int c0 = 0; c0 < rb; c0 += chunk) {
     const int c1 = min(rb, c0 + chunk);
-    const int ne = (c1 - c0) * rk;
+    const int ne = (c1 - c0) * ld;
     __syncthreads();
-    if (chunk == rb) {
-      const double2* B2 = reinterpret_cast<const double2*>(B);
-      double2* T2 = reinterpret_cast<double2*>(T);
-      const int n2 = ne >> 1;
-      int e = tid;
-      for (; e + 256 < n2; e += 512) {
-        double2 v0 = B2[e], v1 = B2[e + 256];
-        T2[e] = v0;
-        T2[e + 256] = v1;
-      }
-      for (; e < n2; e += 256) T2[e] = B2[e];
-      if ((ne & 1) && tid == 0) T[ne - 1] = B[ne - 1];
-    } else {
-      for (int e = tid; e < ne; e += 256) T[e] = B[(int64_t)c0 * rk + e];
-    }
+    for (int e = tid; e < ne; e += 256) T[e] = B[(int64_t)c0 * ld + e];
     __syncthreads();
-    // B out_b: wave w takes every 4th column of the chunk
     for (int i = lane; i < rk; i += 64) {
       double acc = 0.0;
-#pragma unroll 4
-      for (int c = c0 + w; c < c1; c += 4) acc += T[i + (c - c0) * rk] * xb[c];
+      for (int c = c0 + w; c < c1; c += 4) acc += T[i + (c - c0) * ld] * xb[c];
       rf[w * rcap + i] += acc;
     }
-    // B^T out_k: wave w takes a quarter of the rows
     const int q = (rk + 3) >> 2;
     const int i0 = w * q, i1 = min(rk, i0 + q);
     for (int c = c0 + lane; c < c1; c += 64) {
-      const double* Tc = T + (c - c0) * rk;
+      const double* Tc = T + (c - c0) * ld;
       double acc = 0.0;
-#pragma unroll 4
       for (int i = i0; i < i1; i++) acc += Tc[i] * xk[i];
       rt[w * rcap + c] = acc;
     }
@@ -141,122 +216,6 @@ __global__ __launch_bounds__(256) void k_m2l_pairs(const double* __restrict__ m2
     fw[i] = (rf[i] + rf[rcap + i]) + (rf[2 * rcap + i] + rf[3 * rcap + i]);
   for (int c = tid; c < rb; c += 256)
     tw[c] = (rt[c] + rt[rcap + c]) + (rt[2 * rcap + c] + rt[3 * rcap + c]);
-}
-
-// M2L pair kernel, persistent and software-pipelined: each workgroup walks
-// pairs p, p + G, p + 2G, ...; while it computes one pair from LDS the
-// 16-byte loads of the next two pairs are in flight into two register sets
-// (NV double2 per thread each, enough for blocks of up to NV * 512
-// coefficients).  Same products and summation order as k_m2l_pairs.
-template <int NV>
-__global__ __launch_bounds__(256) void k_m2l_pipe(const double* __restrict__ m2l,
-                                                  const PairDesc* __restrict__ pd,
-                                                  const double* __restrict__ outv,
-                                                  double* __restrict__ part, int npairs,
-                                                  int tile, int rcap) {
-  extern __shared__ double lds[];
-  double* T = lds;
-  double* xk = lds + tile;
-  double* xb = xk + rcap;
-  double* rf = xb + rcap;      // [4][rcap]
-  double* rt = rf + 4 * rcap;  // [4][rcap]
-  double2* T2 = reinterpret_cast<double2*>(T);
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  // two staging register sets, A and B (double arrays: an array of double2
-  // is not promoted to registers by the compiler)
-  double ax[NV], ay[NV], bx[NV], by[NV];
-  double at = 0.0, ak = 0.0, ab = 0.0, bt = 0.0, bk = 0.0, bb = 0.0;
-  // Loads are unconditional (indices clamped into the block), so they issue
-  // back to back; each set is consumed by a commit later in the loop.
-#define AFMM_M2L_LOAD(X, Y, TL, XK, XB, D)                                             \
-  {                                                                                    \
-    const int tot_ = (D).rk * (D).rb, n2_ = tot_ >> 1;                                 \
-    const double2* B2_ = reinterpret_cast<const double2*>(m2l + (n2_ > 0 ? (D).off : 0)); \
-    const int last_ = n2_ > 0 ? n2_ - 1 : 0;                                           \
-    _Pragma("unroll") for (int v = 0; v < NV; v++) {                                   \
-      const double2 t_ = B2_[min(tid + 256 * v, last_)];                               \
-      X[v] = t_.x;                                                                     \
-      Y[v] = t_.y;                                                                     \
-    }                                                                                  \
-    TL = m2l[tot_ > 0 ? (D).off + tot_ - 1 : 0];                                       \
-    XK = outv[(D).rk > 0 ? (D).sk + min(tid, (D).rk - 1) : 0];                         \
-    XB = outv[(D).rb > 0 ? (D).sb + min(tid, (D).rb - 1) : 0];                         \
-  }
-#define AFMM_M2L_COMMIT(X, Y, TL, XK, XB, D)                                           \
-  {                                                                                    \
-    const int tot_ = (D).rk * (D).rb, n2_ = tot_ >> 1;                                 \
-    _Pragma("unroll") for (int v = 0; v < NV; v++) {                                   \
-      const int e_ = tid + 256 * v;                                                    \
-      if (e_ < n2_) T2[e_] = make_double2(X[v], Y[v]);                                 \
-    }                                                                                  \
-    if ((tot_ & 1) && tid == 0) T[tot_ - 1] = TL;                                      \
-    if (tid < (D).rk) xk[tid] = XK;                                                    \
-    if (tid < (D).rb) xb[tid] = XB;                                                    \
-  }
-#define AFMM_M2L_COMPUTE(D)                                                            \
-  {                                                                                    \
-    const int rk = (D).rk, rb = (D).rb;                                                \
-    for (int i = lane; i < rk; i += 64) {                                              \
-      double acc = 0.0;                                                                \
-      _Pragma("unroll 4") for (int c = w; c < rb; c += 4) acc += T[i + c * rk] * xb[c]; \
-      rf[w * rcap + i] = acc;                                                          \
-    }                                                                                  \
-    const int q_ = (rk + 3) >> 2;                                                      \
-    const int i0 = w * q_, i1 = min(rk, i0 + q_);                                      \
-    for (int c = lane; c < rb; c += 64) {                                              \
-      const double* Tc = T + c * rk;                                                   \
-      double acc = 0.0;                                                                \
-      _Pragma("unroll 4") for (int i = i0; i < i1; i++) acc += Tc[i] * xk[i];          \
-      rt[w * rcap + c] = acc;                                                          \
-    }                                                                                  \
-    __syncthreads();                                                                   \
-    double* fw = part + (D).part;                                                      \
-    double* tw = fw + rk;                                                              \
-    for (int i = tid; i < rk; i += 256)                                                \
-      fw[i] = (rf[i] + rf[rcap + i]) + (rf[2 * rcap + i] + rf[3 * rcap + i]);          \
-    for (int c = tid; c < rb; c += 256)                                                \
-      tw[c] = (rt[c] + rt[rcap + c]) + (rt[2 * rcap + c] + rt[3 * rcap + c]);          \
-  }
-  const int G = gridDim.x;
-  int p = blockIdx.x;
-  if (p >= npairs) return;
-  PairDesc d0 = pd[p];
-  AFMM_M2L_LOAD(ax, ay, at, ak, ab, d0);
-  PairDesc d1 = d0;
-  if (p + G < npairs) {
-    d1 = pd[p + G];
-    AFMM_M2L_LOAD(bx, by, bt, bk, bb, d1);
-  }
-  AFMM_M2L_COMMIT(ax, ay, at, ak, ab, d0);
-  __syncthreads();
-  for (;;) {
-    // pair p is in LDS, p + G in set B; set A is free
-    PairDesc d2 = d0;
-    if (p + 2 * G < npairs) {
-      d2 = pd[p + 2 * G];
-      AFMM_M2L_LOAD(ax, ay, at, ak, ab, d2);
-    }
-    AFMM_M2L_COMPUTE(d0);
-    if (p + G >= npairs) break;
-    AFMM_M2L_COMMIT(bx, by, bt, bk, bb, d1);
-    __syncthreads();
-    // pair p + G is in LDS, p + 2G in set A; set B is free
-    PairDesc d3 = d1;
-    if (p + 3 * G < npairs) {
-      d3 = pd[p + 3 * G];
-      AFMM_M2L_LOAD(bx, by, bt, bk, bb, d3);
-    }
-    AFMM_M2L_COMPUTE(d1);
-    if (p + 2 * G >= npairs) break;
-    AFMM_M2L_COMMIT(ax, ay, at, ak, ab, d2);
-    __syncthreads();
-    p += 2 * G;
-    d0 = d2;
-    d1 = d3;
-  }
-#undef AFMM_M2L_LOAD
-#undef AFMM_M2L_COMMIT
-#undef AFMM_M2L_COMPUTE
 }
 
 // in_k = sum of its interaction slots' partials, in slot order.
@@ -556,7 +515,10 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
   {
     const int maxpad = (e.max_leaf + 63) & ~63;
     const size_t shm = sizeof(double) * (256 + 4 * (size_t)maxpad);
-    const unsigned grid = (unsigned)std::min<int64_t>(e.near_upairs, 16384);
+    // persistent grid: a few waves per SIMD, leaving wave slots to the
+    // memory-bound kernels that run beside it
+    const unsigned grid = (unsigned)std::min<int64_t>(
+        e.near_upairs, (int64_t)4 * e.num_cus * e.p2p_waves_per_simd);
     AFMM_DISPATCH_KERNEL(e.kid, KID,
                          k_p2p<KID><<<grid, 64, shm, sn>>>(
                              e.x.as<double>(), e.y.as<double>(), e.qs.as<double>(),
@@ -574,32 +536,24 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
   // M2L over every level at once: pair blocks, then per-box reduction
   const int nbox = (int)(e.lv[L].gbase + nleaf);
   const int rcap = std::max(e.max_rank, 1);
-  const size_t shm = sizeof(double) * ((size_t)e.pair_tile + 10 * (size_t)rcap);
   e.prof_begin("m2l", s, &a, &b);
   if (e.npairs > 0) {
-    // pipelined kernel when every block fits the registers of a workgroup
-    // (ranks up to 256 for the outv staging), chunked kernel otherwise
-    const int need = (e.pair_tile + 511) / 512;
-    const unsigned grid = (unsigned)std::min<int64_t>(e.npairs, (int64_t)e.m2l_blocks_per_cu * e.num_cus);
     const PairDesc* pd = e.pair_desc.as<PairDesc>();
     double* part = e.m2l_part.as<double>();
     const double* outv = e.outv.as<double>();
-    const bool ok = e.max_rank <= 256;
     const double* B = e.m2l.as<double>();
     const int np = (int)e.npairs;
-#define AFMM_PIPE(NVV) k_m2l_pipe<NVV><<<grid, 256, shm, s>>>(B, pd, outv, part, np, e.pair_tile, rcap)
-    if (ok && need <= 2) AFMM_PIPE(2);
-    else if (ok && need <= 3) AFMM_PIPE(3);
-    else if (ok && need <= 4) AFMM_PIPE(4);
-    else if (ok && need <= 5) AFMM_PIPE(5);
-    else if (ok && need <= 6) AFMM_PIPE(6);
-    else if (ok && need <= 8) AFMM_PIPE(8);
-    else if (ok && need <= 12) AFMM_PIPE(12);
-    else
-      k_m2l_pairs<<<(unsigned)e.npairs, 256, shm, s>>>(
-          B, e.pair_k.as<int>(), e.pair_b.as<int>(), e.pair_off.as<int64_t>(),
-          e.pair_part.as<int>(), rank, soff, outv, part, e.pair_tile, rcap);
-#undef AFMM_PIPE
+    const int ml = e.pair_max_ld, mb = e.pair_max_rb;
+    if (ml <= 32 && mb <= 32) {
+      k_m2l_reg<2, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np);
+    } else if (ml <= 48 && mb <= 48) {
+      k_m2l_reg<3, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np);
+    } else if (ml <= 64 && mb <= 64) {
+      k_m2l_reg<4, 8><<<np, 64, 0, s>>>(B, pd, outv, part, np);
+    } else {
+      const size_t shm = sizeof(double) * ((size_t)e.pair_tile + 10 * (size_t)rcap);
+      k_m2l_lds<<<np, 256, shm, s>>>(B, pd, outv, part, e.pair_tile, rcap);
+    }
     AFMM_LAUNCH_CHECK();
   }
   e.prof_end(s, b);
