@@ -293,13 +293,19 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
           if (a < b) slot[9 * (size_t)a + 1 + q] = ppart[pidx.at(((int64_t)a << 32) | b)];
           else slot[9 * (size_t)a + 1 + q] = ppart[pidx.at(((int64_t)b << 32) | a)] + e->leaf_count[b];
         }
-      if (off > INT32_MAX) throw Error(AFMM2D_ERR_UNSUPPORTED, "near-field partials exceed 2^31");
+      if (off + e->max_leaf > INT32_MAX)
+        throw Error(AFMM2D_ERR_UNSUPPORTED, "near-field partials exceed 2^31");
+      // absent slots read a zero vector past the last partial, so the
+      // output kernel's loads are unconditional
+      for (int& v : slot)
+        if (v < 0) v = (int)off;
       e->near_upairs = (int64_t)ppart.size();
       e->near_part_total = off;
       e->near_pair.alloc(sizeof(int) * pair.size());
       e->near_pair_part.alloc(sizeof(int) * ppart.size());
       e->near_slot_part.alloc(sizeof(int) * slot.size());
-      e->near_part.alloc(sizeof(double) * std::max<int64_t>(off, 1));
+      e->near_part.alloc(sizeof(double) * (size_t)(off + std::max(e->max_leaf, 1)));
+      AFMM_HIP(hipMemsetAsync(e->near_part.p, 0, e->near_part.bytes, e->s_main));
       AFMM_HIP(hipMemcpyAsync(e->near_pair.p, pair.data(), sizeof(int) * pair.size(),
                               hipMemcpyHostToDevice, e->s_main));
       AFMM_HIP(hipMemcpyAsync(e->near_pair_part.p, ppart.data(), sizeof(int) * ppart.size(),

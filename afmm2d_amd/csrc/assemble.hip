@@ -628,8 +628,12 @@ void assemble_tree(Engine& e) {
   std::vector<int64_t> soff_g(nbox + 1), l2poff_g(nbox + 1);
   for (int j = 2; j <= L; j++) {
     LevelInfo& li = e.lv[j];
+    li.max_nin = 0;
+    li.max_rank = 0;
     for (int k = 0; k < li.nb; k++) {
       int64_t g = li.gbase + k;
+      li.max_nin = std::max(li.max_nin, li.nin[k]);
+      li.max_rank = std::max(li.max_rank, li.rank[k]);
       rank_g[g] = li.rank[k];
       nin_g[g] = li.nin[k];
       soff_g[g] = li.soff[k];
@@ -707,7 +711,7 @@ void assemble_tree(Engine& e) {
     }
   }
   e.h_red_start[nbox] = (int)red_part.size();
-  if (part > INT32_MAX) throw Error(6, "M2L partial buffer exceeds 2^31 entries");
+  if (part + e.max_rank > INT32_MAX) throw Error(6, "M2L partial buffer exceeds 2^31 entries");
   e.npairs = (int64_t)e.h_pair_k.size();
   e.m2l_cols = part;
   // LDS for one staged block: whole blocks up to 48 KiB, larger ones in
@@ -721,6 +725,15 @@ void assemble_tree(Engine& e) {
   upload(e.red_start, e.h_red_start, s);
   upload(e.red_part, red_part, s);
   {
+    std::vector<int> dense((size_t)kMaxIL * nbox, (int)part);
+    for (int64_t g = 0; g < nbox; g++) {
+      const int t0 = e.h_red_start[g], t1 = e.h_red_start[g + 1];
+      if (t1 - t0 > kMaxIL) throw Error(6, "interaction list longer than 27");
+      for (int t = t0; t < t1; t++) dense[(size_t)kMaxIL * g + (t - t0)] = red_part[t];
+    }
+    upload(e.red_list, dense, s);
+  }
+  {
     std::vector<PairDesc> pd(e.npairs);
     for (int64_t q = 0; q < e.npairs; q++) {
       const int gk = e.h_pair_k[q], gbx = e.h_pair_b[q];
@@ -731,7 +744,9 @@ void assemble_tree(Engine& e) {
     upload(e.pair_desc, pd, s);
   }
   e.m2l.alloc(sizeof(double) * std::max<int64_t>(off, 2));
-  e.m2l_part.alloc(sizeof(double) * std::max<int64_t>(part, 2));
+  // partials, then a zero vector of max_rank entries for unused list slots
+  e.m2l_part.alloc(sizeof(double) * (size_t)(part + std::max(e.max_rank, 2)));
+  AFMM_HIP(hipMemsetAsync(e.m2l_part.p, 0, e.m2l_part.bytes, s));
   if (e.npairs > 0) {
     AFMM_DISPATCH_KERNEL(e.kid, KID,
                          k_m2l_pair_build<KID><<<(unsigned)e.npairs, 64, 0, s>>>(

@@ -60,6 +60,7 @@ struct LevelInfo {
   std::vector<int> rank;        // ACA rank
   std::vector<int64_t> soff;    // global skeleton offset per box (nb + 1 entries)
   std::vector<int64_t> l2poff;  // element offset into l2p_* (nb + 1)
+  int max_nin = 0, max_rank = 0;
 };
 
 // Everything the M2L pair kernel needs about one pair, in one record.
@@ -71,6 +72,10 @@ struct PairDesc {
   int64_t sk, sb;  // skeleton offsets of the boxes
   int64_t pad2;
 };
+
+// Largest interaction list: children of the parent's 9-box neighbourhood
+// that are not adjacent, 36 - 9 = 27 (rounded up to 28).
+constexpr int kMaxIL = 28;
 
 struct KernelTimer {
   double total_ms = 0.0;
@@ -102,8 +107,9 @@ struct Engine {
   // near field, symmetric: every unordered neighbour pair (A < B) and every
   // self pair (A, A) once; a pair writes A's sums then (A != B) B's sums to
   // its partial vector; near_slot_part[9 A + s] is the offset of the
-  // partial that slot s (0 = self, 1..8 = neighbours 0..7) adds to leaf A,
-  // -1 if absent
+  // partial that slot s (0 = self, 1..8 = neighbours 0..7) adds to leaf A;
+  // absent slots point at a zero vector after the last partial
+  // (near_part_total)
   int64_t near_upairs = 0, near_part_total = 0;
   DevBuf near_pair, near_pair_part, near_slot_part, near_part;
 
@@ -125,13 +131,16 @@ struct Engine {
   std::vector<int> h_red_start;           // per box gid (nbox + 1)
   std::vector<int> h_red_pair;            // per ordered slot entry: pair index
   DevBuf pair_k, pair_b, pair_off, pair_part, red_start, red_part, m2l, m2l_part;
+  // red_list[kMaxIL g + t]: offset of the t-th partial of box g (slot
+  // order); unused entries point at a zero vector after the last partial
+  DevBuf red_list;
   DevBuf pair_desc;                       // PairDesc per pair (one 32-byte load)
   int num_cus = 256;
   // schedule options (environment, read at create):
   //   AFMM2D_NEAR_OVERLAP=0   near field on the main stream after M2L
   //   AFMM2D_P2P_WAVES=n      near-field waves per SIMD (persistent grid)
   bool near_overlap = true;
-  int p2p_waves_per_simd = 4;
+  int p2p_waves_per_simd = 6;
   // FP64 log table for the near field: (1/c_j, -log(1/c_j)), c_j = 1 + (j + 1/2)/128
   DevBuf log_tab;
   double ln2_hi = 0.0, ln2_lo = 0.0;

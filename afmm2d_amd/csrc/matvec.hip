@@ -23,47 +23,158 @@ namespace afmm {
 
 namespace {
 
+// Wave priority of the HBM-bound kernels that run beside the near field
+// (M2L, downward pass): they issue their loads ahead of the near field's
+// FP64 arithmetic, which fills the remaining issue slots.
+#ifndef AFMM_MEM_PRIO
+#define AFMM_MEM_PRIO 3
+#endif
+
 __global__ void k_gather(const double* __restrict__ b, const int* __restrict__ perm,
                          double* __restrict__ qs, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) qs[i] = b[perm[i]];
 }
 
-// M2M: one workgroup of 4 waves per box; out[c] = sum_i L2P[i][c] src[i]
-// (row-major L2P: lanes over c read each row coalesced, the 4 waves split
-// the rows and are summed through LDS).  src is the leaf's q segment (leaf
-// level) or the children's concatenated multipoles (always contiguous).
-__global__ __launch_bounds__(256) void k_m2m(const double* __restrict__ l2p_rm,
-                                             const int64_t* __restrict__ l2poff,
-                                             const int* __restrict__ rank,
-                                             const int* __restrict__ nin,
-                                             const int64_t* __restrict__ soff,
-                                             const double* __restrict__ src,
-                                             const int64_t* __restrict__ src_off_child,
-                                             const int* __restrict__ src_off_leaf,
-                                             double* __restrict__ outv, int64_t gbase, int nb) {
-  __shared__ double red[4][64];
+// Batched small GEMV, the core of the upward and downward passes:
+// y[cb + c] = sum_i M[i][cb + c] x[i] over one column chunk (rc <= 64
+// columns) of a box's matrix M (nr x nc, row-major, leading dimension nc),
+// x staged in LDS.  Lane l = h rc + c holds column cb + c and the rows
+// i = h + G t (G = 64 / rc row groups, so each step t reads G rc contiguous
+// elements); the waves of the workgroup take the steps round robin, B steps
+// per batch.  Lanes with h >= G use h = nr and contribute nothing.
+struct GemvLane {
+  int rc, G, h, c;
+};
+
+__device__ __forceinline__ GemvLane gemv_lane(int ncols, int cb, int nr) {
+  GemvLane q;
+  q.rc = min(64, ncols - cb);
+  q.G = 64 / q.rc;
+  const int l = threadIdx.x & 63;
+  const int h = l / q.rc;
+  q.c = l - h * q.rc;
+  q.h = h < q.G ? h : nr;
+  return q;
+}
+
+// Issues the loads of one batch (steps t0, t0 + WPB, ...): addresses are
+// clamped into the matrix, so no load is conditional.
+template <int WPB, int B>
+__device__ __forceinline__ void gemv_load(double (&pv)[B], const double* __restrict__ M, int nr,
+                                          int nc, int cb, const GemvLane& q, int t0) {
+  if (nr == 0) return;
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    const int i = q.h + q.G * (t0 + b * WPB);
+    pv[b] = M[(int64_t)(i < nr ? i : nr - 1) * nc + cb + q.c];
+  }
+}
+
+template <int WPB, int B>
+__device__ __forceinline__ double gemv_fma(const double (&pv)[B], double acc, int nr,
+                                           const GemvLane& q, int t0, const double* xs) {
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    const int i = q.h + q.G * (t0 + b * WPB);
+    const double xv = xs[i < nr ? i : 0];
+    acc = i < nr ? fma(pv[b], xv, acc) : acc;
+  }
+  return acc;
+}
+
+// The whole column chunk for this lane, given the first batch already
+// loaded into pv (issued before x was staged, so its latency overlaps the
+// staging).
+template <int WPB, int B>
+__device__ __forceinline__ double gemv_chunk(double (&pv)[B], const double* __restrict__ M, int nr,
+                                             int nc, int cb, const GemvLane& q, const double* xs) {
+  const int w = threadIdx.x >> 6;
+  const int T = (nr + q.G - 1) / q.G;
+  double acc = 0.0;
+  if (w < T) acc = gemv_fma<WPB, B>(pv, acc, nr, q, w, xs);
+  for (int t0 = w + WPB * B; t0 < T; t0 += WPB * B) {
+    gemv_load<WPB, B>(pv, M, nr, nc, cb, q, t0);
+    acc = gemv_fma<WPB, B>(pv, acc, nr, q, t0, xs);
+  }
+  return acc;
+}
+
+// Sum of the row groups and waves for column threadIdx.x < rc, in a fixed
+// order; every thread of the workgroup must call it.
+template <int WPB>
+__device__ __forceinline__ double reduce_groups(double acc, double* red, const GemvLane& q) {
+  if (WPB == 1 && q.G == 1) return acc;
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  double s = 0.0;
+  if ((int)threadIdx.x < q.rc) {
+    for (int w = 0; w < WPB; w++)
+      for (int h = 0; h < q.G; h++) s += red[w * 64 + h * q.rc + threadIdx.x];
+  }
+  __syncthreads();
+  return s;
+}
+
+// A box's local coefficients into xs: (has_parent ? the parent's L2L
+// contribution : 0) + its M2L partials in interaction-slot order.  The list
+// (kMaxIL entries, unused ones pointing at a zero vector) is read from the
+// box's row at kernel entry, with the other per-box scalars, so the partial
+// loads go out in the same round as the matrix loads.
+template <int WPB>
+__device__ __forceinline__ void local_coeffs(double* xs, int r, const double* __restrict__ par,
+                                             int has_parent, const double* __restrict__ part,
+                                             const int (&lst)[kMaxIL]) {
+  for (int c = threadIdx.x; c < r; c += 64 * WPB) {
+    double v[kMaxIL];
+#pragma unroll
+    for (int t = 0; t < kMaxIL; t++) v[t] = part[lst[t] + c];
+    double acc = has_parent ? par[c] : 0.0;
+#pragma unroll
+    for (int t = 0; t < kMaxIL; t++) acc += v[t];
+    xs[c] = acc;
+  }
+}
+
+// M2M for one level: out_k = L2P_k^T src_k with row-major L2P (nin x rank);
+// src is the leaf's q segment (leaf level) or the children's concatenated
+// multipoles (always contiguous).  One workgroup of WPB waves per box.
+template <int WPB, int B>
+__global__ __launch_bounds__(64 * WPB) void k_up(const double* __restrict__ l2p_rm,
+                                                 const int64_t* __restrict__ l2poff,
+                                                 const int* __restrict__ rank,
+                                                 const int* __restrict__ nin,
+                                                 const int64_t* __restrict__ soff,
+                                                 const double* __restrict__ src,
+                                                 const int64_t* __restrict__ src_off_child,
+                                                 const int* __restrict__ src_off_leaf,
+                                                 double* __restrict__ outv, int64_t gbase,
+                                                 int nb) {
+  extern __shared__ double xs[];
+  __shared__ double red[64 * WPB];
   const int k = blockIdx.x;
   if (k >= nb) return;
   const int64_t g = gbase + k;
   const int r = rank[g];
   if (r == 0) return;
   const int n = nin[g];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double* P = l2p_rm + l2poff[g];
   const double* x = src + (src_off_leaf ? (int64_t)src_off_leaf[k] : src_off_child[4 * k]);
   double* out = outv + soff[g];
+  const int w = threadIdx.x >> 6;
+  double pv[B];
+  GemvLane q = gemv_lane(r, 0, n);
+  gemv_load<WPB, B>(pv, P, n, r, 0, q, w);
+  for (int i = threadIdx.x; i < n; i += 64 * WPB) xs[i] = x[i];
+  __syncthreads();
   for (int cb = 0; cb < r; cb += 64) {
-    const int c = cb + lane;
-    double acc = 0.0;
-    if (c < r) {
-#pragma unroll 4
-      for (int i = w; i < n; i += 4) acc += P[(int64_t)i * r + c] * x[i];
+    if (cb > 0) {
+      q = gemv_lane(r, cb, n);
+      gemv_load<WPB, B>(pv, P, n, r, cb, q, w);
     }
-    red[w][lane] = acc;
-    __syncthreads();
-    if (w == 0 && c < r) out[c] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-    __syncthreads();
+    const double acc = gemv_chunk<WPB, B>(pv, P, n, r, cb, q, xs);
+    const double s = reduce_groups<WPB>(acc, red, q);
+    if ((int)threadIdx.x < q.rc) out[cb + threadIdx.x] = s;
   }
 }
 
@@ -105,6 +216,7 @@ __global__ __launch_bounds__(64) void k_m2l_reg(const double* __restrict__ m2l,
                                                 const PairDesc* __restrict__ pd,
                                                 const double* __restrict__ outv,
                                                 double* __restrict__ part, int npairs) {
+  if (AFMM_MEM_PRIO) __builtin_amdgcn_s_setprio(AFMM_MEM_PRIO);
   const int p = blockIdx.x;
   if (p >= npairs) return;
   const PairDesc d = pd[p];
@@ -218,48 +330,49 @@ __global__ __launch_bounds__(256) void k_m2l_lds(const double* __restrict__ m2l,
     tw[c] = (rt[c] + rt[rcap + c]) + (rt[2 * rcap + c] + rt[3 * rcap + c]);
 }
 
-// in_k = sum of its interaction slots' partials, in slot order.
-__global__ __launch_bounds__(64) void k_m2l_reduce(const double* __restrict__ part,
-                                                   const int* __restrict__ red_start,
-                                                   const int* __restrict__ red_part,
+// L2L for a non-leaf level, with the box's M2L sum fused in: in_k =
+// parent contribution + M2L partials (local_coeffs), then the children's
+// parent contribution = L2P_k in_k (column-major L2P, i.e. row-major
+// rank x nin; the children's coefficients are contiguous).
+template <int WPB, int B>
+__global__ __launch_bounds__(64 * WPB) void k_down(const double* __restrict__ l2p_cm,
+                                                   const int64_t* __restrict__ l2poff,
                                                    const int* __restrict__ rank,
+                                                   const int* __restrict__ nin,
                                                    const int64_t* __restrict__ soff,
-                                                   double* __restrict__ inv, int nbox) {
-  const int g = blockIdx.x;
-  if (g >= nbox) return;
-  const int r = rank[g];
-  const int t0 = red_start[g], t1 = red_start[g + 1];
-  double* out = inv + soff[g];
-  for (int i = threadIdx.x; i < r; i += 64) {
-    double acc = 0.0;
-    for (int t = t0; t < t1; t++) acc += part[red_part[t] + i];
-    out[i] = acc;
-  }
-}
-
-// L2L for a non-leaf level: children.in += L2P_k in_k (column-major L2P,
-// threads over rows).  The children's coefficients are contiguous.
-__global__ __launch_bounds__(256) void k_l2l(const double* __restrict__ l2p_cm,
-                                             const int64_t* __restrict__ l2poff,
-                                             const int* __restrict__ rank,
-                                             const int* __restrict__ nin,
-                                             const int64_t* __restrict__ soff,
-                                             const int64_t* __restrict__ child_soff,
-                                             double* __restrict__ inv, int64_t gbase, int nb) {
+                                                   const int64_t* __restrict__ child_soff,
+                                                   const double* __restrict__ part,
+                                                   const int* __restrict__ red_list,
+                                                   double* __restrict__ inv, int has_parent,
+                                                   int64_t gbase, int nb) {
+  extern __shared__ double xs[];
+  __shared__ double red[64 * WPB];
+  if (AFMM_MEM_PRIO) __builtin_amdgcn_s_setprio(AFMM_MEM_PRIO);
   const int k = blockIdx.x;
   if (k >= nb) return;
   const int64_t g = gbase + k;
+  int lst[kMaxIL];
+#pragma unroll
+  for (int t = 0; t < kMaxIL; t++) lst[t] = red_list[(int64_t)kMaxIL * g + t];
   const int r = rank[g];
-  if (r == 0) return;
   const int n = nin[g];
+  if (n == 0) return;
   const double* P = l2p_cm + l2poff[g];
-  const double* x = inv + soff[g];
   double* dst = inv + child_soff[4 * k];
-  for (int i = threadIdx.x; i < n; i += 256) {
-    double acc = 0.0;
-#pragma unroll 8
-    for (int c = 0; c < r; c++) acc += P[i + (int64_t)c * n] * x[c];
-    dst[i] += acc;
+  const int w = threadIdx.x >> 6;
+  double pv[B];
+  GemvLane q = gemv_lane(n, 0, r);
+  gemv_load<WPB, B>(pv, P, r, n, 0, q, w);
+  local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part, lst);
+  __syncthreads();
+  for (int cb = 0; cb < n; cb += 64) {
+    if (cb > 0) {
+      q = gemv_lane(n, cb, r);
+      gemv_load<WPB, B>(pv, P, r, n, cb, q, w);
+    }
+    const double acc = gemv_chunk<WPB, B>(pv, P, r, n, cb, q, xs);
+    const double s = reduce_groups<WPB>(acc, red, q);
+    if ((int)threadIdx.x < q.rc) dst[cb + threadIdx.x] = s;
   }
 }
 
@@ -395,38 +508,71 @@ __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
   }
 }
 
-// Leaves: y[perm[i]] = (L2P_k in_k)[i] + the near-field partials of the
-// leaf's self pair and its neighbour slots 0..7, in that order.
-__global__ __launch_bounds__(64) void k_out(const double* __restrict__ l2p_cm,
-                                            const int64_t* __restrict__ l2poff,
-                                            const int* __restrict__ rank,
-                                            const int64_t* __restrict__ soff,
-                                            const int* __restrict__ leaf_start,
-                                            const int* __restrict__ leaf_count,
-                                            const double* __restrict__ inv,
-                                            const double* __restrict__ near_part,
-                                            const int* __restrict__ slot_part,
-                                            const int* __restrict__ perm,
-                                            double* __restrict__ yout, int64_t gbase, int nb) {
+// Leaf level, fused: in_k (parent contribution + M2L partials), potential
+// = L2P_k in_k + the near-field partials of the leaf's self pair and its
+// neighbour slots 0..7 (in that order; absent slots read zeros), scattered
+// to the caller's order.
+template <int WPB, int B>
+__global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2p_cm,
+                                                   const int64_t* __restrict__ l2poff,
+                                                   const int* __restrict__ rank,
+                                                   const int64_t* __restrict__ soff,
+                                                   const int* __restrict__ leaf_start,
+                                                   const int* __restrict__ leaf_count,
+                                                   const double* __restrict__ part,
+                                                   const int* __restrict__ red_list,
+                                                   const double* __restrict__ inv, int has_parent,
+                                                   const double* __restrict__ near_part,
+                                                   const int* __restrict__ slot_part,
+                                                   const int* __restrict__ perm,
+                                                   double* __restrict__ yout, int64_t gbase,
+                                                   int nb) {
+  extern __shared__ double xs[];
+  __shared__ double red[64 * WPB];
   const int k = blockIdx.x;
   if (k >= nb) return;
   const int64_t g = gbase + k;
+  int lst[kMaxIL], so[9];
+#pragma unroll
+  for (int t = 0; t < kMaxIL; t++) lst[t] = red_list[(int64_t)kMaxIL * g + t];
+#pragma unroll
+  for (int t = 0; t < 9; t++) so[t] = slot_part[9 * k + t];
   const int r = rank[g];
   const int st = leaf_start[k], n = leaf_count[k];
+  if (n == 0) return;
   const double* P = l2p_cm + l2poff[g];
-  const double* x = inv + soff[g];
-  int so[9];
+  const int w = threadIdx.x >> 6;
+  double pv[B];
+  GemvLane q = gemv_lane(n, 0, r);
+  gemv_load<WPB, B>(pv, P, r, n, 0, q, w);
+  // the first chunk's near-field values and destinations, loaded early
+  double nv[9];
+  int dst_i = 0;
+  {
+    const int i = min((int)threadIdx.x, n - 1);
 #pragma unroll
-  for (int s = 0; s < 9; s++) so[s] = slot_part[9 * k + s];
-  for (int i = threadIdx.x; i < n; i += 64) {
-    double acc = 0.0;
-#pragma unroll 8
-    for (int c = 0; c < r; c++) acc += P[i + (int64_t)c * n] * x[c];
-    double nf = 0.0;
+    for (int t = 0; t < 9; t++) nv[t] = near_part[so[t] + i];
+    dst_i = perm[st + i];
+  }
+  local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part, lst);
+  __syncthreads();
+  for (int cb = 0; cb < n; cb += 64) {
+    if (cb > 0) {
+      q = gemv_lane(n, cb, r);
+      gemv_load<WPB, B>(pv, P, r, n, cb, q, w);
+      const int i = min(cb + (int)threadIdx.x, n - 1);
 #pragma unroll
-    for (int s = 0; s < 9; s++)
-      if (so[s] >= 0) nf += near_part[so[s] + i];
-    yout[perm[st + i]] = acc + nf;
+      for (int t = 0; t < 9; t++) nv[t] = near_part[so[t] + i];
+      dst_i = perm[st + i];
+    }
+    const double acc = gemv_chunk<WPB, B>(pv, P, r, n, cb, q, xs);
+    const double s = reduce_groups<WPB>(acc, red, q);
+    if ((int)threadIdx.x < q.rc) {
+      double nf = 0.0;
+#pragma unroll
+      for (int t = 0; t < 9; t++) nf += nv[t];
+      yout[dst_i] = s + nf;
+    }
   }
 }
 
@@ -470,6 +616,58 @@ __global__ void k_sum_parts(const double* __restrict__ part, int nrows, int nchu
 
 }  // namespace
 
+namespace {
+
+#ifndef AFMM_GEMV_BATCH
+#define AFMM_GEMV_BATCH 16
+#endif
+constexpr int kGemvBatch = AFMM_GEMV_BATCH;  // row steps per batch of loads in the GEMV
+
+// Waves per box: the kernels are bound by memory latency (a few dependent
+// round trips per box), so what counts is the number of boxes resident at
+// once.  Large levels get one wave per box; small levels spread each box
+// over up to kMaxWavesPerBox waves, for ~kTargetWaves waves per level.
+#ifndef AFMM_TARGET_WAVES
+#define AFMM_TARGET_WAVES 4096
+#endif
+#ifndef AFMM_MAX_WAVES_PER_BOX
+#define AFMM_MAX_WAVES_PER_BOX 8
+#endif
+constexpr int64_t kTargetWaves = AFMM_TARGET_WAVES;
+constexpr int kMaxWavesPerBox = AFMM_MAX_WAVES_PER_BOX;
+
+int waves_per_box(int nb) {
+  int w = 1;
+  while (w < kMaxWavesPerBox && (int64_t)nb * w < kTargetWaves) w *= 2;
+  return w;
+}
+
+}  // namespace
+
+#define AFMM_DISPATCH_WPB(w, WPB, ...)    \
+  switch (w) {                            \
+    case 1: {                             \
+      constexpr int WPB = 1;              \
+      __VA_ARGS__;                        \
+    } break;                              \
+    case 2: {                             \
+      constexpr int WPB = 2;              \
+      __VA_ARGS__;                        \
+    } break;                              \
+    case 4: {                             \
+      constexpr int WPB = 4;              \
+      __VA_ARGS__;                        \
+    } break;                              \
+    case 8: {                             \
+      constexpr int WPB = 8;              \
+      __VA_ARGS__;                        \
+    } break;                              \
+    default: {                            \
+      constexpr int WPB = 16;             \
+      __VA_ARGS__;                        \
+    } break;                              \
+  }
+
 void gather_sorted(Engine& e, const double* d_b, double* d_qs, hipStream_t s) {
   k_gather<<<ceil_div(e.n, 256), 256, 0, s>>>(d_b, e.perm.as<int>(), d_qs, e.n);
   AFMM_LAUNCH_CHECK();
@@ -491,21 +689,25 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
 
   // upward pass
   e.prof_begin("m2m", s, &a, &b);
-  k_m2m<<<nleaf, 256, 0, s>>>(e.l2p_rm.as<double>(), l2poff, rank, nin, soff, e.qs.as<double>(),
-                              nullptr, e.d_leaf_start.as<int>(), e.outv.as<double>(),
-                              e.lv[L].gbase, nleaf);
-  AFMM_LAUNCH_CHECK();
-  for (int j = L - 1; j >= 2; j--) {
+  for (int j = L; j >= 2; j--) {
     const LevelInfo& li = e.lv[j];
-    k_m2m<<<li.nb, 256, 0, s>>>(e.l2p_rm.as<double>(), l2poff, rank, nin, soff,
-                                e.outv.as<double>(), soff + e.lv[j + 1].gbase, nullptr,
-                                e.outv.as<double>(), li.gbase, li.nb);
+    const int wpb = waves_per_box(li.nb);
+    const size_t shm = sizeof(double) * std::max(1, li.max_nin);
+    const bool leaf = j == L;
+    AFMM_DISPATCH_WPB(wpb, WPB,
+                      k_up<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
+                          e.l2p_rm.as<double>(), l2poff, rank, nin, soff,
+                          leaf ? e.qs.as<double>() : e.outv.as<double>(),
+                          leaf ? nullptr : soff + e.lv[j + 1].gbase,
+                          leaf ? e.d_leaf_start.as<int>() : nullptr, e.outv.as<double>(),
+                          li.gbase, li.nb));
     AFMM_LAUNCH_CHECK();
   }
   e.prof_end(s, b);
 
   // fork the near field onto the auxiliary stream: it overlaps the
-  // memory-bound M2L and L2L, not the short latency-bound M2M chain
+  // HBM-bound M2L and the downward pass (started before the upward pass,
+  // it would crowd out the latency-bound M2M)
   hipStream_t sn = e.near_overlap ? e.s_aux : s;
   if (e.near_overlap) {
     AFMM_HIP(hipEventRecord(e.ev_fork, s));
@@ -532,9 +734,8 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
   if (e.near_overlap) AFMM_HIP(hipEventRecord(e.ev_join, e.s_aux));
 
 
-
-  // M2L over every level at once: pair blocks, then per-box reduction
-  const int nbox = (int)(e.lv[L].gbase + nleaf);
+  // M2L over every level at once: one partial vector per (pair, direction);
+  // each box sums its partials in the downward pass
   const int rcap = std::max(e.max_rank, 1);
   e.prof_begin("m2l", s, &a, &b);
   if (e.npairs > 0) {
@@ -557,29 +758,39 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
     AFMM_LAUNCH_CHECK();
   }
   e.prof_end(s, b);
-  e.prof_begin("m2l_reduce", s, &a, &b);
-  k_m2l_reduce<<<nbox, 64, 0, s>>>(e.m2l_part.as<double>(), e.red_start.as<int>(),
-                                   e.red_part.as<int>(), rank, soff, e.inv.as<double>(), nbox);
-  AFMM_LAUNCH_CHECK();
-  e.prof_end(s, b);
 
-  // downward pass
+  // downward pass: each box adds its M2L partials to its parent's
+  // contribution, then passes L2P_k in_k to its children
+  const double* part = e.m2l_part.as<double>();
   e.prof_begin("l2l", s, &a, &b);
   for (int j = 2; j < L; j++) {
     const LevelInfo& li = e.lv[j];
-    k_l2l<<<li.nb, 256, 0, s>>>(e.l2p_cm.as<double>(), l2poff, rank, nin, soff,
-                                soff + e.lv[j + 1].gbase, e.inv.as<double>(), li.gbase, li.nb);
+    const int wpb = waves_per_box(li.nb);
+    const size_t shm = sizeof(double) * std::max(1, li.max_rank);
+    AFMM_DISPATCH_WPB(wpb, WPB,
+                      k_down<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
+                          e.l2p_cm.as<double>(), l2poff, rank, nin, soff,
+                          soff + e.lv[j + 1].gbase, part, e.red_list.as<int>(),
+                          e.inv.as<double>(), j > 2 ? 1 : 0, li.gbase, li.nb));
     AFMM_LAUNCH_CHECK();
   }
   e.prof_end(s, b);
 
   if (e.near_overlap) AFMM_HIP(hipStreamWaitEvent(s, e.ev_join, 0));
   e.prof_begin("l2p_out", s, &a, &b);
-  k_out<<<nleaf, 64, 0, s>>>(e.l2p_cm.as<double>(), l2poff, rank, soff, e.d_leaf_start.as<int>(),
-                             e.d_leaf_count.as<int>(), e.inv.as<double>(),
-                             e.near_part.as<double>(), e.near_slot_part.as<int>(),
-                             e.perm.as<int>(), d_y, e.lv[L].gbase, nleaf);
-  AFMM_LAUNCH_CHECK();
+  {
+    const LevelInfo& li = e.lv[L];
+    const int wpb = waves_per_box(li.nb);
+    const size_t shm = sizeof(double) * std::max(1, li.max_rank);
+    AFMM_DISPATCH_WPB(wpb, WPB,
+                      k_leaf<WPB, kGemvBatch><<<nleaf, 64 * WPB, shm, s>>>(
+                          e.l2p_cm.as<double>(), l2poff, rank, soff, e.d_leaf_start.as<int>(),
+                          e.d_leaf_count.as<int>(), part, e.red_list.as<int>(),
+                          e.inv.as<double>(), L > 2 ? 1 : 0,
+                          e.near_part.as<double>(), e.near_slot_part.as<int>(), e.perm.as<int>(),
+                          d_y, li.gbase, nleaf));
+    AFMM_LAUNCH_CHECK();
+  }
   e.prof_end(s, b);
 }
 

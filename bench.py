@@ -99,7 +99,7 @@ def run_gpu(A, q, steps, warmup, torch):
     A.set_profiling(False)
     ms = ev0.elapsed_time(ev1) / steps
     kern = {}
-    for name in ("gather", "m2m", "m2l", "m2l_reduce", "l2l", "p2p", "l2p_out"):
+    for name in ("gather", "m2m", "m2l", "l2l", "p2p", "l2p_out"):
         tot, cnt = A.kernel_time(name)
         kern[name] = tot / cnt if cnt else 0.0
     return ms, wall / steps * 1e3, kern, dy.cpu().numpy()
