@@ -14,7 +14,7 @@
 //
 // M2L reads each symmetric block once: the pair kernel holds
 // B = K(skel_k, skel_b) in registers (16-byte loads, all issued up front) and
-// writes B out_b and B^T out_k as partials; the reduction adds each box's
+// writes B out_b and B^T out_k as partials; the downward pass adds each box's
 // partials in its interaction-slot order, so results do not depend on
 // scheduling.
 #include "engine.hpp"

@@ -58,6 +58,27 @@ def inputs(n, seed):
     return xy, q
 
 
+def measured_traffic(kernel, alg_bytes):
+    """HBM bytes per launch of `kernel` as measured by two rocprofv3 --pmc
+    passes (FETCH_SIZE, WRITE_SIZE, calibrated against a copy of known size)
+    and recorded in a summary under profiles/.  Used only when that
+    measurement was taken on an operator with the same algorithmic bytes,
+    i.e. the same workload and build of the operator; otherwise None."""
+    prof = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(prof):
+        return None, None
+    for name in sorted(os.listdir(prof)):
+        if not name.endswith("_hbm_bytes.md"):
+            continue
+        for line in open(os.path.join(prof, name)):
+            if not line.startswith("kernel: "):
+                continue
+            kv = dict(f.strip().split(": ", 1) for f in line.strip().split(";"))
+            if kv.get("kernel") == kernel and int(kv.get("algorithmic_bytes", -1)) == alg_bytes:
+                return int(kv["traffic_bytes_per_launch"]), "profiles/" + name
+    return None, None
+
+
 def calib_copy(mb, torch):
     """A plain copy of `mb` MiB (read mb, write mb) from a source that an
     equally large fill has pushed out of the 256 MB last-level cache: the
@@ -196,9 +217,12 @@ def main():
     # column) and the multipole vector it reads.
     m2l_alg = st["m2l_bytes"] + 8 * st["m2l_columns"] + 8 * st["skeleton_total"]
     ach = m2l_alg / (kern["m2l"] * 1e-3) / 1e9 if kern["m2l"] > 0 else 0.0
+    traffic, traffic_src = measured_traffic("k_m2l_reg", int(m2l_alg))
     roof = {"kernel": "m2l", "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes": int(m2l_alg), "avg_ms": round(kern["m2l"], 4)}
+    if traffic is not None:
+        roof["traffic_source"] = traffic_src
 
     cpu = None
     if not args.no_cpu:
