@@ -60,6 +60,8 @@ class Stats(ctypes.Structure):
         ("m2l_columns", ctypes.c_int64),
         ("l2p_bytes", ctypes.c_int64), ("matvec_bytes", ctypes.c_int64),
         ("create_seconds", ctypes.c_double), ("assemble_seconds", ctypes.c_double),
+        ("m2l_leaf_bytes", ctypes.c_int64), ("m2l_leaf_columns", ctypes.c_int64),
+        ("skeleton_leaf", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -320,8 +322,10 @@ class AFMM:
         _check(lib().afmm2d_get_stats(self._h, ctypes.byref(st)))
         return st.as_dict()
 
-    def set_profiling(self, on):
-        _check(lib().afmm2d_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on, m2l_only=False):
+        """Phase timing with HIP events (see afmm2d_set_profiling); with
+        m2l_only only the leaf-level M2L launch is timed."""
+        _check(lib().afmm2d_set_profiling(self._h, (2 if m2l_only else 1) if on else 0))
 
     def kernel_time(self, name):
         ms, n = ctypes.c_double(), ctypes.c_int64()

@@ -128,13 +128,16 @@ Engine::~Engine() {
   }
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
+  if (ev_tree) (void)hipEventDestroy(ev_tree);
   if (s_main) (void)hipStreamDestroy(s_main);
   if (s_aux) (void)hipStreamDestroy(s_aux);
+  if (s_tree) (void)hipStreamDestroy(s_tree);
 }
 
 void Engine::prof_begin(const char* name, hipStream_t s, hipEvent_t* a, hipEvent_t* b) {
   *a = *b = nullptr;
   if (!profiling) return;
+  if (profiling == 2 && std::strcmp(name, "m2l") != 0) return;
   if (ev_used == ev_pool.size()) {
     std::array<hipEvent_t, 2> p;
     AFMM_HIP(hipEventCreate(&p[0]));
@@ -223,12 +226,16 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
     if (kparams && kernel_id == AFMM2D_KERNEL_REG_INV_R && kparams[1] > 0.0) e->kp.a = kparams[1];
     AFMM_HIP(hipDeviceGetAttribute(&e->num_cus, hipDeviceAttributeMultiprocessorCount, device));
     if (const char* v = std::getenv("AFMM2D_NEAR_OVERLAP")) e->near_overlap = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AFMM2D_TREE_MODE"))
+      e->tree_mode = std::max(0, std::min(2, std::atoi(v)));
     if (const char* v = std::getenv("AFMM2D_P2P_WAVES"))
-      e->p2p_waves_per_simd = std::max(1, std::min(16, std::atoi(v)));
+      e->p2p_waves_per_simd = std::max(0, std::min(16, std::atoi(v)));
     AFMM_HIP(hipStreamCreateWithFlags(&e->s_main, hipStreamNonBlocking));
     AFMM_HIP(hipStreamCreateWithFlags(&e->s_aux, hipStreamNonBlocking));
+    AFMM_HIP(hipStreamCreateWithFlags(&e->s_tree, hipStreamNonBlocking));
     AFMM_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
     AFMM_HIP(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
+    AFMM_HIP(hipEventCreateWithFlags(&e->ev_tree, hipEventDisableTiming));
     double t0 = now_s();
     std::vector<double> hx(n), hy(n);
     for (int i = 0; i < n; i++) {
@@ -265,52 +272,43 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
     e->near_pairs = nleaf;
     for (int v : e->nbr[L]) e->near_pairs += v >= 0;
     {
-      // unordered near-field pairs: self pairs and neighbour pairs A < B
-      std::vector<int> pair, ppart, slot(9 * (size_t)nleaf, -1);
-      std::vector<int64_t> self_off(nleaf, -1);
-      std::map<int64_t, int> pidx;
-      int64_t off = 0;
+      // unordered near-field pairs: self pairs and neighbour pairs A < B.
+      // Partials are leaf-major: leaf A's slot s (0 self, 1 + q neighbour q)
+      // holds count[A] values at 9 start[A] + s count[A]; a pair writes A's
+      // sums to A's slot for B and B's sums to B's slot for A.
+      std::vector<int> pair, ppart;
+      auto slot_off = [&](int a, int s) {
+        return (int64_t)9 * e->leaf_start[a] + (int64_t)s * e->leaf_count[a];
+      };
+      if ((int64_t)9 * e->n > INT32_MAX)
+        throw Error(AFMM2D_ERR_UNSUPPORTED, "near-field partials exceed 2^31");
       for (int a = 0; a < nleaf; a++) {
         pair.push_back(a);
         pair.push_back(a);
-        ppart.push_back((int)off);
-        slot[9 * (size_t)a] = (int)off;
-        off += e->leaf_count[a];
+        ppart.push_back((int)slot_off(a, 0));
+        ppart.push_back((int)slot_off(a, 0));
         for (int q = 0; q < 8; q++) {
           int b = e->nbr[L][8 * (size_t)a + q];
           if (b <= a) continue;
-          pidx[((int64_t)a << 32) | b] = (int)ppart.size();
+          int qb = -1;
+          for (int u = 0; u < 8; u++)
+            if (e->nbr[L][8 * (size_t)b + u] == a) qb = u;
+          if (qb < 0) throw Error(AFMM2D_ERR_UNSUPPORTED, "neighbour relation not symmetric");
           pair.push_back(a);
           pair.push_back(b);
-          ppart.push_back((int)off);
-          off += e->leaf_count[a] + e->leaf_count[b];
+          ppart.push_back((int)slot_off(a, 1 + q));
+          ppart.push_back((int)slot_off(b, 1 + qb));
         }
       }
-      for (int a = 0; a < nleaf; a++)
-        for (int q = 0; q < 8; q++) {
-          int b = e->nbr[L][8 * (size_t)a + q];
-          if (b < 0) continue;
-          if (a < b) slot[9 * (size_t)a + 1 + q] = ppart[pidx.at(((int64_t)a << 32) | b)];
-          else slot[9 * (size_t)a + 1 + q] = ppart[pidx.at(((int64_t)b << 32) | a)] + e->leaf_count[b];
-        }
-      if (off + e->max_leaf > INT32_MAX)
-        throw Error(AFMM2D_ERR_UNSUPPORTED, "near-field partials exceed 2^31");
-      // absent slots read a zero vector past the last partial, so the
-      // output kernel's loads are unconditional
-      for (int& v : slot)
-        if (v < 0) v = (int)off;
-      e->near_upairs = (int64_t)ppart.size();
-      e->near_part_total = off;
+      e->near_upairs = (int64_t)pair.size() / 2;
+      e->near_part_total = (int64_t)9 * e->n;
       e->near_pair.alloc(sizeof(int) * pair.size());
       e->near_pair_part.alloc(sizeof(int) * ppart.size());
-      e->near_slot_part.alloc(sizeof(int) * slot.size());
-      e->near_part.alloc(sizeof(double) * (size_t)(off + std::max(e->max_leaf, 1)));
+      e->near_part.alloc(sizeof(double) * (size_t)std::max<int64_t>(e->near_part_total, 1));
       AFMM_HIP(hipMemsetAsync(e->near_part.p, 0, e->near_part.bytes, e->s_main));
       AFMM_HIP(hipMemcpyAsync(e->near_pair.p, pair.data(), sizeof(int) * pair.size(),
                               hipMemcpyHostToDevice, e->s_main));
       AFMM_HIP(hipMemcpyAsync(e->near_pair_part.p, ppart.data(), sizeof(int) * ppart.size(),
-                              hipMemcpyHostToDevice, e->s_main));
-      AFMM_HIP(hipMemcpyAsync(e->near_slot_part.p, slot.data(), sizeof(int) * slot.size(),
                               hipMemcpyHostToDevice, e->s_main));
       AFMM_HIP(hipStreamSynchronize(e->s_main));
     }
@@ -356,6 +354,7 @@ int afmm2d_destroy(afmm2d_handle h) {
       (void)hipSetDevice(h->e->device);
       (void)hipStreamSynchronize(h->e->s_main);
       (void)hipStreamSynchronize(h->e->s_aux);
+      (void)hipStreamSynchronize(h->e->s_tree);
     }
     delete h;
   });
@@ -661,12 +660,16 @@ int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st) {
                          : 0;
   st->create_seconds = e.create_seconds;
   st->assemble_seconds = e.assemble_seconds;
+  st->m2l_leaf_bytes = e.assembled ? 8 * e.lv[e.L].m2l_coeffs : 0;
+  st->m2l_leaf_columns = e.assembled ? e.lv[e.L].m2l_cols : 0;
+  st->skeleton_leaf = e.assembled ? e.lv[e.L].soff[e.lv[e.L].nb] - e.lv[e.L].soff[0] : 0;
   return AFMM2D_OK;
 }
 
-int afmm2d_set_profiling(afmm2d_handle h, int enable) {
+int afmm2d_set_profiling(afmm2d_handle h, int level) {
   if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL handle");
-  h->e->profiling = enable != 0;
+  if (level < 0 || level > 2) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "profiling level 0..2");
+  h->e->profiling = level;
   return AFMM2D_OK;
 }
 

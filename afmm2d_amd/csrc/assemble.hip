@@ -648,17 +648,17 @@ void assemble_tree(Engine& e) {
   upload(e.l2poff_g, l2poff_g, s);
 
   // M2L layout: every unordered interaction pair once (k < b), in the order
-  // the boxes' slots first meet them; each box then lists, in slot order,
-  // which pair partial feeds its local coefficients.
+  // the boxes' slots first meet them.  Partials are box-major: box g owns
+  // kILSlots slots of rank_g values at kILSlots * soff[g]; its t-th present
+  // interaction slot (slot order) receives the partial of that pair, so a
+  // box's M2L sum reads one contiguous block (unused slots stay zero).
   e.h_pair_k.clear();
   e.h_pair_b.clear();
   e.pair_max_ld = 0;
   e.pair_max_rb = 0;
   e.h_pair_off.clear();
-  e.h_pair_part.clear();
   e.h_red_start.assign(nbox + 1, 0);
   e.h_red_pair.clear();
-  std::vector<int> red_part;
   std::unordered_map<int64_t, int> pair_of;
   int64_t off = 0, part = 0;
   int tile = 2;
@@ -669,6 +669,9 @@ void assemble_tree(Engine& e) {
   };
   for (int j = 2; j <= L; j++) {
     LevelInfo& li = e.lv[j];
+    li.pair_begin = (int64_t)e.h_pair_k.size();
+    li.pair_max_ld = li.pair_max_rb = 0;
+    li.m2l_coeffs = li.m2l_cols = 0;
     for (int k = 0; k < li.nb; k++) {
       for (int q = 0; q < 40; q++) {
         int b = slot_box(j, k, q);
@@ -679,7 +682,6 @@ void assemble_tree(Engine& e) {
         e.h_pair_k.push_back(gk);
         e.h_pair_b.push_back(gbx);
         e.h_pair_off.push_back(off);
-        e.h_pair_part.push_back((int)part);
         const int ld = rk + (rk & 1);
         off += (int64_t)ld * rb;  // ld even: every block and column 16-byte aligned
         part += rk + rb;
@@ -687,32 +689,42 @@ void assemble_tree(Engine& e) {
         tile = std::max(tile, ld * rb);
         e.pair_max_ld = std::max(e.pair_max_ld, ld);
         e.pair_max_rb = std::max(e.pair_max_rb, rb);
+        li.pair_max_ld = std::max(li.pair_max_ld, ld);
+        li.pair_max_rb = std::max(li.pair_max_rb, rb);
+        li.m2l_coeffs += (int64_t)rk * rb;
+        li.m2l_cols += rk + rb;
       }
     }
+    li.pair_end = (int64_t)e.h_pair_k.size();
   }
+  e.npairs = (int64_t)e.h_pair_k.size();
+  if ((int64_t)kILSlots * sb > INT32_MAX) throw Error(6, "M2L partial buffer exceeds 2^31 entries");
+  std::vector<int> part_k(e.npairs, 0), part_b(e.npairs, 0);
   for (int j = 2; j <= L; j++) {
     LevelInfo& li = e.lv[j];
     for (int k = 0; k < li.nb; k++) {
       const int gk = (int)(li.gbase + k);
-      e.h_red_start[gk] = (int)red_part.size();
+      e.h_red_start[gk] = (int)e.h_red_pair.size();
+      int t = 0;
       for (int q = 0; q < 40; q++) {
         int b = slot_box(j, k, q);
         if (b < 0) continue;
+        if (t >= kILSlots) throw Error(6, "interaction list longer than 27");
         const int gbx = (int)(li.gbase + b);
         const int lo = std::min(gk, gbx), hi = std::max(gk, gbx);
         const int pidx = pair_of.at(((int64_t)lo << 32) | hi);
         e.h_red_pair.push_back(pidx);
-        // B out_b lands at part (k is the pair's first box), B^T out_k after
-        // the first box's rank_k values otherwise
-        red_part.push_back(gk < gbx ? e.h_pair_part[pidx]
-                                    : e.h_pair_part[pidx] + li.rank[b]);
+        // the pair writes B out_b to its first box's slot, B^T out_k to its
+        // second box's slot
+        const int dst = (int)(kILSlots * soff_g[gk] + (int64_t)t * li.rank[k]);
+        if (gk < gbx) part_k[pidx] = dst;
+        else part_b[pidx] = dst;
+        t++;
         e.il_pairs++;
       }
     }
   }
-  e.h_red_start[nbox] = (int)red_part.size();
-  if (part + e.max_rank > INT32_MAX) throw Error(6, "M2L partial buffer exceeds 2^31 entries");
-  e.npairs = (int64_t)e.h_pair_k.size();
+  e.h_red_start[nbox] = (int)e.h_red_pair.size();
   e.m2l_cols = part;
   // LDS for one staged block: whole blocks up to 48 KiB, larger ones in
   // column chunks; the pair kernel holds up to 256 rows per lane set.
@@ -721,31 +733,17 @@ void assemble_tree(Engine& e) {
   upload(e.pair_k, e.h_pair_k, s);
   upload(e.pair_b, e.h_pair_b, s);
   upload(e.pair_off, e.h_pair_off, s);
-  upload(e.pair_part, e.h_pair_part, s);
-  upload(e.red_start, e.h_red_start, s);
-  upload(e.red_part, red_part, s);
-  {
-    std::vector<int> dense((size_t)kMaxIL * nbox, (int)part);
-    for (int64_t g = 0; g < nbox; g++) {
-      const int t0 = e.h_red_start[g], t1 = e.h_red_start[g + 1];
-      if (t1 - t0 > kMaxIL) throw Error(6, "interaction list longer than 27");
-      for (int t = t0; t < t1; t++) dense[(size_t)kMaxIL * g + (t - t0)] = red_part[t];
-    }
-    upload(e.red_list, dense, s);
-  }
   {
     std::vector<PairDesc> pd(e.npairs);
     for (int64_t q = 0; q < e.npairs; q++) {
       const int gk = e.h_pair_k[q], gbx = e.h_pair_b[q];
-      pd[q] = PairDesc{rank_g[gk], rank_g[gbx], e.h_pair_part[q], rank_g[gk] + (rank_g[gk] & 1),
-                       e.h_pair_off[q],
-                       soff_g[gk], soff_g[gbx], 0};
+      pd[q] = PairDesc{rank_g[gk], rank_g[gbx], part_k[q], rank_g[gk] + (rank_g[gk] & 1),
+                       e.h_pair_off[q], soff_g[gk], soff_g[gbx], part_b[q], 0};
     }
     upload(e.pair_desc, pd, s);
   }
   e.m2l.alloc(sizeof(double) * std::max<int64_t>(off, 2));
-  // partials, then a zero vector of max_rank entries for unused list slots
-  e.m2l_part.alloc(sizeof(double) * (size_t)(part + std::max(e.max_rank, 2)));
+  e.m2l_part.alloc(sizeof(double) * (size_t)std::max<int64_t>(kILSlots * sb, 2));
   AFMM_HIP(hipMemsetAsync(e.m2l_part.p, 0, e.m2l_part.bytes, s));
   if (e.npairs > 0) {
     AFMM_DISPATCH_KERNEL(e.kid, KID,

@@ -14,9 +14,14 @@
 //                 zero row when rank_k is odd), so every column starts 16-byte
 //                 aligned; K(skeleton_b, skeleton_k) = B^T is not stored (the
 //                 kernels are symmetric)
-//   m2l_part      per pair: B out_b (rank_k values) then B^T out_k (rank_b)
-//   red_part      per box, in interaction-slot order: the offset of the
-//                 partial vector that slot contributes to the box's locals
+//   m2l_part      box-major M2L partials: box gid owns kILSlots slots of
+//                 rank[gid] values at kILSlots * soff[gid]; the t-th present
+//                 interaction slot (slot order) holds that pair's product for
+//                 this box (B out_b for the pair's first box, B^T out_k for
+//                 its second), unused slots stay zero
+//   near_part     leaf-major near-field partials: leaf A owns 9 slots of
+//                 count[A] values at 9 start[A] (slot 0 self, 1 + q
+//                 neighbour q), absent neighbours stay zero
 //   outv, inv     multipole / local coefficients, indexed like skx
 #pragma once
 
@@ -61,21 +66,27 @@ struct LevelInfo {
   std::vector<int64_t> soff;    // global skeleton offset per box (nb + 1 entries)
   std::vector<int64_t> l2poff;  // element offset into l2p_* (nb + 1)
   int max_nin = 0, max_rank = 0;
+  // this level's M2L pairs: [pair_begin, pair_end) of the pair list, their
+  // largest block shape, coefficients and partial columns
+  int64_t pair_begin = 0, pair_end = 0;
+  int pair_max_ld = 0, pair_max_rb = 0;
+  int64_t m2l_coeffs = 0, m2l_cols = 0;
 };
 
 // Everything the M2L pair kernel needs about one pair, in one record.
 struct PairDesc {
   int rk, rb;      // ranks of the pair's boxes
-  int part;        // partial-vector offset
+  int part_k;      // where B out_b goes: a slot of the first box's partials
   int ld;          // leading dimension of the block: rk rounded up to even
   int64_t off;     // element offset of the block
   int64_t sk, sb;  // skeleton offsets of the boxes
-  int64_t pad2;
+  int part_b;      // where B^T out_k goes: a slot of the second box's partials
+  int pad;
 };
 
 // Largest interaction list: children of the parent's 9-box neighbourhood
-// that are not adjacent, 36 - 9 = 27 (rounded up to 28).
-constexpr int kMaxIL = 28;
+// that are not adjacent, 36 - 9 = 27.  Every box owns this many partial slots.
+constexpr int kILSlots = 27;
 
 struct KernelTimer {
   double total_ms = 0.0;
@@ -88,8 +99,8 @@ struct Engine {
   double tol = 0.0;
   KParams kp{0.01, 0.0};
 
-  hipStream_t s_main = nullptr, s_aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t s_main = nullptr, s_aux = nullptr, s_tree = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_tree = nullptr;
 
   // KD partition
   DevBuf x, y, perm;
@@ -105,13 +116,10 @@ struct Engine {
   int64_t il_pairs = 0, near_pairs = 0;
 
   // near field, symmetric: every unordered neighbour pair (A < B) and every
-  // self pair (A, A) once; a pair writes A's sums then (A != B) B's sums to
-  // its partial vector; near_slot_part[9 A + s] is the offset of the
-  // partial that slot s (0 = self, 1..8 = neighbours 0..7) adds to leaf A;
-  // absent slots point at a zero vector after the last partial
-  // (near_part_total)
+  // self pair (A, A) once; near_pair_part[2p], [2p + 1]: where pair p writes
+  // A's sums and B's sums (slots of the leaf-major near_part)
   int64_t near_upairs = 0, near_part_total = 0;
-  DevBuf near_pair, near_pair_part, near_slot_part, near_part;
+  DevBuf near_pair, near_pair_part, near_part;
 
   // assembly
   bool assembled = false;
@@ -127,19 +135,19 @@ struct Engine {
   int pair_max_ld = 0, pair_max_rb = 0;   // largest block shape
   std::vector<int> h_pair_k, h_pair_b;    // box gids, k < b
   std::vector<int64_t> h_pair_off;        // element offset of each block
-  std::vector<int> h_pair_part;           // partial-vector offset of each pair
-  std::vector<int> h_red_start;           // per box gid (nbox + 1)
-  std::vector<int> h_red_pair;            // per ordered slot entry: pair index
-  DevBuf pair_k, pair_b, pair_off, pair_part, red_start, red_part, m2l, m2l_part;
-  // red_list[kMaxIL g + t]: offset of the t-th partial of box g (slot
-  // order); unused entries point at a zero vector after the last partial
-  DevBuf red_list;
-  DevBuf pair_desc;                       // PairDesc per pair (one 32-byte load)
+  std::vector<int> h_red_start;           // per box gid (nbox + 1): first entry of h_red_pair
+  std::vector<int> h_red_pair;            // per present slot, slot order: pair index
+  DevBuf pair_k, pair_b, pair_off, m2l;
+  DevBuf m2l_part;                        // kILSlots slots of rank_g values per box
+  DevBuf pair_desc;                       // PairDesc per pair (48 bytes)
   int num_cus = 256;
   // schedule options (environment, read at create):
   //   AFMM2D_NEAR_OVERLAP=0   near field on the main stream after M2L
   //   AFMM2D_P2P_WAVES=n      near-field waves per SIMD (persistent grid)
+  //   AFMM2D_TREE_MODE=m      which upper-tree passes run beside the
+  //                           leaf-level M2L (matvec_device's schedules)
   bool near_overlap = true;
+  int tree_mode = 0;
   int p2p_waves_per_simd = 6;
   // FP64 log table for the near field: (1/c_j, -log(1/c_j)), c_j = 1 + (j + 1/2)/128
   DevBuf log_tab;
@@ -152,7 +160,7 @@ struct Engine {
   std::vector<double> last_qs, last_ps;  // host copies for compute_error
 
   // profiling
-  bool profiling = false;
+  int profiling = 0;  // 0 off, 1 every phase, 2 the leaf-level M2L only
   std::map<std::string, KernelTimer> timers;
   std::vector<std::array<hipEvent_t, 2>> ev_pool;
   size_t ev_used = 0;

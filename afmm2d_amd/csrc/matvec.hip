@@ -7,10 +7,10 @@
 // (potential += K(leaf, neighbour) q_neighbour for 8 neighbours and self),
 // scatter back to caller order.
 //
-// Stream plan: the near field depends only on the gathered q.  It is forked
-// onto a second stream after the upward pass and runs concurrently with the
-// HBM-bound M2L and L2L; the last kernel joins both (leaf L2P + near field +
-// scatter, fused).
+// Stream plan (matvec_device): after the leaf-level M2M, the near field and
+// the upper tree (levels < L) run on two internal streams beside the
+// HBM-bound leaf-level M2L on the caller's stream; the last kernel joins
+// them (leaf L2P + M2L sums + near field + scatter, fused).
 //
 // M2L reads each symmetric block once: the pair kernel holds
 // B = K(skel_k, skel_b) in registers (16-byte loads, all issued up front) and
@@ -23,9 +23,9 @@ namespace afmm {
 
 namespace {
 
-// Wave priority of the HBM-bound kernels that run beside the near field
-// (M2L, downward pass): they issue their loads ahead of the near field's
-// FP64 arithmetic, which fills the remaining issue slots.
+// Wave priority of the memory-bound kernels that run beside the near field
+// (M2L, M2M, L2L): they issue their loads ahead of the near field's FP64
+// arithmetic, which fills the remaining issue slots.
 #ifndef AFMM_MEM_PRIO
 #define AFMM_MEM_PRIO 3
 #endif
@@ -117,21 +117,19 @@ __device__ __forceinline__ double reduce_groups(double acc, double* red, const G
 }
 
 // A box's local coefficients into xs: (has_parent ? the parent's L2L
-// contribution : 0) + its M2L partials in interaction-slot order.  The list
-// (kMaxIL entries, unused ones pointing at a zero vector) is read from the
-// box's row at kernel entry, with the other per-box scalars, so the partial
-// loads go out in the same round as the matrix loads.
+// contribution : 0) + its M2L partials in interaction-slot order.  The box's
+// kILSlots partial slots (rank values each, unused ones zero) are one
+// contiguous block, so every load is issued at once.
 template <int WPB>
 __device__ __forceinline__ void local_coeffs(double* xs, int r, const double* __restrict__ par,
-                                             int has_parent, const double* __restrict__ part,
-                                             const int (&lst)[kMaxIL]) {
+                                             int has_parent, const double* __restrict__ blk) {
   for (int c = threadIdx.x; c < r; c += 64 * WPB) {
-    double v[kMaxIL];
+    double v[kILSlots];
 #pragma unroll
-    for (int t = 0; t < kMaxIL; t++) v[t] = part[lst[t] + c];
+    for (int t = 0; t < kILSlots; t++) v[t] = blk[t * r + c];
     double acc = has_parent ? par[c] : 0.0;
 #pragma unroll
-    for (int t = 0; t < kMaxIL; t++) acc += v[t];
+    for (int t = 0; t < kILSlots; t++) acc += v[t];
     xs[c] = acc;
   }
 }
@@ -152,6 +150,7 @@ __global__ __launch_bounds__(64 * WPB) void k_up(const double* __restrict__ l2p_
                                                  int nb) {
   extern __shared__ double xs[];
   __shared__ double red[64 * WPB];
+  if (AFMM_MEM_PRIO) __builtin_amdgcn_s_setprio(AFMM_MEM_PRIO);
   const int k = blockIdx.x;
   if (k >= nb) return;
   const int64_t g = gbase + k;
@@ -210,20 +209,16 @@ __device__ __forceinline__ double sum_lanes_stride8(double v) {
 // counts are compile-time: ld <= 16 MAXRT, rb <= 8 MAXCT).  B out_b: each
 // lane accumulates its two rows over its columns, then the 8 lanes sharing
 // rp are summed; B^T out_k: per column tile, the 8 lanes of a column are
-// summed.  Writes B out_b then B^T out_k to the pair's partial vector.
+// summed.  Writes B out_b to the first box's partial slot and B^T out_k to
+// the second box's.
 template <int MAXRT, int MAXCT>
-__global__ __launch_bounds__(64) void k_m2l_reg(const double* __restrict__ m2l,
-                                                const PairDesc* __restrict__ pd,
-                                                const double* __restrict__ outv,
-                                                double* __restrict__ part, int npairs) {
-  if (AFMM_MEM_PRIO) __builtin_amdgcn_s_setprio(AFMM_MEM_PRIO);
-  const int p = blockIdx.x;
-  if (p >= npairs) return;
-  const PairDesc d = pd[p];
+__device__ __forceinline__ void m2l_pair(const double* __restrict__ m2l, const PairDesc& d,
+                                         const double* __restrict__ outv,
+                                         double* __restrict__ part) {
   const int lane = threadIdx.x, rp = lane & 7, ci = lane >> 3;
   const int rk = d.rk, rb = d.rb, ld = d.ld;
-  double* fw = part + d.part;
-  double* tw = fw + rk;
+  double* fw = part + d.part_k;
+  double* tw = part + d.part_b;
   const double* xk = outv + d.sk;
   const double* xb = outv + d.sb;
   const double2* B2 = reinterpret_cast<const double2*>(m2l + d.off);
@@ -279,6 +274,17 @@ __global__ __launch_bounds__(64) void k_m2l_reg(const double* __restrict__ m2l,
   }
 }
 
+template <int MAXRT, int MAXCT>
+__global__ __launch_bounds__(64) void k_m2l_reg(const double* __restrict__ m2l,
+                                                const PairDesc* __restrict__ pd,
+                                                const double* __restrict__ outv,
+                                                double* __restrict__ part, int npairs) {
+  if (AFMM_MEM_PRIO) __builtin_amdgcn_s_setprio(AFMM_MEM_PRIO);
+  const int p = blockIdx.x;
+  if (p >= npairs) return;
+  m2l_pair<MAXRT, MAXCT>(m2l, pd[p], outv, part);
+}
+
 // M2L pair kernel for blocks too large for the register tiles: one
 // workgroup of 4 waves per pair, the block staged through LDS in column
 // chunks of at most 'tile' doubles.  LDS: the chunk, out_k and out_b (rcap
@@ -296,8 +302,8 @@ __global__ __launch_bounds__(256) void k_m2l_lds(const double* __restrict__ m2l,
   const PairDesc d = pd[blockIdx.x];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int rk = d.rk, rb = d.rb, ld = d.ld;
-  double* fw = part + d.part;
-  double* tw = fw + rk;
+  double* fw = part + d.part_k;
+  double* tw = part + d.part_b;
   const double* B = m2l + d.off;
   for (int i = tid; i < rk; i += 256) xk[i] = outv[d.sk + i];
   for (int c = tid; c < rb; c += 256) xb[c] = outv[d.sb + c];
@@ -342,7 +348,6 @@ __global__ __launch_bounds__(64 * WPB) void k_down(const double* __restrict__ l2
                                                    const int64_t* __restrict__ soff,
                                                    const int64_t* __restrict__ child_soff,
                                                    const double* __restrict__ part,
-                                                   const int* __restrict__ red_list,
                                                    double* __restrict__ inv, int has_parent,
                                                    int64_t gbase, int nb) {
   extern __shared__ double xs[];
@@ -351,9 +356,6 @@ __global__ __launch_bounds__(64 * WPB) void k_down(const double* __restrict__ l2
   const int k = blockIdx.x;
   if (k >= nb) return;
   const int64_t g = gbase + k;
-  int lst[kMaxIL];
-#pragma unroll
-  for (int t = 0; t < kMaxIL; t++) lst[t] = red_list[(int64_t)kMaxIL * g + t];
   const int r = rank[g];
   const int n = nin[g];
   if (n == 0) return;
@@ -363,7 +365,7 @@ __global__ __launch_bounds__(64 * WPB) void k_down(const double* __restrict__ l2
   double pv[B];
   GemvLane q = gemv_lane(n, 0, r);
   gemv_load<WPB, B>(pv, P, r, n, 0, q, w);
-  local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part, lst);
+  local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part + kILSlots * soff[g]);
   __syncthreads();
   for (int cb = 0; cb < n; cb += 64) {
     if (cb > 0) {
@@ -456,7 +458,8 @@ __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
       bc[j] = 0.0;
     }
     __syncthreads();
-    double* out = part + pair_part[p];
+    double* out = part + pair_part[2 * p];     // A's sums
+    double* outb = part + pair_part[2 * p + 1];  // B's sums
     for (int ca = 0; ca < pa; ca += 64) {
       const int i = ca + t;
       const bool vi = i < na;
@@ -503,15 +506,14 @@ __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
       for (int i = t; i < na; i += 64) out[i] = scale * (out[i] + bc[i]);
     } else {
       for (int i = t; i < na; i += 64) out[i] = scale * out[i];
-      for (int j = t; j < nb; j += 64) out[na + j] = scale * bc[j];
+      for (int j = t; j < nb; j += 64) outb[j] = scale * bc[j];
     }
   }
 }
 
 // Leaf level, fused: in_k (parent contribution + M2L partials), potential
-// = L2P_k in_k + the near-field partials of the leaf's self pair and its
-// neighbour slots 0..7 (in that order; absent slots read zeros), scattered
-// to the caller's order.
+// = L2P_k in_k + the leaf's 9 near-field partial slots (self, then
+// neighbours 0..7; absent ones are zero), scattered to the caller's order.
 template <int WPB, int B>
 __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2p_cm,
                                                    const int64_t* __restrict__ l2poff,
@@ -520,10 +522,8 @@ __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2
                                                    const int* __restrict__ leaf_start,
                                                    const int* __restrict__ leaf_count,
                                                    const double* __restrict__ part,
-                                                   const int* __restrict__ red_list,
                                                    const double* __restrict__ inv, int has_parent,
                                                    const double* __restrict__ near_part,
-                                                   const int* __restrict__ slot_part,
                                                    const int* __restrict__ perm,
                                                    double* __restrict__ yout, int64_t gbase,
                                                    int nb) {
@@ -532,15 +532,11 @@ __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2
   const int k = blockIdx.x;
   if (k >= nb) return;
   const int64_t g = gbase + k;
-  int lst[kMaxIL], so[9];
-#pragma unroll
-  for (int t = 0; t < kMaxIL; t++) lst[t] = red_list[(int64_t)kMaxIL * g + t];
-#pragma unroll
-  for (int t = 0; t < 9; t++) so[t] = slot_part[9 * k + t];
   const int r = rank[g];
   const int st = leaf_start[k], n = leaf_count[k];
   if (n == 0) return;
   const double* P = l2p_cm + l2poff[g];
+  const double* nblk = near_part + 9 * (int64_t)st;  // 9 slots of n values
   const int w = threadIdx.x >> 6;
   double pv[B];
   GemvLane q = gemv_lane(n, 0, r);
@@ -551,10 +547,10 @@ __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2
   {
     const int i = min((int)threadIdx.x, n - 1);
 #pragma unroll
-    for (int t = 0; t < 9; t++) nv[t] = near_part[so[t] + i];
+    for (int t = 0; t < 9; t++) nv[t] = nblk[t * n + i];
     dst_i = perm[st + i];
   }
-  local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part, lst);
+  local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part + kILSlots * soff[g]);
   __syncthreads();
   for (int cb = 0; cb < n; cb += 64) {
     if (cb > 0) {
@@ -562,7 +558,7 @@ __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2
       gemv_load<WPB, B>(pv, P, r, n, cb, q, w);
       const int i = min(cb + (int)threadIdx.x, n - 1);
 #pragma unroll
-      for (int t = 0; t < 9; t++) nv[t] = near_part[so[t] + i];
+      for (int t = 0; t < 9; t++) nv[t] = nblk[t * n + i];
       dst_i = perm[st + i];
     }
     const double acc = gemv_chunk<WPB, B>(pv, P, r, n, cb, q, xs);
@@ -668,129 +664,197 @@ int waves_per_box(int nb) {
     } break;                              \
   }
 
+namespace {
+
+// M2M of level j (out_k = L2P_k^T src).
+void launch_up(Engine& e, int j, hipStream_t s) {
+  const LevelInfo& li = e.lv[j];
+  const int wpb = waves_per_box(li.nb);
+  const size_t shm = sizeof(double) * std::max(1, li.max_nin);
+  const bool leaf = j == e.L;
+  const int64_t* soff = e.soff_g.as<int64_t>();
+  AFMM_DISPATCH_WPB(wpb, WPB,
+                    k_up<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
+                        e.l2p_rm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
+                        e.nin_g.as<int>(), soff, leaf ? e.qs.as<double>() : e.outv.as<double>(),
+                        leaf ? nullptr : soff + e.lv[j + 1].gbase,
+                        leaf ? e.d_leaf_start.as<int>() : nullptr, e.outv.as<double>(), li.gbase,
+                        li.nb));
+  AFMM_LAUNCH_CHECK();
+}
+
+// M2L pair blocks of levels j0..j1 (one launch; pairs are stored level by
+// level, so the levels' pairs are one contiguous range).
+void launch_m2l(Engine& e, int j0, int j1, hipStream_t s) {
+  const int64_t p0 = e.lv[j0].pair_begin, p1 = e.lv[j1].pair_end;
+  if (p1 <= p0) return;
+  int ml = 0, mb = 0;
+  for (int j = j0; j <= j1; j++) {
+    ml = std::max(ml, e.lv[j].pair_max_ld);
+    mb = std::max(mb, e.lv[j].pair_max_rb);
+  }
+  const PairDesc* pd = e.pair_desc.as<PairDesc>() + p0;
+  double* part = e.m2l_part.as<double>();
+  const double* outv = e.outv.as<double>();
+  const double* B = e.m2l.as<double>();
+  const int np = (int)(p1 - p0);
+  if (ml <= 32 && mb <= 32) {
+    k_m2l_reg<2, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np);
+  } else if (ml <= 48 && mb <= 48) {
+    k_m2l_reg<3, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np);
+  } else if (ml <= 64 && mb <= 64) {
+    k_m2l_reg<4, 8><<<np, 64, 0, s>>>(B, pd, outv, part, np);
+  } else {
+    const int rcap = std::max(e.max_rank, 1);
+    const size_t shm = sizeof(double) * ((size_t)e.pair_tile + 10 * (size_t)rcap);
+    k_m2l_lds<<<np, 256, shm, s>>>(B, pd, outv, part, e.pair_tile, rcap);
+  }
+  AFMM_LAUNCH_CHECK();
+}
+
+// L2L of level j (2 <= j < L), with the level's M2L sums fused in.
+void launch_down(Engine& e, int j, hipStream_t s) {
+  const LevelInfo& li = e.lv[j];
+  const int wpb = waves_per_box(li.nb);
+  const size_t shm = sizeof(double) * std::max(1, li.max_rank);
+  const int64_t* soff = e.soff_g.as<int64_t>();
+  AFMM_DISPATCH_WPB(wpb, WPB,
+                    k_down<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
+                        e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
+                        e.nin_g.as<int>(), soff, soff + e.lv[j + 1].gbase,
+                        e.m2l_part.as<double>(), e.inv.as<double>(), j > 2 ? 1 : 0, li.gbase,
+                        li.nb));
+  AFMM_LAUNCH_CHECK();
+}
+
+// Near field: persistent grid, a few waves per SIMD, leaving wave slots to
+// the memory-bound kernels that run beside it.
+void launch_near(Engine& e, hipStream_t s) {
+  const int maxpad = (e.max_leaf + 63) & ~63;
+  const size_t shm = sizeof(double) * (256 + 4 * (size_t)maxpad);
+  // p2p_waves_per_simd = 0: one workgroup per pair
+  const int64_t cap = e.p2p_waves_per_simd > 0 ? (int64_t)4 * e.num_cus * e.p2p_waves_per_simd
+                                               : e.near_upairs;
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(e.near_upairs, cap));
+  AFMM_DISPATCH_KERNEL(e.kid, KID,
+                       k_p2p<KID><<<grid, 64, shm, s>>>(
+                           e.x.as<double>(), e.y.as<double>(), e.qs.as<double>(),
+                           e.d_leaf_start.as<int>(), e.d_leaf_count.as<int>(),
+                           e.near_pair.as<int>(), e.near_pair_part.as<int>(),
+                           e.near_part.as<double>(), (int)e.near_upairs, maxpad, e.kp,
+                           e.log_tab.as<double2>(), e.ln2_hi, e.ln2_lo));
+  AFMM_LAUNCH_CHECK();
+}
+
+// Leaf level: local coefficients, leaf L2P, near-field sums, scatter.
+void launch_leaf(Engine& e, double* d_y, hipStream_t s) {
+  const int L = e.L;
+  const LevelInfo& li = e.lv[L];
+  const int wpb = waves_per_box(li.nb);
+  const size_t shm = sizeof(double) * std::max(1, li.max_rank);
+  AFMM_DISPATCH_WPB(wpb, WPB,
+                    k_leaf<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
+                        e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
+                        e.soff_g.as<int64_t>(), e.d_leaf_start.as<int>(),
+                        e.d_leaf_count.as<int>(), e.m2l_part.as<double>(), e.inv.as<double>(),
+                        L > 2 ? 1 : 0, e.near_part.as<double>(), e.perm.as<int>(), d_y,
+                        li.gbase, li.nb));
+  AFMM_LAUNCH_CHECK();
+}
+
+void upper_up(Engine& e, hipStream_t s) {
+  hipEvent_t a, b;
+  e.prof_begin("m2m_upper", s, &a, &b);
+  for (int j = e.L - 1; j >= 2; j--) launch_up(e, j, s);
+  e.prof_end(s, b);
+}
+
+void upper_m2l(Engine& e, hipStream_t s) {
+  if (e.L <= 2) return;
+  hipEvent_t a, b;
+  e.prof_begin("m2l_upper", s, &a, &b);
+  launch_m2l(e, 2, e.L - 1, s);
+  e.prof_end(s, b);
+}
+
+void down_chain(Engine& e, hipStream_t s) {
+  hipEvent_t a, b;
+  e.prof_begin("l2l", s, &a, &b);
+  for (int j = 2; j < e.L; j++) launch_down(e, j, s);
+  e.prof_end(s, b);
+}
+
+}  // namespace
+
 void gather_sorted(Engine& e, const double* d_b, double* d_qs, hipStream_t s) {
   k_gather<<<ceil_div(e.n, 256), 256, 0, s>>>(d_b, e.perm.as<int>(), d_qs, e.n);
   AFMM_LAUNCH_CHECK();
 }
 
+// Schedules (Engine::tree_mode; the near field forks onto its own stream
+// unless near_overlap is off, in which case everything runs on s in order):
+//   0  s: gather, M2M (all levels), M2L (upper levels, then leaf level),
+//      L2L, leaf pass; the near field forks after the M2M.
+//   1  after the leaf-level M2M, the upper tree (M2M, M2L and L2L of levels
+//      < L) runs on a third stream beside the leaf-level M2L.
+//   2  as 0, but after the upper levels' M2L the L2L chain runs on a third
+//      stream beside the leaf-level M2L.
 void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
   const int L = e.L;
-  const int nleaf = 1 << (2 * L);
   e.qs.alloc(sizeof(double) * e.n);
   hipEvent_t a, b;
-  const int64_t* soff = e.soff_g.as<int64_t>();
-  const int64_t* l2poff = e.l2poff_g.as<int64_t>();
-  const int* rank = e.rank_g.as<int>();
-  const int* nin = e.nin_g.as<int>();
+  const bool near_side = e.near_overlap;
+  const int mode = e.near_overlap ? e.tree_mode : 0;
 
   e.prof_begin("gather", s, &a, &b);
   gather_sorted(e, d_b, e.qs.as<double>(), s);
   e.prof_end(s, b);
 
-  // upward pass
   e.prof_begin("m2m", s, &a, &b);
-  for (int j = L; j >= 2; j--) {
-    const LevelInfo& li = e.lv[j];
-    const int wpb = waves_per_box(li.nb);
-    const size_t shm = sizeof(double) * std::max(1, li.max_nin);
-    const bool leaf = j == L;
-    AFMM_DISPATCH_WPB(wpb, WPB,
-                      k_up<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
-                          e.l2p_rm.as<double>(), l2poff, rank, nin, soff,
-                          leaf ? e.qs.as<double>() : e.outv.as<double>(),
-                          leaf ? nullptr : soff + e.lv[j + 1].gbase,
-                          leaf ? e.d_leaf_start.as<int>() : nullptr, e.outv.as<double>(),
-                          li.gbase, li.nb));
-    AFMM_LAUNCH_CHECK();
-  }
+  launch_up(e, L, s);
   e.prof_end(s, b);
+  if (mode == 1) {
+    AFMM_HIP(hipEventRecord(e.ev_fork, s));
+    AFMM_HIP(hipStreamWaitEvent(e.s_tree, e.ev_fork, 0));
+    upper_up(e, e.s_tree);
+    upper_m2l(e, e.s_tree);
+    down_chain(e, e.s_tree);
+    AFMM_HIP(hipEventRecord(e.ev_tree, e.s_tree));
+  } else {
+    upper_up(e, s);
+  }
 
-  // fork the near field onto the auxiliary stream: it overlaps the
-  // HBM-bound M2L and the downward pass (started before the upward pass,
-  // it would crowd out the latency-bound M2M)
-  hipStream_t sn = e.near_overlap ? e.s_aux : s;
-  if (e.near_overlap) {
+  if (near_side) {
     AFMM_HIP(hipEventRecord(e.ev_fork, s));
     AFMM_HIP(hipStreamWaitEvent(e.s_aux, e.ev_fork, 0));
+    e.prof_begin("p2p", e.s_aux, &a, &b);
+    launch_near(e, e.s_aux);
+    e.prof_end(e.s_aux, b);
+    AFMM_HIP(hipEventRecord(e.ev_join, e.s_aux));
   }
-  e.prof_begin("p2p", sn, &a, &b);
-  {
-    const int maxpad = (e.max_leaf + 63) & ~63;
-    const size_t shm = sizeof(double) * (256 + 4 * (size_t)maxpad);
-    // persistent grid: a few waves per SIMD, leaving wave slots to the
-    // memory-bound kernels that run beside it
-    const unsigned grid = (unsigned)std::min<int64_t>(
-        e.near_upairs, (int64_t)4 * e.num_cus * e.p2p_waves_per_simd);
-    AFMM_DISPATCH_KERNEL(e.kid, KID,
-                         k_p2p<KID><<<grid, 64, shm, sn>>>(
-                             e.x.as<double>(), e.y.as<double>(), e.qs.as<double>(),
-                             e.d_leaf_start.as<int>(), e.d_leaf_count.as<int>(),
-                             e.near_pair.as<int>(), e.near_pair_part.as<int>(),
-                             e.near_part.as<double>(), (int)e.near_upairs, maxpad, e.kp,
-                             e.log_tab.as<double2>(), e.ln2_hi, e.ln2_lo));
-    AFMM_LAUNCH_CHECK();
+
+  if (mode != 1) upper_m2l(e, s);
+  if (mode == 2) {
+    AFMM_HIP(hipEventRecord(e.ev_fork, s));
+    AFMM_HIP(hipStreamWaitEvent(e.s_tree, e.ev_fork, 0));
+    down_chain(e, e.s_tree);
+    AFMM_HIP(hipEventRecord(e.ev_tree, e.s_tree));
   }
-  e.prof_end(sn, b);
-  if (e.near_overlap) AFMM_HIP(hipEventRecord(e.ev_join, e.s_aux));
 
-
-  // M2L over every level at once: one partial vector per (pair, direction);
-  // each box sums its partials in the downward pass
-  const int rcap = std::max(e.max_rank, 1);
   e.prof_begin("m2l", s, &a, &b);
-  if (e.npairs > 0) {
-    const PairDesc* pd = e.pair_desc.as<PairDesc>();
-    double* part = e.m2l_part.as<double>();
-    const double* outv = e.outv.as<double>();
-    const double* B = e.m2l.as<double>();
-    const int np = (int)e.npairs;
-    const int ml = e.pair_max_ld, mb = e.pair_max_rb;
-    if (ml <= 32 && mb <= 32) {
-      k_m2l_reg<2, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np);
-    } else if (ml <= 48 && mb <= 48) {
-      k_m2l_reg<3, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np);
-    } else if (ml <= 64 && mb <= 64) {
-      k_m2l_reg<4, 8><<<np, 64, 0, s>>>(B, pd, outv, part, np);
-    } else {
-      const size_t shm = sizeof(double) * ((size_t)e.pair_tile + 10 * (size_t)rcap);
-      k_m2l_lds<<<np, 256, shm, s>>>(B, pd, outv, part, e.pair_tile, rcap);
-    }
-    AFMM_LAUNCH_CHECK();
-  }
+  launch_m2l(e, L, L, s);
   e.prof_end(s, b);
 
-  // downward pass: each box adds its M2L partials to its parent's
-  // contribution, then passes L2P_k in_k to its children
-  const double* part = e.m2l_part.as<double>();
-  e.prof_begin("l2l", s, &a, &b);
-  for (int j = 2; j < L; j++) {
-    const LevelInfo& li = e.lv[j];
-    const int wpb = waves_per_box(li.nb);
-    const size_t shm = sizeof(double) * std::max(1, li.max_rank);
-    AFMM_DISPATCH_WPB(wpb, WPB,
-                      k_down<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
-                          e.l2p_cm.as<double>(), l2poff, rank, nin, soff,
-                          soff + e.lv[j + 1].gbase, part, e.red_list.as<int>(),
-                          e.inv.as<double>(), j > 2 ? 1 : 0, li.gbase, li.nb));
-    AFMM_LAUNCH_CHECK();
+  if (mode == 0) down_chain(e, s);
+  if (!near_side) {
+    e.prof_begin("p2p", s, &a, &b);
+    launch_near(e, s);
+    e.prof_end(s, b);
   }
-  e.prof_end(s, b);
-
-  if (e.near_overlap) AFMM_HIP(hipStreamWaitEvent(s, e.ev_join, 0));
+  if (near_side) AFMM_HIP(hipStreamWaitEvent(s, e.ev_join, 0));
+  if (mode != 0) AFMM_HIP(hipStreamWaitEvent(s, e.ev_tree, 0));
   e.prof_begin("l2p_out", s, &a, &b);
-  {
-    const LevelInfo& li = e.lv[L];
-    const int wpb = waves_per_box(li.nb);
-    const size_t shm = sizeof(double) * std::max(1, li.max_rank);
-    AFMM_DISPATCH_WPB(wpb, WPB,
-                      k_leaf<WPB, kGemvBatch><<<nleaf, 64 * WPB, shm, s>>>(
-                          e.l2p_cm.as<double>(), l2poff, rank, soff, e.d_leaf_start.as<int>(),
-                          e.d_leaf_count.as<int>(), part, e.red_list.as<int>(),
-                          e.inv.as<double>(), L > 2 ? 1 : 0,
-                          e.near_part.as<double>(), e.near_slot_part.as<int>(), e.perm.as<int>(),
-                          d_y, li.gbase, nleaf));
-    AFMM_LAUNCH_CHECK();
-  }
+  launch_leaf(e, d_y, s);
   e.prof_end(s, b);
 }
 

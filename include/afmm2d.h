@@ -63,6 +63,9 @@ typedef struct {
   int64_t matvec_bytes;      /* algorithmic HBM bytes of one matvec */
   double create_seconds;     /* KD partition + topology */
   double assemble_seconds;   /* ACA + ID + M2L build */
+  int64_t m2l_leaf_bytes;    /* stored M2L coefficients of the leaf level's pairs */
+  int64_t m2l_leaf_columns;  /* their partial columns (rank_k + rank_b per pair) */
+  int64_t skeleton_leaf;     /* sum of the leaf level's ranks */
 } afmm2d_stats;
 
 /* Message of the calling thread's last failure ("" if none). */
@@ -158,9 +161,12 @@ int afmm2d_kernel_value(int kernel_id, const double* kparams, double a, double x
 
 int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st);
 
-/* Per-kernel device timing with HIP events recorded on the stream each
- * kernel runs on.  names: "gather", "m2m", "m2l", "l2l", "p2p", "l2p_out". */
-int afmm2d_set_profiling(afmm2d_handle h, int enable);
+/* Per-phase device timing with HIP events recorded on the stream each phase
+ * runs on.  level 0: off; 1: every phase ("gather", "m2m", "m2m_upper",
+ * "m2l", "m2l_upper", "l2l", "p2p", "l2p_out"); 2: only "m2l" (the leaf
+ * level's M2L launch), one event pair per matvec, so a timed loop is barely
+ * perturbed. */
+int afmm2d_set_profiling(afmm2d_handle h, int level);
 int afmm2d_get_kernel_time(afmm2d_handle h, const char* name, double* total_ms, int64_t* launches);
 int afmm2d_reset_kernel_times(afmm2d_handle h);
 

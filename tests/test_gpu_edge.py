@@ -137,3 +137,54 @@ def test_cpp_driver_runs():
     out = subprocess.run([exe, "16384", "64", "10"], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "err:" in out.stdout and "Assembly Time" in out.stdout
+
+
+@pytest.mark.parametrize("near,tree,waves", [
+    ("1", "1", "6"), ("1", "2", "6"), ("0", "0", "6"), ("1", "0", "0"), ("1", "1", "2"),
+])
+def test_schedules_are_bit_identical(engine, monkeypatch, near, tree, waves):
+    """Every stream schedule (upper tree and near field beside the leaf-level
+    M2L, or in order) computes the same sums in the same order: the outputs
+    must agree bit for bit with the default schedule, over several calls."""
+    n = 50000
+    xy, q = uniform_points(n, 7)
+    qs = np.stack([q, np.cos(3 * q)], axis=1)
+    A = engine.AFMM(n, 48, 10, xy)
+    A.assemble()
+    ref = A.computeMatVecProduct(qs)
+    A.close()
+    monkeypatch.setenv("AFMM2D_NEAR_OVERLAP", near)
+    monkeypatch.setenv("AFMM2D_TREE_MODE", tree)
+    monkeypatch.setenv("AFMM2D_P2P_WAVES", waves)
+    B = engine.AFMM(n, 48, 10, xy)
+    B.assemble()
+    for _ in range(3):
+        np.testing.assert_array_equal(B.computeMatVecProduct(qs), ref)
+    B.close()
+
+
+def test_profiling_levels(engine):
+    n = 20000
+    xy, q = uniform_points(n, 3)
+    A = engine.AFMM(n, 64, 10, xy)
+    A.assemble()
+    names = ("gather", "m2m", "m2m_upper", "m2l", "m2l_upper", "l2l", "p2p", "l2p_out")
+    A.reset_kernel_times()
+    A.set_profiling(True)
+    A.computeMatVecProduct(q)
+    A.computeMatVecProduct(q)
+    for nm in names:
+        tot, cnt = A.kernel_time(nm)
+        assert cnt == 2 and tot > 0.0, nm
+    A.reset_kernel_times()
+    A.set_profiling(True, m2l_only=True)
+    A.computeMatVecProduct(q)
+    for nm in names:
+        tot, cnt = A.kernel_time(nm)
+        assert cnt == (1 if nm == "m2l" else 0), nm
+    A.set_profiling(False)
+    A.reset_kernel_times()
+    A.computeMatVecProduct(q)
+    assert A.kernel_time("m2l")[1] == 0
+    assert engine.lib().afmm2d_set_profiling(A._h, 3) == 1  # AFMM2D_ERR_INVALID_ARGUMENT
+    A.close()
