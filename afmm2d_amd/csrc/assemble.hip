@@ -647,11 +647,14 @@ void assemble_tree(Engine& e) {
   upload(e.soff_g, soff_g, s);
   upload(e.l2poff_g, l2poff_g, s);
 
-  // M2L layout: every unordered interaction pair once (k < b), in the order
-  // the boxes' slots first meet them.  Partials are box-major: box g owns
-  // kILSlots slots of rank_g values at kILSlots * soff[g]; its t-th present
-  // interaction slot (slot order) receives the partial of that pair, so a
-  // box's M2L sum reads one contiguous block (unused slots stay zero).
+  // M2L layout: every unordered interaction pair once.  Each pair is stored
+  // as B = K(skeleton_first, skeleton_second) in the orientation whose
+  // register-tile class is cheaper (m2l_tile_class), and within two groups
+  // (levels 2..L-1, level L) the pairs are ordered by class, so that one
+  // launch per (group, class) covers them.  Partials are box-major: box g
+  // owns kILSlots slots of rank_g values at kILSlots * soff[g]; its t-th
+  // present interaction slot (slot order) receives that pair's product for
+  // it, so a box's M2L sum reads one contiguous block (unused slots stay 0).
   e.h_pair_k.clear();
   e.h_pair_b.clear();
   e.pair_max_ld = 0;
@@ -667,10 +670,12 @@ void assemble_tree(Engine& e) {
   auto slot_box = [&](int j, int k, int q) {
     return q < 16 ? e.inner[j][16 * (size_t)k + q] : e.outer[j][24 * (size_t)k + q - 16];
   };
+  struct Cand {
+    int first, second, cls, group;
+  };
+  std::vector<Cand> cand;
   for (int j = 2; j <= L; j++) {
     LevelInfo& li = e.lv[j];
-    li.pair_begin = (int64_t)e.h_pair_k.size();
-    li.pair_max_ld = li.pair_max_rb = 0;
     li.m2l_coeffs = li.m2l_cols = 0;
     for (int k = 0; k < li.nb; k++) {
       for (int q = 0; q < 40; q++) {
@@ -678,24 +683,42 @@ void assemble_tree(Engine& e) {
         if (b < 0 || b <= k) continue;
         const int gk = (int)(li.gbase + k), gbx = (int)(li.gbase + b);
         const int rk = li.rank[k], rb = li.rank[b];
-        pair_of[((int64_t)gk << 32) | gbx] = (int)e.h_pair_k.size();
-        e.h_pair_k.push_back(gk);
-        e.h_pair_b.push_back(gbx);
-        e.h_pair_off.push_back(off);
-        const int ld = rk + (rk & 1);
-        off += (int64_t)ld * rb;  // ld even: every block and column 16-byte aligned
+        const int ca = m2l_tile_class(rk + (rk & 1), rb);
+        const int cb = m2l_tile_class(rb + (rb & 1), rk);
+        if (cb < ca) cand.push_back(Cand{gbx, gk, cb, j == L ? 1 : 0});
+        else cand.push_back(Cand{gk, gbx, ca, j == L ? 1 : 0});
         part += rk + rb;
         e.m2l_total += (int64_t)rk * rb;
-        tile = std::max(tile, ld * rb);
-        e.pair_max_ld = std::max(e.pair_max_ld, ld);
-        e.pair_max_rb = std::max(e.pair_max_rb, rb);
-        li.pair_max_ld = std::max(li.pair_max_ld, ld);
-        li.pair_max_rb = std::max(li.pair_max_rb, rb);
         li.m2l_coeffs += (int64_t)rk * rb;
         li.m2l_cols += rk + rb;
       }
     }
-    li.pair_end = (int64_t)e.h_pair_k.size();
+  }
+  std::vector<int> order(cand.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int u, int v) {
+    return cand[u].group != cand[v].group ? cand[u].group < cand[v].group
+                                          : cand[u].cls < cand[v].cls;
+  });
+  for (int g = 0; g < 2; g++)
+    for (int c = 0; c <= kM2LClasses; c++) e.m2l_cls_begin[g][c] = e.m2l_cls_end[g][c] = 0;
+  for (size_t i = 0; i < order.size(); i++) {
+    const Cand& cd = cand[order[i]];
+    const int qidx = (int)i;
+    if (i == 0 || cand[order[i - 1]].group != cd.group || cand[order[i - 1]].cls != cd.cls)
+      e.m2l_cls_begin[cd.group][cd.cls] = qidx;
+    e.m2l_cls_end[cd.group][cd.cls] = qidx + 1;
+    const int rf = rank_g[cd.first], rs = rank_g[cd.second];
+    const int lo = std::min(cd.first, cd.second), hi = std::max(cd.first, cd.second);
+    pair_of[((int64_t)lo << 32) | hi] = qidx;
+    e.h_pair_k.push_back(cd.first);
+    e.h_pair_b.push_back(cd.second);
+    e.h_pair_off.push_back(off);
+    const int ld = rf + (rf & 1);
+    off += (int64_t)ld * rs;  // ld even: every block and column 16-byte aligned
+    tile = std::max(tile, ld * rs);
+    e.pair_max_ld = std::max(e.pair_max_ld, ld);
+    e.pair_max_rb = std::max(e.pair_max_rb, rs);
   }
   e.npairs = (int64_t)e.h_pair_k.size();
   if ((int64_t)kILSlots * sb > INT32_MAX) throw Error(6, "M2L partial buffer exceeds 2^31 entries");
@@ -717,7 +740,7 @@ void assemble_tree(Engine& e) {
         // the pair writes B out_b to its first box's slot, B^T out_k to its
         // second box's slot
         const int dst = (int)(kILSlots * soff_g[gk] + (int64_t)t * li.rank[k]);
-        if (gk < gbx) part_k[pidx] = dst;
+        if (gk == e.h_pair_k[pidx]) part_k[pidx] = dst;
         else part_b[pidx] = dst;
         t++;
         e.il_pairs++;

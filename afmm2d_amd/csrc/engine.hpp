@@ -66,12 +66,25 @@ struct LevelInfo {
   std::vector<int64_t> soff;    // global skeleton offset per box (nb + 1 entries)
   std::vector<int64_t> l2poff;  // element offset into l2p_* (nb + 1)
   int max_nin = 0, max_rank = 0;
-  // this level's M2L pairs: [pair_begin, pair_end) of the pair list, their
-  // largest block shape, coefficients and partial columns
-  int64_t pair_begin = 0, pair_end = 0;
-  int pair_max_ld = 0, pair_max_rb = 0;
+  // this level's M2L coefficients (rank_k rank_b per pair) and partial
+  // columns (rank_k + rank_b per pair)
   int64_t m2l_coeffs = 0, m2l_cols = 0;
 };
+
+// Register-tile classes of the M2L pair kernel, cheapest first (VGPRs per
+// lane on gfx950: 59, 75, 86, 88, 100, 114, 185): class c holds blocks of up
+// to 16 kM2LTileRows[c] rows (leading dimension) and 8 kM2LTileCols[c]
+// columns; class kM2LClasses is the LDS kernel for anything larger.
+constexpr int kM2LClasses = 7;
+constexpr int kM2LTileRows[kM2LClasses] = {2, 2, 2, 3, 3, 3, 4};
+constexpr int kM2LTileCols[kM2LClasses] = {4, 5, 6, 4, 5, 6, 8};
+
+inline int m2l_tile_class(int ld, int cols) {
+  const int rt = (ld + 15) / 16, ct = (cols + 7) / 8;
+  for (int c = 0; c < kM2LClasses; c++)
+    if (rt <= kM2LTileRows[c] && ct <= kM2LTileCols[c]) return c;
+  return kM2LClasses;
+}
 
 // Everything the M2L pair kernel needs about one pair, in one record.
 struct PairDesc {
@@ -133,8 +146,11 @@ struct Engine {
   int64_t npairs = 0;
   int pair_tile = 0;                      // LDS doubles for one block (fallback kernel)
   int pair_max_ld = 0, pair_max_rb = 0;   // largest block shape
-  std::vector<int> h_pair_k, h_pair_b;    // box gids, k < b
+  std::vector<int> h_pair_k, h_pair_b;    // box gids: first (block rows), second
   std::vector<int64_t> h_pair_off;        // element offset of each block
+  // pairs of group g (0: levels 2..L-1, 1: level L) in tile class c:
+  // [m2l_cls_begin[g][c], m2l_cls_end[g][c])
+  int64_t m2l_cls_begin[2][kM2LClasses + 1] = {}, m2l_cls_end[2][kM2LClasses + 1] = {};
   std::vector<int> h_red_start;           // per box gid (nbox + 1): first entry of h_red_pair
   std::vector<int> h_red_pair;            // per present slot, slot order: pair index
   DevBuf pair_k, pair_b, pair_off, m2l;

@@ -683,33 +683,34 @@ void launch_up(Engine& e, int j, hipStream_t s) {
   AFMM_LAUNCH_CHECK();
 }
 
-// M2L pair blocks of levels j0..j1 (one launch; pairs are stored level by
-// level, so the levels' pairs are one contiguous range).
-void launch_m2l(Engine& e, int j0, int j1, hipStream_t s) {
-  const int64_t p0 = e.lv[j0].pair_begin, p1 = e.lv[j1].pair_end;
-  if (p1 <= p0) return;
-  int ml = 0, mb = 0;
-  for (int j = j0; j <= j1; j++) {
-    ml = std::max(ml, e.lv[j].pair_max_ld);
-    mb = std::max(mb, e.lv[j].pair_max_rb);
-  }
-  const PairDesc* pd = e.pair_desc.as<PairDesc>() + p0;
+// M2L pair blocks of group g (0: levels 2..L-1, 1: level L): one launch
+// per register-tile class present (pairs are stored grouped by class).
+void launch_m2l(Engine& e, int g, hipStream_t s) {
+  const PairDesc* pd0 = e.pair_desc.as<PairDesc>();
   double* part = e.m2l_part.as<double>();
   const double* outv = e.outv.as<double>();
   const double* B = e.m2l.as<double>();
-  const int np = (int)(p1 - p0);
-  if (ml <= 32 && mb <= 32) {
-    k_m2l_reg<2, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np);
-  } else if (ml <= 48 && mb <= 48) {
-    k_m2l_reg<3, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np);
-  } else if (ml <= 64 && mb <= 64) {
-    k_m2l_reg<4, 8><<<np, 64, 0, s>>>(B, pd, outv, part, np);
-  } else {
-    const int rcap = std::max(e.max_rank, 1);
-    const size_t shm = sizeof(double) * ((size_t)e.pair_tile + 10 * (size_t)rcap);
-    k_m2l_lds<<<np, 256, shm, s>>>(B, pd, outv, part, e.pair_tile, rcap);
+  for (int c = 0; c <= kM2LClasses; c++) {
+    const int64_t p0 = e.m2l_cls_begin[g][c], p1 = e.m2l_cls_end[g][c];
+    if (p1 <= p0) continue;
+    const PairDesc* pd = pd0 + p0;
+    const int np = (int)(p1 - p0);
+    switch (c) {
+      case 0: k_m2l_reg<2, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 1: k_m2l_reg<2, 5><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 2: k_m2l_reg<2, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 3: k_m2l_reg<3, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 4: k_m2l_reg<3, 5><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 5: k_m2l_reg<3, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 6: k_m2l_reg<4, 8><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      default: {
+        const int rcap = std::max(e.max_rank, 1);
+        const size_t shm = sizeof(double) * ((size_t)e.pair_tile + 10 * (size_t)rcap);
+        k_m2l_lds<<<np, 256, shm, s>>>(B, pd, outv, part, e.pair_tile, rcap);
+      }
+    }
+    AFMM_LAUNCH_CHECK();
   }
-  AFMM_LAUNCH_CHECK();
 }
 
 // L2L of level j (2 <= j < L), with the level's M2L sums fused in.
@@ -773,7 +774,7 @@ void upper_m2l(Engine& e, hipStream_t s) {
   if (e.L <= 2) return;
   hipEvent_t a, b;
   e.prof_begin("m2l_upper", s, &a, &b);
-  launch_m2l(e, 2, e.L - 1, s);
+  launch_m2l(e, 0, s);
   e.prof_end(s, b);
 }
 
@@ -842,7 +843,7 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
   }
 
   e.prof_begin("m2l", s, &a, &b);
-  launch_m2l(e, L, L, s);
+  launch_m2l(e, 1, s);
   e.prof_end(s, b);
 
   if (mode == 0) down_chain(e, s);
