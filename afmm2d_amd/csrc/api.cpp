@@ -129,7 +129,6 @@ Engine::~Engine() {
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
   if (ev_tree) (void)hipEventDestroy(ev_tree);
-  if (s_main) (void)hipStreamDestroy(s_main);
   if (s_aux) (void)hipStreamDestroy(s_aux);
   if (s_tree) (void)hipStreamDestroy(s_tree);
 }
@@ -227,12 +226,13 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
     AFMM_HIP(hipDeviceGetAttribute(&e->num_cus, hipDeviceAttributeMultiprocessorCount, device));
     if (const char* v = std::getenv("AFMM2D_NEAR_OVERLAP")) e->near_overlap = std::atoi(v) != 0;
     if (const char* v = std::getenv("AFMM2D_TREE_MODE"))
-      e->tree_mode = std::max(0, std::min(2, std::atoi(v)));
+      e->tree_mode = std::max(0, std::min(3, std::atoi(v)));
     if (const char* v = std::getenv("AFMM2D_P2P_WAVES"))
       e->p2p_waves_per_simd = std::max(0, std::min(16, std::atoi(v)));
-    AFMM_HIP(hipStreamCreateWithFlags(&e->s_main, hipStreamNonBlocking));
+    // one internal stream per handle (a process has few hardware queues,
+    // and streams that share one run in order): it carries the set-up work
+    // and the near field beside the caller's stream
     AFMM_HIP(hipStreamCreateWithFlags(&e->s_aux, hipStreamNonBlocking));
-    AFMM_HIP(hipStreamCreateWithFlags(&e->s_tree, hipStreamNonBlocking));
     AFMM_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
     AFMM_HIP(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
     AFMM_HIP(hipEventCreateWithFlags(&e->ev_tree, hipEventDisableTiming));
@@ -305,12 +305,12 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
       e->near_pair.alloc(sizeof(int) * pair.size());
       e->near_pair_part.alloc(sizeof(int) * ppart.size());
       e->near_part.alloc(sizeof(double) * (size_t)std::max<int64_t>(e->near_part_total, 1));
-      AFMM_HIP(hipMemsetAsync(e->near_part.p, 0, e->near_part.bytes, e->s_main));
+      AFMM_HIP(hipMemsetAsync(e->near_part.p, 0, e->near_part.bytes, e->s_aux));
       AFMM_HIP(hipMemcpyAsync(e->near_pair.p, pair.data(), sizeof(int) * pair.size(),
-                              hipMemcpyHostToDevice, e->s_main));
+                              hipMemcpyHostToDevice, e->s_aux));
       AFMM_HIP(hipMemcpyAsync(e->near_pair_part.p, ppart.data(), sizeof(int) * ppart.size(),
-                              hipMemcpyHostToDevice, e->s_main));
-      AFMM_HIP(hipStreamSynchronize(e->s_main));
+                              hipMemcpyHostToDevice, e->s_aux));
+      AFMM_HIP(hipStreamSynchronize(e->s_aux));
     }
     // Table of the near-field log: (RN(1/c_j), -log RN(1/c_j)) for
     // c_j = 1 + (j + 1/2)/128, evaluated in extended precision; ln2 split so
@@ -328,10 +328,10 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
       e->ln2_lo = (double)(ln2 - (long double)e->ln2_hi);
       e->log_tab.alloc(sizeof(double) * tab.size());
       AFMM_HIP(hipMemcpyAsync(e->log_tab.p, tab.data(), sizeof(double) * tab.size(),
-                              hipMemcpyHostToDevice, e->s_main));
-      AFMM_HIP(hipStreamSynchronize(e->s_main));
+                              hipMemcpyHostToDevice, e->s_aux));
+      AFMM_HIP(hipStreamSynchronize(e->s_aux));
     }
-    hipStream_t s = e->s_main;
+    hipStream_t s = e->s_aux;
     e->d_leaf_start.alloc(sizeof(int) * nleaf);
     e->d_leaf_count.alloc(sizeof(int) * nleaf);
     e->d_leaf_nbr.alloc(sizeof(int) * 8 * (size_t)nleaf);
@@ -352,9 +352,8 @@ int afmm2d_destroy(afmm2d_handle h) {
   return guarded([&] {
     if (h->e) {
       (void)hipSetDevice(h->e->device);
-      (void)hipStreamSynchronize(h->e->s_main);
       (void)hipStreamSynchronize(h->e->s_aux);
-      (void)hipStreamSynchronize(h->e->s_tree);
+      if (h->e->s_tree) (void)hipStreamSynchronize(h->e->s_tree);
     }
     delete h;
   });
@@ -393,7 +392,7 @@ int afmm2d_matvec(afmm2d_handle h, const double* b, double* y, int nrhs) {
     const size_t nb = sizeof(double) * e.n;
     e.io_b.alloc(nb);
     e.io_y.alloc(nb);
-    hipStream_t s = e.s_main;
+    hipStream_t s = nullptr;  // the null stream plays the caller's stream
     for (int r = 0; r < nrhs; r++) {
       AFMM_HIP(hipMemcpyAsync(e.io_b.p, b + (size_t)r * e.n, nb, hipMemcpyHostToDevice, s));
       matvec_device(e, e.io_b.as<double>(), e.io_y.as<double>(), s);
@@ -551,7 +550,7 @@ int afmm2d_direct_rows(afmm2d_handle h, int nrows, const int64_t* rows, const do
       if (rows[r] < 0 || rows[r] >= e.n) throw Error(AFMM2D_ERR_INVALID_ARGUMENT, "row out of range");
       srows[r] = inv[rows[r]];
     }
-    hipStream_t s = e.s_main;
+    hipStream_t s = nullptr;  // the null stream plays the caller's stream
     DevBuf d_b, d_qs, d_rows, d_out;
     d_b.alloc(sizeof(double) * e.n);
     d_qs.alloc(sizeof(double) * e.n);
@@ -583,7 +582,7 @@ int afmm2d_compute_error(afmm2d_handle h, int box, double* err) {
   return guarded([&] {
     AFMM_HIP(hipSetDevice(e.device));
     const int st = e.leaf_start[box], n = e.leaf_count[box];
-    hipStream_t s = e.s_main;
+    hipStream_t s = nullptr;  // the null stream plays the caller's stream
     DevBuf d_qs, d_rows, d_out;
     d_qs.alloc(sizeof(double) * e.n);
     d_rows.alloc(sizeof(int) * std::max(n, 1));

@@ -432,7 +432,7 @@ size_t free_device_bytes() {
 
 void assemble_tree(Engine& e) {
   const int L = e.L;
-  hipStream_t s = e.s_main;
+  hipStream_t s = e.s_aux;
   e.lv.assign(L + 1, LevelInfo{});
   // per-level skeleton points, kept until the global arrays are built
   std::vector<DevBuf> lskx(L + 2), lsky(L + 2), lskg(L + 2), lrm(L + 1), lcm(L + 1);

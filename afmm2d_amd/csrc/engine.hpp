@@ -112,7 +112,9 @@ struct Engine {
   double tol = 0.0;
   KParams kp{0.01, 0.0};
 
-  hipStream_t s_main = nullptr, s_aux = nullptr, s_tree = nullptr;
+  // s_aux: set-up work and the near field; s_tree: the upper tree in the
+  // schedules that run it beside the leaf-level M2L (created on first use)
+  hipStream_t s_aux = nullptr, s_tree = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_tree = nullptr;
 
   // KD partition
@@ -133,6 +135,7 @@ struct Engine {
   // A's sums and B's sums (slots of the leaf-major near_part)
   int64_t near_upairs = 0, near_part_total = 0;
   DevBuf near_pair, near_pair_part, near_part;
+  DevBuf near_queue;  // chunk counters of the near-field work queue
 
   // assembly
   bool assembled = false;
@@ -164,7 +167,7 @@ struct Engine {
   //                           leaf-level M2L (matvec_device's schedules)
   bool near_overlap = true;
   int tree_mode = 0;
-  int p2p_waves_per_simd = 6;
+  int p2p_waves_per_simd = 4;
   // FP64 log table for the near field: (1/c_j, -log(1/c_j)), c_j = 1 + (j + 1/2)/128
   DevBuf log_tab;
   double ln2_hi = 0.0, ln2_lo = 0.0;

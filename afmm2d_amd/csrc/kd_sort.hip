@@ -199,7 +199,7 @@ void segmented_sort(double* key, int* id, const std::vector<int>& off, DevBuf& d
 
 void kd_partition(Engine& e, const double* h_x, const double* h_y) {
   const int n = e.n, L = e.L;
-  hipStream_t s = e.s_main;
+  hipStream_t s = e.s_aux;
   DevBuf bx[2], by[2], bp[2], key, id, d_off;
   for (int q = 0; q < 2; q++) {
     bx[q].alloc(sizeof(double) * n);

@@ -134,9 +134,11 @@ __device__ __forceinline__ void local_coeffs(double* xs, int r, const double* __
   }
 }
 
-// M2M for one level: out_k = L2P_k^T src_k with row-major L2P (nin x rank);
-// src is the leaf's q segment (leaf level) or the children's concatenated
-// multipoles (always contiguous).  One workgroup of WPB waves per box.
+// M2M for one level: out_k = L2P_k^T src_k with row-major L2P (nin x rank).
+// Leaf level (b != nullptr): src_k is the leaf's charges, gathered here from
+// the caller's order (b[perm[.]]) and also written to qs in KD order for the
+// near field; upper levels: the children's concatenated multipoles (always
+// contiguous).  One workgroup of WPB waves per box.
 template <int WPB, int B>
 __global__ __launch_bounds__(64 * WPB) void k_up(const double* __restrict__ l2p_rm,
                                                  const int64_t* __restrict__ l2poff,
@@ -145,7 +147,10 @@ __global__ __launch_bounds__(64 * WPB) void k_up(const double* __restrict__ l2p_
                                                  const int64_t* __restrict__ soff,
                                                  const double* __restrict__ src,
                                                  const int64_t* __restrict__ src_off_child,
-                                                 const int* __restrict__ src_off_leaf,
+                                                 const double* __restrict__ b,
+                                                 const int* __restrict__ perm,
+                                                 const int* __restrict__ leaf_start,
+                                                 double* __restrict__ qs,
                                                  double* __restrict__ outv, int64_t gbase,
                                                  int nb) {
   extern __shared__ double xs[];
@@ -155,16 +160,28 @@ __global__ __launch_bounds__(64 * WPB) void k_up(const double* __restrict__ l2p_
   if (k >= nb) return;
   const int64_t g = gbase + k;
   const int r = rank[g];
-  if (r == 0) return;
   const int n = nin[g];
   const double* P = l2p_rm + l2poff[g];
-  const double* x = src + (src_off_leaf ? (int64_t)src_off_leaf[k] : src_off_child[4 * k]);
   double* out = outv + soff[g];
   const int w = threadIdx.x >> 6;
   double pv[B];
-  GemvLane q = gemv_lane(r, 0, n);
-  gemv_load<WPB, B>(pv, P, n, r, 0, q, w);
-  for (int i = threadIdx.x; i < n; i += 64 * WPB) xs[i] = x[i];
+  GemvLane q;
+  if (r > 0) {
+    q = gemv_lane(r, 0, n);
+    gemv_load<WPB, B>(pv, P, n, r, 0, q, w);
+  }
+  if (b) {
+    const int st = leaf_start[k];
+    for (int i = threadIdx.x; i < n; i += 64 * WPB) {
+      const double v = b[perm[st + i]];
+      xs[i] = v;
+      qs[st + i] = v;
+    }
+  } else {
+    const double* x = src + src_off_child[4 * k];
+    for (int i = threadIdx.x; i < n; i += 64 * WPB) xs[i] = x[i];
+  }
+  if (r == 0) return;
   __syncthreads();
   for (int cb = 0; cb < r; cb += 64) {
     if (cb > 0) {
@@ -413,6 +430,10 @@ __device__ __forceinline__ double near_kernel(double r2, const KParams& kp, cons
   }
 }
 
+constexpr int kNearSlices = 8;          // work-queue slices of the near field
+constexpr int kNearCounterStride = 32;  // ints between the slice counters (128 B)
+constexpr int kNearChunk = 4;           // pairs per counter increment
+
 // Near field, symmetric: one wave per unordered leaf pair (A, B), A <= B.
 // Lane t holds target point t of an A chunk of 64; at step s it meets
 // source point (t + s) mod 64 of a B chunk staged in LDS, so the 64 lanes
@@ -430,7 +451,8 @@ __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
                                             const int* __restrict__ pair_part,
                                             double* __restrict__ part, int npairs, int maxpad,
                                             KParams kp, const double2* __restrict__ logtab,
-                                            double ln2_hi, double ln2_lo) {
+                                            double ln2_hi, double ln2_lo,
+                                            int* __restrict__ queue) {
   extern __shared__ double sm[];
   double2* tab = reinterpret_cast<double2*>(sm);  // 128 entries
   double* bx = sm + 256;
@@ -443,7 +465,28 @@ __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
   }
   // padding: far away, zero charge; K stays finite there
   const double FAR = 1.0e30;
-  for (int p = blockIdx.x; p < npairs; p += gridDim.x) {
+  // work queue: the pairs are cut into kNearSlices slices, each with its own
+  // chunk counter (one cache line apart); a workgroup drains the slice of
+  // its own index first and then steals from the others, so a workgroup that
+  // becomes resident late (its slots held by the kernels beside it) only
+  // finds less work
+  int slice = blockIdx.x % kNearSlices, tried = 0, p = 0, p_end = 0;
+  for (;;) {
+    if (p >= p_end) {
+      int c = 0;
+      if (t == 0) c = atomicAdd(&queue[slice * kNearCounterStride], 1);
+      c = __builtin_amdgcn_readfirstlane(c);
+      const int s0 = (int)((int64_t)npairs * slice / kNearSlices);
+      const int s1 = (int)((int64_t)npairs * (slice + 1) / kNearSlices);
+      p = s0 + c * kNearChunk;
+      if (p >= s1) {
+        if (++tried == kNearSlices) break;
+        slice = (slice + 1) % kNearSlices;
+        p_end = 0;
+        continue;
+      }
+      p_end = min(p + kNearChunk, s1);
+    }
     const int A = pairs[2 * p], B = pairs[2 * p + 1];
     const bool self = A == B;
     const int sa = leaf_start[A], na = leaf_count[A];
@@ -508,7 +551,13 @@ __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
       for (int i = t; i < na; i += 64) out[i] = scale * out[i];
       for (int j = t; j < nb; j += 64) outb[j] = scale * bc[j];
     }
+    p++;
   }
+}
+
+__global__ void k_zero_ints(int* __restrict__ v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = 0;
 }
 
 // Leaf level, fused: in_k (parent contribution + M2L partials), potential
@@ -618,6 +667,17 @@ namespace {
 #define AFMM_GEMV_BATCH 16
 #endif
 constexpr int kGemvBatch = AFMM_GEMV_BATCH;  // row steps per batch of loads in the GEMV
+// batch sizes of the two largest levels' kernels (leaf-level M2M, level
+// L-1 L2L, leaf pass), where one wave serves one box
+#ifndef AFMM_B_UP_LEAF
+#define AFMM_B_UP_LEAF AFMM_GEMV_BATCH
+#endif
+#ifndef AFMM_B_DOWN_LAST
+#define AFMM_B_DOWN_LAST AFMM_GEMV_BATCH
+#endif
+#ifndef AFMM_B_LEAF
+#define AFMM_B_LEAF AFMM_GEMV_BATCH
+#endif
 
 // Waves per box: the kernels are bound by memory latency (a few dependent
 // round trips per box), so what counts is the number of boxes resident at
@@ -666,20 +726,28 @@ int waves_per_box(int nb) {
 
 namespace {
 
-// M2M of level j (out_k = L2P_k^T src).
-void launch_up(Engine& e, int j, hipStream_t s) {
+// M2M of level j (out_k = L2P_k^T src); the leaf level gathers the charges
+// d_b (caller's order) itself.
+void launch_up(Engine& e, int j, const double* d_b, hipStream_t s) {
   const LevelInfo& li = e.lv[j];
   const int wpb = waves_per_box(li.nb);
   const size_t shm = sizeof(double) * std::max(1, li.max_nin);
   const bool leaf = j == e.L;
   const int64_t* soff = e.soff_g.as<int64_t>();
   AFMM_DISPATCH_WPB(wpb, WPB,
-                    k_up<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
-                        e.l2p_rm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
-                        e.nin_g.as<int>(), soff, leaf ? e.qs.as<double>() : e.outv.as<double>(),
-                        leaf ? nullptr : soff + e.lv[j + 1].gbase,
-                        leaf ? e.d_leaf_start.as<int>() : nullptr, e.outv.as<double>(), li.gbase,
-                        li.nb));
+                    if (leaf) {
+                      k_up<WPB, AFMM_B_UP_LEAF><<<li.nb, 64 * WPB, shm, s>>>(
+                          e.l2p_rm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
+                          e.nin_g.as<int>(), soff, nullptr, nullptr, d_b, e.perm.as<int>(),
+                          e.d_leaf_start.as<int>(), e.qs.as<double>(), e.outv.as<double>(),
+                          li.gbase, li.nb);
+                    } else {
+                      k_up<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
+                          e.l2p_rm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
+                          e.nin_g.as<int>(), soff, e.outv.as<double>(),
+                          soff + e.lv[j + 1].gbase, nullptr, nullptr, nullptr, nullptr,
+                          e.outv.as<double>(), li.gbase, li.nb);
+                    });
   AFMM_LAUNCH_CHECK();
 }
 
@@ -720,17 +788,29 @@ void launch_down(Engine& e, int j, hipStream_t s) {
   const size_t shm = sizeof(double) * std::max(1, li.max_rank);
   const int64_t* soff = e.soff_g.as<int64_t>();
   AFMM_DISPATCH_WPB(wpb, WPB,
-                    k_down<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
-                        e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
-                        e.nin_g.as<int>(), soff, soff + e.lv[j + 1].gbase,
-                        e.m2l_part.as<double>(), e.inv.as<double>(), j > 2 ? 1 : 0, li.gbase,
-                        li.nb));
+                    if (j == e.L - 1) {
+                      k_down<WPB, AFMM_B_DOWN_LAST><<<li.nb, 64 * WPB, shm, s>>>(
+                          e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
+                          e.nin_g.as<int>(), soff, soff + e.lv[j + 1].gbase,
+                          e.m2l_part.as<double>(), e.inv.as<double>(), j > 2 ? 1 : 0,
+                          li.gbase, li.nb);
+                    } else {
+                      k_down<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
+                          e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
+                          e.nin_g.as<int>(), soff, soff + e.lv[j + 1].gbase,
+                          e.m2l_part.as<double>(), e.inv.as<double>(), j > 2 ? 1 : 0,
+                          li.gbase, li.nb);
+                    });
   AFMM_LAUNCH_CHECK();
 }
 
 // Near field: persistent grid, a few waves per SIMD, leaving wave slots to
 // the memory-bound kernels that run beside it.
 void launch_near(Engine& e, hipStream_t s) {
+  const int nq = kNearSlices * kNearCounterStride;
+  e.near_queue.alloc(sizeof(int) * nq);
+  k_zero_ints<<<1, nq, 0, s>>>(e.near_queue.as<int>(), nq);
+  AFMM_LAUNCH_CHECK();
   const int maxpad = (e.max_leaf + 63) & ~63;
   const size_t shm = sizeof(double) * (256 + 4 * (size_t)maxpad);
   // p2p_waves_per_simd = 0: one workgroup per pair
@@ -743,7 +823,7 @@ void launch_near(Engine& e, hipStream_t s) {
                            e.d_leaf_start.as<int>(), e.d_leaf_count.as<int>(),
                            e.near_pair.as<int>(), e.near_pair_part.as<int>(),
                            e.near_part.as<double>(), (int)e.near_upairs, maxpad, e.kp,
-                           e.log_tab.as<double2>(), e.ln2_hi, e.ln2_lo));
+                           e.log_tab.as<double2>(), e.ln2_hi, e.ln2_lo, e.near_queue.as<int>()));
   AFMM_LAUNCH_CHECK();
 }
 
@@ -754,7 +834,7 @@ void launch_leaf(Engine& e, double* d_y, hipStream_t s) {
   const int wpb = waves_per_box(li.nb);
   const size_t shm = sizeof(double) * std::max(1, li.max_rank);
   AFMM_DISPATCH_WPB(wpb, WPB,
-                    k_leaf<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
+                    k_leaf<WPB, AFMM_B_LEAF><<<li.nb, 64 * WPB, shm, s>>>(
                         e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
                         e.soff_g.as<int64_t>(), e.d_leaf_start.as<int>(),
                         e.d_leaf_count.as<int>(), e.m2l_part.as<double>(), e.inv.as<double>(),
@@ -766,7 +846,7 @@ void launch_leaf(Engine& e, double* d_y, hipStream_t s) {
 void upper_up(Engine& e, hipStream_t s) {
   hipEvent_t a, b;
   e.prof_begin("m2m_upper", s, &a, &b);
-  for (int j = e.L - 1; j >= 2; j--) launch_up(e, j, s);
+  for (int j = e.L - 1; j >= 2; j--) launch_up(e, j, nullptr, s);
   e.prof_end(s, b);
 }
 
@@ -794,25 +874,25 @@ void gather_sorted(Engine& e, const double* d_b, double* d_qs, hipStream_t s) {
 
 // Schedules (Engine::tree_mode; the near field forks onto its own stream
 // unless near_overlap is off, in which case everything runs on s in order):
-//   0  s: gather, M2M (all levels), M2L (upper levels, then leaf level),
+//   0  s: M2M (all levels; the leaf level gathers the charges into KD
+//      order), M2L (upper levels, then leaf level),
 //      L2L, leaf pass; the near field forks after the M2M.
 //   1  after the leaf-level M2M, the upper tree (M2M, M2L and L2L of levels
 //      < L) runs on a third stream beside the leaf-level M2L.
 //   2  as 0, but after the upper levels' M2L the L2L chain runs on a third
 //      stream beside the leaf-level M2L.
+//   3  as 0, but the near field forks after the upper levels' M2L.
 void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
   const int L = e.L;
   e.qs.alloc(sizeof(double) * e.n);
   hipEvent_t a, b;
   const bool near_side = e.near_overlap;
   const int mode = e.near_overlap ? e.tree_mode : 0;
-
-  e.prof_begin("gather", s, &a, &b);
-  gather_sorted(e, d_b, e.qs.as<double>(), s);
-  e.prof_end(s, b);
+  if ((mode == 1 || mode == 2) && !e.s_tree)
+    AFMM_HIP(hipStreamCreateWithFlags(&e.s_tree, hipStreamNonBlocking));
 
   e.prof_begin("m2m", s, &a, &b);
-  launch_up(e, L, s);
+  launch_up(e, L, d_b, s);
   e.prof_end(s, b);
   if (mode == 1) {
     AFMM_HIP(hipEventRecord(e.ev_fork, s));
@@ -825,16 +905,17 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
     upper_up(e, s);
   }
 
-  if (near_side) {
+  auto fork_near = [&]() {
     AFMM_HIP(hipEventRecord(e.ev_fork, s));
     AFMM_HIP(hipStreamWaitEvent(e.s_aux, e.ev_fork, 0));
     e.prof_begin("p2p", e.s_aux, &a, &b);
     launch_near(e, e.s_aux);
     e.prof_end(e.s_aux, b);
     AFMM_HIP(hipEventRecord(e.ev_join, e.s_aux));
-  }
-
+  };
+  if (near_side && mode != 3) fork_near();
   if (mode != 1) upper_m2l(e, s);
+  if (near_side && mode == 3) fork_near();
   if (mode == 2) {
     AFMM_HIP(hipEventRecord(e.ev_fork, s));
     AFMM_HIP(hipStreamWaitEvent(e.s_tree, e.ev_fork, 0));
@@ -846,14 +927,15 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
   launch_m2l(e, 1, s);
   e.prof_end(s, b);
 
-  if (mode == 0) down_chain(e, s);
+  const bool tree_side = mode == 1 || mode == 2;
+  if (!tree_side) down_chain(e, s);
   if (!near_side) {
     e.prof_begin("p2p", s, &a, &b);
     launch_near(e, s);
     e.prof_end(s, b);
   }
   if (near_side) AFMM_HIP(hipStreamWaitEvent(s, e.ev_join, 0));
-  if (mode != 0) AFMM_HIP(hipStreamWaitEvent(s, e.ev_tree, 0));
+  if (tree_side) AFMM_HIP(hipStreamWaitEvent(s, e.ev_tree, 0));
   e.prof_begin("l2p_out", s, &a, &b);
   launch_leaf(e, d_y, s);
   e.prof_end(s, b);

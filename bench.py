@@ -131,7 +131,7 @@ def run_gpu(A, q, steps, warmup, torch):
     torch.cuda.synchronize()
     A.set_profiling(False)
     kern = {}
-    for name in ("gather", "m2m", "m2m_upper", "m2l", "m2l_upper", "l2l", "p2p", "l2p_out"):
+    for name in ("m2m", "m2m_upper", "m2l", "m2l_upper", "l2l", "p2p", "l2p_out"):
         tot, cnt = A.kernel_time(name)
         kern[name] = tot / cnt if cnt else 0.0
     kern["m2l_timed_region"] = m2l_ms

@@ -162,8 +162,8 @@ int afmm2d_kernel_value(int kernel_id, const double* kparams, double a, double x
 int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st);
 
 /* Per-phase device timing with HIP events recorded on the stream each phase
- * runs on.  level 0: off; 1: every phase ("gather", "m2m", "m2m_upper",
- * "m2l", "m2l_upper", "l2l", "p2p", "l2p_out"); 2: only "m2l" (the leaf
+ * runs on.  level 0: off; 1: every phase ("m2m", "m2m_upper", "m2l",
+ * "m2l_upper", "l2l", "p2p", "l2p_out"); 2: only "m2l" (the leaf
  * level's M2L launch), one event pair per matvec, so a timed loop is barely
  * perturbed. */
 int afmm2d_set_profiling(afmm2d_handle h, int level);

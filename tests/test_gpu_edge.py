@@ -140,7 +140,8 @@ def test_cpp_driver_runs():
 
 
 @pytest.mark.parametrize("near,tree,waves", [
-    ("1", "1", "6"), ("1", "2", "6"), ("0", "0", "6"), ("1", "0", "0"), ("1", "1", "2"),
+    ("1", "1", "6"), ("1", "2", "6"), ("1", "3", "6"), ("0", "0", "6"), ("1", "0", "0"),
+    ("1", "1", "2"), ("0", "3", "4"),
 ])
 def test_schedules_are_bit_identical(engine, monkeypatch, near, tree, waves):
     """Every stream schedule (upper tree and near field beside the leaf-level
@@ -168,7 +169,7 @@ def test_profiling_levels(engine):
     xy, q = uniform_points(n, 3)
     A = engine.AFMM(n, 64, 10, xy)
     A.assemble()
-    names = ("gather", "m2m", "m2m_upper", "m2l", "m2l_upper", "l2l", "p2p", "l2p_out")
+    names = ("m2m", "m2m_upper", "m2l", "m2l_upper", "l2l", "p2p", "l2p_out")
     A.reset_kernel_times()
     A.set_profiling(True)
     A.computeMatVecProduct(q)
