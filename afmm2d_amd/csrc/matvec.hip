@@ -194,6 +194,108 @@ __global__ __launch_bounds__(64 * WPB) void k_up(const double* __restrict__ l2p_
   }
 }
 
+// Sum over the 32 lanes of each half-wave of 2^S values per lane, as a
+// butterfly: the stage with lane mask m (16, 8, ...) pairs lane li with
+// li ^ m; each keeps the half of its values selected by its bit m and adds
+// the partner's copy of that half.  After S halving stages (masks 16 ..
+// 2^(5-S)) the remaining masks are plain pairwise sums, and lane li holds the
+// 32-lane sum of value li >> (5 - S).
+template <int S>
+__device__ __forceinline__ double half_wave_transpose_sum(double (&p)[1 << S]) {
+  const int li = threadIdx.x & 31;
+#pragma unroll
+  for (int st = 0; st < 5; st++) {
+    const int m = 16 >> st;
+    if (st < S) {
+      constexpr int NV = 1 << S;
+      const int h = NV >> (st + 1);
+      const bool upper = (li & m) != 0;
+#pragma unroll
+      for (int j = 0; j < NV / 2; j++) {
+        if (j < h) {
+          const double keep = upper ? p[j + h] : p[j];
+          const double give = upper ? p[j] : p[j + h];
+          p[j] = keep + __shfl_xor(give, m, 32);
+        }
+      }
+    } else {
+      p[0] += __shfl_xor(p[0], m, 32);
+    }
+  }
+  return p[0];
+}
+
+// M2M for one level on the column-major L2P (transposed GEMV): out[c] =
+// sum_i P[c n + i] x[i].  Half-wave `half` of wave w owns column 2 j + half
+// of the column pairs j = w + WPB u (u < 2^S); its 32 lanes own the rows
+// 64 kc + li and 64 kc + 32 + li of each 64-row chunk kc < NCH, so every
+// load instruction reads 256 contiguous bytes per half-wave with all lanes
+// active (the row-major kernel leaves lanes idle when the rank does not
+// divide 64).  All loads are issued before the first use; each lane's
+// per-column partial sums are then combined over the 32 lanes.  Upper levels
+// only: x is the children's concatenated multipoles.
+template <int S, int NCH, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_up_t(const double* __restrict__ l2p_cm,
+                                                   const int64_t* __restrict__ l2poff,
+                                                   const int* __restrict__ rank,
+                                                   const int* __restrict__ nin,
+                                                   const int64_t* __restrict__ soff,
+                                                   const double* __restrict__ src,
+                                                   const int64_t* __restrict__ src_off_child,
+                                                   double* __restrict__ outv, int64_t gbase,
+                                                   int nb) {
+  constexpr int NV = 1 << S;
+  if (AFMM_MEM_PRIO) __builtin_amdgcn_s_setprio(AFMM_MEM_PRIO);
+  const int k = blockIdx.x;
+  if (k >= nb) return;
+  const int64_t g = gbase + k;
+  const int r = rank[g];
+  const int n = nin[g];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, half = l >> 5, li = l & 31;
+  const double* P = l2p_cm + l2poff[g];
+  // matrix loads first (they need only the box's scalars)
+  double a[NV][NCH][2];
+  if (r > 0) {
+#pragma unroll
+    for (int u = 0; u < NV; u++) {
+      const int c = 2 * (w + WPB * u) + half;
+      const int64_t cb = (int64_t)(c < r ? c : r - 1) * n;
+#pragma unroll
+      for (int kc = 0; kc < NCH; kc++)
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          const int i = 64 * kc + 32 * e + li;
+          a[u][kc][e] = P[cb + (i < n ? i : n - 1)];
+        }
+    }
+  }
+  if (r == 0) return;
+  double xv[NCH][2];
+  const double* x = src + src_off_child[4 * k];
+#pragma unroll
+  for (int kc = 0; kc < NCH; kc++)
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const int i = 64 * kc + 32 * e + li;
+      xv[kc][e] = i < n ? x[i] : 0.0;
+    }
+  double p[NV];
+#pragma unroll
+  for (int u = 0; u < NV; u++) {
+    double s = 0.0;
+#pragma unroll
+    for (int kc = 0; kc < NCH; kc++)
+#pragma unroll
+      for (int e = 0; e < 2; e++) s = fma(a[u][kc][e], xv[kc][e], s);
+    p[u] = 2 * (w + WPB * u) + half < r ? s : 0.0;
+  }
+  const double sum = half_wave_transpose_sum<S>(p);
+  if ((li & ((32 >> S) - 1)) == 0) {
+    const int c = 2 * (w + WPB * (li >> (5 - S))) + half;
+    if (c < r) outv[soff[g] + c] = sum;
+  }
+}
+
 // Wave-level sums through DPP (data-parallel lane moves) on the two 32-bit
 // halves of a double.  Lane layout of the M2L tiles: lane = 8 ci + rp.
 template <int CTRL>
@@ -669,6 +771,21 @@ namespace {
 constexpr int kGemvBatch = AFMM_GEMV_BATCH;  // row steps per batch of loads in the GEMV
 // batch sizes of the two largest levels' kernels (leaf-level M2M, level
 // L-1 L2L, leaf pass), where one wave serves one box
+// M2M by the transposed kernel where the level's shape allows (0: always
+// the row-major kernel)
+#ifndef AFMM_UP_T
+#define AFMM_UP_T 1
+#endif
+#ifndef AFMM_UP_T_MIN_WPB
+#define AFMM_UP_T_MIN_WPB 2
+#endif
+// waves per box at level L-1 (0: the waves_per_box rule)
+#ifndef AFMM_WPB_UP_LM1
+#define AFMM_WPB_UP_LM1 0
+#endif
+#ifndef AFMM_WPB_DOWN_LM1
+#define AFMM_WPB_DOWN_LM1 0
+#endif
 #ifndef AFMM_B_UP_LEAF
 #define AFMM_B_UP_LEAF AFMM_GEMV_BATCH
 #endif
@@ -728,9 +845,75 @@ namespace {
 
 // M2M of level j (out_k = L2P_k^T src); the leaf level gathers the charges
 // d_b (caller's order) itself.
+// Shape of the transposed M2M kernel for a level (upper levels only): NCH
+// row chunks of 64,
+// WPB waves per box, 2^S column pairs per wave; at most 32 matrix loads per
+// lane in flight (2^S NCH <= 16).  False if the level needs the row-major
+// kernel (ranks above 64 or more than 256 rows).
+bool up_t_shape(const LevelInfo& li, int& S, int& NCH, int& WPB) {
+  if (!AFMM_UP_T || li.max_rank < 1 || li.max_rank > 64 || li.max_nin > 256) return false;
+  NCH = ceil_div(std::max(1, li.max_nin), 64);
+  const int max_s = NCH == 1 ? 4 : NCH == 2 ? 3 : 2;
+  const int V = (li.max_rank + 1) / 2;
+  for (int w : {2, 4, 8}) {
+    if (w < AFMM_UP_T_MIN_WPB) continue;
+    const int vw = ceil_div(V, w);
+    int sh = 1;
+    while ((1 << sh) < vw) sh++;
+    if (sh <= max_s) {
+      S = sh;
+      WPB = w;
+      return true;
+    }
+  }
+  return false;
+}
+
+template <int S, int NCH>
+void launch_up_t_wpb(Engine& e, int j, int WPB, hipStream_t s) {
+  const LevelInfo& li = e.lv[j];
+  const int64_t* soff = e.soff_g.as<int64_t>();
+#define AFMM_UP_T_ARGS                                                                          \
+  e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(), e.nin_g.as<int>(), soff, \
+      e.outv.as<double>(), soff + e.lv[j + 1].gbase, e.outv.as<double>(), li.gbase, li.nb
+  switch (WPB) {
+    case 2: k_up_t<S, NCH, 2><<<li.nb, 128, 0, s>>>(AFMM_UP_T_ARGS); break;
+    case 4: k_up_t<S, NCH, 4><<<li.nb, 256, 0, s>>>(AFMM_UP_T_ARGS); break;
+    default: k_up_t<S, NCH, 8><<<li.nb, 512, 0, s>>>(AFMM_UP_T_ARGS); break;
+  }
+#undef AFMM_UP_T_ARGS
+  AFMM_LAUNCH_CHECK();
+}
+
+void launch_up_t(Engine& e, int j, int S, int NCH, int WPB, hipStream_t s) {
+  switch (NCH * 8 + S) {
+    case 8 + 1: launch_up_t_wpb<1, 1>(e, j, WPB, s); break;
+    case 8 + 2: launch_up_t_wpb<2, 1>(e, j, WPB, s); break;
+    case 8 + 3: launch_up_t_wpb<3, 1>(e, j, WPB, s); break;
+    case 8 + 4: launch_up_t_wpb<4, 1>(e, j, WPB, s); break;
+    case 16 + 1: launch_up_t_wpb<1, 2>(e, j, WPB, s); break;
+    case 16 + 2: launch_up_t_wpb<2, 2>(e, j, WPB, s); break;
+    case 16 + 3: launch_up_t_wpb<3, 2>(e, j, WPB, s); break;
+    case 24 + 1: launch_up_t_wpb<1, 3>(e, j, WPB, s); break;
+    case 24 + 2: launch_up_t_wpb<2, 3>(e, j, WPB, s); break;
+    case 32 + 1: launch_up_t_wpb<1, 4>(e, j, WPB, s); break;
+    default: launch_up_t_wpb<2, 4>(e, j, WPB, s); break;
+  }
+}
+
 void launch_up(Engine& e, int j, const double* d_b, hipStream_t s) {
   const LevelInfo& li = e.lv[j];
-  const int wpb = waves_per_box(li.nb);
+  // upper levels: the transposed kernel (measured faster there); the leaf
+  // level gathers its charges from the caller's order (random reads) and
+  // runs as fast on the row-major kernel
+  if (j < e.L) {
+    int S = 0, NCH = 0, WPB = 0;
+    if (up_t_shape(li, S, NCH, WPB)) {
+      launch_up_t(e, j, S, NCH, WPB, s);
+      return;
+    }
+  }
+  const int wpb = (j == e.L - 1 && AFMM_WPB_UP_LM1 > 0) ? AFMM_WPB_UP_LM1 : waves_per_box(li.nb);
   const size_t shm = sizeof(double) * std::max(1, li.max_nin);
   const bool leaf = j == e.L;
   const int64_t* soff = e.soff_g.as<int64_t>();
@@ -784,7 +967,8 @@ void launch_m2l(Engine& e, int g, hipStream_t s) {
 // L2L of level j (2 <= j < L), with the level's M2L sums fused in.
 void launch_down(Engine& e, int j, hipStream_t s) {
   const LevelInfo& li = e.lv[j];
-  const int wpb = waves_per_box(li.nb);
+  const int wpb =
+      (j == e.L - 1 && AFMM_WPB_DOWN_LM1 > 0) ? AFMM_WPB_DOWN_LM1 : waves_per_box(li.nb);
   const size_t shm = sizeof(double) * std::max(1, li.max_rank);
   const int64_t* soff = e.soff_g.as<int64_t>();
   AFMM_DISPATCH_WPB(wpb, WPB,
