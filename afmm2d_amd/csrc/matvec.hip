@@ -58,44 +58,45 @@ __device__ __forceinline__ GemvLane gemv_lane(int ncols, int cb, int nr) {
   return q;
 }
 
-// Issues the loads of one batch (steps t0, t0 + WPB, ...): addresses are
+// Issues the loads of one batch (steps t0, t0 + stride, ...): addresses are
 // clamped into the matrix, so no load is conditional.
-template <int WPB, int B>
+template <int B>
 __device__ __forceinline__ void gemv_load(double (&pv)[B], const double* __restrict__ M, int nr,
-                                          int nc, int cb, const GemvLane& q, int t0) {
+                                          int nc, int cb, const GemvLane& q, int t0, int stride) {
   if (nr == 0) return;
 #pragma unroll
   for (int b = 0; b < B; b++) {
-    const int i = q.h + q.G * (t0 + b * WPB);
+    const int i = q.h + q.G * (t0 + b * stride);
     pv[b] = M[(int64_t)(i < nr ? i : nr - 1) * nc + cb + q.c];
   }
 }
 
-template <int WPB, int B>
+template <int B>
 __device__ __forceinline__ double gemv_fma(const double (&pv)[B], double acc, int nr,
-                                           const GemvLane& q, int t0, const double* xs) {
+                                           const GemvLane& q, int t0, int stride,
+                                           const double* xs) {
 #pragma unroll
   for (int b = 0; b < B; b++) {
-    const int i = q.h + q.G * (t0 + b * WPB);
+    const int i = q.h + q.G * (t0 + b * stride);
     const double xv = xs[i < nr ? i : 0];
     acc = i < nr ? fma(pv[b], xv, acc) : acc;
   }
   return acc;
 }
 
-// The whole column chunk for this lane, given the first batch already
-// loaded into pv (issued before x was staged, so its latency overlaps the
-// staging).
-template <int WPB, int B>
+// The whole column chunk for this lane over the steps t_first, t_first +
+// stride, ..., given the first batch already loaded into pv (issued before x
+// was staged, so its latency overlaps the staging).
+template <int B>
 __device__ __forceinline__ double gemv_chunk(double (&pv)[B], const double* __restrict__ M, int nr,
-                                             int nc, int cb, const GemvLane& q, const double* xs) {
-  const int w = threadIdx.x >> 6;
+                                             int nc, int cb, const GemvLane& q, const double* xs,
+                                             int t_first, int stride) {
   const int T = (nr + q.G - 1) / q.G;
   double acc = 0.0;
-  if (w < T) acc = gemv_fma<WPB, B>(pv, acc, nr, q, w, xs);
-  for (int t0 = w + WPB * B; t0 < T; t0 += WPB * B) {
-    gemv_load<WPB, B>(pv, M, nr, nc, cb, q, t0);
-    acc = gemv_fma<WPB, B>(pv, acc, nr, q, t0, xs);
+  if (t_first < T) acc = gemv_fma<B>(pv, acc, nr, q, t_first, stride, xs);
+  for (int t0 = t_first + stride * B; t0 < T; t0 += stride * B) {
+    gemv_load<B>(pv, M, nr, nc, cb, q, t0, stride);
+    acc = gemv_fma<B>(pv, acc, nr, q, t0, stride, xs);
   }
   return acc;
 }
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(64 * WPB) void k_up(const double* __restrict__ l2p_
   GemvLane q;
   if (r > 0) {
     q = gemv_lane(r, 0, n);
-    gemv_load<WPB, B>(pv, P, n, r, 0, q, w);
+    gemv_load<B>(pv, P, n, r, 0, q, w, WPB);
   }
   if (b) {
     const int st = leaf_start[k];
@@ -186,9 +187,9 @@ __global__ __launch_bounds__(64 * WPB) void k_up(const double* __restrict__ l2p_
   for (int cb = 0; cb < r; cb += 64) {
     if (cb > 0) {
       q = gemv_lane(r, cb, n);
-      gemv_load<WPB, B>(pv, P, n, r, cb, q, w);
+      gemv_load<B>(pv, P, n, r, cb, q, w, WPB);
     }
-    const double acc = gemv_chunk<WPB, B>(pv, P, n, r, cb, q, xs);
+    const double acc = gemv_chunk<B>(pv, P, n, r, cb, q, xs, w, WPB);
     const double s = reduce_groups<WPB>(acc, red, q);
     if ((int)threadIdx.x < q.rc) out[cb + threadIdx.x] = s;
   }
@@ -482,16 +483,38 @@ __global__ __launch_bounds__(64 * WPB) void k_down(const double* __restrict__ l2
   double* dst = inv + child_soff[4 * k];
   const int w = threadIdx.x >> 6;
   double pv[B];
+  const int nch = (n + 63) >> 6;
+  if (nch > 1 && nch <= WPB) {
+    // the column chunks in parallel: wave w takes chunk w % nch and every
+    // nsub-th step of it (the waves w' = w mod nch); one reduction at the end
+    const int ch = w % nch, sub = w / nch, nsub = (WPB - 1 - ch) / nch + 1;
+    const GemvLane q = gemv_lane(n, 64 * ch, r);
+    gemv_load<B>(pv, P, r, n, 64 * ch, q, sub, nsub);
+    local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part + kILSlots * soff[g]);
+    __syncthreads();
+    red[threadIdx.x] = gemv_chunk<B>(pv, P, r, n, 64 * ch, q, xs, sub, nsub);
+    __syncthreads();
+    const int i = threadIdx.x;
+    if (i < n) {
+      const int ci = i >> 6, c = i & 63;
+      const int rc = min(64, n - 64 * ci), G = 64 / rc, ns = (WPB - 1 - ci) / nch + 1;
+      double acc = 0.0;
+      for (int u = 0; u < ns; u++)
+        for (int h = 0; h < G; h++) acc += red[(ci + nch * u) * 64 + h * rc + c];
+      dst[i] = acc;
+    }
+    return;
+  }
   GemvLane q = gemv_lane(n, 0, r);
-  gemv_load<WPB, B>(pv, P, r, n, 0, q, w);
+  gemv_load<B>(pv, P, r, n, 0, q, w, WPB);
   local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part + kILSlots * soff[g]);
   __syncthreads();
   for (int cb = 0; cb < n; cb += 64) {
     if (cb > 0) {
       q = gemv_lane(n, cb, r);
-      gemv_load<WPB, B>(pv, P, r, n, cb, q, w);
+      gemv_load<B>(pv, P, r, n, cb, q, w, WPB);
     }
-    const double acc = gemv_chunk<WPB, B>(pv, P, r, n, cb, q, xs);
+    const double acc = gemv_chunk<B>(pv, P, r, n, cb, q, xs, w, WPB);
     const double s = reduce_groups<WPB>(acc, red, q);
     if ((int)threadIdx.x < q.rc) dst[cb + threadIdx.x] = s;
   }
@@ -691,7 +714,7 @@ __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2
   const int w = threadIdx.x >> 6;
   double pv[B];
   GemvLane q = gemv_lane(n, 0, r);
-  gemv_load<WPB, B>(pv, P, r, n, 0, q, w);
+  gemv_load<B>(pv, P, r, n, 0, q, w, WPB);
   // the first chunk's near-field values and destinations, loaded early
   double nv[9];
   int dst_i = 0;
@@ -706,13 +729,13 @@ __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2
   for (int cb = 0; cb < n; cb += 64) {
     if (cb > 0) {
       q = gemv_lane(n, cb, r);
-      gemv_load<WPB, B>(pv, P, r, n, cb, q, w);
+      gemv_load<B>(pv, P, r, n, cb, q, w, WPB);
       const int i = min(cb + (int)threadIdx.x, n - 1);
 #pragma unroll
       for (int t = 0; t < 9; t++) nv[t] = nblk[t * n + i];
       dst_i = perm[st + i];
     }
-    const double acc = gemv_chunk<WPB, B>(pv, P, r, n, cb, q, xs);
+    const double acc = gemv_chunk<B>(pv, P, r, n, cb, q, xs, w, WPB);
     const double s = reduce_groups<WPB>(acc, red, q);
     if ((int)threadIdx.x < q.rc) {
       double nf = 0.0;
