@@ -566,7 +566,7 @@ constexpr int kNearChunk = 4;           // pairs per counter increment
 // accumulated in LDS with one FP64 LDS add per step.  Each kernel value is
 // evaluated once and used for both directions.  A self pair (A, A) runs
 // steps 0..32 on its diagonal chunk pairs (step 32 on lanes 0..31 only).
-template <int KID>
+template <int KID, int MAXPAD>
 __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
                                             const double* __restrict__ y,
                                             const double* __restrict__ qs,
@@ -574,16 +574,15 @@ __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
                                             const int* __restrict__ leaf_count,
                                             const int* __restrict__ pairs,
                                             const int* __restrict__ pair_part,
-                                            double* __restrict__ part, int npairs, int maxpad,
-                                            KParams kp, const double2* __restrict__ logtab,
+                                            double* __restrict__ part, int npairs, KParams kp,
+                                            const double2* __restrict__ logtab,
                                             double ln2_hi, double ln2_lo,
                                             int* __restrict__ queue) {
-  extern __shared__ double sm[];
-  double2* tab = reinterpret_cast<double2*>(sm);  // 128 entries
-  double* bx = sm + 256;
-  double* by = bx + maxpad;
-  double* bq = by + maxpad;
-  double* bc = bq + maxpad;  // source-side sums
+  // LDS: the log table, then the source points' x, y, charge and
+  // source-side sums as four arrays at compile-time offsets (one address per
+  // step, 8-byte lane stride: no bank conflicts)
+  __shared__ double2 tab[128];
+  __shared__ double bx[MAXPAD], by[MAXPAD], bq[MAXPAD], bc[MAXPAD];
   const int t = threadIdx.x;
   if constexpr (KID == K_LOG) {
     for (int u = t; u < 128; u += 64) tab[u] = logtab[u];
@@ -1018,19 +1017,24 @@ void launch_near(Engine& e, hipStream_t s) {
   e.near_queue.alloc(sizeof(int) * nq);
   k_zero_ints<<<1, nq, 0, s>>>(e.near_queue.as<int>(), nq);
   AFMM_LAUNCH_CHECK();
-  const int maxpad = (e.max_leaf + 63) & ~63;
-  const size_t shm = sizeof(double) * (256 + 4 * (size_t)maxpad);
   // p2p_waves_per_simd = 0: one workgroup per pair
   const int64_t cap = e.p2p_waves_per_simd > 0 ? (int64_t)4 * e.num_cus * e.p2p_waves_per_simd
                                                : e.near_upairs;
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(e.near_upairs, cap));
+  // LDS arrays sized for the largest leaf, padded to a multiple of 64
+  const int maxpad = (e.max_leaf + 63) & ~63;
+#define AFMM_NEAR_LAUNCH(KID, MAXPAD)                                                       \
+  k_p2p<KID, MAXPAD><<<grid, 64, 0, s>>>(                                                  \
+      e.x.as<double>(), e.y.as<double>(), e.qs.as<double>(), e.d_leaf_start.as<int>(),     \
+      e.d_leaf_count.as<int>(), e.near_pair.as<int>(), e.near_pair_part.as<int>(),         \
+      e.near_part.as<double>(), (int)e.near_upairs, e.kp, e.log_tab.as<double2>(), e.ln2_hi, \
+      e.ln2_lo, e.near_queue.as<int>())
   AFMM_DISPATCH_KERNEL(e.kid, KID,
-                       k_p2p<KID><<<grid, 64, shm, s>>>(
-                           e.x.as<double>(), e.y.as<double>(), e.qs.as<double>(),
-                           e.d_leaf_start.as<int>(), e.d_leaf_count.as<int>(),
-                           e.near_pair.as<int>(), e.near_pair_part.as<int>(),
-                           e.near_part.as<double>(), (int)e.near_upairs, maxpad, e.kp,
-                           e.log_tab.as<double2>(), e.ln2_hi, e.ln2_lo, e.near_queue.as<int>()));
+                       if (maxpad <= 64) AFMM_NEAR_LAUNCH(KID, 64);
+                       else if (maxpad <= 128) AFMM_NEAR_LAUNCH(KID, 128);
+                       else if (maxpad <= 256) AFMM_NEAR_LAUNCH(KID, 256);
+                       else AFMM_NEAR_LAUNCH(KID, 512));
+#undef AFMM_NEAR_LAUNCH
   AFMM_LAUNCH_CHECK();
 }
 
