@@ -61,7 +61,7 @@ class Stats(ctypes.Structure):
         ("l2p_bytes", ctypes.c_int64), ("matvec_bytes", ctypes.c_int64),
         ("create_seconds", ctypes.c_double), ("assemble_seconds", ctypes.c_double),
         ("m2l_leaf_bytes", ctypes.c_int64), ("m2l_leaf_columns", ctypes.c_int64),
-        ("skeleton_leaf", ctypes.c_int64),
+        ("skeleton_leaf", ctypes.c_int64), ("near_waves", ctypes.c_int64),
     ]
 
     def as_dict(self):

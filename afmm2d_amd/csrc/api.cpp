@@ -168,6 +168,31 @@ void Engine::prof_collect() {
   ev_used = 0;
 }
 
+// Near-field waves per SIMD for the overlapped schedule: the near field runs
+// beside the HBM-bound M2L and should end about when the M2L does.  With
+// R = (near-field time alone) / (M2L time alone), estimated from the
+// problem's sizes and the rates measured on MI355X (~9e11 near-field kernel
+// evaluations/s, ~5 TB/s for the M2L blocks and partials), the best counts
+// measured were 3 at R ~ 0.55 (N = 2^20, tolerance 1e-12) and 6 at R ~ 1.2
+// (tolerance 1e-8): round(5 R), clamped to 2..8.
+int near_waves_per_simd(const Engine& e) {
+  double evals = 0.0;
+  const int nleaf = (int)e.leaf_count.size();
+  for (int a = 0; a < nleaf; a++) {
+    const double na = e.leaf_count[a];
+    evals += 0.5 * na * na;  // self pair: each unordered point pair once
+    for (int q = 0; q < 8; q++) {
+      const int b = e.nbr[e.L][8 * (size_t)a + q];
+      if (b > a) evals += na * e.leaf_count[b];
+    }
+  }
+  const double t_near = evals / 9.0e11;
+  const double t_m2l = (8.0 * e.m2l_total + 16.0 * e.m2l_cols) / 5.0e12;
+  if (t_m2l <= 0.0) return 8;
+  const double w = 5.0 * t_near / t_m2l;
+  return std::max(2, std::min(8, (int)std::lround(w)));
+}
+
 }  // namespace afmm
 
 struct afmm2d_handle_s {
@@ -227,8 +252,10 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
     if (const char* v = std::getenv("AFMM2D_NEAR_OVERLAP")) e->near_overlap = std::atoi(v) != 0;
     if (const char* v = std::getenv("AFMM2D_TREE_MODE"))
       e->tree_mode = std::max(0, std::min(3, std::atoi(v)));
-    if (const char* v = std::getenv("AFMM2D_P2P_WAVES"))
+    if (const char* v = std::getenv("AFMM2D_P2P_WAVES")) {
       e->p2p_waves_per_simd = std::max(0, std::min(16, std::atoi(v)));
+      e->p2p_waves_fixed = true;
+    }
     // one internal stream per handle (a process has few hardware queues,
     // and streams that share one run in order): it carries the set-up work
     // and the near field beside the caller's stream
@@ -368,6 +395,7 @@ int afmm2d_assemble(afmm2d_handle h) {
     double t0 = now_s();
     assemble_tree(e);
     e.assembled = true;
+    if (!e.p2p_waves_fixed) e.p2p_waves_per_simd = near_waves_per_simd(e);
     e.assemble_seconds = now_s() - t0;
   });
 }
@@ -662,6 +690,7 @@ int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st) {
   st->m2l_leaf_bytes = e.assembled ? 8 * e.lv[e.L].m2l_coeffs : 0;
   st->m2l_leaf_columns = e.assembled ? e.lv[e.L].m2l_cols : 0;
   st->skeleton_leaf = e.assembled ? e.lv[e.L].soff[e.lv[e.L].nb] - e.lv[e.L].soff[0] : 0;
+  st->near_waves = e.p2p_waves_per_simd;
   return AFMM2D_OK;
 }
 

@@ -213,7 +213,7 @@ def main():
     if not args.no_side_run:
         Ac = E.AFMM(args.n, args.leaf, args.config_tol_pow, xy, kernel="log")
         Ac.assemble()
-        ms_c, _, kern_c, y_c = run_gpu(Ac, q, max(5, args.steps // 5), 2, torch)
+        ms_c, _, kern_c, y_c = run_gpu(Ac, q, max(20, args.steps // 2), args.warmup, torch)
         err_c = float(np.linalg.norm(y_c[rows] - ref_rows) / np.linalg.norm(ref_rows))
         st_c = Ac.stats()
         Ac.close()
@@ -275,6 +275,7 @@ def main():
             "kernel_avg_ms": {k: round(v, 4) for k, v in kern.items()},
             "dominant_kernel": dom,
             "m2l_bytes": st["m2l_bytes"], "l2p_bytes": st["l2p_bytes"],
+            "near_waves_per_simd": st["near_waves"],
             "matvec_bytes": st["matvec_bytes"],
             "create_s": round(t_create, 3), "assemble_s": round(t_asm, 3),
             "at_tol_pow_%d" % args.config_tol_pow: side,

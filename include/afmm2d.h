@@ -66,6 +66,7 @@ typedef struct {
   int64_t m2l_leaf_bytes;    /* stored M2L coefficients of the leaf level's pairs */
   int64_t m2l_leaf_columns;  /* their partial columns (rank_k + rank_b per pair) */
   int64_t skeleton_leaf;     /* sum of the leaf level's ranks */
+  int64_t near_waves;        /* near-field waves per SIMD used beside the M2L */
 } afmm2d_stats;
 
 /* Message of the calling thread's last failure ("" if none). */
