@@ -38,7 +38,8 @@ EXPORTS = [
     "afmm2d_assemble", "afmm2d_matvec", "afmm2d_matvec_device", "afmm2d_levels",
     "afmm2d_get_permutation", "afmm2d_get_sorted_locations", "afmm2d_get_leaf_counts",
     "afmm2d_get_leaf_range", "afmm2d_get_interactions", "afmm2d_get_ranks",
-    "afmm2d_get_skeleton", "afmm2d_get_l2p", "afmm2d_get_m2l", "afmm2d_direct",
+    "afmm2d_get_skeleton", "afmm2d_get_l2p", "afmm2d_get_m2l", "afmm2d_near_field",
+    "afmm2d_direct",
     "afmm2d_direct_rows", "afmm2d_compute_error", "afmm2d_evaluate_error",
     "afmm2d_get_matrix_entry", "afmm2d_kernel_value", "afmm2d_get_stats",
     "afmm2d_set_profiling", "afmm2d_get_kernel_time", "afmm2d_reset_kernel_times",
@@ -107,6 +108,7 @@ def lib():
         "afmm2d_get_skeleton": (i, [_vp, i, i, _ip, i, _ip]),
         "afmm2d_get_l2p": (i, [_vp, i, i, _dp, ctypes.c_int64, _ip, _ip]),
         "afmm2d_get_m2l": (i, [_vp, i, i, i, _dp, ctypes.c_int64, _ip, _ip]),
+        "afmm2d_near_field": (i, [_vp, _dp, _dp]),
         "afmm2d_direct": (i, [_vp, _dp, _dp]),
         "afmm2d_direct_rows": (i, [_vp, i, _lp, _dp, _dp]),
         "afmm2d_compute_error": (i, [_vp, i, _dp]),
@@ -302,6 +304,14 @@ class AFMM:
         return v.value
 
     # checks and measurement -------------------------------------------------
+    def near_field(self, b):
+        """The near-field term alone (leaf self and neighbour blocks) on the
+        device, caller order (AFMM2DTree.cpp:1199-1213)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        y = np.empty(self.N, dtype=np.float64)
+        _check(lib().afmm2d_near_field(self._h, _d(b), _d(y)))
+        return y
+
     def direct(self, b):
         """Dense direct sum y = K b on the device (tutorial.cpp:97-131)."""
         b = np.ascontiguousarray(b, dtype=np.float64)

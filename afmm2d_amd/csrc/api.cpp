@@ -595,6 +595,22 @@ int afmm2d_direct_rows(afmm2d_handle h, int nrows, const int64_t* rows, const do
   });
 }
 
+int afmm2d_near_field(afmm2d_handle h, const double* b, double* y) {
+  if (!h || !b || !y) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  return guarded([&] {
+    Engine& e = *h->e;
+    AFMM_HIP(hipSetDevice(e.device));
+    const size_t nb = sizeof(double) * e.n;
+    e.io_b.alloc(nb);
+    e.io_y.alloc(nb);
+    hipStream_t s = nullptr;
+    AFMM_HIP(hipMemcpyAsync(e.io_b.p, b, nb, hipMemcpyHostToDevice, s));
+    near_field_device(e, e.io_b.as<double>(), e.io_y.as<double>(), s);
+    AFMM_HIP(hipMemcpyAsync(y, e.io_y.p, nb, hipMemcpyDeviceToHost, s));
+    AFMM_HIP(hipStreamSynchronize(s));
+  });
+}
+
 int afmm2d_direct(afmm2d_handle h, const double* b, double* y) {
   if (!h || !b || !y) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
   std::vector<int64_t> rows(h->e->n);

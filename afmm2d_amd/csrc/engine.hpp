@@ -207,6 +207,7 @@ int near_waves_per_simd(const Engine& e);
 
 // matvec.hip
 void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s);
+void near_field_device(Engine& e, const double* d_b, double* d_y, hipStream_t s);
 void direct_rows_device(Engine& e, int nrows, const int* d_rows, const double* d_qs, double* d_out,
                         hipStream_t s);
 void gather_sorted(Engine& e, const double* d_b, double* d_qs, hipStream_t s);

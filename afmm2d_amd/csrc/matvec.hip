@@ -544,14 +544,41 @@ __device__ __forceinline__ double log_tang(double x, const double2* __restrict__
   return fma(e, ln2_hi, t.y) + fma(e, ln2_lo, p);
 }
 
+// 1/sqrt(x) for normal x > 0: the hardware estimate y0 = (1 + d)/sqrt(x)
+// (v_rsq_f64) refined by one second-order Newton step.  With
+// e = 1 - x y0^2 = -2d - d^2, sqrt(x) y0 (1 + e/2 + 3e^2/8) = 1 + O(d^3);
+// what remains is the rounding of x y0 in e and of the last FMA, about one
+// ulp.  x = 0 gives +inf from the estimate and NaN after the step: callers
+// select the diagonal value separately.
+__device__ __forceinline__ double rsqrt_refined(double x) {
+  const double y0 = __builtin_amdgcn_rsq(x);
+  const double e = fma(-(x * y0), y0, 1.0);
+  return fma(y0 * e, fma(e, 0.375, 0.5), y0);
+}
+
+// The kernel from the squared distance for the near field.  Same values as
+// kernel_r2 to a few ulp, without its branches and correctly rounded
+// divisions and square roots: the log by Tang's table, the 1/R kernels
+// through rsqrt_refined (one estimate, no division), the Gaussian with the
+// reciprocal of its width taken once (ninv_p0 = -1/p0).
 template <int KID>
-__device__ __forceinline__ double near_kernel(double r2, const KParams& kp, const double2* tab,
-                                              double ln2_hi, double ln2_lo) {
+__device__ __forceinline__ double near_kernel(double r2, const KParams& kp, double ninv_p0,
+                                              double inv_a, const double2* tab, double ln2_hi,
+                                              double ln2_lo) {
   if constexpr (KID == K_LOG) {
     // 2 x the log kernel; the factor 1/2 is applied to the sums
     return r2 == 0.0 ? 0.0 : log_tang(r2, tab, ln2_hi, ln2_lo);
+  } else if constexpr (KID == K_INV_R) {
+    const double rs = rsqrt_refined(r2);
+    return r2 == 0.0 ? 0.0 : rs;
+  } else if constexpr (KID == K_GAUSS) {
+    return exp(r2 * ninv_p0);
   } else {
-    return kernel_r2<KID>(r2, kp);
+    // R < a ? R/a : a/R with R = r2 / sqrt(r2); both sides are 1 at R = a
+    const double rs = rsqrt_refined(r2);
+    const double r = r2 * rs;
+    const double v = r < kp.a ? r * inv_a : kp.a * rs;
+    return r2 == 0.0 ? 0.0 : v;
   }
 }
 
@@ -589,6 +616,8 @@ __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
   }
   // padding: far away, zero charge; K stays finite there
   const double FAR = 1.0e30;
+  const double ninv_p0 = KID == K_GAUSS ? -1.0 / kp.p0 : 0.0;
+  const double inv_a = KID == K_REG_INV_R ? 1.0 / kp.a : 0.0;
   // work queue: the pairs are cut into kNearSlices slices, each with its own
   // chunk counter (one cache line apart); a workgroup drains the slice of
   // its own index first and then steals from the others, so a workgroup that
@@ -639,7 +668,7 @@ __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
         int s = 0;
         if (diag) {
           // step 0: the point with itself (R2 = 0)
-          row += near_kernel<KID>(0.0, kp, tab, ln2_hi, ln2_lo) * qi;
+          row += near_kernel<KID>(0.0, kp, ninv_p0, inv_a, tab, ln2_hi, ln2_lo) * qi;
           s = 1;
         }
         const int s_end = diag ? 32 : 64;
@@ -647,14 +676,16 @@ __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
         for (; s < s_end; s++) {
           const int j = cb + ((t + s) & 63);
           const double dx = xi - bx[j], dy = yi - by[j];
-          const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, tab, ln2_hi, ln2_lo);
+          const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab, ln2_hi,
+                                                   ln2_lo);
           row = fma(v, bq[j], row);
           atomicAdd(&bc[j], v * qi);
         }
         if (diag && t < 32) {
           const int j = cb + t + 32;
           const double dx = xi - bx[j], dy = yi - by[j];
-          const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, tab, ln2_hi, ln2_lo);
+          const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab, ln2_hi,
+                                                   ln2_lo);
           row = fma(v, bq[j], row);
           atomicAdd(&bc[j], v * qi);
         }
@@ -676,6 +707,25 @@ __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
       for (int j = t; j < nb; j += 64) outb[j] = scale * bc[j];
     }
     p++;
+  }
+}
+
+// Near-field term alone: the 9 partial slots of every leaf point summed and
+// scattered to the caller's order.  One workgroup per leaf box.
+__global__ __launch_bounds__(256) void k_near_out(const double* __restrict__ near_part,
+                                                  const int* __restrict__ leaf_start,
+                                                  const int* __restrict__ leaf_count,
+                                                  const int* __restrict__ perm,
+                                                  double* __restrict__ yout, int nleaves) {
+  const int k = blockIdx.x;
+  if (k >= nleaves) return;
+  const int st = leaf_start[k], n = leaf_count[k];
+  const double* nblk = near_part + 9 * (int64_t)st;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < 9; t++) s += nblk[t * n + i];
+    yout[perm[st + i]] = s;
   }
 }
 
@@ -1150,6 +1200,16 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
   e.prof_begin("l2p_out", s, &a, &b);
   launch_leaf(e, d_y, s);
   e.prof_end(s, b);
+}
+
+void near_field_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
+  e.qs.alloc(sizeof(double) * e.n);
+  gather_sorted(e, d_b, e.qs.as<double>(), s);
+  launch_near(e, s);
+  const int nleaves = 1 << (2 * e.L);
+  k_near_out<<<nleaves, 256, 0, s>>>(e.near_part.as<double>(), e.d_leaf_start.as<int>(),
+                                      e.d_leaf_count.as<int>(), e.perm.as<int>(), d_y, nleaves);
+  AFMM_LAUNCH_CHECK();
 }
 
 void direct_rows_device(Engine& e, int nrows, const int* d_rows, const double* d_qs,

@@ -136,6 +136,12 @@ int afmm2d_get_l2p(afmm2d_handle h, int level, int box, double* out, int64_t cap
 int afmm2d_get_m2l(afmm2d_handle h, int level, int box, int slot, double* out, int64_t capacity,
                    int* rows, int* cols);
 
+/* The near-field term alone, y = sum over each leaf's neighbours and itself
+ * of K(leaf, box) b(box) (FMM2DTree::evaluate_NearField,
+ * AFMM2DTree.cpp:1199-1213), host arrays in caller order.  Needs no
+ * assembly; runs the matvec's own near-field kernel. */
+int afmm2d_near_field(afmm2d_handle h, const double* b, double* y);
+
 /* Dense direct sum y = K b on the device (the check of tutorial.cpp:97-131),
  * host arrays in caller order. */
 int afmm2d_direct(afmm2d_handle h, const double* b, double* y);

@@ -51,6 +51,7 @@ def lib():
         L.or_assemble.restype = ctypes.c_int
         L.or_assemble.argtypes = [ctypes.c_void_p]
         L.or_matvec.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.or_near_field.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.or_set_parallel_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_levels.restype = ctypes.c_int
         L.or_levels.argtypes = [ctypes.c_void_p]
@@ -182,6 +183,13 @@ class Tree:
         q = np.ascontiguousarray(q, dtype=np.float64)
         y = np.empty(self.n, dtype=np.float64)
         lib().or_matvec(self.h, _d(q), _d(y))
+        return y
+
+    def near_field(self, q):
+        """The near-field term alone, caller order (AFMM2DTree.cpp:1199-1213)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        y = np.empty(self.n, dtype=np.float64)
+        lib().or_near_field(self.h, _d(q), _d(y))
         return y
 
     def direct(self, q):

@@ -772,6 +772,35 @@ static void near_box(or_tree* t, long k) {
   free(R);
 }
 
+/* The near-field term alone (evaluate_NearField, AFMM2DTree.cpp:1199-1213):
+ * for every leaf, the sum over its 8 neighbours and itself of K(leaf,
+ * neighbour) q(neighbour), in the order near_box adds them; y in the
+ * caller's order.  Needs no assembly. */
+void or_near_field(const or_tree* t, const double* q, double* y) {
+  int L = t->L;
+  long nl = 1L << (2 * L);
+  double* qs = (double*)malloc(sizeof(double) * (size_t)(t->n > 0 ? t->n : 1));
+  double* ps = (double*)calloc((size_t)(t->n > 0 ? t->n : 1), sizeof(double));
+  for (int i = 0; i < t->n; i++) qs[i] = q[t->perm[i]];
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long k = 0; k < nl; k++) {
+    int n = t->leaf_count[k], st = t->leaf_start[k];
+    for (int s = 0; s < 9; s++) {
+      long kb = s < 8 ? t->nbr[L][8 * k + s] : k;
+      if (kb < 0) continue;
+      int m = t->leaf_count[kb], sb = t->leaf_start[kb];
+      for (int i = 0; i < n; i++) {
+        double acc = 0.0;
+        for (int c = 0; c < m; c++) acc += t->entry(&t->k, st + i, sb + c) * qs[sb + c];
+        ps[st + i] += acc;
+      }
+    }
+  }
+  for (int i = 0; i < t->n; i++) y[t->perm[i]] = ps[i];
+  free(qs);
+  free(ps);
+}
+
 void or_matvec(or_tree* t, const double* q, double* y) {
   int L = t->L;
   for (int i = 0; i < t->n; i++) t->qs[i] = q[t->perm[i]];

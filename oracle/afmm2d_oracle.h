@@ -59,6 +59,9 @@ or_tree* or_create(int n, int leaf, int tol_pow, const double* xy, int kernel_id
 void or_destroy(or_tree* t);
 int or_assemble(or_tree* t);
 void or_matvec(or_tree* t, const double* q, double* y);
+/* The near-field term alone (leaf self + neighbour blocks), caller order;
+ * needs no assembly. */
+void or_near_field(const or_tree* t, const double* q, double* y);
 /* 0: OpenMP placement of the reference (M2L parallel over levels only);
  * 1: parallel over boxes everywhere.  Results are identical. */
 void or_set_parallel_mode(or_tree* t, int mode);

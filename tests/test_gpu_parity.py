@@ -82,3 +82,31 @@ def test_direct_sum_matches_oracle(engine, oracle_mod):
     assert np.linalg.norm(yd - yo) / np.linalg.norm(yo) <= 1e-13
     T.close()
     a.close()
+
+
+@pytest.mark.parametrize("n,leaf,kernel,quant", [
+    (3000, 40, "log", 0),          # 47 points per leaf: LDS arrays of 64
+    (7000, 100, "reg_inv_r", 0),   # 109: 128
+    (20000, 64, "gaussian", 0),    # 78: 128
+    (12000, 64, "log", 0),         # 187: 256
+    (12000, 64, "reg_inv_r", 0),
+    (5000, 100, "inv_r", 0),       # 312: 512
+    (5000, 100, "reg_inv_r", 16),  # duplicated points: R = 0 off the diagonal
+    (6000, 64, "inv_r", 16),
+])
+def test_near_field_matches_oracle(engine, oracle_mod, n, leaf, kernel, quant):
+    """The near-field kernel alone (AFMM2DTree.cpp:1199-1213) against the
+    oracle's: the rsqrt-based 1/R kernels and the table log agree to a few
+    ulp per entry, so the sums agree to 1e-13 relative."""
+    kid = engine.KERNELS[kernel]
+    xy, q = uniform_points(n, 40 + n % 97)
+    if quant:
+        xy = np.round(xy * quant) / quant
+    T = oracle_mod.Tree(xy, leaf, 8, kid)
+    ref = T.near_field(q)
+    a = engine.AFMM(n, leaf, 8, xy, kernel=kernel)
+    got = a.near_field(q)
+    a.close()
+    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    assert err <= 1e-13, err
+    assert float(np.max(np.abs(got - ref))) <= 1e-12 * float(np.max(np.abs(ref)))

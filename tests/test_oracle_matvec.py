@@ -107,3 +107,36 @@ def test_invalid_levels(oracle_mod):
     xy, _ = uniform_points(100, 1)
     with pytest.raises(ValueError):
         oracle_mod.Tree(xy, 64, 8)
+
+
+@pytest.mark.parametrize("kid", [0, 1, 2, 3])
+def test_near_field_brute_force(oracle_mod, kid):
+    """The oracle's near-field term (AFMM2DTree.cpp:1199-1213) against a
+    numpy brute force over each leaf's own points and its 8 neighbours'."""
+    xy, q = uniform_points(2000, 21)
+    T = oracle_mod.Tree(xy, 30, 8, kid)
+    L = T.L
+    assert L == 3
+    sxy, perm = T.sorted_xy(), T.perm()
+    qs = q[perm]
+    nbr, _, _ = oracle_mod.topology(L)
+    a = T.a
+    ys = np.zeros(len(q))
+    for k in range(4 ** L):
+        st, n = T.leaf_range(k)
+        boxes = [k] + [b for b in nbr[k] if b >= 0]
+        for b in boxes:
+            sb, m = T.leaf_range(b)
+            d = sxy[st:st + n, None, :] - sxy[None, sb:sb + m, :]
+            r2 = (d ** 2).sum(-1)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                r = np.sqrt(r2)
+                K = {0: np.where(r2 > 0, 0.5 * np.log(r2), 0.0),
+                     1: np.where(r2 > 0, 1.0 / r, 0.0),
+                     2: np.exp(-r2 / 0.01),
+                     3: np.where(r < a, r / a, a / r)}[kid]
+            ys[st:st + n] += K @ qs[sb:sb + m]
+    y = np.empty_like(ys)
+    y[perm] = ys
+    got = T.near_field(q)
+    assert rel(got, y) <= 1e-13
