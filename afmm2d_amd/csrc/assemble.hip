@@ -664,7 +664,6 @@ void assemble_tree(Engine& e) {
   e.h_red_pair.clear();
   std::unordered_map<int64_t, int> pair_of;
   int64_t off = 0, part = 0;
-  int tile = 2;
   e.il_pairs = 0;
   e.m2l_total = 0;
   auto slot_box = [&](int j, int k, int q) {
@@ -701,7 +700,7 @@ void assemble_tree(Engine& e) {
                                           : cand[u].cls < cand[v].cls;
   });
   for (int g = 0; g < 2; g++)
-    for (int c = 0; c <= kM2LClasses; c++) e.m2l_cls_begin[g][c] = e.m2l_cls_end[g][c] = 0;
+    for (int c = 0; c <= kM2LAllClasses; c++) e.m2l_cls_begin[g][c] = e.m2l_cls_end[g][c] = 0;
   for (size_t i = 0; i < order.size(); i++) {
     const Cand& cd = cand[order[i]];
     const int qidx = (int)i;
@@ -716,7 +715,6 @@ void assemble_tree(Engine& e) {
     e.h_pair_off.push_back(off);
     const int ld = rf + (rf & 1);
     off += (int64_t)ld * rs;  // ld even: every block and column 16-byte aligned
-    tile = std::max(tile, ld * rs);
     e.pair_max_ld = std::max(e.pair_max_ld, ld);
     e.pair_max_rb = std::max(e.pair_max_rb, rs);
   }
@@ -749,9 +747,7 @@ void assemble_tree(Engine& e) {
   }
   e.h_red_start[nbox] = (int)e.h_red_pair.size();
   e.m2l_cols = part;
-  // LDS for one staged block: whole blocks up to 48 KiB, larger ones in
-  // column chunks; the pair kernel holds up to 256 rows per lane set.
-  e.pair_tile = std::min(tile, 6144);
+  // the streamed pair kernel holds up to 256 rows per block
   if (e.max_rank > 256) throw Error(6, "ranks above 256 are not supported by the M2L kernel");
   upload(e.pair_k, e.h_pair_k, s);
   upload(e.pair_b, e.h_pair_b, s);
@@ -765,7 +761,9 @@ void assemble_tree(Engine& e) {
     }
     upload(e.pair_desc, pd, s);
   }
-  e.m2l.alloc(sizeof(double) * std::max<int64_t>(off, 2));
+  // one spare double2 / double past the end: the pair kernels clamp the
+  // addresses of empty blocks (rank 0) to their start, which may be the end
+  e.m2l.alloc(sizeof(double) * (off + 2));
   e.m2l_part.alloc(sizeof(double) * (size_t)std::max<int64_t>(kILSlots * sb, 2));
   AFMM_HIP(hipMemsetAsync(e.m2l_part.p, 0, e.m2l_part.bytes, s));
   if (e.npairs > 0) {
@@ -777,7 +775,7 @@ void assemble_tree(Engine& e) {
     AFMM_LAUNCH_CHECK();
   }
   AFMM_HIP(hipStreamSynchronize(s));
-  e.outv.alloc(sizeof(double) * std::max<int64_t>(sb, 1));
+  e.outv.alloc(sizeof(double) * (sb + 1));
   e.inv.alloc(sizeof(double) * std::max<int64_t>(sb, 1));
   AFMM_HIP(hipMemsetAsync(e.outv.p, 0, e.outv.bytes, s));
   AFMM_HIP(hipMemsetAsync(e.inv.p, 0, e.inv.bytes, s));

@@ -74,16 +74,25 @@ struct LevelInfo {
 // Register-tile classes of the M2L pair kernel, cheapest first (VGPRs per
 // lane on gfx950: 59, 75, 86, 88, 100, 114, 185): class c holds blocks of up
 // to 16 kM2LTileRows[c] rows (leading dimension) and 8 kM2LTileCols[c]
-// columns; class kM2LClasses is the LDS kernel for anything larger.
+// columns, the whole block in registers.
 constexpr int kM2LClasses = 7;
 constexpr int kM2LTileRows[kM2LClasses] = {2, 2, 2, 3, 3, 3, 4};
 constexpr int kM2LTileCols[kM2LClasses] = {4, 5, 6, 4, 5, 6, 8};
+// Streamed classes, after those: class kM2LClasses + s holds blocks of up to
+// 16 kM2LStreamRows[s] rows and any number of columns, streamed through
+// registers in chunks of columns.  256 rows covers every rank the assembly
+// accepts.
+constexpr int kM2LStreamClasses = 6;
+constexpr int kM2LStreamRows[kM2LStreamClasses] = {4, 6, 8, 10, 12, 16};
+constexpr int kM2LAllClasses = kM2LClasses + kM2LStreamClasses;
 
 inline int m2l_tile_class(int ld, int cols) {
   const int rt = (ld + 15) / 16, ct = (cols + 7) / 8;
   for (int c = 0; c < kM2LClasses; c++)
     if (rt <= kM2LTileRows[c] && ct <= kM2LTileCols[c]) return c;
-  return kM2LClasses;
+  for (int s = 0; s < kM2LStreamClasses; s++)
+    if (rt <= kM2LStreamRows[s]) return kM2LClasses + s;
+  return kM2LAllClasses;  // more than 256 rows: rejected at assembly
 }
 
 // Everything the M2L pair kernel needs about one pair, in one record.
@@ -147,13 +156,12 @@ struct Engine {
   DevBuf l2p_rm, l2p_cm;
   // M2L, symmetric pair storage
   int64_t npairs = 0;
-  int pair_tile = 0;                      // LDS doubles for one block (fallback kernel)
   int pair_max_ld = 0, pair_max_rb = 0;   // largest block shape
   std::vector<int> h_pair_k, h_pair_b;    // box gids: first (block rows), second
   std::vector<int64_t> h_pair_off;        // element offset of each block
   // pairs of group g (0: levels 2..L-1, 1: level L) in tile class c:
   // [m2l_cls_begin[g][c], m2l_cls_end[g][c])
-  int64_t m2l_cls_begin[2][kM2LClasses + 1] = {}, m2l_cls_end[2][kM2LClasses + 1] = {};
+  int64_t m2l_cls_begin[2][kM2LAllClasses + 1] = {}, m2l_cls_end[2][kM2LAllClasses + 1] = {};
   std::vector<int> h_red_start;           // per box gid (nbox + 1): first entry of h_red_pair
   std::vector<int> h_red_pair;            // per present slot, slot order: pair index
   DevBuf pair_k, pair_b, pair_off, m2l;

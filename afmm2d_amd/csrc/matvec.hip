@@ -405,55 +405,118 @@ __global__ __launch_bounds__(64) void k_m2l_reg(const double* __restrict__ m2l,
   m2l_pair<MAXRT, MAXCT>(m2l, pd[p], outv, part);
 }
 
-// M2L pair kernel for blocks too large for the register tiles: one
-// workgroup of 4 waves per pair, the block staged through LDS in column
-// chunks of at most 'tile' doubles.  LDS: the chunk, out_k and out_b (rcap
-// each) and the per-wave partial sums (2 x 4 x rcap).
-__global__ __launch_bounds__(256) void k_m2l_lds(const double* __restrict__ m2l,
-                                                 const PairDesc* __restrict__ pd,
-                                                 const double* __restrict__ outv,
-                                                 double* __restrict__ part, int tile, int rcap) {
-  extern __shared__ double lds[];
-  double* T = lds;
-  double* xk = lds + tile;
-  double* xb = xk + rcap;
-  double* rf = xb + rcap;      // [4][rcap]
-  double* rt = rf + 4 * rcap;  // [4][rcap]
-  const PairDesc d = pd[blockIdx.x];
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+// M2L pair kernel for blocks taller or wider than the register tiles: one
+// wave per pair, the same lane layout (lane 8 ci + rp holds rows 2 rp,
+// 2 rp + 1 of each 16-row tile of column ci of an 8-column tile), all RT row
+// tiles of a chunk of CT column tiles per step.  The block streams through
+// two register buffers: the next chunk's loads are issued before the current
+// chunk is used.  Per chunk, B^T out_k of its columns is summed over the 8
+// lanes of each column and written; B out_b accumulates per lane (two rows
+// per row tile) and is summed over the column lanes at the end.
+template <int RT, int CT>
+__device__ __forceinline__ void m2l_load_chunk(const double2* __restrict__ B2, int ld, int rb,
+                                               int c0, int ci, const int (&rr)[RT],
+                                               double (&ax)[CT][RT], double (&ay)[CT][RT],
+                                               double (&xv)[CT], const double* __restrict__ xb) {
+#pragma unroll
+  for (int u = 0; u < CT; u++) {
+    const int c = c0 + 8 * u + ci;
+    const int cc = min(c, rb - 1);
+    xv[u] = xb[cc];
+    if (c >= rb) xv[u] = 0.0;
+    const double2* col = B2 + ((cc * ld) >> 1);
+#pragma unroll
+    for (int t = 0; t < RT; t++) {
+      const double2 a = col[rr[t]];
+      ax[u][t] = a.x;
+      ay[u][t] = a.y;
+    }
+  }
+}
+
+template <int RT, int CT>
+__device__ __forceinline__ void m2l_use_chunk(const double (&ax)[CT][RT], const double (&ay)[CT][RT],
+                                              const double (&xv)[CT], const double (&k0)[RT],
+                                              const double (&k1)[RT], double (&f0)[RT],
+                                              double (&f1)[RT], int c0, int ci, int rp, int rb,
+                                              double* __restrict__ tw) {
+#pragma unroll
+  for (int u = 0; u < CT; u++) {
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < RT; t++) s += ax[u][t] * k0[t] + ay[u][t] * k1[t];
+    s = sum_lanes8(s);
+    const int c = c0 + 8 * u + ci;
+    if (rp == 0 && c < rb) tw[c] = s;
+#pragma unroll
+    for (int t = 0; t < RT; t++) {
+      f0[t] = fma(ax[u][t], xv[u], f0[t]);
+      f1[t] = fma(ay[u][t], xv[u], f1[t]);
+    }
+  }
+}
+
+template <int RT, int CT>
+__global__ __launch_bounds__(64) void k_m2l_stream(const double* __restrict__ m2l,
+                                                   const PairDesc* __restrict__ pd,
+                                                   const double* __restrict__ outv,
+                                                   double* __restrict__ part, int npairs) {
+  if (AFMM_MEM_PRIO) __builtin_amdgcn_s_setprio(AFMM_MEM_PRIO);
+  const int p = blockIdx.x;
+  if (p >= npairs) return;
+  const PairDesc d = pd[p];
+  const int lane = threadIdx.x, rp = lane & 7, ci = lane >> 3;
   const int rk = d.rk, rb = d.rb, ld = d.ld;
   double* fw = part + d.part_k;
   double* tw = part + d.part_b;
-  const double* B = m2l + d.off;
-  for (int i = tid; i < rk; i += 256) xk[i] = outv[d.sk + i];
-  for (int c = tid; c < rb; c += 256) xb[c] = outv[d.sb + c];
-  for (int u = tid; u < 4 * rcap; u += 256) rf[u] = 0.0;
-  const int chunk = max(1, tile / ld);
-  for (int c0 = 0; c0 < rb; c0 += chunk) {
-    const int c1 = min(rb, c0 + chunk);
-    const int ne = (c1 - c0) * ld;
-    __syncthreads();
-    for (int e = tid; e < ne; e += 256) T[e] = B[(int64_t)c0 * ld + e];
-    __syncthreads();
-    for (int i = lane; i < rk; i += 64) {
-      double acc = 0.0;
-      for (int c = c0 + w; c < c1; c += 4) acc += T[i + (c - c0) * ld] * xb[c];
-      rf[w * rcap + i] += acc;
-    }
-    const int q = (rk + 3) >> 2;
-    const int i0 = w * q, i1 = min(rk, i0 + q);
-    for (int c = c0 + lane; c < c1; c += 64) {
-      const double* Tc = T + (c - c0) * ld;
-      double acc = 0.0;
-      for (int i = i0; i < i1; i++) acc += Tc[i] * xk[i];
-      rt[w * rcap + c] = acc;
+  const double* xk = outv + d.sk;
+  const double* xb = outv + d.sb;
+  const double2* B2 = reinterpret_cast<const double2*>(m2l + d.off);
+  if (ld == 0 || rb == 0) {
+    // an empty side: both products are zero
+    for (int i = lane; i < rk; i += 64) fw[i] = 0.0;
+    for (int i = lane; i < rb; i += 64) tw[i] = 0.0;
+    return;
+  }
+  // Addresses are clamped into the block instead of masking the loads: a
+  // row past ld meets k0 = k1 = 0 in B^T out_k and its B out_b sum is never
+  // stored; a column past rb meets xv = 0 in B out_b and its B^T out_k sum
+  // is never stored.  rr[t]: the double2 index of rows 16 t + 2 rp, + 1
+  // within a column, clamped to the last row pair.
+  constexpr int W = 8 * CT;  // columns per chunk
+  int rr[RT];
+#pragma unroll
+  for (int t = 0; t < RT; t++) rr[t] = min(16 * t + 2 * rp, ld - 2) >> 1;
+  double ax[CT][RT], ay[CT][RT], xv[CT], bx[CT][RT], by[CT][RT], bv[CT];
+  m2l_load_chunk<RT, CT>(B2, ld, rb, 0, ci, rr, ax, ay, xv, xb);
+  double k0[RT], k1[RT], f0[RT], f1[RT];
+#pragma unroll
+  for (int t = 0; t < RT; t++) {
+    const int r = 16 * t + 2 * rp;
+    k0[t] = xk[min(r, rk - 1)];
+    k1[t] = xk[min(r + 1, rk - 1)];
+    if (!(r < rk)) k0[t] = 0.0;
+    if (!(r + 1 < rk)) k1[t] = 0.0;
+    f0[t] = 0.0;
+    f1[t] = 0.0;
+  }
+  for (int c0 = 0; c0 < rb; c0 += 2 * W) {
+    if (c0 + W < rb) m2l_load_chunk<RT, CT>(B2, ld, rb, c0 + W, ci, rr, bx, by, bv, xb);
+    m2l_use_chunk<RT, CT>(ax, ay, xv, k0, k1, f0, f1, c0, ci, rp, rb, tw);
+    if (c0 + W >= rb) break;
+    if (c0 + 2 * W < rb) m2l_load_chunk<RT, CT>(B2, ld, rb, c0 + 2 * W, ci, rr, ax, ay, xv, xb);
+    m2l_use_chunk<RT, CT>(bx, by, bv, k0, k1, f0, f1, c0 + W, ci, rp, rb, tw);
+  }
+#pragma unroll
+  for (int t = 0; t < RT; t++) {
+    const double g0 = sum_lanes_stride8(f0[t]);
+    const double g1 = sum_lanes_stride8(f1[t]);
+    const int r = 16 * t + 2 * rp;
+    if (ci == 0) {
+      if (r < rk) fw[r] = g0;
+      if (r + 1 < rk) fw[r + 1] = g1;
     }
   }
-  __syncthreads();
-  for (int i = tid; i < rk; i += 256)
-    fw[i] = (rf[i] + rf[rcap + i]) + (rf[2 * rcap + i] + rf[3 * rcap + i]);
-  for (int c = tid; c < rb; c += 256)
-    tw[c] = (rt[c] + rt[rcap + c]) + (rt[2 * rcap + c] + rt[3 * rcap + c]);
 }
 
 // L2L for a non-leaf level, with the box's M2L sum fused in: in_k =
@@ -1008,12 +1071,16 @@ void launch_up(Engine& e, int j, const double* d_b, hipStream_t s) {
 
 // M2L pair blocks of group g (0: levels 2..L-1, 1: level L): one launch
 // per register-tile class present (pairs are stored grouped by class).
+static_assert(kM2LAllClasses == 13 && kM2LStreamRows[0] == 4 && kM2LStreamRows[1] == 6 &&
+                  kM2LStreamRows[2] == 8 && kM2LStreamRows[3] == 10 &&
+                  kM2LStreamRows[4] == 12 && kM2LStreamRows[5] == 16,
+              "launch_m2l's cases follow the class tables");
 void launch_m2l(Engine& e, int g, hipStream_t s) {
   const PairDesc* pd0 = e.pair_desc.as<PairDesc>();
   double* part = e.m2l_part.as<double>();
   const double* outv = e.outv.as<double>();
   const double* B = e.m2l.as<double>();
-  for (int c = 0; c <= kM2LClasses; c++) {
+  for (int c = 0; c <= kM2LAllClasses; c++) {
     const int64_t p0 = e.m2l_cls_begin[g][c], p1 = e.m2l_cls_end[g][c];
     if (p1 <= p0) continue;
     const PairDesc* pd = pd0 + p0;
@@ -1026,11 +1093,13 @@ void launch_m2l(Engine& e, int g, hipStream_t s) {
       case 4: k_m2l_reg<3, 5><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
       case 5: k_m2l_reg<3, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
       case 6: k_m2l_reg<4, 8><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      default: {
-        const int rcap = std::max(e.max_rank, 1);
-        const size_t shm = sizeof(double) * ((size_t)e.pair_tile + 10 * (size_t)rcap);
-        k_m2l_lds<<<np, 256, shm, s>>>(B, pd, outv, part, e.pair_tile, rcap);
-      }
+      case 7: k_m2l_stream<4, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 8: k_m2l_stream<6, 3><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 9: k_m2l_stream<8, 2><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 10: k_m2l_stream<10, 1><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 11: k_m2l_stream<12, 1><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 12: k_m2l_stream<16, 1><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      default: throw Error(6, "M2L block taller than 256 rows");
     }
     AFMM_LAUNCH_CHECK();
   }

@@ -37,7 +37,9 @@ def test_topology_matches_oracle(engine, oracle_mod):
 
 @pytest.mark.parametrize("n,leaf,tol_pow,kernel", [
     (4096, 64, 10, "log"), (4096, 64, 8, "log"), (16384, 64, 12, "log"),
-    (8192, 64, 10, "inv_r"), (8192, 64, 10, "reg_inv_r"), (8192, 64, 6, "gaussian")])
+    (8192, 64, 10, "inv_r"), (8192, 64, 10, "reg_inv_r"), (8192, 64, 6, "gaussian"),
+    # ranks above 64: the streamed M2L classes (leaf level and upper levels)
+    (10000, 100, 12, "reg_inv_r"), (5000, 100, 14, "reg_inv_r")])
 def test_assembly_and_matvec_match_oracle(engine, oracle_mod, n, leaf, tol_pow, kernel):
     kid = engine.KERNELS[kernel]
     xy, q = uniform_points(n, 11 + tol_pow)
@@ -45,6 +47,8 @@ def test_assembly_and_matvec_match_oracle(engine, oracle_mod, n, leaf, tol_pow, 
     T.assemble()
     a = engine.AFMM(n, leaf, tol_pow, xy, kernel=kernel)
     a.assemble()
+    if leaf == 100:
+        assert a.stats()["max_rank"] > 64  # blocks beyond the register tiles
     mism = 0
     tot = 0
     for j in range(2, T.L + 1):
