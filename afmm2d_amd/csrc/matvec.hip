@@ -649,38 +649,46 @@ constexpr int kNearSlices = 8;          // work-queue slices of the near field
 constexpr int kNearCounterStride = 32;  // ints between the slice counters (128 B)
 constexpr int kNearChunk = 4;           // pairs per counter increment
 
-// Near field, symmetric: one wave per unordered leaf pair (A, B), A <= B.
-// Lane t holds target point t of an A chunk of 64; at step s it meets
-// source point (t + s) mod 64 of a B chunk staged in LDS, so the 64 lanes
-// touch 64 distinct sources and the source-side sums (for B's points) are
-// accumulated in LDS with one FP64 LDS add per step.  Each kernel value is
-// evaluated once and used for both directions.  A self pair (A, A) runs
-// steps 0..32 on its diagonal chunk pairs (step 32 on lanes 0..31 only).
-template <int KID, int MAXPAD>
-__global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
-                                            const double* __restrict__ y,
-                                            const double* __restrict__ qs,
-                                            const int* __restrict__ leaf_start,
-                                            const int* __restrict__ leaf_count,
-                                            const int* __restrict__ pairs,
-                                            const int* __restrict__ pair_part,
-                                            double* __restrict__ part, int npairs, KParams kp,
-                                            const double2* __restrict__ logtab,
-                                            double ln2_hi, double ln2_lo,
-                                            int* __restrict__ queue) {
-  // LDS: the log table, then the source points' x, y, charge and
-  // source-side sums as four arrays at compile-time offsets (one address per
-  // step, 8-byte lane stride: no bank conflicts)
+// Near field, symmetric: one unordered leaf pair (A, B), A <= B, per
+// workgroup of nw waves at a time.  B is taken in chunks of 64 points
+// staged in LDS (x, y, charge: 1.5 KiB); A's chunks stay in registers
+// (compile-time count NCH, so the arrays are statically indexed), wave w
+// taking the chunks ca = w, w + nw, ...  For a B chunk, lane t of an A chunk
+// meets source (t + s) mod 64 at step s, so the 64 lanes touch 64 distinct
+// sources; each kernel value is evaluated once and used for both
+// directions: the A-side sum in a register, the B-side sum in LDS (one FP64
+// LDS add per step) in a row of its own per A chunk, so every row has one
+// writer and the rows are added in chunk order: the sums do not depend on
+// how the chunks were spread over waves.  A B chunk's sums are complete
+// after its A chunks and written out then; a self pair (A, A) visits the
+// chunk pairs ca <= cb, the diagonal ones by steps 1..32 (step 32 on lanes
+// 0..31 only) and folds chunk cb's B-side sums into that chunk's A-side
+// sums.
+template <int KID, int NCH>
+__global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
+                                             const double* __restrict__ y,
+                                             const double* __restrict__ qs,
+                                             const int* __restrict__ leaf_start,
+                                             const int* __restrict__ leaf_count,
+                                             const int* __restrict__ pairs,
+                                             const int* __restrict__ pair_part,
+                                             double* __restrict__ part, int npairs, KParams kp,
+                                             const double2* __restrict__ logtab,
+                                             double ln2_hi, double ln2_lo,
+                                             int* __restrict__ queue) {
   __shared__ double2 tab[128];
-  __shared__ double bx[MAXPAD], by[MAXPAD], bq[MAXPAD], bc[MAXPAD];
-  const int t = threadIdx.x;
+  // one address per step for each array, 8-byte lane stride: no bank conflicts
+  __shared__ double bx[64], by[64], bq[64], bc[NCH][64];
+  __shared__ int grab;
+  const int t = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if constexpr (KID == K_LOG) {
-    for (int u = t; u < 128; u += 64) tab[u] = logtab[u];
+    for (int u = threadIdx.x; u < 128; u += blockDim.x) tab[u] = logtab[u];
   }
   // padding: far away, zero charge; K stays finite there
   const double FAR = 1.0e30;
   const double ninv_p0 = KID == K_GAUSS ? -1.0 / kp.p0 : 0.0;
   const double inv_a = KID == K_REG_INV_R ? 1.0 / kp.a : 0.0;
+  const double scale = KID == K_LOG ? 0.5 : 1.0;
   // work queue: the pairs are cut into kNearSlices slices, each with its own
   // chunk counter (one cache line apart); a workgroup drains the slice of
   // its own index first and then steals from the others, so a workgroup that
@@ -689,9 +697,10 @@ __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
   int slice = blockIdx.x % kNearSlices, tried = 0, p = 0, p_end = 0;
   for (;;) {
     if (p >= p_end) {
-      int c = 0;
-      if (t == 0) c = atomicAdd(&queue[slice * kNearCounterStride], 1);
-      c = __builtin_amdgcn_readfirstlane(c);
+      __syncthreads();
+      if (threadIdx.x == 0) grab = atomicAdd(&queue[slice * kNearCounterStride], 1);
+      __syncthreads();
+      const int c = grab;
       const int s0 = (int)((int64_t)npairs * slice / kNearSlices);
       const int s1 = (int)((int64_t)npairs * (slice + 1) / kNearSlices);
       p = s0 + c * kNearChunk;
@@ -707,67 +716,83 @@ __global__ __launch_bounds__(64) void k_p2p(const double* __restrict__ x,
     const bool self = A == B;
     const int sa = leaf_start[A], na = leaf_count[A];
     const int sb = leaf_start[B], nb = leaf_count[B];
-    const int pb = (nb + 63) & ~63, pa = (na + 63) & ~63;
-    __syncthreads();
-    for (int j = t; j < pb; j += 64) {
-      const bool v = j < nb;
-      bx[j] = v ? x[sb + j] : FAR;
-      by[j] = v ? y[sb + j] : FAR;
-      bq[j] = v ? qs[sb + j] : 0.0;
-      bc[j] = 0.0;
-    }
-    __syncthreads();
-    double* out = part + pair_part[2 * p];     // A's sums
+    const int nca = (na + 63) >> 6, ncb = (nb + 63) >> 6;
+    double* out = part + pair_part[2 * p];       // A's sums
     double* outb = part + pair_part[2 * p + 1];  // B's sums
-    for (int ca = 0; ca < pa; ca += 64) {
-      const int i = ca + t;
-      const bool vi = i < na;
-      const double xi = vi ? x[sa + i] : FAR;
-      const double yi = vi ? y[sa + i] : FAR;
-      const double qi = vi ? qs[sa + i] : 0.0;
-      double row = 0.0;
-      for (int cb = self ? ca : 0; cb < pb; cb += 64) {
-        const bool diag = self && cb == ca;
+    double xa[NCH], ya[NCH], qa[NCH], row[NCH];
+#pragma unroll
+    for (int ca = 0; ca < NCH; ca++) {
+      const int i = 64 * ca + t;
+      const bool v = ca < nca && ca % nw == w && i < na;
+      xa[ca] = v ? x[sa + i] : FAR;
+      ya[ca] = v ? y[sa + i] : FAR;
+      qa[ca] = v ? qs[sa + i] : 0.0;
+      row[ca] = 0.0;
+    }
+    for (int cb = 0; cb < ncb; cb++) {
+      __syncthreads();  // the previous chunk's sums have been read
+      if (w == 0) {
+        const int j = 64 * cb + t;
+        const bool v = j < nb;
+        bx[t] = v ? x[sb + j] : FAR;
+        by[t] = v ? y[sb + j] : FAR;
+        bq[t] = v ? qs[sb + j] : 0.0;
+      }
+#pragma unroll
+      for (int ca = 0; ca < NCH; ca++)
+        if (ca % nw == w) bc[ca][t] = 0.0;
+      __syncthreads();
+#pragma unroll
+      for (int ca = 0; ca < NCH; ca++) {
+        if (ca >= nca || ca % nw != w || (self && ca > cb)) continue;
+        const bool diag = self && ca == cb;
+        const double xi = xa[ca], yi = ya[ca], qi = qa[ca];
+        double acc = row[ca];
         int s = 0;
         if (diag) {
           // step 0: the point with itself (R2 = 0)
-          row += near_kernel<KID>(0.0, kp, ninv_p0, inv_a, tab, ln2_hi, ln2_lo) * qi;
+          acc += near_kernel<KID>(0.0, kp, ninv_p0, inv_a, tab, ln2_hi, ln2_lo) * qi;
           s = 1;
         }
         const int s_end = diag ? 32 : 64;
 #pragma unroll 2
         for (; s < s_end; s++) {
-          const int j = cb + ((t + s) & 63);
+          const int j = (t + s) & 63;
           const double dx = xi - bx[j], dy = yi - by[j];
-          const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab, ln2_hi,
-                                                   ln2_lo);
-          row = fma(v, bq[j], row);
-          atomicAdd(&bc[j], v * qi);
+          const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab,
+                                            ln2_hi, ln2_lo);
+          acc = fma(v, bq[j], acc);
+          atomicAdd(&bc[ca][j], v * qi);
         }
         if (diag && t < 32) {
-          const int j = cb + t + 32;
+          const int j = t + 32;
           const double dx = xi - bx[j], dy = yi - by[j];
-          const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab, ln2_hi,
-                                                   ln2_lo);
-          row = fma(v, bq[j], row);
-          atomicAdd(&bc[j], v * qi);
+          const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab,
+                                            ln2_hi, ln2_lo);
+          acc = fma(v, bq[j], acc);
+          atomicAdd(&bc[ca][j], v * qi);
         }
+        row[ca] = acc;
       }
+      __syncthreads();
+      // chunk cb's B-side sums are complete: the rows of A's chunks, in order
+      double colsum = 0.0;
+#pragma unroll
+      for (int ca = 0; ca < NCH; ca++)
+        if (ca < nca) colsum += bc[ca][t];
       if (self) {
-        // B = A: the source-side sums of this chunk arrive from chunks <= ca,
-        // all of which precede it in this loop
-        if (vi) out[i] = row;
-      } else {
-        if (vi) out[i] = row;
+#pragma unroll
+        for (int ca = 0; ca < NCH; ca++)
+          if (ca == cb && ca % nw == w) row[ca] += colsum;
+      } else if (w == 0) {
+        const int j = 64 * cb + t;
+        if (j < nb) outb[j] = scale * colsum;
       }
     }
-    __syncthreads();
-    const double scale = KID == K_LOG ? 0.5 : 1.0;
-    if (self) {
-      for (int i = t; i < na; i += 64) out[i] = scale * (out[i] + bc[i]);
-    } else {
-      for (int i = t; i < na; i += 64) out[i] = scale * out[i];
-      for (int j = t; j < nb; j += 64) outb[j] = scale * bc[j];
+#pragma unroll
+    for (int ca = 0; ca < NCH; ca++) {
+      const int i = 64 * ca + t;
+      if (ca < nca && ca % nw == w && i < na) out[i] = scale * row[ca];
     }
     p++;
   }
@@ -1136,23 +1161,30 @@ void launch_near(Engine& e, hipStream_t s) {
   e.near_queue.alloc(sizeof(int) * nq);
   k_zero_ints<<<1, nq, 0, s>>>(e.near_queue.as<int>(), nq);
   AFMM_LAUNCH_CHECK();
-  // p2p_waves_per_simd = 0: one workgroup per pair
-  const int64_t cap = e.p2p_waves_per_simd > 0 ? (int64_t)4 * e.num_cus * e.p2p_waves_per_simd
-                                               : e.near_upairs;
+  // Wave slots the near field may take beside the memory-bound kernels
+  // (p2p_waves_per_simd per SIMD); 0: one workgroup per pair.  With fewer
+  // pairs than slots, several waves share a pair (up to 4, and no more than
+  // the chunks of the largest leaf).
+  const int nch = (e.max_leaf + 63) / 64;
+  const int64_t slots = (int64_t)4 * e.num_cus * e.p2p_waves_per_simd;
+  int nw = 1;
+  if (e.p2p_waves_per_simd > 0 && e.near_upairs > 0 && e.near_upairs < slots)
+    nw = (int)std::min<int64_t>(std::min(4, nch), (slots + e.near_upairs - 1) / e.near_upairs);
+  nw = std::max(1, nw);
+  const int64_t cap = e.p2p_waves_per_simd > 0 ? slots / nw : e.near_upairs;
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(e.near_upairs, cap));
-  // LDS arrays sized for the largest leaf, padded to a multiple of 64
-  const int maxpad = (e.max_leaf + 63) & ~63;
-#define AFMM_NEAR_LAUNCH(KID, MAXPAD)                                                       \
-  k_p2p<KID, MAXPAD><<<grid, 64, 0, s>>>(                                                  \
+#define AFMM_NEAR_LAUNCH(KID, NCH)                                                          \
+  k_p2p<KID, NCH><<<grid, 64 * nw, 0, s>>>(                                                \
       e.x.as<double>(), e.y.as<double>(), e.qs.as<double>(), e.d_leaf_start.as<int>(),     \
       e.d_leaf_count.as<int>(), e.near_pair.as<int>(), e.near_pair_part.as<int>(),         \
       e.near_part.as<double>(), (int)e.near_upairs, e.kp, e.log_tab.as<double2>(), e.ln2_hi, \
       e.ln2_lo, e.near_queue.as<int>())
+  // A's chunks in registers: arrays for the largest leaf (up to 512 points)
   AFMM_DISPATCH_KERNEL(e.kid, KID,
-                       if (maxpad <= 64) AFMM_NEAR_LAUNCH(KID, 64);
-                       else if (maxpad <= 128) AFMM_NEAR_LAUNCH(KID, 128);
-                       else if (maxpad <= 256) AFMM_NEAR_LAUNCH(KID, 256);
-                       else AFMM_NEAR_LAUNCH(KID, 512));
+                       if (nch <= 1) AFMM_NEAR_LAUNCH(KID, 1);
+                       else if (nch <= 2) AFMM_NEAR_LAUNCH(KID, 2);
+                       else if (nch <= 4) AFMM_NEAR_LAUNCH(KID, 4);
+                       else AFMM_NEAR_LAUNCH(KID, 8));
 #undef AFMM_NEAR_LAUNCH
   AFMM_LAUNCH_CHECK();
 }
