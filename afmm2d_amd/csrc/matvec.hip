@@ -743,36 +743,67 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
         if (ca % nw == w) bc[ca][t] = 0.0;
       __syncthreads();
 #pragma unroll
-      for (int ca = 0; ca < NCH; ca++) {
-        if (ca >= nca || ca % nw != w || (self && ca > cb)) continue;
-        const bool diag = self && ca == cb;
-        const double xi = xa[ca], yi = ya[ca], qi = qa[ca];
-        double acc = row[ca];
-        int s = 0;
-        if (diag) {
-          // step 0: the point with itself (R2 = 0)
-          acc += near_kernel<KID>(0.0, kp, ninv_p0, inv_a, tab, ln2_hi, ln2_lo) * qi;
-          s = 1;
-        }
-        const int s_end = diag ? 32 : 64;
+      for (int cp = 0; cp < NCH; cp += 2) {
+        if (cp + 1 < NCH && nw == 1 && !self && cp + 1 < nca) {
+          // two A chunks per step against one staged source: half the LDS
+          // reads per evaluation; each row gets the same adds in the same
+          // order as one chunk at a time
+          const double x0 = xa[cp], y0 = ya[cp], q0 = qa[cp];
+          const double x1 = xa[cp + 1], y1 = ya[cp + 1], q1 = qa[cp + 1];
+          double acc0 = row[cp], acc1 = row[cp + 1];
+          double* bc0 = bc[cp];
+          double* bc1 = bc[cp + 1];
 #pragma unroll 2
-        for (; s < s_end; s++) {
-          const int j = (t + s) & 63;
-          const double dx = xi - bx[j], dy = yi - by[j];
-          const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab,
-                                            ln2_hi, ln2_lo);
-          acc = fma(v, bq[j], acc);
-          atomicAdd(&bc[ca][j], v * qi);
+          for (int s = 0; s < 64; s++) {
+            const int j = (t + s) & 63;
+            const double sx = bx[j], sy = by[j], sq = bq[j];
+            const double dx0 = x0 - sx, dy0 = y0 - sy;
+            const double dx1 = x1 - sx, dy1 = y1 - sy;
+            const double v0 = near_kernel<KID>(fma(dx0, dx0, dy0 * dy0), kp, ninv_p0, inv_a,
+                                               tab, ln2_hi, ln2_lo);
+            const double v1 = near_kernel<KID>(fma(dx1, dx1, dy1 * dy1), kp, ninv_p0, inv_a,
+                                               tab, ln2_hi, ln2_lo);
+            acc0 = fma(v0, sq, acc0);
+            acc1 = fma(v1, sq, acc1);
+            atomicAdd(&bc0[j], v0 * q0);
+            atomicAdd(&bc1[j], v1 * q1);
+          }
+          row[cp] = acc0;
+          row[cp + 1] = acc1;
+          continue;
         }
-        if (diag && t < 32) {
-          const int j = t + 32;
-          const double dx = xi - bx[j], dy = yi - by[j];
-          const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab,
-                                            ln2_hi, ln2_lo);
-          acc = fma(v, bq[j], acc);
-          atomicAdd(&bc[ca][j], v * qi);
+#pragma unroll
+        for (int ca = cp; ca < cp + 2 && ca < NCH; ca++) {
+          if (ca >= nca || ca % nw != w || (self && ca > cb)) continue;
+          const bool diag = self && ca == cb;
+          const double xi = xa[ca], yi = ya[ca], qi = qa[ca];
+          double acc = row[ca];
+          int s = 0;
+          if (diag) {
+            // step 0: the point with itself (R2 = 0)
+            acc += near_kernel<KID>(0.0, kp, ninv_p0, inv_a, tab, ln2_hi, ln2_lo) * qi;
+            s = 1;
+          }
+          const int s_end = diag ? 32 : 64;
+#pragma unroll 2
+          for (; s < s_end; s++) {
+            const int j = (t + s) & 63;
+            const double dx = xi - bx[j], dy = yi - by[j];
+            const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab,
+                                              ln2_hi, ln2_lo);
+            acc = fma(v, bq[j], acc);
+            atomicAdd(&bc[ca][j], v * qi);
+          }
+          if (diag && t < 32) {
+            const int j = t + 32;
+            const double dx = xi - bx[j], dy = yi - by[j];
+            const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab,
+                                              ln2_hi, ln2_lo);
+            acc = fma(v, bq[j], acc);
+            atomicAdd(&bc[ca][j], v * qi);
+          }
+          row[ca] = acc;
         }
-        row[ca] = acc;
       }
       __syncthreads();
       // chunk cb's B-side sums are complete: the rows of A's chunks, in order
