@@ -174,7 +174,10 @@ void Engine::prof_collect() {
 // problem's sizes and the rates measured on MI355X (~9e11 near-field kernel
 // evaluations/s, ~5 TB/s for the M2L blocks and partials), the best counts
 // measured were 3 at R ~ 0.55 (N = 2^20, tolerance 1e-12) and 6 at R ~ 1.2
-// (tolerance 1e-8): round(5 R), clamped to 2..8.
+// (tolerance 1e-8): round(5 R), clamped to 2..8.  With larger leaves (the
+// published shapes: 100-391 points, a/R kernel) the count matters little
+// from 4 up (N = 1e6: within 5% for 2..8; N = 1e5: 0.55 ms at 4..8,
+// 0.91 ms at 2).
 int near_waves_per_simd(const Engine& e) {
   double evals = 0.0;
   const int nleaf = (int)e.leaf_count.size();

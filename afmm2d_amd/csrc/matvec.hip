@@ -637,11 +637,11 @@ __device__ __forceinline__ double near_kernel(double r2, const KParams& kp, doub
   } else if constexpr (KID == K_GAUSS) {
     return exp(r2 * ninv_p0);
   } else {
-    // R < a ? R/a : a/R with R = r2 / sqrt(r2); both sides are 1 at R = a
-    const double rs = rsqrt_refined(r2);
+    // R < a ? R/a : a/R = min(R/a, a/R) with R = r2 / sqrt(r2); r2 is kept
+    // off zero inside the square root only, so R = 0 gives R/a = 0 exactly
+    const double rs = rsqrt_refined(fmax(r2, 1e-300));
     const double r = r2 * rs;
-    const double v = r < kp.a ? r * inv_a : kp.a * rs;
-    return r2 == 0.0 ? 0.0 : v;
+    return fmin(r * inv_a, kp.a * rs);
   }
 }
 

@@ -37,6 +37,8 @@ def parse():
     p.add_argument("--n", type=int, default=1 << 20)
     p.add_argument("--leaf", type=int, default=64)
     p.add_argument("--tol-pow", type=int, default=12)
+    p.add_argument("--kernel", default="log", choices=["log", "inv_r", "gaussian", "reg_inv_r"],
+                   help="kernel of the operator (the headline metric is quoted on log)")
     p.add_argument("--config-tol-pow", type=int, default=8)
     p.add_argument("--check-rows", type=int, default=4096)
     p.add_argument("--seed", type=int, default=2024)
@@ -149,12 +151,12 @@ def rel_err_rows(A, q, y, nrows, seed):
     return float(np.linalg.norm(y[rows] - ref) / np.linalg.norm(ref)), rows, ref
 
 
-def cpu_baseline(xy, q, leaf, tol_pow, threads, reps):
+def cpu_baseline(xy, q, leaf, tol_pow, threads, reps, kernel_id=0):
     """The oracle's OpenMP FP64 matvec (reference pragma placement) on the
     same workload, timed with its own wall clock around the matvec only."""
     import oracle
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    T = oracle.Tree(xy, leaf, tol_pow)
+    T = oracle.Tree(xy, leaf, tol_pow, kernel_id)
     t0 = time.perf_counter()
     T.assemble()
     t_asm = time.perf_counter() - t0
@@ -286,7 +288,7 @@ def main():
 
     # headline: smallest TOL_POW meeting the 1e-8 error bar
     t0 = time.perf_counter()
-    A = E.AFMM(args.n, args.leaf, args.tol_pow, xy, kernel="log")
+    A = E.AFMM(args.n, args.leaf, args.tol_pow, xy, kernel=args.kernel)
     t_create = time.perf_counter() - t0
     t0 = time.perf_counter()
     A.assemble()
@@ -301,7 +303,7 @@ def main():
     # configured tolerance, reported beside the headline
     side = None
     if not args.no_side_run:
-        Ac = E.AFMM(args.n, args.leaf, args.config_tol_pow, xy, kernel="log")
+        Ac = E.AFMM(args.n, args.leaf, args.config_tol_pow, xy, kernel=args.kernel)
         Ac.assemble()
         ms_c, _, kern_c, y_c = run_gpu(Ac, q, max(20, args.steps // 2), args.warmup, torch)
         err_c = float(np.linalg.norm(y_c[rows] - ref_rows) / np.linalg.norm(ref_rows))
@@ -332,12 +334,13 @@ def main():
     cpu = None
     if not args.no_cpu:
         t_cpu, t_cpu_asm, y_cpu = cpu_baseline(xy, q, args.leaf, args.tol_pow, threads,
-                                               args.cpu_reps)
+                                               args.cpu_reps, E.KERNELS[args.kernel])
         cpu = {"value": round(args.n / t_cpu, 1), "unit": "point*RHS/s", "cores": threads,
                "kind": "port",
-               "sample": "full workload (N=2^20, tol 1e-12): oracle OpenMP FP64 matvec, "
+               "sample": "full workload (N=%d, tol 1e-%d): oracle OpenMP FP64 matvec, "
                          "reference pragma placement, median of %d after 1 warm-up; "
-                         "assembly %.1f s untimed" % (args.cpu_reps, t_cpu_asm),
+                         "assembly %.1f s untimed" % (args.n, args.tol_pow, args.cpu_reps,
+                                                      t_cpu_asm),
                "ms_per_matvec": round(t_cpu * 1e3, 2), "cpu_model": cpu_model(),
                "gpu_vs_cpu_rel_diff": float(np.linalg.norm(y - y_cpu) / np.linalg.norm(y_cpu))}
 
@@ -356,9 +359,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "N=2^20 uniform [-1,1]^2, log kernel, leaf 64 (L=7), 1 RHS, "
-                        "ACA tol 1e-%d" % args.tol_pow,
-            "n": args.n, "leaf": args.leaf, "tol_pow": args.tol_pow, "kernel": "log",
+            "workload": "N=%d uniform [-1,1]^2, %s kernel, leaf %d (L=%d), 1 RHS, "
+                        "ACA tol 1e-%d" % (args.n, args.kernel, args.leaf, st["levels"],
+                                           args.tol_pow),
+            "n": args.n, "leaf": args.leaf, "tol_pow": args.tol_pow, "kernel": args.kernel,
             "nrhs": 1, "levels": st["levels"], "max_rank": st["max_rank"],
             "rel_err": err, "rel_err_rows": int(len(rows)),
             "wall_ms_per_step": round(wall_ms, 4),
