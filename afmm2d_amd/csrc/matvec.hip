@@ -1127,6 +1127,22 @@ void launch_up(Engine& e, int j, const double* d_b, hipStream_t s) {
 
 // M2L pair blocks of group g (0: levels 2..L-1, 1: level L): one launch
 // per register-tile class present (pairs are stored grouped by class).
+// column tiles (8 columns each) per streamed chunk of the streamed classes;
+// narrow chunks keep the registers low, so more M2L waves fit beside the
+// near field's (N = 1e6, leaf 100, a/R, tol 1e-14: 2.46 ms against 2.62 ms
+// with 4/3/2 tiles; equal at 1e-10 and 1e-12)
+#ifndef AFMM_SCT4
+#define AFMM_SCT4 2
+#endif
+#ifndef AFMM_SCT6
+#define AFMM_SCT6 2
+#endif
+#ifndef AFMM_SCT8
+#define AFMM_SCT8 1
+#endif
+#ifndef AFMM_SCT10
+#define AFMM_SCT10 1
+#endif
 static_assert(kM2LAllClasses == 13 && kM2LStreamRows[0] == 4 && kM2LStreamRows[1] == 6 &&
                   kM2LStreamRows[2] == 8 && kM2LStreamRows[3] == 10 &&
                   kM2LStreamRows[4] == 12 && kM2LStreamRows[5] == 16,
@@ -1149,10 +1165,10 @@ void launch_m2l(Engine& e, int g, hipStream_t s) {
       case 4: k_m2l_reg<3, 5><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
       case 5: k_m2l_reg<3, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
       case 6: k_m2l_reg<4, 8><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 7: k_m2l_stream<4, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 8: k_m2l_stream<6, 3><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 9: k_m2l_stream<8, 2><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 10: k_m2l_stream<10, 1><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 7: k_m2l_stream<4, AFMM_SCT4><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 8: k_m2l_stream<6, AFMM_SCT6><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 9: k_m2l_stream<8, AFMM_SCT8><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 10: k_m2l_stream<10, AFMM_SCT10><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
       case 11: k_m2l_stream<12, 1><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
       case 12: k_m2l_stream<16, 1><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
       default: throw Error(6, "M2L block taller than 256 rows");
