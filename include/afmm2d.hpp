@@ -135,6 +135,14 @@ class AFMM {
     return y;
   }
 
+  // The near-field term alone (FMM2DTree::evaluate_NearField); needs no assemble().
+  std::vector<double> nearField(const std::vector<double>& b) {
+    if ((int)b.size() != N) throw Error(AFMM2D_ERR_INVALID_ARGUMENT, "b size");
+    std::vector<double> y(N);
+    check(afmm2d_near_field(h_, b.data(), y.data()));
+    return y;
+  }
+
   // One leaf's relative error against the direct sum; returns it.
   double evaluateError(unsigned seed = 0) {
     int box = 0;
