@@ -226,7 +226,7 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
   *out = nullptr;
   if (!loc) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "loc is NULL");
   if (n <= 1 || leaf <= 0) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "need N > 1 and leaf > 0");
-  if (kernel_id < 0 || kernel_id > 3) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown kernel id");
+  if (kernel_id < 0 || kernel_id > 4) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown kernel id");
   if (layout != AFMM2D_LOC_INTERLEAVED && layout != AFMM2D_LOC_COLUMN_MAJOR)
     return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown location layout");
   if (tol_pow < 0 || tol_pow > 300) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "tol_pow out of range");
@@ -251,6 +251,7 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
     e->kp.p0 = (kparams && kparams[0] > 0.0) ? kparams[0] : 0.01;
     e->kp.a = std::ldexp(1.0, -L);  // a = 0.5^L (AFMM2DTree.cpp:13-19)
     if (kparams && kernel_id == AFMM2D_KERNEL_REG_INV_R && kparams[1] > 0.0) e->kp.a = kparams[1];
+    for (int i = 0; i < 4; i++) e->kp.u[i] = kparams ? kparams[i] : 0.0;
     AFMM_HIP(hipDeviceGetAttribute(&e->num_cus, hipDeviceAttributeMultiprocessorCount, device));
     if (const char* v = std::getenv("AFMM2D_NEAR_OVERLAP")) e->near_overlap = std::atoi(v) != 0;
     if (const char* v = std::getenv("AFMM2D_TREE_MODE"))
@@ -670,8 +671,9 @@ int afmm2d_get_matrix_entry(afmm2d_handle h, int i, int j, double* value) {
 int afmm2d_kernel_value(int kernel_id, const double* kparams, double a, double xi, double yi,
                         double xj, double yj, double* value) {
   if (!value) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (kernel_id < 0 || kernel_id > 3) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown kernel id");
-  KParams p{(kparams && kparams[0] > 0.0) ? kparams[0] : 0.01, a};
+  if (kernel_id < 0 || kernel_id > 4) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown kernel id");
+  KParams p{(kparams && kparams[0] > 0.0) ? kparams[0] : 0.01, a, {0.0, 0.0, 0.0, 0.0}};
+  for (int i = 0; i < 4; i++) p.u[i] = kparams ? kparams[i] : 0.0;
   *value = kernel_host(kernel_id, p, xi, yi, xj, yj);
   return AFMM2D_OK;
 }

@@ -28,14 +28,27 @@ struct Error : std::runtime_error {
 #define AFMM_LAUNCH_CHECK() AFMM_HIP(hipGetLastError())
 
 // Kernel ids match afmm2d_kernel_id.
-enum KernelId { K_LOG = 0, K_INV_R = 1, K_GAUSS = 2, K_REG_INV_R = 3 };
+enum KernelId { K_LOG = 0, K_INV_R = 1, K_GAUSS = 2, K_REG_INV_R = 3, K_USER = 4 };
 
 // Parameters every kernel needs: p0 = Gaussian width (s^2), a = the
-// regularisation radius of K_REG_INV_R (0.5^L).
+// regularisation radius of K_REG_INV_R (0.5^L), u = the kparams as given
+// (zeros if none), for the user kernel.
 struct KParams {
   double p0;
   double a;
+  double u[4];
 };
+}  // namespace afmm
+
+// The user kernel (K_USER), from the header named by AFMM2D_USER_KERNEL_HEADER
+// at build time.  It is compiled without contracting multiplies and adds into
+// fused operations, like the host compiler and the CPU oracle evaluate it.
+#define AFMM2D_USER_FN __host__ __device__ inline
+#pragma clang fp contract(off)
+#include AFMM2D_USER_KERNEL_HEADER
+#pragma clang fp contract(fast)
+
+namespace afmm {
 
 // Kernel value from the squared distance.  R2 == 0 (the diagonal) gives the
 // kernel's diagonal value: 0 for the singular kernels, exp(0) = 1 for the
@@ -69,7 +82,11 @@ __host__ __device__ __forceinline__ double dist2_exact(double xi, double yi, dou
 template <int KID>
 __host__ __device__ __forceinline__ double kernel_xy(double xi, double yi, double xj, double yj,
                                                      const KParams& p) {
-  return kernel_r2<KID>(dist2_exact(xi, yi, xj, yj), p);
+  if constexpr (KID == K_USER) {
+    return ::afmm2d_user_kernel(xi, yi, xj, yj, p.u, p.a);
+  } else {
+    return kernel_r2<KID>(dist2_exact(xi, yi, xj, yj), p);
+  }
 }
 
 inline double kernel_host(int kid, const KParams& p, double xi, double yi, double xj, double yj) {
@@ -77,7 +94,8 @@ inline double kernel_host(int kid, const KParams& p, double xi, double yi, doubl
     case K_LOG: return kernel_xy<K_LOG>(xi, yi, xj, yj, p);
     case K_INV_R: return kernel_xy<K_INV_R>(xi, yi, xj, yj, p);
     case K_GAUSS: return kernel_xy<K_GAUSS>(xi, yi, xj, yj, p);
-    default: return kernel_xy<K_REG_INV_R>(xi, yi, xj, yj, p);
+    case K_REG_INV_R: return kernel_xy<K_REG_INV_R>(xi, yi, xj, yj, p);
+    default: return kernel_xy<K_USER>(xi, yi, xj, yj, p);
   }
 }
 
@@ -87,7 +105,8 @@ inline double kernel_host(int kid, const KParams& p, double xi, double yi, doubl
     case ::afmm::K_LOG: { constexpr int KID = 0; __VA_ARGS__; break; }     \
     case ::afmm::K_INV_R: { constexpr int KID = 1; __VA_ARGS__; break; }   \
     case ::afmm::K_GAUSS: { constexpr int KID = 2; __VA_ARGS__; break; }   \
-    default: { constexpr int KID = 3; __VA_ARGS__; break; }                \
+    case ::afmm::K_REG_INV_R: { constexpr int KID = 3; __VA_ARGS__; break; } \
+    default: { constexpr int KID = 4; __VA_ARGS__; break; }                \
   }
 
 inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }

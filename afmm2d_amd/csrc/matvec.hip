@@ -645,6 +645,20 @@ __device__ __forceinline__ double near_kernel(double r2, const KParams& kp, doub
   }
 }
 
+// K(target, source) for the near field: the user kernel takes the points,
+// the built-in ones the squared distance.
+template <int KID>
+__device__ __forceinline__ double pair_value(double xi, double yi, double sx, double sy,
+                                             const KParams& kp, double ninv_p0, double inv_a,
+                                             const double2* tab, double ln2_hi, double ln2_lo) {
+  if constexpr (KID == K_USER) {
+    return kernel_xy<K_USER>(xi, yi, sx, sy, kp);
+  } else {
+    const double dx = xi - sx, dy = yi - sy;
+    return near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab, ln2_hi, ln2_lo);
+  }
+}
+
 constexpr int kNearSlices = 8;          // work-queue slices of the near field
 constexpr int kNearCounterStride = 32;  // ints between the slice counters (128 B)
 constexpr int kNearChunk = 4;           // pairs per counter increment
@@ -684,8 +698,9 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
   if constexpr (KID == K_LOG) {
     for (int u = threadIdx.x; u < 128; u += blockDim.x) tab[u] = logtab[u];
   }
-  // padding: far away, zero charge; K stays finite there
-  const double FAR = 1.0e30;
+  // padding: zero charge, far away for the built-in kernels (K stays finite
+  // there); the user kernel is only known to be finite at real points, so
+  // its padding repeats the leaf's first point
   const double ninv_p0 = KID == K_GAUSS ? -1.0 / kp.p0 : 0.0;
   const double inv_a = KID == K_REG_INV_R ? 1.0 / kp.a : 0.0;
   const double scale = KID == K_LOG ? 0.5 : 1.0;
@@ -719,13 +734,15 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
     const int nca = (na + 63) >> 6, ncb = (nb + 63) >> 6;
     double* out = part + pair_part[2 * p];       // A's sums
     double* outb = part + pair_part[2 * p + 1];  // B's sums
+    const double pax = KID == K_USER ? x[sa] : 1.0e30, pay = KID == K_USER ? y[sa] : 1.0e30;
+    const double pbx = KID == K_USER ? x[sb] : 1.0e30, pby = KID == K_USER ? y[sb] : 1.0e30;
     double xa[NCH], ya[NCH], qa[NCH], row[NCH];
 #pragma unroll
     for (int ca = 0; ca < NCH; ca++) {
       const int i = 64 * ca + t;
       const bool v = ca < nca && ca % nw == w && i < na;
-      xa[ca] = v ? x[sa + i] : FAR;
-      ya[ca] = v ? y[sa + i] : FAR;
+      xa[ca] = v ? x[sa + i] : pax;
+      ya[ca] = v ? y[sa + i] : pay;
       qa[ca] = v ? qs[sa + i] : 0.0;
       row[ca] = 0.0;
     }
@@ -734,8 +751,8 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
       if (w == 0) {
         const int j = 64 * cb + t;
         const bool v = j < nb;
-        bx[t] = v ? x[sb + j] : FAR;
-        by[t] = v ? y[sb + j] : FAR;
+        bx[t] = v ? x[sb + j] : pbx;
+        by[t] = v ? y[sb + j] : pby;
         bq[t] = v ? qs[sb + j] : 0.0;
       }
 #pragma unroll
@@ -757,12 +774,10 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
           for (int s = 0; s < 64; s++) {
             const int j = (t + s) & 63;
             const double sx = bx[j], sy = by[j], sq = bq[j];
-            const double dx0 = x0 - sx, dy0 = y0 - sy;
-            const double dx1 = x1 - sx, dy1 = y1 - sy;
-            const double v0 = near_kernel<KID>(fma(dx0, dx0, dy0 * dy0), kp, ninv_p0, inv_a,
-                                               tab, ln2_hi, ln2_lo);
-            const double v1 = near_kernel<KID>(fma(dx1, dx1, dy1 * dy1), kp, ninv_p0, inv_a,
-                                               tab, ln2_hi, ln2_lo);
+            const double v0 = pair_value<KID>(x0, y0, sx, sy, kp, ninv_p0, inv_a, tab, ln2_hi,
+                                              ln2_lo);
+            const double v1 = pair_value<KID>(x1, y1, sx, sy, kp, ninv_p0, inv_a, tab, ln2_hi,
+                                              ln2_lo);
             acc0 = fma(v0, sq, acc0);
             acc1 = fma(v1, sq, acc1);
             atomicAdd(&bc0[j], v0 * q0);
@@ -781,24 +796,25 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
           int s = 0;
           if (diag) {
             // step 0: the point with itself (R2 = 0)
-            acc += near_kernel<KID>(0.0, kp, ninv_p0, inv_a, tab, ln2_hi, ln2_lo) * qi;
+            acc += (KID == K_USER ? kernel_xy<K_USER>(xi, yi, xi, yi, kp)
+                                  : near_kernel<KID>(0.0, kp, ninv_p0, inv_a, tab, ln2_hi,
+                                                     ln2_lo)) *
+                   qi;
             s = 1;
           }
           const int s_end = diag ? 32 : 64;
 #pragma unroll 2
           for (; s < s_end; s++) {
             const int j = (t + s) & 63;
-            const double dx = xi - bx[j], dy = yi - by[j];
-            const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab,
-                                              ln2_hi, ln2_lo);
+            const double v = pair_value<KID>(xi, yi, bx[j], by[j], kp, ninv_p0, inv_a, tab,
+                                             ln2_hi, ln2_lo);
             acc = fma(v, bq[j], acc);
             atomicAdd(&bc[ca][j], v * qi);
           }
           if (diag && t < 32) {
             const int j = t + 32;
-            const double dx = xi - bx[j], dy = yi - by[j];
-            const double v = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab,
-                                              ln2_hi, ln2_lo);
+            const double v = pair_value<KID>(xi, yi, bx[j], by[j], kp, ninv_p0, inv_a, tab,
+                                             ln2_hi, ln2_lo);
             acc = fma(v, bq[j], acc);
             atomicAdd(&bc[ca][j], v * qi);
           }

@@ -38,7 +38,11 @@ typedef enum {
   AFMM2D_KERNEL_LOG = 0,       /* 0.5*log(R2), 0 when R2 == 0 */
   AFMM2D_KERNEL_INV_R = 1,     /* 1/sqrt(R2), 0 when R2 == 0 */
   AFMM2D_KERNEL_GAUSSIAN = 2,  /* exp(-R2/kparams[0]), kparams[0] defaults to 0.01 */
-  AFMM2D_KERNEL_REG_INV_R = 3  /* R < a ? R/a : a/R, a = 0.5^nLevels (testAFMM2D.cpp:9-20) */
+  AFMM2D_KERNEL_REG_INV_R = 3, /* R < a ? R/a : a/R, a = 0.5^nLevels (testAFMM2D.cpp:9-20) */
+  AFMM2D_KERNEL_USER = 4       /* the user kernel compiled into this library: a header defining
+                                  afmm2d_user_kernel(xi, yi, xj, yj, kparams, a), chosen at build
+                                  time (the reference's userkernel::getMatrixEntry, kernel.hpp:58);
+                                  default afmm2d_amd/user_kernels/anisotropic_log.h */
 } afmm2d_kernel_id;
 
 typedef enum {

@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libafmm2d_oracle.so")
 _lib = None
 
-KERNEL_LOG, KERNEL_INV_R, KERNEL_GAUSS, KERNEL_REG_INV_R = 0, 1, 2, 3
+KERNEL_LOG, KERNEL_INV_R, KERNEL_GAUSS, KERNEL_REG_INV_R, KERNEL_USER = 0, 1, 2, 3, 4
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)

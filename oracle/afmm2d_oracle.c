@@ -38,6 +38,14 @@
 /* ------------------------------------------------------------------------- */
 /* Kernels: examples/testAFMM2D.cpp:9-20 (a/R), kernel.hpp:39-54 (1/r, log). */
 
+/* The user kernel (kernel id 4): the same header the engine is built with
+ * (see afmm2d_amd/user_kernels/anisotropic_log.h for the contract). */
+#define AFMM2D_USER_FN static inline
+#ifndef AFMM2D_USER_KERNEL_HEADER
+#define AFMM2D_USER_KERNEL_HEADER "../afmm2d_amd/user_kernels/anisotropic_log.h"
+#endif
+#include AFMM2D_USER_KERNEL_HEADER
+
 double or_kernel_value(int kid, const double* kp, double a, double xi, double yi,
                        double xj, double yj) {
   double dx = xi - xj, dy = yi - yj;
@@ -54,6 +62,10 @@ double or_kernel_value(int kid, const double* kp, double a, double xi, double yi
     case OR_KERNEL_REG_INV_R: {
       double r = sqrt(r2);
       return r < a ? r / a : a / r;
+    }
+    case OR_KERNEL_USER: {
+      static const double zero[4] = {0.0, 0.0, 0.0, 0.0};
+      return afmm2d_user_kernel(xi, yi, xj, yj, kp ? kp : zero, a);
     }
   }
   return 0.0;

@@ -18,7 +18,8 @@
 extern "C" {
 #endif
 
-enum { OR_KERNEL_LOG = 0, OR_KERNEL_INV_R = 1, OR_KERNEL_GAUSS = 2, OR_KERNEL_REG_INV_R = 3 };
+enum { OR_KERNEL_LOG = 0, OR_KERNEL_INV_R = 1, OR_KERNEL_GAUSS = 2, OR_KERNEL_REG_INV_R = 3,
+       OR_KERNEL_USER = 4 };
 
 typedef struct or_tree or_tree;
 

@@ -34,10 +34,11 @@ def test_status_strings(engine):
     assert L.afmm2d_status_string(99).decode() == "unknown status"
 
 
-@pytest.mark.parametrize("kernel,kid", [("log", 0), ("inv_r", 1), ("gaussian", 2), ("reg_inv_r", 3)])
-def test_host_kernel_matches_oracle(engine, oracle_mod, kernel, kid):
+@pytest.mark.parametrize("kernel,kid,kp", [
+    ("log", 0, None), ("inv_r", 1, None), ("gaussian", 2, [0.02, 0, 0, 0]),
+    ("reg_inv_r", 3, None), ("user", 4, None), ("user", 4, [3.0, 0, 0, 0])])
+def test_host_kernel_matches_oracle(engine, oracle_mod, kernel, kid, kp):
     rng = np.random.default_rng(kid)
-    kp = [0.02, 0, 0, 0] if kernel == "gaussian" else None
     for _ in range(200):
         xi, yi, xj, yj = rng.uniform(-1, 1, 4)
         a = 0.25
@@ -84,3 +85,33 @@ def test_null_handle_calls_fail_cleanly(engine):
 def test_levels_formula(engine, oracle_mod):
     for n, m in [(4096, 64), (1 << 20, 64), (6400, 200), (1000000, 100), (1 << 23, 128)]:
         assert engine.n_levels(n, m) == oracle_mod.nlevels(n, m)
+
+
+def test_user_kernel_header_plugs_in(tmp_path):
+    """The plug-in path a user takes: build the library with another kernel
+    header (here a multiquadric, sqrt(R^2 + c^2), c = kparams[0]) and get that
+    kernel as id 4 (host evaluation; the device kernels include the same
+    header)."""
+    from afmm2d_amd import build as B
+    hdr = tmp_path / "multiquadric.h"
+    hdr.write_text(
+        "AFMM2D_USER_FN double afmm2d_user_kernel(double xi, double yi, double xj, double yj,\n"
+        "                                         const double* kp, double a) {\n"
+        "  const double c = kp[0] > 0.0 ? kp[0] : 0.1;\n"
+        "  const double dx = xi - xj, dy = yi - yj;\n"
+        "  return sqrt(dx * dx + dy * dy + c * c);\n"
+        "}\n")
+    lib_path = B.build(user_kernel=str(hdr), lib_path=str(tmp_path / "libafmm2d_mq.so"),
+                       build_dir=str(tmp_path / "obj"), example=False)
+    L = ctypes.CDLL(lib_path)
+    L.afmm2d_kernel_value.restype = ctypes.c_int
+    L.afmm2d_kernel_value.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.c_double] + [ctypes.c_double] * 4 + \
+        [ctypes.POINTER(ctypes.c_double)]
+    v = ctypes.c_double()
+    assert L.afmm2d_kernel_value(4, None, 0.0, 0.1, 0.2, 0.4, -0.3, ctypes.byref(v)) == 0
+    assert abs(v.value - np.sqrt(0.3 ** 2 + 0.5 ** 2 + 0.01)) <= 1e-15
+    kp = (ctypes.c_double * 4)(0.5, 0.0, 0.0, 0.0)
+    assert L.afmm2d_kernel_value(4, kp, 0.0, 0.1, 0.2, 0.4, -0.3, ctypes.byref(v)) == 0
+    assert abs(v.value - np.sqrt(0.3 ** 2 + 0.5 ** 2 + 0.25)) <= 1e-15
+    assert L.afmm2d_kernel_value(5, None, 0.0, 0.1, 0.2, 0.4, -0.3, ctypes.byref(v)) == 1

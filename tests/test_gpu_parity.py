@@ -39,7 +39,9 @@ def test_topology_matches_oracle(engine, oracle_mod):
     (4096, 64, 10, "log"), (4096, 64, 8, "log"), (16384, 64, 12, "log"),
     (8192, 64, 10, "inv_r"), (8192, 64, 10, "reg_inv_r"), (8192, 64, 6, "gaussian"),
     # ranks above 64: the streamed M2L classes (leaf level and upper levels)
-    (10000, 100, 12, "reg_inv_r"), (5000, 100, 14, "reg_inv_r")])
+    (10000, 100, 12, "reg_inv_r"), (5000, 100, 14, "reg_inv_r"),
+    # the compiled-in user kernel (anisotropic log)
+    (8192, 64, 10, "user"), (20000, 64, 12, "user")])
 def test_assembly_and_matvec_match_oracle(engine, oracle_mod, n, leaf, tol_pow, kernel):
     kid = engine.KERNELS[kernel]
     xy, q = uniform_points(n, 11 + tol_pow)
@@ -97,6 +99,8 @@ def test_direct_sum_matches_oracle(engine, oracle_mod):
     (5000, 100, "inv_r", 0),       # 312: 512
     (5000, 100, "reg_inv_r", 16),  # duplicated points: R = 0 off the diagonal
     (6000, 64, "inv_r", 16),
+    (7000, 100, "user", 0),        # user kernel: padding repeats a real point
+    (6000, 64, "user", 16),
 ])
 def test_near_field_matches_oracle(engine, oracle_mod, n, leaf, kernel, quant):
     """The near-field kernel alone (AFMM2DTree.cpp:1199-1213) against the
@@ -114,3 +118,24 @@ def test_near_field_matches_oracle(engine, oracle_mod, n, leaf, kernel, quant):
     err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
     assert err <= 1e-13, err
     assert float(np.max(np.abs(got - ref))) <= 1e-12 * float(np.max(np.abs(ref)))
+
+
+def test_user_kernel_parameters_reach_the_device(engine, oracle_mod):
+    """kparams of the user kernel (here the anisotropy g = 3) used by the
+    device kernels: near field and whole operator against the oracle."""
+    xy, q = uniform_points(8192, 77)
+    kp = [3.0, 0.0, 0.0, 0.0]
+    T = oracle_mod.Tree(xy, 64, 10, 4, kparams=kp)
+    a = engine.AFMM(8192, 64, 10, xy, kernel="user", kparams=kp)
+    nf, nf_ref = a.near_field(q), T.near_field(q)
+    assert np.linalg.norm(nf - nf_ref) <= 1e-13 * np.linalg.norm(nf_ref)
+    T.assemble()
+    a.assemble()
+    y, yo = a.computeMatVecProduct(q), T.matvec(q)
+    assert np.linalg.norm(y - yo) <= 1e-11 * np.linalg.norm(yo)
+    # a different g is a different operator
+    a2 = engine.AFMM(8192, 64, 10, xy, kernel="user")
+    assert np.linalg.norm(a2.near_field(q) - nf) > 1e-3 * np.linalg.norm(nf)
+    a.close()
+    a2.close()
+    T.close()

@@ -11,15 +11,18 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("kid,kp", [(0, None), (1, None), (3, None), (2, [0.05, 0, 0, 0])])
+@pytest.mark.parametrize("kid,kp", [(0, None), (1, None), (3, None), (2, [0.05, 0, 0, 0]),
+                                    (4, None), (4, [0.5, 0, 0, 0])])
 def test_kernel_values(oracle_mod, kid, kp):
     rng = np.random.default_rng(0)
     a = 0.125
     for _ in range(50):
         xi, yi, xj, yj = rng.uniform(-1, 1, 4)
         r = np.hypot(xi - xj, yi - yj)
+        g = kp[0] if kp else 2.0  # the example user kernel: anisotropic log
         expect = {0: np.log(r), 1: 1 / r, 3: (r / a if r < a else a / r),
-                  2: np.exp(-r * r / (kp[0] if kp else 0.01))}[kid]
+                  2: np.exp(-r * r / (kp[0] if kp else 0.01)),
+                  4: np.log(np.hypot(xi - xj, g * (yi - yj)))}[kid]
         got = oracle_mod.kernel_value(kid, xi, yi, xj, yj, a=a, kparams=kp)
         assert abs(got - expect) <= 1e-14 * max(1.0, abs(expect))
     # diagonal values
@@ -109,7 +112,7 @@ def test_invalid_levels(oracle_mod):
         oracle_mod.Tree(xy, 64, 8)
 
 
-@pytest.mark.parametrize("kid", [0, 1, 2, 3])
+@pytest.mark.parametrize("kid", [0, 1, 2, 3, 4])
 def test_near_field_brute_force(oracle_mod, kid):
     """The oracle's near-field term (AFMM2DTree.cpp:1199-1213) against a
     numpy brute force over each leaf's own points and its 8 neighbours'."""
@@ -131,12 +134,27 @@ def test_near_field_brute_force(oracle_mod, kid):
             r2 = (d ** 2).sum(-1)
             with np.errstate(divide="ignore", invalid="ignore"):
                 r = np.sqrt(r2)
+                r2g = d[..., 0] ** 2 + (2.0 * d[..., 1]) ** 2
                 K = {0: np.where(r2 > 0, 0.5 * np.log(r2), 0.0),
                      1: np.where(r2 > 0, 1.0 / r, 0.0),
                      2: np.exp(-r2 / 0.01),
-                     3: np.where(r < a, r / a, a / r)}[kid]
+                     3: np.where(r < a, r / a, a / r),
+                     4: np.where(r2g > 0, 0.5 * np.log(r2g), 0.0)}[kid]
             ys[st:st + n] += K @ qs[sb:sb + m]
     y = np.empty_like(ys)
     y[perm] = ys
     got = T.near_field(q)
     assert rel(got, y) <= 1e-13
+
+
+def test_user_kernel_operator(oracle_mod):
+    """The compiled-in user kernel (id 4) through the whole oracle operator:
+    error against the direct sum falls with the tolerance."""
+    xy, q = uniform_points(4096, 31)
+    errs = []
+    for tp in (6, 10):
+        T = oracle_mod.Tree(xy, 64, tp, 4)
+        T.assemble()
+        errs.append(rel(T.matvec(q), T.direct(q)))
+        T.close()
+    assert errs[1] < errs[0] and errs[1] <= 1e-7, errs
