@@ -21,7 +21,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # AFMM2D_LIB names another build of the library (e.g. a tuned variant)
 LIB_PATH = os.environ.get("AFMM2D_LIB") or os.path.join(_HERE, "libafmm2d.so")
 
-KERNELS = {"log": 0, "inv_r": 1, "gaussian": 2, "reg_inv_r": 3, "user": 4}
+KERNELS = {"log": 0, "inv_r": 1, "gaussian": 2, "reg_inv_r": 3, "user": 4, "user_sym": 5,
+           "user_anti": 6}
 
 OK = 0
 ERR_INVALID_ARGUMENT = 1

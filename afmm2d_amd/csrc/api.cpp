@@ -198,9 +198,45 @@ int near_waves_per_simd(const Engine& e) {
 
 }  // namespace afmm
 
+// e: the operator, or the symmetric part of a non-symmetric user kernel;
+// e2: that kernel's antisymmetric part (null otherwise).  y = e(b) + e2(b).
 struct afmm2d_handle_s {
   std::unique_ptr<Engine> e;
+  std::unique_ptr<Engine> e2;
 };
+
+namespace {
+
+// Whether the user kernel is symmetric on this point set: K(i, j) == K(j, i)
+// bit for bit on 4096 sampled pairs (a fixed seed), about half of them
+// within a leaf's neighbourhood in KD order, where most kernels vary fastest.
+bool user_kernel_symmetric(const Engine& e) {
+  std::mt19937_64 rng(12345);
+  for (int t = 0; t < 4096; t++) {
+    const int i = (int)(rng() % (uint64_t)e.n);
+    int j = (int)(rng() % (uint64_t)e.n);
+    if (t & 1) j = std::min(e.n - 1, i + (int)(rng() % 64));
+    const double xi = e.h_xy[2 * (size_t)i], yi = e.h_xy[2 * (size_t)i + 1];
+    const double xj = e.h_xy[2 * (size_t)j], yj = e.h_xy[2 * (size_t)j + 1];
+    const double a = kernel_host(K_USER, e.kp, xi, yi, xj, yj);
+    const double b = kernel_host(K_USER, e.kp, xj, yj, xi, yi);
+    if (!(a == b)) return false;
+  }
+  return true;
+}
+
+// y = the operator applied to b, device arrays in caller order, on stream s.
+void run_matvec(afmm2d_handle h, const double* d_b, double* d_y, hipStream_t s) {
+  matvec_device(*h->e, d_b, d_y, s);
+  if (h->e2) {
+    Engine& e2 = *h->e2;
+    e2.io_y.alloc(sizeof(double) * e2.n);
+    matvec_device(e2, d_b, e2.io_y.as<double>(), s);
+    add_device(d_y, e2.io_y.as<const double>(), e2.n, s);
+  }
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -226,7 +262,8 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
   *out = nullptr;
   if (!loc) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "loc is NULL");
   if (n <= 1 || leaf <= 0) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "need N > 1 and leaf > 0");
-  if (kernel_id < 0 || kernel_id > 4) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown kernel id");
+  if (kernel_id < 0 || kernel_id >= kNumKernels)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown kernel id");
   if (layout != AFMM2D_LOC_INTERLEAVED && layout != AFMM2D_LOC_COLUMN_MAJOR)
     return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown location layout");
   if (tol_pow < 0 || tol_pow > 300) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "tol_pow out of range");
@@ -238,6 +275,9 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
   if (n / (1 << (2 * L)) + 1 > 512)
     return fail(AFMM2D_ERR_UNSUPPORTED, "leaves above 512 points are not supported");
   return guarded([&] {
+  // one engine per operator: the kernel itself, or for a non-symmetric user
+  // kernel its symmetric and its antisymmetric part (same points and tree)
+  auto make_engine = [&](int kid) {
     auto e = std::make_unique<Engine>();
     if (device < 0) AFMM_HIP(hipGetDevice(&device));
     AFMM_HIP(hipSetDevice(device));
@@ -247,7 +287,8 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
     e->tol_pow = tol_pow;
     e->tol = std::pow(10.0, -1.0 * tol_pow);  // ACA.cpp:7
     e->L = L;
-    e->kid = kernel_id;
+    e->kid = kid;
+    e->kid_full = kernel_id;
     e->kp.p0 = (kparams && kparams[0] > 0.0) ? kparams[0] : 0.01;
     e->kp.a = std::ldexp(1.0, -L);  // a = 0.5^L (AFMM2DTree.cpp:13-19)
     if (kparams && kernel_id == AFMM2D_KERNEL_REG_INV_R && kparams[1] > 0.0) e->kp.a = kparams[1];
@@ -374,17 +415,26 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
                             hipMemcpyHostToDevice, s));
     AFMM_HIP(hipStreamSynchronize(s));
     e->create_seconds = now_s() - t0;
-    *out = new afmm2d_handle_s{std::move(e)};
+    return e;
+  };
+    auto e = make_engine(kernel_id);
+    std::unique_ptr<Engine> e2;
+    if (kernel_id == K_USER && !user_kernel_symmetric(*e)) {
+      e->kid = K_USER_SYM;
+      e2 = make_engine(K_USER_ANTI);
+    }
+    *out = new afmm2d_handle_s{std::move(e), std::move(e2)};
   });
 }
 
 int afmm2d_destroy(afmm2d_handle h) {
   if (!h) return AFMM2D_OK;
   return guarded([&] {
-    if (h->e) {
-      (void)hipSetDevice(h->e->device);
-      (void)hipStreamSynchronize(h->e->s_aux);
-      if (h->e->s_tree) (void)hipStreamSynchronize(h->e->s_tree);
+    for (Engine* e : {h->e.get(), h->e2.get()}) {
+      if (!e) continue;
+      (void)hipSetDevice(e->device);
+      (void)hipStreamSynchronize(e->s_aux);
+      if (e->s_tree) (void)hipStreamSynchronize(e->s_tree);
     }
     delete h;
   });
@@ -393,14 +443,17 @@ int afmm2d_destroy(afmm2d_handle h) {
 int afmm2d_assemble(afmm2d_handle h) {
   if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "handle is NULL");
   return guarded([&] {
-    Engine& e = *h->e;
-    AFMM_HIP(hipSetDevice(e.device));
-    if (e.assembled) return;
-    double t0 = now_s();
-    assemble_tree(e);
-    e.assembled = true;
-    if (!e.p2p_waves_fixed) e.p2p_waves_per_simd = near_waves_per_simd(e);
-    e.assemble_seconds = now_s() - t0;
+    for (Engine* ep : {h->e.get(), h->e2.get()}) {
+      if (!ep) continue;
+      Engine& e = *ep;
+      AFMM_HIP(hipSetDevice(e.device));
+      if (e.assembled) continue;
+      double t0 = now_s();
+      assemble_tree(e);
+      e.assembled = true;
+      if (!e.p2p_waves_fixed) e.p2p_waves_per_simd = near_waves_per_simd(e);
+      e.assemble_seconds = now_s() - t0;
+    }
   });
 }
 
@@ -411,7 +464,7 @@ int afmm2d_matvec_device(afmm2d_handle h, const double* d_b, double* d_y, void* 
     Engine& e = *h->e;
     AFMM_HIP(hipSetDevice(e.device));
     hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)0;
-    matvec_device(e, d_b, d_y, s);
+    run_matvec(h, d_b, d_y, s);
   });
 }
 
@@ -427,7 +480,7 @@ int afmm2d_matvec(afmm2d_handle h, const double* b, double* y, int nrhs) {
     hipStream_t s = nullptr;  // the null stream plays the caller's stream
     for (int r = 0; r < nrhs; r++) {
       AFMM_HIP(hipMemcpyAsync(e.io_b.p, b + (size_t)r * e.n, nb, hipMemcpyHostToDevice, s));
-      matvec_device(e, e.io_b.as<double>(), e.io_y.as<double>(), s);
+      run_matvec(h, e.io_b.as<double>(), e.io_y.as<double>(), s);
       AFMM_HIP(hipMemcpyAsync(y + (size_t)r * e.n, e.io_y.p, nb, hipMemcpyDeviceToHost, s));
       AFMM_HIP(hipStreamSynchronize(s));
       if (e.profiling) e.prof_collect();
@@ -610,6 +663,12 @@ int afmm2d_near_field(afmm2d_handle h, const double* b, double* y) {
     hipStream_t s = nullptr;
     AFMM_HIP(hipMemcpyAsync(e.io_b.p, b, nb, hipMemcpyHostToDevice, s));
     near_field_device(e, e.io_b.as<double>(), e.io_y.as<double>(), s);
+    if (h->e2) {
+      Engine& e2 = *h->e2;
+      e2.io_y.alloc(nb);
+      near_field_device(e2, e.io_b.as<double>(), e2.io_y.as<double>(), s);
+      add_device(e.io_y.as<double>(), e2.io_y.as<const double>(), e.n, s);
+    }
     AFMM_HIP(hipMemcpyAsync(y, e.io_y.p, nb, hipMemcpyDeviceToHost, s));
     AFMM_HIP(hipStreamSynchronize(s));
   });
@@ -663,7 +722,7 @@ int afmm2d_get_matrix_entry(afmm2d_handle h, int i, int j, double* value) {
   if (!h || !value) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
   Engine& e = *h->e;
   if (i < 0 || j < 0 || i >= e.n || j >= e.n) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "index out of range");
-  *value = kernel_host(e.kid, e.kp, e.h_xy[2 * (size_t)i], e.h_xy[2 * (size_t)i + 1],
+  *value = kernel_host(e.kid_full, e.kp, e.h_xy[2 * (size_t)i], e.h_xy[2 * (size_t)i + 1],
                        e.h_xy[2 * (size_t)j], e.h_xy[2 * (size_t)j + 1]);
   return AFMM2D_OK;
 }
@@ -671,7 +730,8 @@ int afmm2d_get_matrix_entry(afmm2d_handle h, int i, int j, double* value) {
 int afmm2d_kernel_value(int kernel_id, const double* kparams, double a, double xi, double yi,
                         double xj, double yj, double* value) {
   if (!value) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (kernel_id < 0 || kernel_id > 4) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown kernel id");
+  if (kernel_id < 0 || kernel_id >= kNumKernels)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown kernel id");
   KParams p{(kparams && kparams[0] > 0.0) ? kparams[0] : 0.01, a, {0.0, 0.0, 0.0, 0.0}};
   for (int i = 0; i < 4; i++) p.u[i] = kparams ? kparams[i] : 0.0;
   *value = kernel_host(kernel_id, p, xi, yi, xj, yj);
@@ -685,7 +745,7 @@ int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st) {
   st->n = e.n;
   st->levels = e.L;
   st->tol_pow = e.tol_pow;
-  st->kernel_id = e.kid;
+  st->kernel_id = e.kid_full;
   st->max_rank = e.max_rank;
   st->skeleton_total = e.skel_total;
   st->il_pairs = e.il_pairs;

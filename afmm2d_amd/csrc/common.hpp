@@ -28,7 +28,25 @@ struct Error : std::runtime_error {
 #define AFMM_LAUNCH_CHECK() AFMM_HIP(hipGetLastError())
 
 // Kernel ids match afmm2d_kernel_id.
-enum KernelId { K_LOG = 0, K_INV_R = 1, K_GAUSS = 2, K_REG_INV_R = 3, K_USER = 4 };
+// K_USER_SYM and K_USER_ANTI are the symmetric and antisymmetric parts of
+// the user kernel, (K(x, y) + K(y, x)) / 2 and (K(x, y) - K(y, x)) / 2: a
+// non-symmetric user kernel runs as the sum of two operators, one for each.
+enum KernelId {
+  K_LOG = 0,
+  K_INV_R = 1,
+  K_GAUSS = 2,
+  K_REG_INV_R = 3,
+  K_USER = 4,
+  K_USER_SYM = 5,
+  K_USER_ANTI = 6
+};
+constexpr int kNumKernels = 7;
+
+// K(y, x) = kTransposeSign * K(x, y): -1 for the antisymmetric part, whose
+// M2L blocks and near-field values are used with this sign in the
+// transposed direction.
+template <int KID>
+constexpr double kTransposeSign = KID == K_USER_ANTI ? -1.0 : 1.0;
 
 // Parameters every kernel needs: p0 = Gaussian width (s^2), a = the
 // regularisation radius of K_REG_INV_R (0.5^L), u = the kparams as given
@@ -84,6 +102,12 @@ __host__ __device__ __forceinline__ double kernel_xy(double xi, double yi, doubl
                                                      const KParams& p) {
   if constexpr (KID == K_USER) {
     return ::afmm2d_user_kernel(xi, yi, xj, yj, p.u, p.a);
+  } else if constexpr (KID == K_USER_SYM) {
+    return 0.5 * (::afmm2d_user_kernel(xi, yi, xj, yj, p.u, p.a) +
+                  ::afmm2d_user_kernel(xj, yj, xi, yi, p.u, p.a));
+  } else if constexpr (KID == K_USER_ANTI) {
+    return 0.5 * (::afmm2d_user_kernel(xi, yi, xj, yj, p.u, p.a) -
+                  ::afmm2d_user_kernel(xj, yj, xi, yi, p.u, p.a));
   } else {
     return kernel_r2<KID>(dist2_exact(xi, yi, xj, yj), p);
   }
@@ -95,7 +119,9 @@ inline double kernel_host(int kid, const KParams& p, double xi, double yi, doubl
     case K_INV_R: return kernel_xy<K_INV_R>(xi, yi, xj, yj, p);
     case K_GAUSS: return kernel_xy<K_GAUSS>(xi, yi, xj, yj, p);
     case K_REG_INV_R: return kernel_xy<K_REG_INV_R>(xi, yi, xj, yj, p);
-    default: return kernel_xy<K_USER>(xi, yi, xj, yj, p);
+    case K_USER: return kernel_xy<K_USER>(xi, yi, xj, yj, p);
+    case K_USER_SYM: return kernel_xy<K_USER_SYM>(xi, yi, xj, yj, p);
+    default: return kernel_xy<K_USER_ANTI>(xi, yi, xj, yj, p);
   }
 }
 
@@ -106,7 +132,9 @@ inline double kernel_host(int kid, const KParams& p, double xi, double yi, doubl
     case ::afmm::K_INV_R: { constexpr int KID = 1; __VA_ARGS__; break; }   \
     case ::afmm::K_GAUSS: { constexpr int KID = 2; __VA_ARGS__; break; }   \
     case ::afmm::K_REG_INV_R: { constexpr int KID = 3; __VA_ARGS__; break; } \
-    default: { constexpr int KID = 4; __VA_ARGS__; break; }                \
+    case ::afmm::K_USER: { constexpr int KID = 4; __VA_ARGS__; break; }    \
+    case ::afmm::K_USER_SYM: { constexpr int KID = 5; __VA_ARGS__; break; } \
+    default: { constexpr int KID = 6; __VA_ARGS__; break; }                \
   }
 
 inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }

@@ -12,8 +12,8 @@
 //                 B = K(skeleton_k, skeleton_b), rank_k x rank_b column-major
 //                 with leading dimension ld = rank_k rounded up to even (a
 //                 zero row when rank_k is odd), so every column starts 16-byte
-//                 aligned; K(skeleton_b, skeleton_k) = B^T is not stored (the
-//                 kernels are symmetric)
+//                 aligned; K(skeleton_b, skeleton_k) = +-B^T is not stored
+//                 (symmetric kernels; -B^T for an antisymmetric part)
 //   m2l_part      box-major M2L partials: box gid owns kILSlots slots of
 //                 rank[gid] values at kILSlots * soff[gid]; the t-th present
 //                 interaction slot (slot order) holds that pair's product for
@@ -118,6 +118,10 @@ struct KernelTimer {
 struct Engine {
   // problem
   int n = 0, leaf = 0, tol_pow = 0, L = 0, kid = 0, device = 0;
+  // the kernel of the whole operator, for direct sums and single entries: kid
+  // itself, or K_USER when this engine carries one part of a non-symmetric
+  // user kernel
+  int kid_full = 0;
   double tol = 0.0;
   KParams kp{0.01, 0.0};
 
@@ -216,6 +220,8 @@ int near_waves_per_simd(const Engine& e);
 // matvec.hip
 void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s);
 void near_field_device(Engine& e, const double* d_b, double* d_y, hipStream_t s);
+// y += t (n values)
+void add_device(double* d_y, const double* d_t, int n, hipStream_t s);
 void direct_rows_device(Engine& e, int nrows, const int* d_rows, const double* d_qs, double* d_out,
                         hipStream_t s);
 void gather_sorted(Engine& e, const double* d_b, double* d_qs, hipStream_t s);

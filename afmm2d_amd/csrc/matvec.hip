@@ -334,7 +334,7 @@ __device__ __forceinline__ double sum_lanes_stride8(double v) {
 template <int MAXRT, int MAXCT>
 __device__ __forceinline__ void m2l_pair(const double* __restrict__ m2l, const PairDesc& d,
                                          const double* __restrict__ outv,
-                                         double* __restrict__ part) {
+                                         double* __restrict__ part, double tsign) {
   const int lane = threadIdx.x, rp = lane & 7, ci = lane >> 3;
   const int rk = d.rk, rb = d.rb, ld = d.ld;
   double* fw = part + d.part_k;
@@ -374,7 +374,7 @@ __device__ __forceinline__ void m2l_pair(const double* __restrict__ m2l, const P
     for (int t = 0; t < MAXRT; t++) s += ax[u][t] * k0[t] + ay[u][t] * k1[t];
     s = sum_lanes8(s);
     const int c = 8 * u + ci;
-    if (rp == 0 && c < rb) tw[c] = s;
+    if (rp == 0 && c < rb) tw[c] = tsign * s;
   }
 #pragma unroll
   for (int t = 0; t < MAXRT; t++) {
@@ -398,11 +398,12 @@ template <int MAXRT, int MAXCT>
 __global__ __launch_bounds__(64) void k_m2l_reg(const double* __restrict__ m2l,
                                                 const PairDesc* __restrict__ pd,
                                                 const double* __restrict__ outv,
-                                                double* __restrict__ part, int npairs) {
+                                                double* __restrict__ part, int npairs,
+                                                double tsign) {
   if (AFMM_MEM_PRIO) __builtin_amdgcn_s_setprio(AFMM_MEM_PRIO);
   const int p = blockIdx.x;
   if (p >= npairs) return;
-  m2l_pair<MAXRT, MAXCT>(m2l, pd[p], outv, part);
+  m2l_pair<MAXRT, MAXCT>(m2l, pd[p], outv, part, tsign);
 }
 
 // M2L pair kernel for blocks taller or wider than the register tiles: one
@@ -439,7 +440,7 @@ __device__ __forceinline__ void m2l_use_chunk(const double (&ax)[CT][RT], const 
                                               const double (&xv)[CT], const double (&k0)[RT],
                                               const double (&k1)[RT], double (&f0)[RT],
                                               double (&f1)[RT], int c0, int ci, int rp, int rb,
-                                              double* __restrict__ tw) {
+                                              double* __restrict__ tw, double tsign) {
 #pragma unroll
   for (int u = 0; u < CT; u++) {
     double s = 0.0;
@@ -447,7 +448,7 @@ __device__ __forceinline__ void m2l_use_chunk(const double (&ax)[CT][RT], const 
     for (int t = 0; t < RT; t++) s += ax[u][t] * k0[t] + ay[u][t] * k1[t];
     s = sum_lanes8(s);
     const int c = c0 + 8 * u + ci;
-    if (rp == 0 && c < rb) tw[c] = s;
+    if (rp == 0 && c < rb) tw[c] = tsign * s;
 #pragma unroll
     for (int t = 0; t < RT; t++) {
       f0[t] = fma(ax[u][t], xv[u], f0[t]);
@@ -460,7 +461,8 @@ template <int RT, int CT>
 __global__ __launch_bounds__(64) void k_m2l_stream(const double* __restrict__ m2l,
                                                    const PairDesc* __restrict__ pd,
                                                    const double* __restrict__ outv,
-                                                   double* __restrict__ part, int npairs) {
+                                                   double* __restrict__ part, int npairs,
+                                                   double tsign) {
   if (AFMM_MEM_PRIO) __builtin_amdgcn_s_setprio(AFMM_MEM_PRIO);
   const int p = blockIdx.x;
   if (p >= npairs) return;
@@ -502,10 +504,10 @@ __global__ __launch_bounds__(64) void k_m2l_stream(const double* __restrict__ m2
   }
   for (int c0 = 0; c0 < rb; c0 += 2 * W) {
     if (c0 + W < rb) m2l_load_chunk<RT, CT>(B2, ld, rb, c0 + W, ci, rr, bx, by, bv, xb);
-    m2l_use_chunk<RT, CT>(ax, ay, xv, k0, k1, f0, f1, c0, ci, rp, rb, tw);
+    m2l_use_chunk<RT, CT>(ax, ay, xv, k0, k1, f0, f1, c0, ci, rp, rb, tw, tsign);
     if (c0 + W >= rb) break;
     if (c0 + 2 * W < rb) m2l_load_chunk<RT, CT>(B2, ld, rb, c0 + 2 * W, ci, rr, ax, ay, xv, xb);
-    m2l_use_chunk<RT, CT>(bx, by, bv, k0, k1, f0, f1, c0 + W, ci, rp, rb, tw);
+    m2l_use_chunk<RT, CT>(bx, by, bv, k0, k1, f0, f1, c0 + W, ci, rp, rb, tw, tsign);
   }
 #pragma unroll
   for (int t = 0; t < RT; t++) {
@@ -651,8 +653,8 @@ template <int KID>
 __device__ __forceinline__ double pair_value(double xi, double yi, double sx, double sy,
                                              const KParams& kp, double ninv_p0, double inv_a,
                                              const double2* tab, double ln2_hi, double ln2_lo) {
-  if constexpr (KID == K_USER) {
-    return kernel_xy<K_USER>(xi, yi, sx, sy, kp);
+  if constexpr (KID >= K_USER) {
+    return kernel_xy<KID>(xi, yi, sx, sy, kp);
   } else {
     const double dx = xi - sx, dy = yi - sy;
     return near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab, ln2_hi, ln2_lo);
@@ -734,8 +736,8 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
     const int nca = (na + 63) >> 6, ncb = (nb + 63) >> 6;
     double* out = part + pair_part[2 * p];       // A's sums
     double* outb = part + pair_part[2 * p + 1];  // B's sums
-    const double pax = KID == K_USER ? x[sa] : 1.0e30, pay = KID == K_USER ? y[sa] : 1.0e30;
-    const double pbx = KID == K_USER ? x[sb] : 1.0e30, pby = KID == K_USER ? y[sb] : 1.0e30;
+    const double pax = KID >= K_USER ? x[sa] : 1.0e30, pay = KID >= K_USER ? y[sa] : 1.0e30;
+    const double pbx = KID >= K_USER ? x[sb] : 1.0e30, pby = KID >= K_USER ? y[sb] : 1.0e30;
     double xa[NCH], ya[NCH], qa[NCH], row[NCH];
 #pragma unroll
     for (int ca = 0; ca < NCH; ca++) {
@@ -780,8 +782,8 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
                                               ln2_lo);
             acc0 = fma(v0, sq, acc0);
             acc1 = fma(v1, sq, acc1);
-            atomicAdd(&bc0[j], v0 * q0);
-            atomicAdd(&bc1[j], v1 * q1);
+            atomicAdd(&bc0[j], kTransposeSign<KID> * v0 * q0);
+            atomicAdd(&bc1[j], kTransposeSign<KID> * v1 * q1);
           }
           row[cp] = acc0;
           row[cp + 1] = acc1;
@@ -796,7 +798,7 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
           int s = 0;
           if (diag) {
             // step 0: the point with itself (R2 = 0)
-            acc += (KID == K_USER ? kernel_xy<K_USER>(xi, yi, xi, yi, kp)
+            acc += (KID >= K_USER ? kernel_xy<KID>(xi, yi, xi, yi, kp)
                                   : near_kernel<KID>(0.0, kp, ninv_p0, inv_a, tab, ln2_hi,
                                                      ln2_lo)) *
                    qi;
@@ -809,14 +811,14 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
             const double v = pair_value<KID>(xi, yi, bx[j], by[j], kp, ninv_p0, inv_a, tab,
                                              ln2_hi, ln2_lo);
             acc = fma(v, bq[j], acc);
-            atomicAdd(&bc[ca][j], v * qi);
+            atomicAdd(&bc[ca][j], kTransposeSign<KID> * v * qi);
           }
           if (diag && t < 32) {
             const int j = t + 32;
             const double v = pair_value<KID>(xi, yi, bx[j], by[j], kp, ninv_p0, inv_a, tab,
                                              ln2_hi, ln2_lo);
             acc = fma(v, bq[j], acc);
-            atomicAdd(&bc[ca][j], v * qi);
+            atomicAdd(&bc[ca][j], kTransposeSign<KID> * v * qi);
           }
           row[ca] = acc;
         }
@@ -862,6 +864,11 @@ __global__ __launch_bounds__(256) void k_near_out(const double* __restrict__ nea
     for (int t = 0; t < 9; t++) s += nblk[t * n + i];
     yout[perm[st + i]] = s;
   }
+}
+
+__global__ void k_add(double* __restrict__ y, const double* __restrict__ t, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += t[i];
 }
 
 __global__ void k_zero_ints(int* __restrict__ v, int n) {
@@ -1173,20 +1180,22 @@ void launch_m2l(Engine& e, int g, hipStream_t s) {
     if (p1 <= p0) continue;
     const PairDesc* pd = pd0 + p0;
     const int np = (int)(p1 - p0);
+    // K(second, first) = ts K(first, second)^T: -1 for an antisymmetric kernel
+    const double ts = e.kid == K_USER_ANTI ? -1.0 : 1.0;
     switch (c) {
-      case 0: k_m2l_reg<2, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 1: k_m2l_reg<2, 5><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 2: k_m2l_reg<2, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 3: k_m2l_reg<3, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 4: k_m2l_reg<3, 5><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 5: k_m2l_reg<3, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 6: k_m2l_reg<4, 8><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 7: k_m2l_stream<4, AFMM_SCT4><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 8: k_m2l_stream<6, AFMM_SCT6><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 9: k_m2l_stream<8, AFMM_SCT8><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 10: k_m2l_stream<10, AFMM_SCT10><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 11: k_m2l_stream<12, 1><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
-      case 12: k_m2l_stream<16, 1><<<np, 64, 0, s>>>(B, pd, outv, part, np); break;
+      case 0: k_m2l_reg<2, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
+      case 1: k_m2l_reg<2, 5><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
+      case 2: k_m2l_reg<2, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
+      case 3: k_m2l_reg<3, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
+      case 4: k_m2l_reg<3, 5><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
+      case 5: k_m2l_reg<3, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
+      case 6: k_m2l_reg<4, 8><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
+      case 7: k_m2l_stream<4, AFMM_SCT4><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
+      case 8: k_m2l_stream<6, AFMM_SCT6><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
+      case 9: k_m2l_stream<8, AFMM_SCT8><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
+      case 10: k_m2l_stream<10, AFMM_SCT10><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
+      case 11: k_m2l_stream<12, 1><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
+      case 12: k_m2l_stream<16, 1><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
       default: throw Error(6, "M2L block taller than 256 rows");
     }
     AFMM_LAUNCH_CHECK();
@@ -1366,6 +1375,11 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
   e.prof_end(s, b);
 }
 
+void add_device(double* d_y, const double* d_t, int n, hipStream_t s) {
+  k_add<<<ceil_div(n, 256), 256, 0, s>>>(d_y, d_t, n);
+  AFMM_LAUNCH_CHECK();
+}
+
 void near_field_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
   e.qs.alloc(sizeof(double) * e.n);
   gather_sorted(e, d_b, e.qs.as<double>(), s);
@@ -1387,7 +1401,7 @@ void direct_rows_device(Engine& e, int nrows, const int* d_rows, const double* d
   DevBuf part;
   part.alloc(sizeof(double) * (size_t)nrows * nchunk);
   dim3 grid(rb, nchunk);
-  AFMM_DISPATCH_KERNEL(e.kid, KID,
+  AFMM_DISPATCH_KERNEL(e.kid_full, KID,
                        k_direct<KID><<<grid, 256, 0, s>>>(e.x.as<double>(), e.y.as<double>(), d_qs,
                                                           d_rows, nrows, e.n, chunk,
                                                           part.as<double>(), e.kp));

@@ -39,10 +39,14 @@ typedef enum {
   AFMM2D_KERNEL_INV_R = 1,     /* 1/sqrt(R2), 0 when R2 == 0 */
   AFMM2D_KERNEL_GAUSSIAN = 2,  /* exp(-R2/kparams[0]), kparams[0] defaults to 0.01 */
   AFMM2D_KERNEL_REG_INV_R = 3, /* R < a ? R/a : a/R, a = 0.5^nLevels (testAFMM2D.cpp:9-20) */
-  AFMM2D_KERNEL_USER = 4       /* the user kernel compiled into this library: a header defining
+  AFMM2D_KERNEL_USER = 4,      /* the user kernel compiled into this library: a header defining
                                   afmm2d_user_kernel(xi, yi, xj, yj, kparams, a), chosen at build
                                   time (the reference's userkernel::getMatrixEntry, kernel.hpp:58);
-                                  default afmm2d_amd/user_kernels/anisotropic_log.h */
+                                  default afmm2d_amd/user_kernels/anisotropic_log.h.  May be
+                                  non-symmetric: create samples K(i,j) against K(j,i) and, if they
+                                  differ, runs the operator as the sum of the two below */
+  AFMM2D_KERNEL_USER_SYM = 5,  /* (K(x,y) + K(y,x)) / 2 of the user kernel */
+  AFMM2D_KERNEL_USER_ANTI = 6  /* (K(x,y) - K(y,x)) / 2 of the user kernel */
 } afmm2d_kernel_id;
 
 typedef enum {

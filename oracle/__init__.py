@@ -15,6 +15,7 @@ _LIB_PATH = os.path.join(_HERE, "libafmm2d_oracle.so")
 _lib = None
 
 KERNEL_LOG, KERNEL_INV_R, KERNEL_GAUSS, KERNEL_REG_INV_R, KERNEL_USER = 0, 1, 2, 3, 4
+KERNEL_USER_SYM, KERNEL_USER_ANTI = 5, 6
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)

@@ -63,9 +63,18 @@ double or_kernel_value(int kid, const double* kp, double a, double xi, double yi
       double r = sqrt(r2);
       return r < a ? r / a : a / r;
     }
-    case OR_KERNEL_USER: {
+    case OR_KERNEL_USER:
+    case OR_KERNEL_USER_SYM:
+    case OR_KERNEL_USER_ANTI: {
+      /* the user kernel, or its symmetric / antisymmetric part: the
+       * reference's algorithm assumes a symmetric kernel (AFMM2DTree.cpp:
+       * 1081-1082, 1151); K = S + A runs as one operator per part */
       static const double zero[4] = {0.0, 0.0, 0.0, 0.0};
-      return afmm2d_user_kernel(xi, yi, xj, yj, kp ? kp : zero, a);
+      const double* p = kp ? kp : zero;
+      const double kij = afmm2d_user_kernel(xi, yi, xj, yj, p, a);
+      if (kid == OR_KERNEL_USER) return kij;
+      const double kji = afmm2d_user_kernel(xj, yj, xi, yi, p, a);
+      return kid == OR_KERNEL_USER_SYM ? 0.5 * (kij + kji) : 0.5 * (kij - kji);
     }
   }
   return 0.0;

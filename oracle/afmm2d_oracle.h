@@ -19,7 +19,7 @@ extern "C" {
 #endif
 
 enum { OR_KERNEL_LOG = 0, OR_KERNEL_INV_R = 1, OR_KERNEL_GAUSS = 2, OR_KERNEL_REG_INV_R = 3,
-       OR_KERNEL_USER = 4 };
+       OR_KERNEL_USER = 4, OR_KERNEL_USER_SYM = 5, OR_KERNEL_USER_ANTI = 6 };
 
 typedef struct or_tree or_tree;
 

@@ -36,7 +36,9 @@ def test_status_strings(engine):
 
 @pytest.mark.parametrize("kernel,kid,kp", [
     ("log", 0, None), ("inv_r", 1, None), ("gaussian", 2, [0.02, 0, 0, 0]),
-    ("reg_inv_r", 3, None), ("user", 4, None), ("user", 4, [3.0, 0, 0, 0])])
+    ("reg_inv_r", 3, None), ("user", 4, None), ("user", 4, [3.0, 0, 0, 0]),
+    ("user", 4, [0.0, 0.25, 0, 0]), ("user_sym", 5, [0.0, 0.25, 0, 0]),
+    ("user_anti", 6, [0.0, 0.25, 0, 0])])
 def test_host_kernel_matches_oracle(engine, oracle_mod, kernel, kid, kp):
     rng = np.random.default_rng(kid)
     for _ in range(200):
@@ -114,4 +116,4 @@ def test_user_kernel_header_plugs_in(tmp_path):
     kp = (ctypes.c_double * 4)(0.5, 0.0, 0.0, 0.0)
     assert L.afmm2d_kernel_value(4, kp, 0.0, 0.1, 0.2, 0.4, -0.3, ctypes.byref(v)) == 0
     assert abs(v.value - np.sqrt(0.3 ** 2 + 0.5 ** 2 + 0.25)) <= 1e-15
-    assert L.afmm2d_kernel_value(5, None, 0.0, 0.1, 0.2, 0.4, -0.3, ctypes.byref(v)) == 1
+    assert L.afmm2d_kernel_value(7, None, 0.0, 0.1, 0.2, 0.4, -0.3, ctypes.byref(v)) == 1

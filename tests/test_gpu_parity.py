@@ -139,3 +139,42 @@ def test_user_kernel_parameters_reach_the_device(engine, oracle_mod):
     a.close()
     a2.close()
     T.close()
+
+
+@pytest.mark.parametrize("tol_pow", [8, 11])
+def test_nonsymmetric_user_kernel(engine, oracle_mod, tol_pow):
+    """A non-symmetric user kernel (the example with a dipole term) runs as
+    its symmetric and antisymmetric parts: each part against the oracle
+    (exact ranks, matvec to 1e-11), the whole operator against their sum,
+    the direct sum of the full kernel and the oracle's full near field."""
+    kp = [0.0, 0.25, 0.0, 0.0]
+    n = 12000
+    xy, q = uniform_points(n, 91 + tol_pow)
+    parts = np.zeros(n)
+    for name, kid in (("user_sym", 5), ("user_anti", 6)):
+        T = oracle_mod.Tree(xy, 64, tol_pow, kid, kparams=kp)
+        T.assemble()
+        a = engine.AFMM(n, 64, tol_pow, xy, kernel=name, kparams=kp)
+        a.assemble()
+        for j in range(2, T.L + 1):
+            np.testing.assert_array_equal(a.ranks(j), T.ranks(j))
+        y, yo = a.computeMatVecProduct(q), T.matvec(q)
+        assert np.linalg.norm(y - yo) <= 1e-11 * np.linalg.norm(yo), name
+        parts += yo
+        a.close()
+        T.close()
+    a = engine.AFMM(n, 64, tol_pow, xy, kernel="user", kparams=kp)
+    a.assemble()
+    y = a.computeMatVecProduct(q)
+    assert np.linalg.norm(y - parts) <= 1e-11 * np.linalg.norm(parts)
+    T4 = oracle_mod.Tree(xy, 64, tol_pow, 4, kparams=kp)
+    yd = a.direct(q)
+    rows = np.arange(0, n, 97)
+    np.testing.assert_allclose(yd[rows], T4.direct(q)[rows], rtol=0, atol=1e-13 * np.abs(yd).max())
+    err = np.linalg.norm(y - yd) / np.linalg.norm(yd)
+    assert err <= (1e-4 if tol_pow == 8 else 1e-8), err
+    nf, nf_ref = a.near_field(q), T4.near_field(q)
+    assert np.linalg.norm(nf - nf_ref) <= 1e-13 * np.linalg.norm(nf_ref)
+    assert a.getMatrixEntry(0, 1) != a.getMatrixEntry(1, 0)
+    a.close()
+    T4.close()

@@ -12,17 +12,19 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("kid,kp", [(0, None), (1, None), (3, None), (2, [0.05, 0, 0, 0]),
-                                    (4, None), (4, [0.5, 0, 0, 0])])
+                                    (4, None), (4, [0.5, 0, 0, 0]), (4, [0.5, 0.25, 0, 0])])
 def test_kernel_values(oracle_mod, kid, kp):
     rng = np.random.default_rng(0)
     a = 0.125
     for _ in range(50):
         xi, yi, xj, yj = rng.uniform(-1, 1, 4)
         r = np.hypot(xi - xj, yi - yj)
-        g = kp[0] if kp else 2.0  # the example user kernel: anisotropic log
+        # the example user kernel: anisotropic log plus a dipole term
+        g = kp[0] if kp else 2.0
+        b = kp[1] if kp else 0.0
         expect = {0: np.log(r), 1: 1 / r, 3: (r / a if r < a else a / r),
                   2: np.exp(-r * r / (kp[0] if kp else 0.01)),
-                  4: np.log(np.hypot(xi - xj, g * (yi - yj)))}[kid]
+                  4: np.log(np.hypot(xi - xj, g * (yi - yj))) + b * (xi - xj) / r ** 2}[kid]
         got = oracle_mod.kernel_value(kid, xi, yi, xj, yj, a=a, kparams=kp)
         assert abs(got - expect) <= 1e-14 * max(1.0, abs(expect))
     # diagonal values
@@ -112,12 +114,13 @@ def test_invalid_levels(oracle_mod):
         oracle_mod.Tree(xy, 64, 8)
 
 
-@pytest.mark.parametrize("kid", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("kid", [0, 1, 2, 3, 4, 6])
 def test_near_field_brute_force(oracle_mod, kid):
     """The oracle's near-field term (AFMM2DTree.cpp:1199-1213) against a
     numpy brute force over each leaf's own points and its 8 neighbours'."""
     xy, q = uniform_points(2000, 21)
-    T = oracle_mod.Tree(xy, 30, 8, kid)
+    kp = [0.0, 0.25, 0.0, 0.0] if kid == 6 else None  # dipole part of the user kernel
+    T = oracle_mod.Tree(xy, 30, 8, kid, kparams=kp)
     L = T.L
     assert L == 3
     sxy, perm = T.sorted_xy(), T.perm()
@@ -139,7 +142,8 @@ def test_near_field_brute_force(oracle_mod, kid):
                      1: np.where(r2 > 0, 1.0 / r, 0.0),
                      2: np.exp(-r2 / 0.01),
                      3: np.where(r < a, r / a, a / r),
-                     4: np.where(r2g > 0, 0.5 * np.log(r2g), 0.0)}[kid]
+                     4: np.where(r2g > 0, 0.5 * np.log(r2g), 0.0),
+                     6: np.where(r2 > 0, 0.25 * d[..., 0] / r2, 0.0)}[kid]
             ys[st:st + n] += K @ qs[sb:sb + m]
     y = np.empty_like(ys)
     y[perm] = ys
@@ -157,4 +161,48 @@ def test_user_kernel_operator(oracle_mod):
         T.assemble()
         errs.append(rel(T.matvec(q), T.direct(q)))
         T.close()
+    assert errs[1] < errs[0] and errs[1] <= 1e-7, errs
+
+
+def test_user_kernel_parts(oracle_mod):
+    """Kernel ids 5 and 6: the symmetric and antisymmetric parts of the user
+    kernel, exactly (anti)symmetric and summing to it."""
+    rng = np.random.default_rng(3)
+    kp = [0.0, 0.25, 0.0, 0.0]
+    for _ in range(100):
+        xi, yi, xj, yj = rng.uniform(-1, 1, 4)
+        k = oracle_mod.kernel_value(4, xi, yi, xj, yj, kparams=kp)
+        kt = oracle_mod.kernel_value(4, xj, yj, xi, yi, kparams=kp)
+        s_ij = oracle_mod.kernel_value(5, xi, yi, xj, yj, kparams=kp)
+        a_ij = oracle_mod.kernel_value(6, xi, yi, xj, yj, kparams=kp)
+        assert s_ij == oracle_mod.kernel_value(5, xj, yj, xi, yi, kparams=kp)
+        assert a_ij == -oracle_mod.kernel_value(6, xj, yj, xi, yi, kparams=kp)
+        assert abs(s_ij + a_ij - k) <= 1e-15 * max(1.0, abs(k))
+        assert abs(a_ij - 0.25 * (xi - xj) / ((xi - xj) ** 2 + (yi - yj) ** 2)) <= \
+            1e-12 * max(1.0, abs(a_ij))
+        assert k != kt
+
+
+def test_nonsymmetric_operator_as_two_parts(oracle_mod):
+    """A non-symmetric kernel through the reference's algorithm, one operator
+    per part: the sum approaches the direct sum of the full kernel as the
+    tolerance falls (the antisymmetric part uses the same L2P^T multipole
+    basis, since A(far, box) = -A(box, far)^T)."""
+    kp = [0.0, 0.25, 0.0, 0.0]
+    xy, q = uniform_points(4096, 37)
+    T4 = oracle_mod.Tree(xy, 64, 6, 4, kparams=kp)
+    yd = T4.direct(q)
+    errs = []
+    for tp in (6, 10):
+        y = np.zeros_like(q)
+        for kid in (5, 6):
+            T = oracle_mod.Tree(xy, 64, tp, kid, kparams=kp)
+            T.assemble()
+            y += T.matvec(q)
+            T.close()
+        errs.append(rel(y, yd))
+    # the near field of the full kernel is the two parts' near fields
+    ns = oracle_mod.Tree(xy, 64, 6, 5, kparams=kp).near_field(q)
+    na = oracle_mod.Tree(xy, 64, 6, 6, kparams=kp).near_field(q)
+    assert rel(ns + na, T4.near_field(q)) <= 1e-14
     assert errs[1] < errs[0] and errs[1] <= 1e-7, errs
