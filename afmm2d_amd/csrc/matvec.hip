@@ -631,8 +631,11 @@ __device__ __forceinline__ double near_kernel(double r2, const KParams& kp, doub
                                               double inv_a, const double2* tab, double ln2_hi,
                                               double ln2_lo) {
   if constexpr (KID == K_LOG) {
-    // 2 x the log kernel; the factor 1/2 is applied to the sums
-    return r2 == 0.0 ? 0.0 : log_tang(r2, tab, ln2_hi, ln2_lo);
+    // 2 x the log kernel; the factor 1/2 is applied to the sums.  The log is
+    // taken of max(r2, tiny) and the diagonal value selected after, so no
+    // branch splits the wave
+    const double lg = log_tang(fmax(r2, 1e-300), tab, ln2_hi, ln2_lo);
+    return r2 == 0.0 ? 0.0 : lg;
   } else if constexpr (KID == K_INV_R) {
     const double rs = rsqrt_refined(r2);
     return r2 == 0.0 ? 0.0 : rs;
@@ -693,8 +696,12 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
                                              double ln2_hi, double ln2_lo,
                                              int* __restrict__ queue) {
   __shared__ double2 tab[128];
-  // one address per step for each array, 8-byte lane stride: no bank conflicts
-  __shared__ double bx[64], by[64], bq[64], bc[NCH][64];
+  // one address per step for each array, 8-byte lane stride: no bank
+  // conflicts.  The staged chunk is stored twice (entries t and t + 64), so
+  // source t + s of lane t needs no wrap: each lane's addresses are its base
+  // plus an immediate offset.  The B-side sums of source j collect in
+  // entries j and j + 64 of a row and are added when the chunk is done.
+  __shared__ double bx[128], by[128], bq[128], bc[NCH][128];
   __shared__ int grab;
   const int t = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if constexpr (KID == K_LOG) {
@@ -753,13 +760,21 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
       if (w == 0) {
         const int j = 64 * cb + t;
         const bool v = j < nb;
-        bx[t] = v ? x[sb + j] : pbx;
-        by[t] = v ? y[sb + j] : pby;
-        bq[t] = v ? qs[sb + j] : 0.0;
+        const double vx = v ? x[sb + j] : pbx, vy = v ? y[sb + j] : pby;
+        const double vq = v ? qs[sb + j] : 0.0;
+        bx[t] = vx;
+        bx[t + 64] = vx;
+        by[t] = vy;
+        by[t + 64] = vy;
+        bq[t] = vq;
+        bq[t + 64] = vq;
       }
 #pragma unroll
       for (int ca = 0; ca < NCH; ca++)
-        if (ca % nw == w) bc[ca][t] = 0.0;
+        if (ca % nw == w) {
+          bc[ca][t] = 0.0;
+          bc[ca][t + 64] = 0.0;
+        }
       __syncthreads();
 #pragma unroll
       for (int cp = 0; cp < NCH; cp += 2) {
@@ -772,9 +787,9 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
           double acc0 = row[cp], acc1 = row[cp + 1];
           double* bc0 = bc[cp];
           double* bc1 = bc[cp + 1];
-#pragma unroll 2
+#pragma unroll 8
           for (int s = 0; s < 64; s++) {
-            const int j = (t + s) & 63;
+            const int j = t + s;
             const double sx = bx[j], sy = by[j], sq = bq[j];
             const double v0 = pair_value<KID>(x0, y0, sx, sy, kp, ninv_p0, inv_a, tab, ln2_hi,
                                               ln2_lo);
@@ -807,7 +822,7 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
           const int s_end = diag ? 32 : 64;
 #pragma unroll 2
           for (; s < s_end; s++) {
-            const int j = (t + s) & 63;
+            const int j = t + s;
             const double v = pair_value<KID>(xi, yi, bx[j], by[j], kp, ninv_p0, inv_a, tab,
                                              ln2_hi, ln2_lo);
             acc = fma(v, bq[j], acc);
@@ -828,7 +843,7 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
       double colsum = 0.0;
 #pragma unroll
       for (int ca = 0; ca < NCH; ca++)
-        if (ca < nca) colsum += bc[ca][t];
+        if (ca < nca) colsum += bc[ca][t] + bc[ca][t + 64];
       if (self) {
 #pragma unroll
         for (int ca = 0; ca < NCH; ca++)
