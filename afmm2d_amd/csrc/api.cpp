@@ -2,6 +2,7 @@
 // interaction lists, and the host side of assemble/matvec.  The numerical
 // work runs in the HIP kernels of kd_sort.hip, assemble.hip and matvec.hip;
 // this file only moves data and builds index tables.
+#include <algorithm>
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -192,8 +193,63 @@ int near_waves_per_simd(const Engine& e) {
   const double t_near = evals / 9.0e11;
   const double t_m2l = (8.0 * e.m2l_total + 16.0 * e.m2l_cols) / 5.0e12;
   if (t_m2l <= 0.0) return 8;
-  const double w = 5.0 * t_near / t_m2l;
-  return std::max(2, std::min(8, (int)std::lround(w)));
+  return std::max(2, std::min(8, (int)std::lround(5.0 * t_near / t_m2l)));
+}
+
+// The near-field wave count measured on the assembled operator: matvecs
+// back to back with each candidate (the estimate above, 2, 4 and 8 waves per
+// SIMD; the better of two batches of 4 after 2 warm-ups), keeping the
+// fastest.  The estimate cannot see
+// every interaction between the two: the near field's persistent workgroups
+// keep their registers until its queue drains, and with M2L blocks of rank
+// above 64 (~200-VGPR kernels) the M2L beside it may find no room (N = 1e6,
+// leaf 100, a/R, tolerance 1e-12: 1.90 ms at 8 waves, 1.66 ms at 2), while a
+// near field that dominates needs its waves (N = 1e5: 0.55 ms at 4, 0.91 at
+// 2).  Results do not depend on the count.
+int tune_near_waves(Engine& e, int estimate) {
+  std::vector<int> cand = {estimate, 2, 4, 8};
+  std::sort(cand.begin(), cand.end());
+  cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+  const size_t nb = sizeof(double) * e.n;
+  e.io_b.alloc(nb);
+  e.io_y.alloc(nb);
+  {
+    std::vector<double> q(e.n);
+    std::mt19937_64 rng(7);
+    for (double& v : q) v = (double)(rng() >> 11) * 0x1.0p-53 - 0.5;
+    AFMM_HIP(hipMemcpy(e.io_b.p, q.data(), nb, hipMemcpyHostToDevice));
+  }
+  hipStream_t s = nullptr;
+  hipEvent_t a, b;
+  AFMM_HIP(hipEventCreate(&a));
+  AFMM_HIP(hipEventCreate(&b));
+  const int saved_prof = e.profiling;
+  e.profiling = 0;
+  int best = estimate;
+  float best_ms = 1e30f;
+  for (int c : cand) {
+    e.p2p_waves_per_simd = c;
+    for (int i = 0; i < 2; i++) matvec_device(e, e.io_b.as<double>(), e.io_y.as<double>(), s);
+    // back to back, as a caller's loop runs them: the better of two batches
+    float t = 1e30f;
+    for (int rep = 0; rep < 2; rep++) {
+      AFMM_HIP(hipEventRecord(a, s));
+      for (int i = 0; i < 4; i++) matvec_device(e, e.io_b.as<double>(), e.io_y.as<double>(), s);
+      AFMM_HIP(hipEventRecord(b, s));
+      AFMM_HIP(hipEventSynchronize(b));
+      float x = 0.f;
+      AFMM_HIP(hipEventElapsedTime(&x, a, b));
+      t = std::min(t, x / 4);
+    }
+    if (t < best_ms * 0.99f || (c == estimate && t <= best_ms * 1.01f)) {
+      best_ms = t;
+      best = c;
+    }
+  }
+  e.profiling = saved_prof;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return best;
 }
 
 }  // namespace afmm
@@ -297,6 +353,7 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
     if (const char* v = std::getenv("AFMM2D_NEAR_OVERLAP")) e->near_overlap = std::atoi(v) != 0;
     if (const char* v = std::getenv("AFMM2D_TREE_MODE"))
       e->tree_mode = std::max(0, std::min(3, std::atoi(v)));
+    if (const char* v = std::getenv("AFMM2D_NEAR_TUNE")) e->near_tune = std::atoi(v) != 0;
     if (const char* v = std::getenv("AFMM2D_P2P_WAVES")) {
       e->p2p_waves_per_simd = std::max(0, std::min(16, std::atoi(v)));
       e->p2p_waves_fixed = true;
@@ -451,7 +508,10 @@ int afmm2d_assemble(afmm2d_handle h) {
       double t0 = now_s();
       assemble_tree(e);
       e.assembled = true;
-      if (!e.p2p_waves_fixed) e.p2p_waves_per_simd = near_waves_per_simd(e);
+      if (!e.p2p_waves_fixed) {
+        e.p2p_waves_per_simd = near_waves_per_simd(e);
+        if (e.near_tune) e.p2p_waves_per_simd = tune_near_waves(e, e.p2p_waves_per_simd);
+      }
       e.assemble_seconds = now_s() - t0;
     }
   });

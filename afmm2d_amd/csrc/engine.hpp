@@ -175,13 +175,15 @@ struct Engine {
   // schedule options (environment, read at create):
   //   AFMM2D_NEAR_OVERLAP=0   near field on the main stream after M2L
   //   AFMM2D_P2P_WAVES=n      near-field waves per SIMD (persistent grid;
-  //                           default: near_waves_per_simd at assembly)
+  //                           default: timed at assembly, tune_near_waves)
+  //   AFMM2D_NEAR_TUNE=0      the estimate near_waves_per_simd, untimed
   //   AFMM2D_TREE_MODE=m      which upper-tree passes run beside the
   //                           leaf-level M2L (matvec_device's schedules)
   bool near_overlap = true;
   int tree_mode = 0;
   int p2p_waves_per_simd = 4;   // set at assembly from the problem's sizes
   bool p2p_waves_fixed = false;  // AFMM2D_P2P_WAVES given
+  bool near_tune = true;         // AFMM2D_NEAR_TUNE=0: the estimate, untimed
   // FP64 log table for the near field: (1/c_j, -log(1/c_j)), c_j = 1 + (j + 1/2)/128
   DevBuf log_tab;
   double ln2_hi = 0.0, ln2_lo = 0.0;
@@ -216,6 +218,7 @@ void assemble_tree(Engine& e);
 
 // api.cpp
 int near_waves_per_simd(const Engine& e);
+int tune_near_waves(Engine& e, int estimate);
 
 // matvec.hip
 void matvec_device(Engine& e, const double* d_b, double* d_y, hipStream_t s);
