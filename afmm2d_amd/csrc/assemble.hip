@@ -19,8 +19,11 @@
 // ID (AFMM2DTree.cpp:1042-1060): L2P = Ac D^{-1} with D = K(skeleton rows,
 // skeleton columns), through LU with partial pivoting of D^T.
 //
-// M2L (AFMM2DTree.cpp:1070-1097): K(skeleton_k, skeleton_kIL) for every
-// ordered pair, written per target box as one column-major matrix.
+// M2L (AFMM2DTree.cpp:1070-1097): the reference builds K(skeleton_k,
+// skeleton_kIL) for every ordered pair; here each unordered pair is built
+// once, B = K(skeleton_first, skeleton_second) column-major, in the
+// orientation whose register-tile class is cheaper, and the matvec applies
+// B and +-B^T (the kernel is symmetric, or the antisymmetric part of one).
 #include <algorithm>
 #include <cfloat>
 #include <numeric>

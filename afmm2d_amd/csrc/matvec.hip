@@ -7,16 +7,19 @@
 // (potential += K(leaf, neighbour) q_neighbour for 8 neighbours and self),
 // scatter back to caller order.
 //
-// Stream plan (matvec_device): after the leaf-level M2M, the near field and
-// the upper tree (levels < L) run on two internal streams beside the
-// HBM-bound leaf-level M2L on the caller's stream; the last kernel joins
-// them (leaf L2P + M2L sums + near field + scatter, fused).
+// Stream plan (matvec_device, default schedule): the M2M of every level, the
+// M2L, the L2L chain and the leaf pass run in order on the caller's stream;
+// after the upward pass the near field forks onto the handle's internal
+// stream and runs beside the M2L and the L2L chain, and the leaf pass (leaf
+// L2P + M2L sums + near field + scatter, fused) joins it.  Other schedules
+// are options (see matvec_device).
 //
-// M2L reads each symmetric block once: the pair kernel holds
-// B = K(skel_k, skel_b) in registers (16-byte loads, all issued up front) and
-// writes B out_b and B^T out_k as partials; the downward pass adds each box's
-// partials in its interaction-slot order, so results do not depend on
-// scheduling.
+// M2L reads each stored block once: a pair kernel holds B = K(skel_k,
+// skel_b) in registers (the whole block up to 64 x 64, larger ones streamed
+// in column chunks; 16-byte loads, issued before use) and writes B out_b and
+// +-B^T out_k (the sign of an antisymmetric kernel part) as partials; the
+// downward pass adds each box's partials in its interaction-slot order, so
+// results do not depend on scheduling.
 #include "engine.hpp"
 
 namespace afmm {
