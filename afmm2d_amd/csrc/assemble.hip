@@ -8,7 +8,8 @@
 // children 4k..4k+3 are consecutive and skeletons are stored box after box,
 // every box's rows are one contiguous range of the level's source points.
 //
-// ACA (ACA.cpp:27-133): one workgroup per box, the rank loop is sequential,
+// ACA (ACA.cpp:27-133): one workgroup per box (256 threads, 1024 on levels
+// with few boxes), the rank loop is sequential,
 // each step's row/column evaluation, residual update, argmax and norm
 // reductions are parallel over the workgroup.  Pivot choice and stopping rule
 // follow the reference exactly (first maximum, |prev - normS| >= tol*prev,
@@ -99,8 +100,8 @@ struct AcaArgs {
   KParams kp;
 };
 
-template <int KID>
-__global__ __launch_bounds__(ACA_THREADS) void k_aca(AcaArgs a) {
+template <int KID, int NT>
+__global__ __launch_bounds__(NT) void k_aca(AcaArgs a) {
 #pragma clang fp contract(off)
   __shared__ double coefU[MAXR];
   __shared__ double coefV[MAXR];
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(AcaArgs a) {
   __shared__ double dV[MAXR + 1];
   __shared__ int il_start[40], il_pref[41];
   __shared__ int nslot;
-  __shared__ Cand s4[4];
+  __shared__ Cand s4[NT / 64];
   __shared__ double s_pt[4];
   __shared__ double s_norm[2];
 
@@ -140,8 +141,8 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(AcaArgs a) {
     nslot = c;
   }
   __syncthreads();
-  // column points, each owned by thread (j mod 256)
-  for (int j = tid; j < n2; j += ACA_THREADS) {
+  // column points, each owned by thread (j mod NT)
+  for (int j = tid; j < n2; j += NT) {
     int s = 0;
     while (s + 1 < nslot && il_pref[s + 1] <= j) s++;
     int p = il_start[s] + (j - il_pref[s]);
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(AcaArgs a) {
   for (int l = 0; l < n1; l++) {
     const double px = a.src_x[r0 + l], py = a.src_y[r0 + l];
     Cand c{0.0, 0.0, -1};
-    for (int j = tid, m = 0; j < n2; j += ACA_THREADS, m++) {
+    for (int j = tid, m = 0; j < n2; j += NT, m++) {
       double v = kernel_xy<KID>(px, py, cx[j], cy[j], a.kp);
       V[j] = v;
       c = cbetter(c, Cand{fabs(v), v, j});
@@ -173,13 +174,13 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(AcaArgs a) {
     if (tid == 0) a.rank[b] = 0;
     return;
   }
-  if (tid == (jstar & (ACA_THREADS - 1))) {
+  if (tid == (jstar & (NT - 1))) {
     s_pt[0] = cx[jstar];
     s_pt[1] = cy[jstar];
-    cmask |= 1ull << (jstar / ACA_THREADS);
+    cmask |= 1ull << (jstar / NT);
   }
-  if (tid == (istar & (ACA_THREADS - 1))) rmask |= 1ull << (istar / ACA_THREADS);
-  for (int j = tid; j < n2; j += ACA_THREADS) V[j] = V[j] / piv;
+  if (tid == (istar & (NT - 1))) rmask |= 1ull << (istar / NT);
+  for (int j = tid; j < n2; j += NT) V[j] = V[j] / piv;
   __syncthreads();
   double cjx = s_pt[0], cjy = s_pt[1];
   if (tid == 0) {
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(AcaArgs a) {
     a.pcy[po] = cjy;
   }
   Cand cr{0.0, 0.0, -1};
-  for (int i = tid, m = 0; i < n1; i += ACA_THREADS, m++) {
+  for (int i = tid, m = 0; i < n1; i += NT, m++) {
     double u = kernel_xy<KID>(a.src_x[r0 + i], a.src_y[r0 + i], cjx, cjy, a.kp);
     U[i] = u;
     if (!((rmask >> m) & 1)) cr = cbetter(cr, Cand{fabs(u), u, i});
@@ -200,13 +201,13 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(AcaArgs a) {
 
   while (fabs(prev - normS) >= a.tol * prev && rank < mn && istar >= 0) {
     const int t = rank;
-    for (int l = tid; l < t; l += ACA_THREADS) coefU[l] = U[(int64_t)l * n1 + istar];
+    for (int l = tid; l < t; l += NT) coefU[l] = U[(int64_t)l * n1 + istar];
     __syncthreads();
     // Phase A: residual row of pivot row istar.
     const double px = a.src_x[r0 + istar], py = a.src_y[r0 + istar];
     double* Vt = V + (int64_t)t * n2;
     Cand c{0.0, 0.0, -1};
-    for (int j = tid, m = 0; j < n2; j += ACA_THREADS, m++) {
+    for (int j = tid, m = 0; j < n2; j += NT, m++) {
       double r = kernel_xy<KID>(px, py, cx[j], cy[j], a.kp);
       for (int l = 0; l < t; l++) r -= coefU[l] * V[(int64_t)l * n2 + j];
       Vt[j] = r;
@@ -216,13 +217,13 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(AcaArgs a) {
     jstar = rc.i;
     piv = rc.v;
     if (jstar < 0 || piv == 0.0) break;  // a zero pivot ends the ACA
-    for (int l = tid; l < t; l += ACA_THREADS) coefV[l] = V[(int64_t)l * n2 + jstar];
-    if (tid == (jstar & (ACA_THREADS - 1))) {
+    for (int l = tid; l < t; l += NT) coefV[l] = V[(int64_t)l * n2 + jstar];
+    if (tid == (jstar & (NT - 1))) {
       s_pt[0] = cx[jstar];
       s_pt[1] = cy[jstar];
-      cmask |= 1ull << (jstar / ACA_THREADS);
+      cmask |= 1ull << (jstar / NT);
     }
-    if (tid == (istar & (ACA_THREADS - 1))) rmask |= 1ull << (istar / ACA_THREADS);
+    if (tid == (istar & (NT - 1))) rmask |= 1ull << (istar / NT);
     __syncthreads();
     cjx = s_pt[0];
     cjy = s_pt[1];
@@ -232,10 +233,10 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(AcaArgs a) {
       a.pcy[po + t] = cjy;
     }
     // Phase B: v = row / pivot; residual column u.
-    for (int j = tid; j < n2; j += ACA_THREADS) Vt[j] = Vt[j] / piv;
+    for (int j = tid; j < n2; j += NT) Vt[j] = Vt[j] / piv;
     double* Ut = U + (int64_t)t * n1;
     Cand cu{0.0, 0.0, -1};
-    for (int i = tid, m = 0; i < n1; i += ACA_THREADS, m++) {
+    for (int i = tid, m = 0; i < n1; i += NT, m++) {
       double u = kernel_xy<KID>(a.src_x[r0 + i], a.src_y[r0 + i], cjx, cjy, a.kp);
       for (int l = 0; l < t; l++) u -= coefV[l] * U[(int64_t)l * n1 + i];
       Ut[i] = u;
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(AcaArgs a) {
     // Phase C: U_l.u and V_l.v for l = 0..t.
     {
       const int w = tid >> 6, lane = tid & 63;
-      for (int l = w; l <= t; l += ACA_THREADS / 64) {
+      for (int l = w; l <= t; l += NT / 64) {
         double su = 0.0, sv = 0.0;
         const double* Ul = U + (int64_t)l * n1;
         const double* Vl = V + (int64_t)l * n2;
@@ -479,6 +480,11 @@ void assemble_tree(Engine& e) {
         if (il[40 * k + q] >= 0) c += n1[il[40 * k + q]];
       n2[k] = c;
     }
+    // Levels with few boxes (large ones) give each box a workgroup of 1024
+    // threads; the ranks and pivots are the same for either size (every
+    // value is computed alike, the argmax is order-free and each dot product
+    // is one wave's).
+    const int aca_nt = nb < 2 * e.num_cus ? 1024 : ACA_THREADS;
     std::vector<int64_t> pivoff(nb + 1, 0);
     int maxn = 0;
     for (int k = 0; k < nb; k++) {
@@ -488,8 +494,7 @@ void assemble_tree(Engine& e) {
       if (mn > MAXR)
         throw Error(6, "ACA block " + std::to_string(n1[k]) + "x" + std::to_string(n2[k]) +
                            " exceeds the supported rank capacity");
-      if ((n1[k] + ACA_THREADS - 1) / ACA_THREADS > MAXOWN ||
-          (n2[k] + ACA_THREADS - 1) / ACA_THREADS > MAXOWN)
+      if ((n1[k] + aca_nt - 1) / aca_nt > MAXOWN || (n2[k] + aca_nt - 1) / aca_nt > MAXOWN)
         throw Error(6, "ACA block too large for one workgroup");
     }
     upload(d_rstart, rstart, s);
@@ -503,8 +508,12 @@ void assemble_tree(Engine& e) {
     d_pcx.alloc(sizeof(double) * npiv);
     d_pcy.alloc(sizeof(double) * npiv);
 
-    // ACA in batches bounded by a scratch budget.
-    const size_t budget = std::max<size_t>((size_t)1 << 30, free_device_bytes() / 4);
+    // ACA in batches bounded by a scratch budget of at most 16 GiB, one
+    // buffer reused by every batch and level: a level's whole scratch at once
+    // (24-55 GB at N = 1e6) made assembly take 4-5 s instead of ~1 s in some
+    // runs; 4 GiB batches left too few boxes per launch at the leaf level.
+    const size_t budget = std::min<size_t>((size_t)16 << 30,
+                                           std::max<size_t>((size_t)1 << 30, free_device_bytes() / 4));
     int k0 = 0;
     while (k0 < nb) {
       std::vector<int> boxes;
@@ -528,8 +537,12 @@ void assemble_tree(Engine& e) {
                 d_boxes.as<int>(), d_scroff.as<int64_t>(), d_scr.as<double>(),
                 d_pivoff.as<int64_t>(), d_rank.as<int>(), d_prow.as<int>(), d_pcx.as<double>(),
                 d_pcy.as<double>(), e.tol, e.kp};
-      AFMM_DISPATCH_KERNEL(e.kid, KID,
-                           k_aca<KID><<<(unsigned)boxes.size(), ACA_THREADS, 0, s>>>(a));
+      if (aca_nt == 1024) {
+        AFMM_DISPATCH_KERNEL(e.kid, KID, k_aca<KID, 1024><<<(unsigned)boxes.size(), 1024, 0, s>>>(a));
+      } else {
+        AFMM_DISPATCH_KERNEL(e.kid, KID,
+                             k_aca<KID, ACA_THREADS><<<(unsigned)boxes.size(), ACA_THREADS, 0, s>>>(a));
+      }
       AFMM_LAUNCH_CHECK();
     }
     li.rank.resize(nb);
