@@ -635,11 +635,22 @@ int afmm2d_get_l2p(afmm2d_handle h, int level, int box, double* out, int64_t cap
     const LevelInfo& li = e.lv[level];
     *rows = li.nin[box];
     *cols = li.rank[box];
-    int64_t m = (int64_t)*rows * *cols;
+    const int n = *rows, r = *cols, nc = n - r;
+    int64_t m = (int64_t)n * r;
     if (m > capacity || (!out && m > 0)) throw Error(AFMM2D_ERR_INVALID_ARGUMENT, "capacity too small");
-    if (m > 0)
-      AFMM_HIP(hipMemcpy(out, e.l2p_rm.as<double>() + li.l2poff[box], sizeof(double) * m,
+    if (m == 0) return;
+    // stored rows plus the identity on the skeleton rows, in box row order
+    std::vector<double> st((size_t)nc * r);
+    std::vector<int> rmap(n);
+    const int64_t roff = level == e.L ? e.leaf_start[box] : e.n + e.lv[level + 1].soff[4 * (size_t)box];
+    if (nc > 0)
+      AFMM_HIP(hipMemcpy(st.data(), e.l2p_rm.as<double>() + li.l2poff[box], sizeof(double) * st.size(),
                          hipMemcpyDeviceToHost));
+    AFMM_HIP(hipMemcpy(rmap.data(), e.rowmap.as<int>() + roff, sizeof(int) * n, hipMemcpyDeviceToHost));
+    std::fill(out, out + m, 0.0);
+    for (int t = 0; t < nc; t++)
+      std::copy(st.begin() + (size_t)t * r, st.begin() + (size_t)(t + 1) * r, out + (int64_t)rmap[t] * r);
+    for (int c = 0; c < r; c++) out[(int64_t)rmap[nc + c] * r + c] = 1.0;
   });
 }
 
@@ -812,19 +823,20 @@ int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st) {
   st->near_pairs = e.near_pairs;
   st->m2l_bytes = 8 * e.m2l_total;
   st->m2l_columns = e.m2l_cols;
-  st->l2p_bytes = 8 * e.l2p_total;
+  st->l2p_bytes = 8 * e.l2p_total;  // stored rows (the skeleton rows are the identity)
   // One matvec reads q (gather) and the coordinates (near field), L2P twice
   // (M2M and L2L), every stored M2L coefficient and its column index once,
   // the multipole/local vectors, and writes y.
   // One matvec reads b and perm (gather), writes and rereads q in KD order,
   // reads the coordinates (near field), writes and rereads the near-field
-  // potentials, reads perm again and writes y; it reads L2P twice (M2M, L2L),
-  // every stored M2L block once, writes and rereads the M2L partials, and
-  // touches the multipole/local vectors a few times.
+  // potentials, reads perm again and writes y; it reads the stored L2P rows
+  // twice (M2M, L2L) with their row maps, every stored M2L block once,
+  // writes and rereads the M2L partials, and touches the multipole/local
+  // vectors a few times.
   st->matvec_bytes = e.assembled
-                         ? (int64_t)e.n * (8 + 4 + 8 + 8 + 16 + 8 + 8 + 4 + 8) +
+                         ? (int64_t)e.n * (8 + 4 + 8 + 8 + 16 + 8 + 8 + 4 + 8 + 2 * 4) +
                                2 * 8 * e.l2p_total + 8 * e.m2l_total + 16 * e.m2l_cols +
-                               4 * e.il_pairs + 6 * 8 * e.skel_total
+                               4 * e.il_pairs + (6 * 8 + 2 * 4) * e.skel_total
                          : 0;
   st->create_seconds = e.create_seconds;
   st->assemble_seconds = e.assemble_seconds;

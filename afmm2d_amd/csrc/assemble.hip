@@ -18,7 +18,10 @@
 // residuals round as the CPU oracle's do.
 //
 // ID (AFMM2DTree.cpp:1042-1060): L2P = Ac D^{-1} with D = K(skeleton rows,
-// skeleton columns), through LU with partial pivoting of D^T.
+// skeleton columns), through LU with partial pivoting of D^T.  Row skel_q of
+// Ac is row q of D, so row skel_q of L2P is e_q (the LU solve returns it up to
+// rounding amplified by cond(D)); only the other rows are solved and stored,
+// and the skeleton rows are applied as the identity.
 //
 // M2L (AFMM2DTree.cpp:1070-1097): the reference builds K(skeleton_k,
 // skeleton_kIL) for every ordered pair; here each unordered pair is built
@@ -293,6 +296,7 @@ struct IdArgs {
   const int64_t* l2p_off;  // per box, local level offset
   double* l2p_rm;
   double* l2p_cm;
+  int* rowmap;             // per row of the level's sources (rstart-indexed)
   double* mscr;            // global LU scratch when rank > RLDS
   const int64_t* m_off;
   int use_lds;
@@ -307,9 +311,13 @@ __global__ __launch_bounds__(256) void k_id(IdArgs a) {
   __shared__ int s_p;
   const int b = blockIdx.x, tid = threadIdx.x;
   const int r = a.rank[b];
-  if (r <= 0) return;
   const int n1 = a.n1[b];
   const int r0 = a.rstart[b];
+  int* rmap = a.rowmap + r0;
+  if (r <= 0) {
+    for (int i = tid; i < n1; i += blockDim.x) rmap[i] = i;
+    return;
+  }
   const int64_t po = a.piv_off[b];
   double* M = a.use_lds ? lds : a.mscr + a.m_off[b];
   // M = D^T: M[i][q] = K(skeleton row q, skeleton column i)
@@ -352,8 +360,22 @@ __global__ __launch_bounds__(256) void k_id(IdArgs a) {
   }
   double* rm = a.l2p_rm + a.l2p_off[b];
   double* cm = a.l2p_cm + a.l2p_off[b];
+  const int nc = n1 - r;  // stored rows
   for (int i = tid; i < n1; i += blockDim.x) {
-    double* x = rm + (int64_t)i * r;
+    // skeleton row q goes to position nc + q; the others keep their order
+    int sq = -1, below = 0;
+    for (int u = 0; u < r; u++) {
+      const int pr = a.prow[po + u];
+      sq = pr == i ? u : sq;
+      below += pr < i;
+    }
+    if (sq >= 0) {
+      rmap[nc + sq] = i;
+      continue;
+    }
+    const int t = i - below;
+    rmap[t] = i;
+    double* x = rm + (int64_t)t * r;
     const double xi = a.src_x[r0 + i], yi = a.src_y[r0 + i];
     for (int q = 0; q < r; q++) x[q] = kernel_xy<KID>(xi, yi, a.pcx[po + q], a.pcy[po + q], a.kp);
     for (int k = 0; k < r; k++) {
@@ -374,7 +396,7 @@ __global__ __launch_bounds__(256) void k_id(IdArgs a) {
       for (int q = k + 1; q < r; q++) s = s - M[k * r + q] * x[q];
       x[k] = s / M[k * r + k];
     }
-    for (int q = 0; q < r; q++) cm[i + (int64_t)q * n1] = x[q];
+    for (int q = 0; q < r; q++) cm[t + (int64_t)q * nc] = x[q];
   }
 }
 
@@ -439,7 +461,9 @@ void assemble_tree(Engine& e) {
   hipStream_t s = e.s_aux;
   e.lv.assign(L + 1, LevelInfo{});
   // per-level skeleton points, kept until the global arrays are built
-  std::vector<DevBuf> lskx(L + 2), lsky(L + 2), lskg(L + 2), lrm(L + 1), lcm(L + 1);
+  std::vector<DevBuf> lskx(L + 2), lsky(L + 2), lskg(L + 2), lrm(L + 1), lcm(L + 1),
+      lrow(L + 1);
+  std::vector<int64_t> nrows(L + 1, 0);  // source rows of each level
   std::vector<std::vector<int64_t>> sloc(L + 2);
   DevBuf d_rstart, d_n1, d_n2, d_il, d_boxes, d_scroff, d_scr, d_pivoff, d_rank, d_prow, d_pcx,
       d_pcy, d_sloc, d_l2poff, d_moff, d_mscr;
@@ -557,7 +581,7 @@ void assemble_tree(Engine& e) {
       int r = li.rank[k];
       maxr = std::max(maxr, r);
       sloc[j][k + 1] = sloc[j][k] + r;
-      l2poff[k + 1] = l2poff[k] + (int64_t)r * n1[k];
+      l2poff[k + 1] = l2poff[k] + (int64_t)r * (n1[k] - r);
       moff[k + 1] = moff[k] + (int64_t)r * r;
     }
     e.max_rank = std::max(e.max_rank, maxr);
@@ -575,6 +599,9 @@ void assemble_tree(Engine& e) {
     lrm[j].alloc(sizeof(double) * std::max<int64_t>(l2poff[nb], 1));
     lcm[j].alloc(sizeof(double) * std::max<int64_t>(l2poff[nb], 1));
     upload(d_l2poff, l2poff, s);
+    // a leaf's points start at leaf_start (KD order, not box order)
+    nrows[j] = j == L ? (int64_t)e.n : sloc[j + 1][4 * (size_t)nb];
+    lrow[j].alloc(sizeof(int) * std::max<int64_t>(nrows[j], 1));
     const int use_lds = maxr <= RLDS;
     if (!use_lds) {
       upload(d_moff, moff, s);
@@ -583,7 +610,7 @@ void assemble_tree(Engine& e) {
     if (maxr > MAXR) throw Error(6, "rank exceeds the supported capacity");
     IdArgs ia{src_x, src_y, d_rstart.as<int>(), d_n1.as<int>(), d_rank.as<int>(),
               d_pivoff.as<int64_t>(), d_prow.as<int>(), d_pcx.as<double>(), d_pcy.as<double>(),
-              d_l2poff.as<int64_t>(), lrm[j].as<double>(), lcm[j].as<double>(),
+              d_l2poff.as<int64_t>(), lrm[j].as<double>(), lcm[j].as<double>(), lrow[j].as<int>(),
               use_lds ? nullptr : d_mscr.as<double>(), use_lds ? nullptr : d_moff.as<int64_t>(),
               use_lds, e.kp};
     const size_t shm = use_lds ? sizeof(double) * (size_t)maxr * maxr : 0;
@@ -612,6 +639,15 @@ void assemble_tree(Engine& e) {
   e.skidx.alloc(sizeof(int) * std::max<int64_t>(sb, 1));
   e.l2p_rm.alloc(sizeof(double) * std::max<int64_t>(lb, 1));
   e.l2p_cm.alloc(sizeof(double) * std::max<int64_t>(lb, 1));
+  e.rowmap.alloc(sizeof(int) * (size_t)(e.n + std::max<int64_t>(sb, 1)));
+  for (int j = 2; j <= L; j++) {
+    // the leaves' rows are the points; an upper level's rows are the
+    // skeletons of the level below
+    const int64_t rbase = j == L ? 0 : e.n + e.lv[j + 1].soff[0];
+    if (nrows[j] > 0)
+      AFMM_HIP(hipMemcpyAsync(e.rowmap.as<int>() + rbase, lrow[j].p, sizeof(int) * nrows[j],
+                              hipMemcpyDeviceToDevice, s));
+  }
   for (int j = 2; j <= L; j++) {
     LevelInfo& li = e.lv[j];
     int64_t ns = li.soff[li.nb] - li.soff[0];
@@ -635,7 +671,7 @@ void assemble_tree(Engine& e) {
   for (int j = 0; j <= L + 1; j++) {
     lskx[j].release(); lsky[j].release(); lskg[j].release();
   }
-  for (int j = 0; j <= L; j++) { lrm[j].release(); lcm[j].release(); }
+  for (int j = 0; j <= L; j++) { lrm[j].release(); lcm[j].release(); lrow[j].release(); }
   d_scr.release();
   d_mscr.release();
 
@@ -646,10 +682,12 @@ void assemble_tree(Engine& e) {
     LevelInfo& li = e.lv[j];
     li.max_nin = 0;
     li.max_rank = 0;
+    li.max_stored = 0;
     for (int k = 0; k < li.nb; k++) {
       int64_t g = li.gbase + k;
       li.max_nin = std::max(li.max_nin, li.nin[k]);
       li.max_rank = std::max(li.max_rank, li.rank[k]);
+      li.max_stored = std::max(li.max_stored, li.nin[k] - li.rank[k]);
       rank_g[g] = li.rank[k];
       nin_g[g] = li.nin[k];
       soff_g[g] = li.soff[k];

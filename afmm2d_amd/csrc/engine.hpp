@@ -6,8 +6,15 @@
 //   perm          KD-sorted index -> caller index (N int32)
 //   skx, sky      skeleton points of every box, concatenated in (level, box)
 //                 order; box gid owns [soff[gid], soff[gid] + rank[gid])
-//   l2p_rm        L2P of every box, rows x rank row-major   (read by M2M)
+//   l2p_rm        L2P of every box without its skeleton rows (those are the
+//                 identity: row skel_q of L2P is e_q), (nin - rank) x rank
+//                 row-major, the other rows in ascending order  (read by M2M)
 //   l2p_cm        the same matrices column-major             (read by L2L)
+//   rowmap        row of each stored L2P row: box rows t < nin - rank are the
+//                 stored rows, rows nin - rank + q the skeleton rows in
+//                 skeleton order; a leaf's rows (its points) at
+//                 leaf_start, an upper box's rows (its children's skeletons)
+//                 at N + the global skeleton offset of its first child
 //   m2l           one block per unordered interaction pair (k, b), k < b:
 //                 B = K(skeleton_k, skeleton_b), rank_k x rank_b column-major
 //                 with leading dimension ld = rank_k rounded up to even (a
@@ -66,6 +73,7 @@ struct LevelInfo {
   std::vector<int64_t> soff;    // global skeleton offset per box (nb + 1 entries)
   std::vector<int64_t> l2poff;  // element offset into l2p_* (nb + 1)
   int max_nin = 0, max_rank = 0;
+  int max_stored = 0;           // largest nin - rank (stored L2P rows)
   // this level's M2L coefficients (rank_k rank_b per pair) and partial
   // columns (rank_k + rank_b per pair)
   int64_t m2l_coeffs = 0, m2l_cols = 0;
@@ -157,7 +165,7 @@ struct Engine {
   int max_rank = 0;
   DevBuf skx, sky, skidx;
   DevBuf rank_g, nin_g, soff_g, l2poff_g;
-  DevBuf l2p_rm, l2p_cm;
+  DevBuf l2p_rm, l2p_cm, rowmap;
   // M2L, symmetric pair storage
   int64_t npairs = 0;
   int pair_max_ld = 0, pair_max_rb = 0;   // largest block shape

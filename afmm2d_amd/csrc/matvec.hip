@@ -138,17 +138,21 @@ __device__ __forceinline__ void local_coeffs(double* xs, int r, const double* __
   }
 }
 
-// M2M for one level: out_k = L2P_k^T src_k with row-major L2P (nin x rank).
-// Leaf level (b != nullptr): src_k is the leaf's charges, gathered here from
-// the caller's order (b[perm[.]]) and also written to qs in KD order for the
-// near field; upper levels: the children's concatenated multipoles (always
-// contiguous).  One workgroup of WPB waves per box.
+// M2M for one level: out_k = L2P_k^T src_k with the stored rows of L2P
+// (row-major, (nin - rank) x rank) and the identity on its skeleton rows:
+// out_k[c] = sum_t P[t][c] src[rowmap[t]] + src[rowmap[nc + c]], nc = nin -
+// rank.  src is staged in LDS in rowmap order.  Leaf level (b != nullptr):
+// src_k is the leaf's charges, gathered here from the caller's order
+// (b[perm[.]]) and also written to qs in KD order for the near field; upper
+// levels: the children's concatenated multipoles (always contiguous).  One
+// workgroup of WPB waves per box.
 template <int WPB, int B>
 __global__ __launch_bounds__(64 * WPB) void k_up(const double* __restrict__ l2p_rm,
                                                  const int64_t* __restrict__ l2poff,
                                                  const int* __restrict__ rank,
                                                  const int* __restrict__ nin,
                                                  const int64_t* __restrict__ soff,
+                                                 const int* __restrict__ rowmap,
                                                  const double* __restrict__ src,
                                                  const int64_t* __restrict__ src_off_child,
                                                  const double* __restrict__ b,
@@ -165,36 +169,45 @@ __global__ __launch_bounds__(64 * WPB) void k_up(const double* __restrict__ l2p_
   const int64_t g = gbase + k;
   const int r = rank[g];
   const int n = nin[g];
+  const int nc = n - r;
   const double* P = l2p_rm + l2poff[g];
   double* out = outv + soff[g];
   const int w = threadIdx.x >> 6;
   double pv[B];
   GemvLane q;
-  if (r > 0) {
-    q = gemv_lane(r, 0, n);
-    gemv_load<B>(pv, P, n, r, 0, q, w, WPB);
+  if (r > 0 && nc > 0) {
+    q = gemv_lane(r, 0, nc);
+    gemv_load<B>(pv, P, nc, r, 0, q, w, WPB);
   }
   if (b) {
     const int st = leaf_start[k];
-    for (int i = threadIdx.x; i < n; i += 64 * WPB) {
+    const int* rm = rowmap + st;
+    for (int t = threadIdx.x; t < n; t += 64 * WPB) {
+      const int i = rm[t];
       const double v = b[perm[st + i]];
-      xs[i] = v;
+      xs[t] = v;
       qs[st + i] = v;
     }
   } else {
-    const double* x = src + src_off_child[4 * k];
-    for (int i = threadIdx.x; i < n; i += 64 * WPB) xs[i] = x[i];
+    const int64_t so = src_off_child[4 * k];
+    const double* x = src + so;
+    const int* rm = rowmap + so;
+    for (int t = threadIdx.x; t < n; t += 64 * WPB) xs[t] = x[rm[t]];
   }
   if (r == 0) return;
   __syncthreads();
+  if (nc == 0) {
+    for (int c = threadIdx.x; c < r; c += 64 * WPB) out[c] = xs[c];
+    return;
+  }
   for (int cb = 0; cb < r; cb += 64) {
     if (cb > 0) {
-      q = gemv_lane(r, cb, n);
-      gemv_load<B>(pv, P, n, r, cb, q, w, WPB);
+      q = gemv_lane(r, cb, nc);
+      gemv_load<B>(pv, P, nc, r, cb, q, w, WPB);
     }
-    const double acc = gemv_chunk<B>(pv, P, n, r, cb, q, xs, w, WPB);
+    const double acc = gemv_chunk<B>(pv, P, nc, r, cb, q, xs, w, WPB);
     const double s = reduce_groups<WPB>(acc, red, q);
-    if ((int)threadIdx.x < q.rc) out[cb + threadIdx.x] = s;
+    if ((int)threadIdx.x < q.rc) out[cb + threadIdx.x] = s + xs[nc + cb + threadIdx.x];
   }
 }
 
@@ -229,21 +242,23 @@ __device__ __forceinline__ double half_wave_transpose_sum(double (&p)[1 << S]) {
   return p[0];
 }
 
-// M2M for one level on the column-major L2P (transposed GEMV): out[c] =
-// sum_i P[c n + i] x[i].  Half-wave `half` of wave w owns column 2 j + half
-// of the column pairs j = w + WPB u (u < 2^S); its 32 lanes own the rows
-// 64 kc + li and 64 kc + 32 + li of each 64-row chunk kc < NCH, so every
-// load instruction reads 256 contiguous bytes per half-wave with all lanes
-// active (the row-major kernel leaves lanes idle when the rank does not
-// divide 64).  All loads are issued before the first use; each lane's
-// per-column partial sums are then combined over the 32 lanes.  Upper levels
-// only: x is the children's concatenated multipoles.
+// M2M for one level on the column-major stored L2P rows (transposed GEMV):
+// out[c] = sum_t P[c nc + t] x[rowmap[t]] + x[rowmap[nc + c]], nc = nin -
+// rank.  Half-wave `half` of wave w owns column 2 j + half of the column
+// pairs j = w + WPB u (u < 2^S); its 32 lanes own the rows 64 kc + li and
+// 64 kc + 32 + li of each 64-row chunk kc < NCH, so every load instruction
+// reads 256 contiguous bytes per half-wave with all lanes active (the
+// row-major kernel leaves lanes idle when the rank does not divide 64).  All
+// loads are issued before the first use; each lane's per-column partial sums
+// are then combined over the 32 lanes.  Upper levels only: x is the
+// children's concatenated multipoles.
 template <int S, int NCH, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_up_t(const double* __restrict__ l2p_cm,
                                                    const int64_t* __restrict__ l2poff,
                                                    const int* __restrict__ rank,
                                                    const int* __restrict__ nin,
                                                    const int64_t* __restrict__ soff,
+                                                   const int* __restrict__ rowmap,
                                                    const double* __restrict__ src,
                                                    const int64_t* __restrict__ src_off_child,
                                                    double* __restrict__ outv, int64_t gbase,
@@ -254,35 +269,46 @@ __global__ __launch_bounds__(64 * WPB) void k_up_t(const double* __restrict__ l2
   if (k >= nb) return;
   const int64_t g = gbase + k;
   const int r = rank[g];
-  const int n = nin[g];
+  const int nc = nin[g] - r;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, half = l >> 5, li = l & 31;
   const double* P = l2p_cm + l2poff[g];
   // matrix loads first (they need only the box's scalars)
   double a[NV][NCH][2];
-  if (r > 0) {
+  if (r > 0 && nc > 0) {
 #pragma unroll
     for (int u = 0; u < NV; u++) {
       const int c = 2 * (w + WPB * u) + half;
-      const int64_t cb = (int64_t)(c < r ? c : r - 1) * n;
+      const int64_t cb = (int64_t)(c < r ? c : r - 1) * nc;
 #pragma unroll
       for (int kc = 0; kc < NCH; kc++)
 #pragma unroll
         for (int e = 0; e < 2; e++) {
           const int i = 64 * kc + 32 * e + li;
-          a[u][kc][e] = P[cb + (i < n ? i : n - 1)];
+          a[u][kc][e] = P[cb + (i < nc ? i : nc - 1)];
         }
     }
+  } else {
+#pragma unroll
+    for (int u = 0; u < NV; u++)
+#pragma unroll
+      for (int kc = 0; kc < NCH; kc++) a[u][kc][0] = a[u][kc][1] = 0.0;
   }
   if (r == 0) return;
   double xv[NCH][2];
-  const double* x = src + src_off_child[4 * k];
+  const int64_t so = src_off_child[4 * k];
+  const double* x = src + so;
+  const int* rm = rowmap + so;
 #pragma unroll
   for (int kc = 0; kc < NCH; kc++)
 #pragma unroll
     for (int e = 0; e < 2; e++) {
       const int i = 64 * kc + 32 * e + li;
-      xv[kc][e] = i < n ? x[i] : 0.0;
+      xv[kc][e] = i < nc ? x[rm[i]] : 0.0;
     }
+  // the skeleton-row term of the column this lane writes (if it writes one)
+  const bool writer = (li & ((32 >> S) - 1)) == 0;
+  const int cw = 2 * (w + WPB * (li >> (5 - S))) + half;
+  const double xsk = writer && cw < r ? x[rm[nc + cw]] : 0.0;
   double p[NV];
 #pragma unroll
   for (int u = 0; u < NV; u++) {
@@ -294,10 +320,7 @@ __global__ __launch_bounds__(64 * WPB) void k_up_t(const double* __restrict__ l2
     p[u] = 2 * (w + WPB * u) + half < r ? s : 0.0;
   }
   const double sum = half_wave_transpose_sum<S>(p);
-  if ((li & ((32 >> S) - 1)) == 0) {
-    const int c = 2 * (w + WPB * (li >> (5 - S))) + half;
-    if (c < r) outv[soff[g] + c] = sum;
-  }
+  if (writer && cw < r) outv[soff[g] + cw] = sum + xsk;
 }
 
 // Wave-level sums through DPP (data-parallel lane moves) on the two 32-bit
@@ -526,8 +549,10 @@ __global__ __launch_bounds__(64) void k_m2l_stream(const double* __restrict__ m2
 
 // L2L for a non-leaf level, with the box's M2L sum fused in: in_k =
 // parent contribution + M2L partials (local_coeffs), then the children's
-// parent contribution = L2P_k in_k (column-major L2P, i.e. row-major
-// rank x nin; the children's coefficients are contiguous).
+// parent contribution = L2P_k in_k: stored row t (column-major stored rows,
+// i.e. row-major rank x nc) goes to child coefficient rowmap[t], skeleton row
+// q (the identity) to rowmap[nc + q].  The children's coefficients are
+// contiguous.
 template <int WPB, int B>
 __global__ __launch_bounds__(64 * WPB) void k_down(const double* __restrict__ l2p_cm,
                                                    const int64_t* __restrict__ l2poff,
@@ -535,6 +560,7 @@ __global__ __launch_bounds__(64 * WPB) void k_down(const double* __restrict__ l2
                                                    const int* __restrict__ nin,
                                                    const int64_t* __restrict__ soff,
                                                    const int64_t* __restrict__ child_soff,
+                                                   const int* __restrict__ rowmap,
                                                    const double* __restrict__ part,
                                                    double* __restrict__ inv, int has_parent,
                                                    int64_t gbase, int nb) {
@@ -547,44 +573,52 @@ __global__ __launch_bounds__(64 * WPB) void k_down(const double* __restrict__ l2
   const int r = rank[g];
   const int n = nin[g];
   if (n == 0) return;
+  const int nc = n - r;
   const double* P = l2p_cm + l2poff[g];
-  double* dst = inv + child_soff[4 * k];
+  const int64_t co = child_soff[4 * k];
+  double* dst = inv + co;
+  const int* rm = rowmap + co;
   const int w = threadIdx.x >> 6;
   double pv[B];
-  const int nch = (n + 63) >> 6;
-  if (nch > 1 && nch <= WPB) {
-    // the column chunks in parallel: wave w takes chunk w % nch and every
-    // nsub-th step of it (the waves w' = w mod nch); one reduction at the end
-    const int ch = w % nch, sub = w / nch, nsub = (WPB - 1 - ch) / nch + 1;
-    const GemvLane q = gemv_lane(n, 64 * ch, r);
-    gemv_load<B>(pv, P, r, n, 64 * ch, q, sub, nsub);
+  const int nch = (nc + 63) >> 6;
+  if (nch <= 1 || nch > WPB) {
+    GemvLane q;
+    if (nc > 0) {
+      q = gemv_lane(nc, 0, r);
+      gemv_load<B>(pv, P, r, nc, 0, q, w, WPB);
+    }
     local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part + kILSlots * soff[g]);
     __syncthreads();
-    red[threadIdx.x] = gemv_chunk<B>(pv, P, r, n, 64 * ch, q, xs, sub, nsub);
-    __syncthreads();
-    const int i = threadIdx.x;
-    if (i < n) {
-      const int ci = i >> 6, c = i & 63;
-      const int rc = min(64, n - 64 * ci), G = 64 / rc, ns = (WPB - 1 - ci) / nch + 1;
-      double acc = 0.0;
-      for (int u = 0; u < ns; u++)
-        for (int h = 0; h < G; h++) acc += red[(ci + nch * u) * 64 + h * rc + c];
-      dst[i] = acc;
+    for (int c = threadIdx.x; c < r; c += 64 * WPB) dst[rm[nc + c]] = xs[c];
+    for (int cb = 0; cb < nc; cb += 64) {
+      if (cb > 0) {
+        q = gemv_lane(nc, cb, r);
+        gemv_load<B>(pv, P, r, nc, cb, q, w, WPB);
+      }
+      const double acc = gemv_chunk<B>(pv, P, r, nc, cb, q, xs, w, WPB);
+      const double s = reduce_groups<WPB>(acc, red, q);
+      if ((int)threadIdx.x < q.rc) dst[rm[cb + threadIdx.x]] = s;
     }
     return;
   }
-  GemvLane q = gemv_lane(n, 0, r);
-  gemv_load<B>(pv, P, r, n, 0, q, w, WPB);
+  // the column chunks in parallel: wave w takes chunk w % nch and every
+  // nsub-th step of it (the waves w' = w mod nch); one reduction at the end
+  const int ch = w % nch, sub = w / nch, nsub = (WPB - 1 - ch) / nch + 1;
+  const GemvLane q = gemv_lane(nc, 64 * ch, r);
+  gemv_load<B>(pv, P, r, nc, 64 * ch, q, sub, nsub);
   local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part + kILSlots * soff[g]);
   __syncthreads();
-  for (int cb = 0; cb < n; cb += 64) {
-    if (cb > 0) {
-      q = gemv_lane(n, cb, r);
-      gemv_load<B>(pv, P, r, n, cb, q, w, WPB);
-    }
-    const double acc = gemv_chunk<B>(pv, P, r, n, cb, q, xs, w, WPB);
-    const double s = reduce_groups<WPB>(acc, red, q);
-    if ((int)threadIdx.x < q.rc) dst[cb + threadIdx.x] = s;
+  for (int c = threadIdx.x; c < r; c += 64 * WPB) dst[rm[nc + c]] = xs[c];
+  red[threadIdx.x] = gemv_chunk<B>(pv, P, r, nc, 64 * ch, q, xs, sub, nsub);
+  __syncthreads();
+  const int i = threadIdx.x;
+  if (i < nc) {
+    const int ci = i >> 6, c = i & 63;
+    const int rc = min(64, nc - 64 * ci), G = 64 / rc, ns = (WPB - 1 - ci) / nch + 1;
+    double acc = 0.0;
+    for (int u = 0; u < ns; u++)
+      for (int h = 0; h < G; h++) acc += red[(ci + nch * u) * 64 + h * rc + c];
+    dst[rm[i]] = acc;
   }
 }
 
@@ -897,6 +931,8 @@ __global__ void k_zero_ints(int* __restrict__ v, int n) {
 // Leaf level, fused: in_k (parent contribution + M2L partials), potential
 // = L2P_k in_k + the leaf's 9 near-field partial slots (self, then
 // neighbours 0..7; absent ones are zero), scattered to the caller's order.
+// Stored L2P row t (column-major stored rows) is point rowmap[t]; skeleton
+// point rowmap[nc + q] takes in_k[q] (the identity row).
 template <int WPB, int B>
 __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2p_cm,
                                                    const int64_t* __restrict__ l2poff,
@@ -904,6 +940,7 @@ __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2
                                                    const int64_t* __restrict__ soff,
                                                    const int* __restrict__ leaf_start,
                                                    const int* __restrict__ leaf_count,
+                                                   const int* __restrict__ rowmap,
                                                    const double* __restrict__ part,
                                                    const double* __restrict__ inv, int has_parent,
                                                    const double* __restrict__ near_part,
@@ -918,33 +955,41 @@ __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2
   const int r = rank[g];
   const int st = leaf_start[k], n = leaf_count[k];
   if (n == 0) return;
+  const int nc = n - r;
   const double* P = l2p_cm + l2poff[g];
   const double* nblk = near_part + 9 * (int64_t)st;  // 9 slots of n values
+  const int* rm = rowmap + st;
   const int w = threadIdx.x >> 6;
   double pv[B];
-  GemvLane q = gemv_lane(n, 0, r);
-  gemv_load<B>(pv, P, r, n, 0, q, w, WPB);
-  // the first chunk's near-field values and destinations, loaded early
+  GemvLane q;
+  if (nc > 0) {
+    q = gemv_lane(nc, 0, r);
+    gemv_load<B>(pv, P, r, nc, 0, q, w, WPB);
+  }
+  // the near-field values and destinations of this thread's first row (stored
+  // rows first, then the skeleton rows), loaded early
   double nv[9];
   int dst_i = 0;
   {
-    const int i = min((int)threadIdx.x, n - 1);
+    const int i = rm[min((int)threadIdx.x, n - 1)];
 #pragma unroll
     for (int t = 0; t < 9; t++) nv[t] = nblk[t * n + i];
     dst_i = perm[st + i];
   }
   local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part + kILSlots * soff[g]);
   __syncthreads();
-  for (int cb = 0; cb < n; cb += 64) {
+  int row = threadIdx.x;  // the row whose nv/dst_i this thread holds
+  for (int cb = 0; cb < nc; cb += 64) {
     if (cb > 0) {
-      q = gemv_lane(n, cb, r);
-      gemv_load<B>(pv, P, r, n, cb, q, w, WPB);
-      const int i = min(cb + (int)threadIdx.x, n - 1);
+      q = gemv_lane(nc, cb, r);
+      gemv_load<B>(pv, P, r, nc, cb, q, w, WPB);
+      row = cb + threadIdx.x;
+      const int i = rm[min(row, n - 1)];
 #pragma unroll
       for (int t = 0; t < 9; t++) nv[t] = nblk[t * n + i];
       dst_i = perm[st + i];
     }
-    const double acc = gemv_chunk<B>(pv, P, r, n, cb, q, xs, w, WPB);
+    const double acc = gemv_chunk<B>(pv, P, r, nc, cb, q, xs, w, WPB);
     const double s = reduce_groups<WPB>(acc, red, q);
     if ((int)threadIdx.x < q.rc) {
       double nf = 0.0;
@@ -952,6 +997,22 @@ __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2
       for (int t = 0; t < 9; t++) nf += nv[t];
       yout[dst_i] = s + nf;
     }
+  }
+  // skeleton rows: thread u takes the rows v >= nc with v = u mod 64 WPB;
+  // the values it already holds are reused when their row is v
+  int v0 = threadIdx.x;
+  while (v0 < nc) v0 += 64 * WPB;
+  for (int v = v0; v < n; v += 64 * WPB) {
+    if (v != row) {
+      const int i = rm[v];
+#pragma unroll
+      for (int t = 0; t < 9; t++) nv[t] = nblk[t * n + i];
+      dst_i = perm[st + i];
+    }
+    double nf = 0.0;
+#pragma unroll
+    for (int t = 0; t < 9; t++) nf += nv[t];
+    yout[dst_i] = xs[v - nc] + nf;
   }
 }
 
@@ -1081,10 +1142,10 @@ namespace {
 // row chunks of 64,
 // WPB waves per box, 2^S column pairs per wave; at most 32 matrix loads per
 // lane in flight (2^S NCH <= 16).  False if the level needs the row-major
-// kernel (ranks above 64 or more than 256 rows).
+// kernel (ranks above 64 or more than 256 stored rows).
 bool up_t_shape(const LevelInfo& li, int& S, int& NCH, int& WPB) {
-  if (!AFMM_UP_T || li.max_rank < 1 || li.max_rank > 64 || li.max_nin > 256) return false;
-  NCH = ceil_div(std::max(1, li.max_nin), 64);
+  if (!AFMM_UP_T || li.max_rank < 1 || li.max_rank > 64 || li.max_stored > 256) return false;
+  NCH = ceil_div(std::max(1, li.max_stored), 64);
   const int max_s = NCH == 1 ? 4 : NCH == 2 ? 3 : 2;
   const int V = (li.max_rank + 1) / 2;
   for (int w : {2, 4, 8}) {
@@ -1107,7 +1168,8 @@ void launch_up_t_wpb(Engine& e, int j, int WPB, hipStream_t s) {
   const int64_t* soff = e.soff_g.as<int64_t>();
 #define AFMM_UP_T_ARGS                                                                          \
   e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(), e.nin_g.as<int>(), soff, \
-      e.outv.as<double>(), soff + e.lv[j + 1].gbase, e.outv.as<double>(), li.gbase, li.nb
+      e.rowmap.as<int>() + e.n, e.outv.as<double>(), soff + e.lv[j + 1].gbase,                 \
+      e.outv.as<double>(), li.gbase, li.nb
   switch (WPB) {
     case 2: k_up_t<S, NCH, 2><<<li.nb, 128, 0, s>>>(AFMM_UP_T_ARGS); break;
     case 4: k_up_t<S, NCH, 4><<<li.nb, 256, 0, s>>>(AFMM_UP_T_ARGS); break;
@@ -1153,13 +1215,14 @@ void launch_up(Engine& e, int j, const double* d_b, hipStream_t s) {
                     if (leaf) {
                       k_up<WPB, AFMM_B_UP_LEAF><<<li.nb, 64 * WPB, shm, s>>>(
                           e.l2p_rm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
-                          e.nin_g.as<int>(), soff, nullptr, nullptr, d_b, e.perm.as<int>(),
+                          e.nin_g.as<int>(), soff, e.rowmap.as<int>(), nullptr, nullptr, d_b,
+                          e.perm.as<int>(),
                           e.d_leaf_start.as<int>(), e.qs.as<double>(), e.outv.as<double>(),
                           li.gbase, li.nb);
                     } else {
                       k_up<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
                           e.l2p_rm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
-                          e.nin_g.as<int>(), soff, e.outv.as<double>(),
+                          e.nin_g.as<int>(), soff, e.rowmap.as<int>() + e.n, e.outv.as<double>(),
                           soff + e.lv[j + 1].gbase, nullptr, nullptr, nullptr, nullptr,
                           e.outv.as<double>(), li.gbase, li.nb);
                     });
@@ -1232,13 +1295,13 @@ void launch_down(Engine& e, int j, hipStream_t s) {
                       k_down<WPB, AFMM_B_DOWN_LAST><<<li.nb, 64 * WPB, shm, s>>>(
                           e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
                           e.nin_g.as<int>(), soff, soff + e.lv[j + 1].gbase,
-                          e.m2l_part.as<double>(), e.inv.as<double>(), j > 2 ? 1 : 0,
+                          e.rowmap.as<int>() + e.n, e.m2l_part.as<double>(), e.inv.as<double>(), j > 2 ? 1 : 0,
                           li.gbase, li.nb);
                     } else {
                       k_down<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
                           e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
                           e.nin_g.as<int>(), soff, soff + e.lv[j + 1].gbase,
-                          e.m2l_part.as<double>(), e.inv.as<double>(), j > 2 ? 1 : 0,
+                          e.rowmap.as<int>() + e.n, e.m2l_part.as<double>(), e.inv.as<double>(), j > 2 ? 1 : 0,
                           li.gbase, li.nb);
                     });
   AFMM_LAUNCH_CHECK();
@@ -1289,8 +1352,8 @@ void launch_leaf(Engine& e, double* d_y, hipStream_t s) {
                     k_leaf<WPB, AFMM_B_LEAF><<<li.nb, 64 * WPB, shm, s>>>(
                         e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
                         e.soff_g.as<int64_t>(), e.d_leaf_start.as<int>(),
-                        e.d_leaf_count.as<int>(), e.m2l_part.as<double>(), e.inv.as<double>(),
-                        L > 2 ? 1 : 0, e.near_part.as<double>(), e.perm.as<int>(), d_y,
+                        e.d_leaf_count.as<int>(), e.rowmap.as<int>(), e.m2l_part.as<double>(),
+                        e.inv.as<double>(), L > 2 ? 1 : 0, e.near_part.as<double>(), e.perm.as<int>(), d_y,
                         li.gbase, li.nb));
   AFMM_LAUNCH_CHECK();
 }
