@@ -54,6 +54,7 @@ def lib():
         L.or_matvec.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.or_near_field.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.or_set_parallel_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_set_exact_skeleton_rows.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_levels.restype = ctypes.c_int
         L.or_levels.argtypes = [ctypes.c_void_p]
         L.or_kernel_a.restype = ctypes.c_double
@@ -176,6 +177,10 @@ class Tree:
 
     def set_parallel_mode(self, mode):
         lib().or_set_parallel_mode(self.h, mode)
+
+    def set_exact_skeleton_rows(self, on=True):
+        """Skeleton rows of L2P as the exact identity (the product's form)."""
+        lib().or_set_exact_skeleton_rows(self.h, 1 if on else 0)
 
     def assemble(self):
         lib().or_assemble(self.h)

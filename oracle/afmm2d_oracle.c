@@ -24,6 +24,10 @@
  *   - a box of rank 0 contributes no far field (the reference aborts);
  *   - R2 == 0 gives the kernel's diagonal value (the reference evaluates
  *     log(0) for distinct coincident points).
+ * Product-only deviation, switchable here (or_set_exact_skeleton_rows): the
+ * skeleton rows of L2P are exactly the identity rows the LU solve returns up
+ * to rounding; pinned by the matvec agreeing with the solved rows to
+ * rounding (tests/test_oracle_matvec.py).
  */
 #include "afmm2d_oracle.h"
 
@@ -499,6 +503,7 @@ struct or_tree {
   double* ps;
   int assembled;
   int par_mode;
+  int exact_skel_rows;
 };
 
 int or_levels(const or_tree* t) { return t->L; }
@@ -554,6 +559,7 @@ or_tree* or_create(int n, int leaf, int tol_pow, const double* xy, int kid, cons
 }
 
 void or_set_parallel_mode(or_tree* t, int mode) { t->par_mode = mode; }
+void or_set_exact_skeleton_rows(or_tree* t, int on) { t->exact_skel_rows = on; }
 
 static void free_box(or_box* b) {
   free(b->rows); free(b->skel); free(b->l2p);
@@ -654,6 +660,11 @@ static void nodes_box(or_tree* t, int j, long k) {
         for (int q = 0; q < r; q++) Ac[(size_t)i * r + q] = t->entry(&t->k, b->rows[i], sc[q]);
       b->l2p = (double*)malloc(sizeof(double) * (size_t)n1 * r);
       l2p_solve(r, D, n1, Ac, b->l2p);
+      /* Row rp[q] of Ac is row q of D, so row rp[q] of L2P is e_q up to the
+       * solve's rounding; the product applies it as the identity. */
+      if (t->exact_skel_rows)
+        for (int q = 0; q < r; q++)
+          for (int c = 0; c < r; c++) b->l2p[(size_t)rp[q] * r + c] = q == c ? 1.0 : 0.0;
       free(D); free(Ac); free(sc);
     }
     free(U); free(V); free(row); free(ur); free(uc); free(rp); free(cp);

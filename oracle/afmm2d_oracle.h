@@ -66,6 +66,10 @@ void or_near_field(const or_tree* t, const double* q, double* y);
 /* 0: OpenMP placement of the reference (M2L parallel over levels only);
  * 1: parallel over boxes everywhere.  Results are identical. */
 void or_set_parallel_mode(or_tree* t, int mode);
+/* 1: store the skeleton rows of every L2P as the exact identity rows (as the
+ * product applies them) instead of the LU solve's result; set before
+ * or_assemble. */
+void or_set_exact_skeleton_rows(or_tree* t, int on);
 
 /* Accessors. */
 int or_levels(const or_tree* t);

@@ -108,6 +108,46 @@ def test_parallel_modes_agree(oracle_mod):
     np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("kid,tol_pow", [(0, 12), (0, 8), (2, 10), (3, 6)])
+def test_exact_skeleton_rows_change_the_matvec_by_rounding_only(oracle_mod, kid, tol_pow):
+    # The product stores L2P without its skeleton rows and applies them as the
+    # identity: row skel_q of Ac D^-1 is e_q in exact arithmetic.  The solved
+    # rows differ from e_q by rounding amplified by cond(D); the operator must
+    # not notice.
+    xy, q = uniform_points(20000, 40 + kid)
+    T = oracle_mod.Tree(xy, 64, tol_pow, kid)
+    T.assemble()
+    X = oracle_mod.Tree(xy, 64, tol_pow, kid)
+    X.set_exact_skeleton_rows()
+    X.assemble()
+    for j in (2, T.L):
+        for box in (0, 4 ** j // 2, 4 ** j - 1):
+            P, Px = T.l2p(j, box), X.l2p(j, box)
+            sk = T.skeleton(j, box)
+            np.testing.assert_array_equal(sk, X.skeleton(j, box))
+            if not P.size:
+                continue
+            # the rows of the box that are skeleton rows (as indices of the
+            # box's own rows): leaf rows are its points, upper rows are the
+            # children's skeletons in child order
+            if j == T.L:
+                st, cnt = T.leaf_range(box)
+                rows = np.arange(st, st + cnt)
+            else:
+                rows = np.concatenate([T.skeleton(j + 1, 4 * box + c) for c in range(4)])
+            pos = np.array([int(np.nonzero(rows == g)[0][0]) for g in sk])
+            np.testing.assert_array_equal(Px[pos], np.eye(len(sk)))
+            other = np.setdiff1d(np.arange(P.shape[0]), pos)
+            np.testing.assert_array_equal(Px[other], P[other])
+            assert np.abs(P[pos] - np.eye(len(sk))).max() < 1e-1
+    y, yx, yd = T.matvec(q), X.matvec(q), T.direct(q)
+    assert np.linalg.norm(yx - y) <= 1e-12 * np.linalg.norm(y)
+    e, ex = (np.linalg.norm(v - yd) / np.linalg.norm(yd) for v in (y, yx))
+    assert ex <= 1.01 * e + 1e-15
+    T.close()
+    X.close()
+
+
 def test_invalid_levels(oracle_mod):
     xy, _ = uniform_points(100, 1)
     with pytest.raises(ValueError):
