@@ -705,10 +705,10 @@ void assemble_tree(Engine& e) {
   // as B = K(skeleton_first, skeleton_second) in the orientation whose
   // register-tile class is cheaper (m2l_tile_class), and within two groups
   // (levels 2..L-1, level L) the pairs are ordered by class, so that one
-  // launch per (group, class) covers them.  Partials are box-major: box g
-  // owns kILSlots slots of rank_g values at kILSlots * soff[g]; its t-th
-  // present interaction slot (slot order) receives that pair's product for
-  // it, so a box's M2L sum reads one contiguous block (unused slots stay 0).
+  // launch per (group, class) covers them.  Partials are pair-major: pair q
+  // writes its two products next to each other, pairs in storage order, so
+  // the pair kernels' writes form one sequential stream; box g's t-th present
+  // interaction slot (slot order) reads its product at slot_off[27 g + t].
   e.h_pair_k.clear();
   e.h_pair_b.clear();
   e.pair_max_ld = 0;
@@ -773,8 +773,18 @@ void assemble_tree(Engine& e) {
     e.pair_max_rb = std::max(e.pair_max_rb, rs);
   }
   e.npairs = (int64_t)e.h_pair_k.size();
-  if ((int64_t)kILSlots * sb > INT32_MAX) throw Error(6, "M2L partial buffer exceeds 2^31 entries");
+  if (part > INT32_MAX) throw Error(6, "M2L partial buffer exceeds 2^31 entries");
+  // pair-major partials: pair q writes B out_b then B^T out_k at pp[q]
   std::vector<int> part_k(e.npairs, 0), part_b(e.npairs, 0);
+  {
+    int64_t pp = 0;
+    for (int64_t q = 0; q < e.npairs; q++) {
+      part_k[q] = (int)pp;
+      part_b[q] = (int)(pp + rank_g[e.h_pair_k[q]]);
+      pp += rank_g[e.h_pair_k[q]] + rank_g[e.h_pair_b[q]];
+    }
+  }
+  std::vector<int> slot_off((size_t)kILSlots * nbox, -1);
   for (int j = 2; j <= L; j++) {
     LevelInfo& li = e.lv[j];
     for (int k = 0; k < li.nb; k++) {
@@ -789,11 +799,8 @@ void assemble_tree(Engine& e) {
         const int lo = std::min(gk, gbx), hi = std::max(gk, gbx);
         const int pidx = pair_of.at(((int64_t)lo << 32) | hi);
         e.h_red_pair.push_back(pidx);
-        // the pair writes B out_b to its first box's slot, B^T out_k to its
-        // second box's slot
-        const int dst = (int)(kILSlots * soff_g[gk] + (int64_t)t * li.rank[k]);
-        if (gk == e.h_pair_k[pidx]) part_k[pidx] = dst;
-        else part_b[pidx] = dst;
+        // the box's t-th present slot reads the pair's product for it
+        slot_off[(size_t)kILSlots * gk + t] = gk == e.h_pair_k[pidx] ? part_k[pidx] : part_b[pidx];
         t++;
         e.il_pairs++;
       }
@@ -803,6 +810,7 @@ void assemble_tree(Engine& e) {
   e.m2l_cols = part;
   // the streamed pair kernel holds up to 256 rows per block
   if (e.max_rank > 256) throw Error(6, "ranks above 256 are not supported by the M2L kernel");
+  upload(e.slot_off, slot_off, s);
   upload(e.pair_k, e.h_pair_k, s);
   upload(e.pair_b, e.h_pair_b, s);
   upload(e.pair_off, e.h_pair_off, s);
@@ -818,7 +826,7 @@ void assemble_tree(Engine& e) {
   // one spare double2 / double past the end: the pair kernels clamp the
   // addresses of empty blocks (rank 0) to their start, which may be the end
   e.m2l.alloc(sizeof(double) * (off + 2));
-  e.m2l_part.alloc(sizeof(double) * (size_t)std::max<int64_t>(kILSlots * sb, 2));
+  e.m2l_part.alloc(sizeof(double) * (size_t)std::max<int64_t>(part, 2));
   AFMM_HIP(hipMemsetAsync(e.m2l_part.p, 0, e.m2l_part.bytes, s));
   if (e.npairs > 0) {
     AFMM_DISPATCH_KERNEL(e.kid, KID,

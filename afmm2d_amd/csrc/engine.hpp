@@ -21,11 +21,12 @@
 //                 zero row when rank_k is odd), so every column starts 16-byte
 //                 aligned; K(skeleton_b, skeleton_k) = +-B^T is not stored
 //                 (symmetric kernels; -B^T for an antisymmetric part)
-//   m2l_part      box-major M2L partials: box gid owns kILSlots slots of
-//                 rank[gid] values at kILSlots * soff[gid]; the t-th present
-//                 interaction slot (slot order) holds that pair's product for
-//                 this box (B out_b for the pair's first box, B^T out_k for
-//                 its second), unused slots stay zero
+//   m2l_part      pair-major M2L partials: pair q holds B out_b (rank_first
+//                 values) then B^T out_k (rank_second values), pairs in block
+//                 storage order
+//   slot_off      kILSlots ints per box gid: where the box's t-th present
+//                 interaction slot (slot order) finds its product in
+//                 m2l_part, -1 past the present slots
 //   near_part     leaf-major near-field partials: leaf A owns 9 slots of
 //                 count[A] values at 9 start[A] (slot 0 self, 1 + q
 //                 neighbour q), absent neighbours stay zero
@@ -106,11 +107,11 @@ inline int m2l_tile_class(int ld, int cols) {
 // Everything the M2L pair kernel needs about one pair, in one record.
 struct PairDesc {
   int rk, rb;      // ranks of the pair's boxes
-  int part_k;      // where B out_b goes: a slot of the first box's partials
+  int part_k;      // where B out_b goes (the pair's partials)
   int ld;          // leading dimension of the block: rk rounded up to even
   int64_t off;     // element offset of the block
   int64_t sk, sb;  // skeleton offsets of the boxes
-  int part_b;      // where B^T out_k goes: a slot of the second box's partials
+  int part_b;      // where B^T out_k goes: part_k + rk
   int pad;
 };
 
@@ -177,7 +178,8 @@ struct Engine {
   std::vector<int> h_red_start;           // per box gid (nbox + 1): first entry of h_red_pair
   std::vector<int> h_red_pair;            // per present slot, slot order: pair index
   DevBuf pair_k, pair_b, pair_off, m2l;
-  DevBuf m2l_part;                        // kILSlots slots of rank_g values per box
+  DevBuf m2l_part;                        // pair-major: rank_first + rank_second per pair
+  DevBuf slot_off;                        // kILSlots per box gid: partial offset, -1 absent
   DevBuf pair_desc;                       // PairDesc per pair (48 bytes)
   int num_cus = 256;
   // schedule options (environment, read at create):

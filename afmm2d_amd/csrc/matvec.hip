@@ -121,16 +121,21 @@ __device__ __forceinline__ double reduce_groups(double acc, double* red, const G
 }
 
 // A box's local coefficients into xs: (has_parent ? the parent's L2L
-// contribution : 0) + its M2L partials in interaction-slot order.  The box's
-// kILSlots partial slots (rank values each, unused ones zero) are one
-// contiguous block, so every load is issued at once.
+// contribution : 0) + its M2L partials in interaction-slot order.  so[t] is
+// where the box's t-th present slot finds its product in part (-1: none, a
+// zero term); the offsets are the same for the whole workgroup, and every
+// load is issued at once.
 template <int WPB>
 __device__ __forceinline__ void local_coeffs(double* xs, int r, const double* __restrict__ par,
-                                             int has_parent, const double* __restrict__ blk) {
+                                             int has_parent, const double* __restrict__ part,
+                                             const int* __restrict__ so) {
+  int o[kILSlots];
+#pragma unroll
+  for (int t = 0; t < kILSlots; t++) o[t] = so[t];
   for (int c = threadIdx.x; c < r; c += 64 * WPB) {
     double v[kILSlots];
 #pragma unroll
-    for (int t = 0; t < kILSlots; t++) v[t] = blk[t * r + c];
+    for (int t = 0; t < kILSlots; t++) v[t] = o[t] >= 0 ? part[o[t] + c] : 0.0;
     double acc = has_parent ? par[c] : 0.0;
 #pragma unroll
     for (int t = 0; t < kILSlots; t++) acc += v[t];
@@ -355,8 +360,8 @@ __device__ __forceinline__ double sum_lanes_stride8(double v) {
 // counts are compile-time: ld <= 16 MAXRT, rb <= 8 MAXCT).  B out_b: each
 // lane accumulates its two rows over its columns, then the 8 lanes sharing
 // rp are summed; B^T out_k: per column tile, the 8 lanes of a column are
-// summed.  Writes B out_b to the first box's partial slot and B^T out_k to
-// the second box's.
+// summed.  Writes B out_b and B^T out_k to the pair's partials (part_k,
+// part_b).
 template <int MAXRT, int MAXCT>
 __device__ __forceinline__ void m2l_pair(const double* __restrict__ m2l, const PairDesc& d,
                                          const double* __restrict__ outv,
@@ -562,6 +567,7 @@ __global__ __launch_bounds__(64 * WPB) void k_down(const double* __restrict__ l2
                                                    const int64_t* __restrict__ child_soff,
                                                    const int* __restrict__ rowmap,
                                                    const double* __restrict__ part,
+                                                   const int* __restrict__ slot_off,
                                                    double* __restrict__ inv, int has_parent,
                                                    int64_t gbase, int nb) {
   extern __shared__ double xs[];
@@ -587,7 +593,7 @@ __global__ __launch_bounds__(64 * WPB) void k_down(const double* __restrict__ l2
       q = gemv_lane(nc, 0, r);
       gemv_load<B>(pv, P, r, nc, 0, q, w, WPB);
     }
-    local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part + kILSlots * soff[g]);
+    local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part, slot_off + (int64_t)kILSlots * g);
     __syncthreads();
     for (int c = threadIdx.x; c < r; c += 64 * WPB) dst[rm[nc + c]] = xs[c];
     for (int cb = 0; cb < nc; cb += 64) {
@@ -606,7 +612,7 @@ __global__ __launch_bounds__(64 * WPB) void k_down(const double* __restrict__ l2
   const int ch = w % nch, sub = w / nch, nsub = (WPB - 1 - ch) / nch + 1;
   const GemvLane q = gemv_lane(nc, 64 * ch, r);
   gemv_load<B>(pv, P, r, nc, 64 * ch, q, sub, nsub);
-  local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part + kILSlots * soff[g]);
+  local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part, slot_off + (int64_t)kILSlots * g);
   __syncthreads();
   for (int c = threadIdx.x; c < r; c += 64 * WPB) dst[rm[nc + c]] = xs[c];
   red[threadIdx.x] = gemv_chunk<B>(pv, P, r, nc, 64 * ch, q, xs, sub, nsub);
@@ -942,6 +948,7 @@ __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2
                                                    const int* __restrict__ leaf_count,
                                                    const int* __restrict__ rowmap,
                                                    const double* __restrict__ part,
+                                                   const int* __restrict__ slot_off,
                                                    const double* __restrict__ inv, int has_parent,
                                                    const double* __restrict__ near_part,
                                                    const int* __restrict__ perm,
@@ -976,7 +983,7 @@ __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2
     for (int t = 0; t < 9; t++) nv[t] = nblk[t * n + i];
     dst_i = perm[st + i];
   }
-  local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part + kILSlots * soff[g]);
+  local_coeffs<WPB>(xs, r, inv + soff[g], has_parent, part, slot_off + (int64_t)kILSlots * g);
   __syncthreads();
   int row = threadIdx.x;  // the row whose nv/dst_i this thread holds
   for (int cb = 0; cb < nc; cb += 64) {
@@ -1295,13 +1302,15 @@ void launch_down(Engine& e, int j, hipStream_t s) {
                       k_down<WPB, AFMM_B_DOWN_LAST><<<li.nb, 64 * WPB, shm, s>>>(
                           e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
                           e.nin_g.as<int>(), soff, soff + e.lv[j + 1].gbase,
-                          e.rowmap.as<int>() + e.n, e.m2l_part.as<double>(), e.inv.as<double>(), j > 2 ? 1 : 0,
+                          e.rowmap.as<int>() + e.n, e.m2l_part.as<double>(), e.slot_off.as<int>(),
+                          e.inv.as<double>(), j > 2 ? 1 : 0,
                           li.gbase, li.nb);
                     } else {
                       k_down<WPB, kGemvBatch><<<li.nb, 64 * WPB, shm, s>>>(
                           e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
                           e.nin_g.as<int>(), soff, soff + e.lv[j + 1].gbase,
-                          e.rowmap.as<int>() + e.n, e.m2l_part.as<double>(), e.inv.as<double>(), j > 2 ? 1 : 0,
+                          e.rowmap.as<int>() + e.n, e.m2l_part.as<double>(), e.slot_off.as<int>(),
+                          e.inv.as<double>(), j > 2 ? 1 : 0,
                           li.gbase, li.nb);
                     });
   AFMM_LAUNCH_CHECK();
@@ -1353,7 +1362,7 @@ void launch_leaf(Engine& e, double* d_y, hipStream_t s) {
                         e.l2p_cm.as<double>(), e.l2poff_g.as<int64_t>(), e.rank_g.as<int>(),
                         e.soff_g.as<int64_t>(), e.d_leaf_start.as<int>(),
                         e.d_leaf_count.as<int>(), e.rowmap.as<int>(), e.m2l_part.as<double>(),
-                        e.inv.as<double>(), L > 2 ? 1 : 0, e.near_part.as<double>(), e.perm.as<int>(), d_y,
+                        e.slot_off.as<int>(), e.inv.as<double>(), L > 2 ? 1 : 0, e.near_part.as<double>(), e.perm.as<int>(), d_y,
                         li.gbase, li.nb));
   AFMM_LAUNCH_CHECK();
 }

@@ -70,6 +70,14 @@ def test_assembly_and_matvec_match_oracle(engine, oracle_mod, n, leaf, tol_pow, 
                 # relative 1e-4 while the operator they define agrees far
                 # tighter (the matvec check below).
                 assert np.linalg.norm(P - Po) <= 1e-4 * np.linalg.norm(Po)
+                # skeleton rows are stored as exactly the identity
+                if j == T.L:
+                    st, cnt = T.leaf_range(box)
+                    rows = np.arange(st, st + cnt)
+                else:
+                    rows = np.concatenate([T.skeleton(j + 1, 4 * box + c) for c in range(4)])
+                pos = [int(np.nonzero(rows == g)[0][0]) for g in sk]
+                np.testing.assert_array_equal(P[pos], np.eye(len(sk)))
     y = a.computeMatVecProduct(q)
     yo = T.matvec(q)
     rel = np.linalg.norm(y - yo) / np.linalg.norm(yo)
