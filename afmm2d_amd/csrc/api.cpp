@@ -350,10 +350,12 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
     if (kparams && kernel_id == AFMM2D_KERNEL_REG_INV_R && kparams[1] > 0.0) e->kp.a = kparams[1];
     for (int i = 0; i < 4; i++) e->kp.u[i] = kparams ? kparams[i] : 0.0;
     AFMM_HIP(hipDeviceGetAttribute(&e->num_cus, hipDeviceAttributeMultiprocessorCount, device));
+    AFMM_HIP(hipDeviceGetAttribute(&e->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     if (const char* v = std::getenv("AFMM2D_NEAR_OVERLAP")) e->near_overlap = std::atoi(v) != 0;
     if (const char* v = std::getenv("AFMM2D_TREE_MODE"))
       e->tree_mode = std::max(0, std::min(3, std::atoi(v)));
     if (const char* v = std::getenv("AFMM2D_NEAR_TUNE")) e->near_tune = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AFMM2D_TREE_FUSED")) e->tree_fused = std::atoi(v) != 0;
     if (const char* v = std::getenv("AFMM2D_P2P_WAVES")) {
       e->p2p_waves_per_simd = std::max(0, std::min(16, std::atoi(v)));
       e->p2p_waves_fixed = true;

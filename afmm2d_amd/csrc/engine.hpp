@@ -182,6 +182,7 @@ struct Engine {
   DevBuf slot_off;                        // kILSlots per box gid: partial offset, -1 absent
   DevBuf pair_desc;                       // PairDesc per pair (48 bytes)
   int num_cus = 256;
+  int max_lds = 65536;  // LDS bytes a workgroup may allocate
   // schedule options (environment, read at create):
   //   AFMM2D_NEAR_OVERLAP=0   near field on the main stream after M2L
   //   AFMM2D_P2P_WAVES=n      near-field waves per SIMD (persistent grid;
@@ -189,8 +190,11 @@ struct Engine {
   //   AFMM2D_NEAR_TUNE=0      the estimate near_waves_per_simd, untimed
   //   AFMM2D_TREE_MODE=m      which upper-tree passes run beside the
   //                           leaf-level M2L (matvec_device's schedules)
+  //   AFMM2D_TREE_FUSED=0     upper-tree M2M and L2L one launch per level
+  //                           (default: fused over 3-level subtrees)
   bool near_overlap = true;
   int tree_mode = 0;
+  bool tree_fused = true;
   int p2p_waves_per_simd = 4;   // set at assembly from the problem's sizes
   bool p2p_waves_fixed = false;  // AFMM2D_P2P_WAVES given
   bool near_tune = true;         // AFMM2D_NEAR_TUNE=0: the estimate, untimed

@@ -20,6 +20,8 @@
 // +-B^T out_k (the sign of an antisymmetric kernel part) as partials; the
 // downward pass adds each box's partials in its interaction-slot order, so
 // results do not depend on scheduling.
+#include <cstdlib>
+
 #include "engine.hpp"
 
 namespace afmm {
@@ -625,6 +627,195 @@ __global__ __launch_bounds__(64 * WPB) void k_down(const double* __restrict__ l2
     for (int u = 0; u < ns; u++)
       for (int h = 0; h < G; h++) acc += red[(ci + nch * u) * 64 + h * rc + c];
     dst[rm[i]] = acc;
+  }
+}
+
+// Upper-tree passes fused over subtrees: one workgroup of 16 waves per box of
+// the chunk's top level jt covers that box's descendants on D <= 3 levels
+// (jt .. jt + D - 1; at depth d there are 4^d of them, quad-tree ids
+// [4^d k, 4^d (k + 1))).  Each level's boxes split the 16 waves evenly
+// (16 / 4^d waves per box), and the levels are separated by workgroup
+// barriers only, so a chunk costs one launch instead of D.  Every wave runs
+// the same sequence of barriers whatever its box's shape.
+struct TreeArgs {
+  const double* l2p;      // stored L2P rows: row-major (up) / column-major (down)
+  const int64_t* l2poff;
+  const int* rank;
+  const int* nin;
+  const int64_t* soff;
+  const int* rowmap_up;   // rowmap of the upper levels (indexed like skx)
+  double* vec;            // outv (up) / inv (down)
+  const double* part;     // M2L partials (down)
+  const int* slot_off;    // partial offsets per box (down)
+  int64_t gb[4];          // gbase of levels jt .. jt + 3
+  int D;                  // levels in the chunk
+  int top_has_parent;     // down: the top level has an L2L contribution
+  int xs_stride;          // doubles of LDS per box
+};
+
+constexpr int kTreeWaves = 16;
+
+// sum_{k in [k0, k1)} P[k stride + ic] xs[k]: loads in batches of 16
+// through two register sets, the next batch issued before the current one is
+// used; addresses past k1 are clamped to k1 - 1 and their terms dropped.
+// The first batch is already issued into pa.
+__device__ __forceinline__ void tree_load(double (&pv)[16], const double* __restrict__ P,
+                                          int stride, int ic, int k, int k1) {
+#pragma unroll
+  for (int q = 0; q < 16; q++) pv[q] = P[(unsigned)(min(k + q, k1 - 1) * stride + ic)];
+}
+__device__ __forceinline__ double tree_use(const double (&pv)[16], double acc, int k, int k1,
+                                           const double* xs) {
+#pragma unroll
+  for (int q = 0; q < 16; q++) acc = k + q < k1 ? fma(pv[q], xs[k + q], acc) : acc;
+  return acc;
+}
+__device__ __forceinline__ double tree_dot_pre(double (&pa)[16], const double* __restrict__ P,
+                                               int stride, int ic, int k0, int k1,
+                                               const double* xs) {
+  double acc = 0.0;
+  if (k0 >= k1) return acc;
+  double pb[16];
+  for (int k = k0;; k += 32) {
+    if (k + 16 < k1) tree_load(pb, P, stride, ic, k + 16, k1);
+    acc = tree_use(pa, acc, k, k1, xs);
+    if (k + 16 >= k1) break;
+    if (k + 32 < k1) tree_load(pa, P, stride, ic, k + 32, k1);
+    acc = tree_use(pb, acc, k + 16, k1, xs);
+    if (k + 32 >= k1) break;
+  }
+  return acc;
+}
+// M2M over a chunk, bottom level first: out_k[c] = sum_t P[t r + c]
+// x[rowmap[t]] + x[rowmap[nc + c]], x the children's multipoles (staged in
+// LDS in rowmap order).  A box's waves split its column chunks (64 columns)
+// and, when they outnumber them, its rows; the row parts are summed in order.
+__global__ __launch_bounds__(64 * kTreeWaves) void k_up_tree(TreeArgs a) {
+  extern __shared__ double lds[];
+  __shared__ double red[kTreeWaves][64];
+  if (AFMM_MEM_PRIO) __builtin_amdgcn_s_setprio(AFMM_MEM_PRIO);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+  for (int d = a.D - 1; d >= 0; d--) {
+    const int m = 1 << (2 * d), WB = kTreeWaves / m;
+    const int ub = w / WB, u = w - ub * WB;
+    const int64_t k = (int64_t)blockIdx.x * m + ub;
+    const int64_t g = a.gb[d] + k;
+    const int r = a.rank[g], n = a.nin[g], nc = n - r;
+    const int64_t co = a.soff[a.gb[d + 1] + 4 * k];
+    double* xs = lds + (int64_t)ub * a.xs_stride;
+    {
+      const double* x = a.vec + co;
+      const int* rm = a.rowmap_up + co;
+      for (int t = u * 64 + l; t < n; t += WB * 64) xs[t] = x[rm[t]];
+    }
+    __syncthreads();
+    const double* P = a.l2p + a.l2poff[g];
+    double* out = a.vec + a.soff[g];
+    const int ncc = (r + 63) >> 6;
+    const int np = (ncc > 0 && ncc < WB) ? WB / ncc : 1;
+    // np == 1: column chunks u, u + WB, ... over all rows; np > 1: the one
+    // part (chunk u mod ncc, rows of part u / ncc) into the reduction rows
+    int cc0 = u, t0 = 0, t1 = nc;
+    if (np > 1) {
+      const int p = u / ncc;
+      cc0 = u - p * ncc;
+      t0 = p < np ? p * nc / np : 0;
+      t1 = p < np ? (p + 1) * nc / np : 0;
+    }
+    for (int cc = cc0; cc < ncc; cc += WB) {
+      const int ic = min(cc * 64 + l, r - 1);
+      double pa[16];
+      if (t0 < t1) tree_load(pa, P, r, ic, t0, t1);
+      const double acc = tree_dot_pre(pa, P, r, ic, t0, t1, xs);
+      if (np > 1) {
+        red[w][l] = acc;
+        break;
+      }
+      const int c = cc * 64 + l;
+      if (c < r) out[c] = acc + xs[nc + c];
+    }
+    __syncthreads();
+    if (np > 1 && u < ncc) {
+      const int c = u * 64 + l;
+      double acc = 0.0;
+      for (int p = 0; p < np; p++) acc += red[ub * WB + p * ncc + u][l];
+      if (c < r) out[c] = acc + xs[nc + c];
+    }
+    __syncthreads();
+  }
+}
+
+// L2L over a chunk, top level first: in_k = parent contribution + M2L
+// partials (slot order), then the children's contributions: stored row t
+// (column-major stored rows, P[c nc + t]) to child coefficient rowmap[t],
+// skeleton row q to rowmap[nc + q].  A box's waves split its 64-row output
+// chunks and, when they outnumber them, the sum over c.
+__global__ __launch_bounds__(64 * kTreeWaves) void k_down_tree(TreeArgs a) {
+  extern __shared__ double lds[];
+  __shared__ double red[kTreeWaves][64];
+  if (AFMM_MEM_PRIO) __builtin_amdgcn_s_setprio(AFMM_MEM_PRIO);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+  for (int d = 0; d < a.D; d++) {
+    const int m = 1 << (2 * d), WB = kTreeWaves / m;
+    const int ub = w / WB, u = w - ub * WB;
+    const int64_t k = (int64_t)blockIdx.x * m + ub;
+    const int64_t g = a.gb[d] + k;
+    const int r = a.rank[g], n = a.nin[g], nc = n - r;
+    double* xs = lds + (int64_t)ub * a.xs_stride;
+    {
+      const bool hp = d > 0 || a.top_has_parent;
+      const double* par = a.vec + a.soff[g];
+      const int* so = a.slot_off + (int64_t)kILSlots * g;
+      int o[kILSlots];
+#pragma unroll
+      for (int t = 0; t < kILSlots; t++) o[t] = so[t];
+      for (int c = u * 64 + l; c < r; c += WB * 64) {
+        double v[kILSlots];
+#pragma unroll
+        for (int t = 0; t < kILSlots; t++) v[t] = o[t] >= 0 ? a.part[o[t] + c] : 0.0;
+        double acc = hp ? par[c] : 0.0;
+#pragma unroll
+        for (int t = 0; t < kILSlots; t++) acc += v[t];
+        xs[c] = acc;
+      }
+    }
+    __syncthreads();
+    const double* P = a.l2p + a.l2poff[g];
+    const int64_t co = a.soff[a.gb[d + 1] + 4 * k];
+    double* dst = a.vec + co;
+    const int* rm = a.rowmap_up + co;
+    const int ntc = (nc + 63) >> 6;
+    const int np = (ntc > 0 && ntc < WB) ? WB / ntc : 1;
+    // np == 1: row chunks u, u + WB, ... over all c; np > 1: the one part
+    // (chunk u mod ntc, the c range of part u / ntc) into the reduction rows
+    int tc0 = u, c0 = 0, c1 = r;
+    if (np > 1) {
+      const int p = u / ntc;
+      tc0 = u - p * ntc;
+      c0 = p < np ? p * r / np : 0;
+      c1 = p < np ? (p + 1) * r / np : 0;
+    }
+    for (int tc = tc0; tc < ntc; tc += WB) {
+      const int it = min(tc * 64 + l, nc - 1);
+      double pa[16];
+      if (c0 < c1) tree_load(pa, P, nc, it, c0, c1);
+      const double acc = tree_dot_pre(pa, P, nc, it, c0, c1, xs);
+      if (np > 1) {
+        red[w][l] = acc;
+        break;
+      }
+      const int t = tc * 64 + l;
+      if (t < nc) dst[rm[t]] = acc;
+    }
+    for (int c = u * 64 + l; c < r; c += WB * 64) dst[rm[nc + c]] = xs[c];
+    __syncthreads();
+    if (np > 1 && u < ntc) {
+      const int t = u * 64 + l;
+      double acc = 0.0;
+      for (int p = 0; p < np; p++) acc += red[ub * WB + p * ntc + u][l];
+      if (t < nc) dst[rm[t]] = acc;
+    }
+    __syncthreads();
   }
 }
 
@@ -1367,10 +1558,63 @@ void launch_leaf(Engine& e, double* d_y, hipStream_t s) {
   AFMM_LAUNCH_CHECK();
 }
 
+// Chunks of the fused upper-tree passes: levels [jt, jb], 3 at a time from
+// level L - 1 up, down to level 4 (4^jt >= 256 workgroups: with fewer, one
+// launch per level measured faster).  Levels above the chunks, or all levels
+// when the pass runs level by level (AFMM2D_TREE_FUSED=0 or a level whose
+// rows do not fit the workgroup's LDS), use the per-level kernels.
+// (the static LDS of the tree kernels: the reduction rows)
+constexpr size_t kTreeStaticLds = 8 * 1024;
+constexpr int kTreeMinTop = 4;
+std::vector<std::pair<int, int>> tree_chunks(const Engine& e, bool up) {
+  std::vector<std::pair<int, int>> ch;
+  if (!e.tree_fused || e.L < 3) return ch;
+  for (int j = 2; j < e.L; j++) {
+    const int need = up ? e.lv[j].max_nin : e.lv[j].max_rank;
+    if ((size_t)kTreeWaves * need * sizeof(double) + kTreeStaticLds > (size_t)e.max_lds) return ch;
+  }
+  for (int jb = e.L - 1; jb >= kTreeMinTop;) {
+    const int jt = std::max(kTreeMinTop, jb - 2);
+    ch.push_back({jt, jb});
+    jb = jt - 1;
+  }
+  return ch;
+}
+
+void launch_tree(Engine& e, int jt, int jb, bool up, hipStream_t s) {
+  TreeArgs a{};
+  a.l2p = up ? e.l2p_rm.as<double>() : e.l2p_cm.as<double>();
+  a.l2poff = e.l2poff_g.as<int64_t>();
+  a.rank = e.rank_g.as<int>();
+  a.nin = e.nin_g.as<int>();
+  a.soff = e.soff_g.as<int64_t>();
+  a.rowmap_up = e.rowmap.as<int>() + e.n;
+  a.vec = up ? e.outv.as<double>() : e.inv.as<double>();
+  a.part = e.m2l_part.as<double>();
+  a.slot_off = e.slot_off.as<int>();
+  a.D = jb - jt + 1;
+  for (int d = 0; d <= a.D; d++) a.gb[d] = e.lv[jt + d].gbase;
+  a.top_has_parent = jt > 2 ? 1 : 0;
+  int stride = 1;
+  for (int j = jt; j <= jb; j++) stride = std::max(stride, up ? e.lv[j].max_nin : e.lv[j].max_rank);
+  a.xs_stride = stride;
+  const size_t shm = sizeof(double) * (size_t)kTreeWaves * stride;
+  const unsigned grid = 1u << (2 * jt);
+  if (up) k_up_tree<<<grid, 64 * kTreeWaves, shm, s>>>(a);
+  else k_down_tree<<<grid, 64 * kTreeWaves, shm, s>>>(a);
+  AFMM_LAUNCH_CHECK();
+}
+
 void upper_up(Engine& e, hipStream_t s) {
   hipEvent_t a, b;
   e.prof_begin("m2m_upper", s, &a, &b);
-  for (int j = e.L - 1; j >= 2; j--) launch_up(e, j, nullptr, s);
+  const auto ch = tree_chunks(e, true);
+  int j = e.L - 1;
+  for (const auto& c : ch) {
+    launch_tree(e, c.first, c.second, true, s);
+    j = c.first - 1;
+  }
+  for (; j >= 2; j--) launch_up(e, j, nullptr, s);
   e.prof_end(s, b);
 }
 
@@ -1385,7 +1629,10 @@ void upper_m2l(Engine& e, hipStream_t s) {
 void down_chain(Engine& e, hipStream_t s) {
   hipEvent_t a, b;
   e.prof_begin("l2l", s, &a, &b);
-  for (int j = 2; j < e.L; j++) launch_down(e, j, s);
+  const auto ch = tree_chunks(e, false);
+  const int jf = ch.empty() ? e.L : ch.back().first;  // first fused level
+  for (int j = 2; j < jf; j++) launch_down(e, j, s);
+  for (size_t i = ch.size(); i-- > 0;) launch_tree(e, ch[i].first, ch[i].second, false, s);
   e.prof_end(s, b);
 }
 
