@@ -198,8 +198,10 @@ int near_waves_per_simd(const Engine& e) {
 
 // The near-field wave count measured on the assembled operator: matvecs
 // back to back with each candidate (the estimate above, 2, 4 and 8 waves per
-// SIMD; the better of two batches of 4 after 2 warm-ups), keeping the
-// fastest.  The estimate cannot see
+// SIMD), keeping the fastest.  After 8 warm-up matvecs the candidates take
+// turns over 3 rounds (one warm-up and a timed batch of 4 each), and each
+// keeps its best batch, so a clock still settling favours none of them.  The
+// estimate cannot see
 // every interaction between the two: the near field's persistent workgroups
 // keep their registers until its queue drains, and with M2L blocks of rank
 // above 64 (~200-VGPR kernels) the M2L beside it may find no room (N = 1e6,
@@ -225,27 +227,33 @@ int tune_near_waves(Engine& e, int estimate) {
   AFMM_HIP(hipEventCreate(&b));
   const int saved_prof = e.profiling;
   e.profiling = 0;
-  int best = estimate;
-  float best_ms = 1e30f;
-  for (int c : cand) {
-    e.p2p_waves_per_simd = c;
-    for (int i = 0; i < 2; i++) matvec_device(e, e.io_b.as<double>(), e.io_y.as<double>(), s);
-    // back to back, as a caller's loop runs them: the better of two batches
-    float t = 1e30f;
-    for (int rep = 0; rep < 2; rep++) {
+  e.p2p_waves_per_simd = estimate;
+  for (int i = 0; i < 8; i++) matvec_device(e, e.io_b.as<double>(), e.io_y.as<double>(), s);
+  std::vector<float> t(cand.size(), 1e30f);
+  for (int round = 0; round < 3; round++) {
+    for (size_t ci = 0; ci < cand.size(); ci++) {
+      e.p2p_waves_per_simd = cand[ci];
+      matvec_device(e, e.io_b.as<double>(), e.io_y.as<double>(), s);
+      // back to back, as a caller's loop runs them
       AFMM_HIP(hipEventRecord(a, s));
       for (int i = 0; i < 4; i++) matvec_device(e, e.io_b.as<double>(), e.io_y.as<double>(), s);
       AFMM_HIP(hipEventRecord(b, s));
       AFMM_HIP(hipEventSynchronize(b));
       float x = 0.f;
       AFMM_HIP(hipEventElapsedTime(&x, a, b));
-      t = std::min(t, x / 4);
-    }
-    if (t < best_ms * 0.99f || (c == estimate && t <= best_ms * 1.01f)) {
-      best_ms = t;
-      best = c;
+      t[ci] = std::min(t[ci], x / 4);
     }
   }
+  int best = estimate;
+  float best_ms = 1e30f;
+  for (size_t ci = 0; ci < cand.size(); ci++)
+    if (t[ci] < best_ms) {
+      best_ms = t[ci];
+      best = cand[ci];
+    }
+  // the estimate when it is within 1% of the fastest
+  for (size_t ci = 0; ci < cand.size(); ci++)
+    if (cand[ci] == estimate && t[ci] <= best_ms * 1.01f) best = estimate;
   e.profiling = saved_prof;
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
