@@ -364,7 +364,6 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
       e->tree_mode = std::max(0, std::min(3, std::atoi(v)));
     if (const char* v = std::getenv("AFMM2D_NEAR_TUNE")) e->near_tune = std::atoi(v) != 0;
     if (const char* v = std::getenv("AFMM2D_TREE_FUSED")) e->tree_fused = std::atoi(v) != 0;
-    if (const char* v = std::getenv("AFMM2D_M2L_PROMOTE")) e->m2l_promote = std::max(0, std::min(2, std::atoi(v)));
     if (const char* v = std::getenv("AFMM2D_P2P_WAVES")) {
       e->p2p_waves_per_simd = std::max(0, std::min(16, std::atoi(v)));
       e->p2p_waves_fixed = true;

@@ -747,35 +747,6 @@ void assemble_tree(Engine& e) {
       }
     }
   }
-  // A register-tile class with few pairs in a group costs a launch of its
-  // own (5-10 us, mostly ramp-up and drain); its pairs move into the group's
-  // most populated register class when that class holds the block in either
-  // orientation.
-  if (e.m2l_promote) {
-    constexpr int64_t kFewPairs = 2048;
-    for (int grp = 0; grp < e.m2l_promote; grp++) {
-      std::vector<int64_t> cnt(kM2LAllClasses + 1, 0);
-      for (const Cand& c : cand)
-        if (c.group == grp) cnt[c.cls]++;
-      int big = 0;
-      for (int c = 1; c < kM2LClasses; c++)
-        if (cnt[c] > cnt[big]) big = c;
-      if (cnt[big] < kFewPairs) continue;
-      auto fits = [&](int rows, int cols) {
-        return (rows + (rows & 1) + 15) / 16 <= kM2LTileRows[big] && (cols + 7) / 8 <= kM2LTileCols[big];
-      };
-      for (Cand& c : cand) {
-        if (c.group != grp || c.cls == big || c.cls >= kM2LClasses || cnt[c.cls] >= kFewPairs) continue;
-        const int rf = rank_g[c.first], rs = rank_g[c.second];
-        if (fits(rf, rs)) {
-          c.cls = big;
-        } else if (fits(rs, rf)) {
-          std::swap(c.first, c.second);
-          c.cls = big;
-        }
-      }
-    }
-  }
   std::vector<int> order(cand.size());
   for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int u, int v) {

@@ -195,10 +195,6 @@ struct Engine {
   bool near_overlap = true;
   int tree_mode = 0;
   bool tree_fused = true;
-  //   AFMM2D_M2L_PROMOTE=n    move the pairs of sparse M2L tile classes into
-  //                           the group's main class in the first n groups
-  //                           (default 2: both; 0: every class its own launch)
-  int m2l_promote = 2;
   int p2p_waves_per_simd = 4;   // set at assembly from the problem's sizes
   bool p2p_waves_fixed = false;  // AFMM2D_P2P_WAVES given
   bool near_tune = true;         // AFMM2D_NEAR_TUNE=0: the estimate, untimed

@@ -164,6 +164,35 @@ def test_schedules_are_bit_identical(engine, monkeypatch, near, tree, waves):
     B.close()
 
 
+@pytest.mark.parametrize("n,leaf,tol_pow,kernel", [
+    (1 << 16, 16, 10, "log"),        # L = 6: levels 5-4 in one fused launch
+    (1 << 17, 32, 12, "reg_inv_r"),  # L = 6, ranks near 64
+    (300000, 64, 9, "inv_r"),        # L = 6, ragged leaves
+])
+def test_fused_tree_passes_match_per_level_launches(engine, oracle_mod, monkeypatch, n, leaf,
+                                                    tol_pow, kernel):
+    # The fused upper-tree passes sum the same terms in another order: the
+    # results agree with one launch per level to rounding, and with the
+    # oracle to the usual bar.
+    xy, q = uniform_points(n, n + tol_pow)
+    A = engine.AFMM(n, leaf, tol_pow, xy, kernel=kernel)
+    A.assemble()
+    assert A.nLevels == 6
+    y = A.computeMatVecProduct(q)
+    A.close()
+    monkeypatch.setenv("AFMM2D_TREE_FUSED", "0")
+    B = engine.AFMM(n, leaf, tol_pow, xy, kernel=kernel)
+    B.assemble()
+    yb = B.computeMatVecProduct(q)
+    B.close()
+    assert rel(y, yb) <= 1e-13, rel(y, yb)
+    if n == 1 << 16:
+        T = oracle_mod.Tree(xy, leaf, tol_pow, engine.KERNELS[kernel])
+        T.assemble()
+        assert rel(y, T.matvec(q)) <= 1e-11
+        T.close()
+
+
 def test_profiling_levels(engine):
     n = 20000
     xy, q = uniform_points(n, 3)
