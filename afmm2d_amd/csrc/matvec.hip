@@ -338,6 +338,8 @@ __device__ __forceinline__ double dpp_move(double v) {
   const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
+// wave_rol:1: lane i receives lane i + 1's value, lane 63 lane 0's
+constexpr int kDppWaveRol1 = 0x134;
 // sum over the 8 consecutive lanes 8g..8g+7 (quad_perm xor 1, xor 2, then
 // row_half_mirror, which pairs each quad with the other half of its 8 lanes)
 __device__ __forceinline__ double sum_lanes8(double v) {
@@ -933,8 +935,10 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
   // one address per step for each array, 8-byte lane stride: no bank
   // conflicts.  The staged chunk is stored twice (entries t and t + 64), so
   // source t + s of lane t needs no wrap: each lane's addresses are its base
-  // plus an immediate offset.  The B-side sums of source j collect in
-  // entries j and j + 64 of a row and are added when the chunk is done.
+  // plus an immediate offset.  The B-side sums of an A chunk collect in a
+  // row of bc: in entry t for source t when they travel in registers, in
+  // entries j and j + 64 (LDS adds) on the diagonal chunk of a self pair;
+  // the rows are added when the chunk is done.
   __shared__ double bx[128], by[128], bq[128], bc[NCH][128];
   __shared__ int grab;
   const int t = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -1019,8 +1023,11 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
           const double x0 = xa[cp], y0 = ya[cp], q0 = qa[cp];
           const double x1 = xa[cp + 1], y1 = ya[cp + 1], q1 = qa[cp + 1];
           double acc0 = row[cp], acc1 = row[cp + 1];
-          double* bc0 = bc[cp];
-          double* bc1 = bc[cp + 1];
+          // B-side sums in registers that travel with the sources: at step
+          // s lane t holds source t + s's sum, adds its term and passes it
+          // to lane t - 1 (wave_rol:1), so after 64 steps lane t holds
+          // source t's
+          double bs0 = 0.0, bs1 = 0.0;
 #pragma unroll 8
           for (int s = 0; s < 64; s++) {
             const int j = t + s;
@@ -1031,9 +1038,11 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
                                               ln2_lo);
             acc0 = fma(v0, sq, acc0);
             acc1 = fma(v1, sq, acc1);
-            atomicAdd(&bc0[j], kTransposeSign<KID> * v0 * q0);
-            atomicAdd(&bc1[j], kTransposeSign<KID> * v1 * q1);
+            bs0 = dpp_move<kDppWaveRol1>(fma(kTransposeSign<KID> * v0, q0, bs0));
+            bs1 = dpp_move<kDppWaveRol1>(fma(kTransposeSign<KID> * v1, q1, bs1));
           }
+          bc[cp][t] = bs0;  // entries t + 64 of both rows stay zero
+          bc[cp + 1][t] = bs1;
           row[cp] = acc0;
           row[cp + 1] = acc1;
           continue;
@@ -1053,7 +1062,20 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
                    qi;
             s = 1;
           }
-          const int s_end = diag ? 32 : 64;
+          if (!diag) {
+            // the B-side sums travel with the sources (as above)
+            double bs = 0.0;
+#pragma unroll 2
+            for (; s < 64; s++) {
+              const int j = t + s;
+              const double v = pair_value<KID>(xi, yi, bx[j], by[j], kp, ninv_p0, inv_a, tab,
+                                               ln2_hi, ln2_lo);
+              acc = fma(v, bq[j], acc);
+              bs = dpp_move<kDppWaveRol1>(fma(kTransposeSign<KID> * v, qi, bs));
+            }
+            bc[ca][t] = bs;
+          }
+          const int s_end = diag ? 32 : 0;
 #pragma unroll 2
           for (; s < s_end; s++) {
             const int j = t + s;
