@@ -134,7 +134,9 @@ int afmm2d_get_ranks(afmm2d_handle h, int level, int* ranks);
  * point indices in pivot order.  *count receives the rank. */
 int afmm2d_get_skeleton(afmm2d_handle h, int level, int box, int* idx, int capacity, int* count);
 
-/* L2P of a box (AFMM2DBox.hpp:25), rows x rank, row-major. */
+/* L2P of a box (AFMM2DBox.hpp:25), rows x rank, row-major.  Its skeleton
+ * rows are exactly the identity rows (the engine stores only the other rows;
+ * the reference's LU solve returns them up to rounding). */
 int afmm2d_get_l2p(afmm2d_handle h, int level, int box, double* out, int64_t capacity, int* rows,
                    int* cols);
 
