@@ -193,6 +193,26 @@ def test_fused_tree_passes_match_per_level_launches(engine, oracle_mod, monkeypa
         T.close()
 
 
+def test_boxes_whose_rows_are_all_skeleton(engine, oracle_mod):
+    # Leaves of 4 points at a tight tolerance: many boxes keep every row as a
+    # skeleton row (no stored L2P rows at all), others keep a few.
+    n = 4096
+    xy, q = uniform_points(n, 77)
+    T = oracle_mod.Tree(xy, 4, 12)
+    T.assemble()
+    A = engine.AFMM(n, 4, 12, xy)
+    A.assemble()
+    L = T.L
+    full = sum(int(A.ranks(j)[k] == T.nin(j, k)) for j in range(2, L + 1) for k in range(4 ** j))
+    part = sum(int(0 < A.ranks(j)[k] < T.nin(j, k)) for j in range(2, L + 1) for k in range(4 ** j))
+    assert full > 0 and part > 0, (full, part)
+    for j in (2, L):
+        np.testing.assert_array_equal(A.ranks(j), T.ranks(j))
+    assert rel(A.computeMatVecProduct(q), T.matvec(q)) <= 1e-11
+    A.close()
+    T.close()
+
+
 def test_profiling_levels(engine):
     n = 20000
     xy, q = uniform_points(n, 3)
