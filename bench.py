@@ -29,6 +29,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
+# the headline metric, as the benchmark's definition words it
+METRIC = "FMM matvec point\u00b7RHS/s at rel.err\u22641e-8, N=2^20; % HBM roofline"
+UNIT = "point\u00b7RHS/s"
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -335,7 +340,7 @@ def main():
     if not args.no_cpu:
         t_cpu, t_cpu_asm, y_cpu = cpu_baseline(xy, q, args.leaf, args.tol_pow, threads,
                                                args.cpu_reps, E.KERNELS[args.kernel])
-        cpu = {"value": round(args.n / t_cpu, 1), "unit": "point*RHS/s", "cores": threads,
+        cpu = {"value": round(args.n / t_cpu, 1), "unit": UNIT, "cores": threads,
                "kind": "port",
                "sample": "full workload (N=%d, tol 1e-%d): oracle OpenMP FP64 matvec, "
                          "reference pragma placement, median of %d after 1 warm-up; "
@@ -346,9 +351,9 @@ def main():
 
     value = args.n / (ms * 1e-3)
     line = {
-        "metric": "FMM matvec point*RHS/s at rel.err<=1e-8, N=2^20; % HBM roofline",
+        "metric": METRIC,
         "value": round(value, 1),
-        "unit": "point*RHS/s",
+        "unit": UNIT,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
