@@ -1014,8 +1014,44 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
           bc[ca][t + 64] = 0.0;
         }
       __syncthreads();
+      // all A chunks of a full leaf four at a time against one staged
+      // source: four independent evaluation chains per step and a quarter of
+      // the LDS reads per evaluation
+      const bool quad = NCH % 4 == 0 && nw == 1 && !self && nca == NCH;
+      if (quad) {
+#pragma unroll
+        for (int cq = 0; cq < NCH; cq += 4) {
+          double xq[4], yq[4], qq[4], aq[4], bs[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            xq[u] = xa[cq + u];
+            yq[u] = ya[cq + u];
+            qq[u] = qa[cq + u];
+            aq[u] = row[cq + u];
+            bs[u] = 0.0;
+          }
+#pragma unroll 4
+          for (int s = 0; s < 64; s++) {
+            const int j = t + s;
+            const double sx = bx[j], sy = by[j], sq = bq[j];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const double v = pair_value<KID>(xq[u], yq[u], sx, sy, kp, ninv_p0, inv_a, tab,
+                                               ln2_hi, ln2_lo);
+              aq[u] = fma(v, sq, aq[u]);
+              bs[u] = dpp_move<kDppWaveRol1>(fma(kTransposeSign<KID> * v, qq[u], bs[u]));
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            bc[cq + u][t] = bs[u];
+            row[cq + u] = aq[u];
+          }
+        }
+      }
 #pragma unroll
       for (int cp = 0; cp < NCH; cp += 2) {
+        if (quad) break;
         if (cp + 1 < NCH && nw == 1 && !self && cp + 1 < nca) {
           // two A chunks per step against one staged source: half the LDS
           // reads per evaluation; each row gets the same adds in the same
