@@ -128,24 +128,26 @@ def test_near_field_matches_oracle(engine, oracle_mod, n, leaf, kernel, quant):
     assert float(np.max(np.abs(got - ref))) <= 1e-12 * float(np.max(np.abs(ref)))
 
 
-@pytest.mark.parametrize("n,leaf,kernel", [
-    (14336, 200, "log"),        # 224 points per leaf: four full 64-point chunks
-    (14336, 200, "reg_inv_r"),
-    (7680, 400, "inv_r"),       # 480: eight chunks, two groups of four
-    (14336, 200, "user"),
+@pytest.mark.parametrize("n,leaf,kernel,chunks", [
+    (14336, 200, "log", 4),        # 224 points per leaf: four 64-point chunks
+    (14336, 200, "reg_inv_r", 4),
+    (7680, 400, "inv_r", 8),       # 480: eight chunks, two groups of four
+    (14336, 200, "user", 4),
+    (20000, 300, "log", 5),        # 312: chunks two at a time, then one
+    (100000, 100, "reg_inv_r", 7),  # 390
 ])
-def test_near_field_full_leaves_one_wave_per_pair(engine, oracle_mod, monkeypatch, n, leaf,
-                                                  kernel):
+def test_near_field_long_leaves_one_wave_per_pair(engine, oracle_mod, monkeypatch, n, leaf,
+                                                  kernel, chunks):
     # One workgroup per leaf pair (P2P_WAVES = 0), so a wave holds every A
-    # chunk of a full leaf and takes them four at a time; near field and
-    # matvec against the oracle.
+    # chunk of a leaf: four at a time when every chunk is full, else two at
+    # a time and one; near field and matvec against the oracle.
     monkeypatch.setenv("AFMM2D_P2P_WAVES", "0")
     kid = engine.KERNELS[kernel]
     xy, q = uniform_points(n, 5 + n % 89)
     T = oracle_mod.Tree(xy, leaf, 10, kid)
     a = engine.AFMM(n, leaf, 10, xy, kernel=kernel)
     counts = np.asarray(a.NumberOfParticlesInLeaves)
-    assert ((counts + 63) // 64).min() == ((counts + 63) // 64).max()  # every leaf full-width
+    assert ((counts + 63) // 64).min() == ((counts + 63) // 64).max() == chunks
     ref, got = T.near_field(q), a.near_field(q)
     assert float(np.linalg.norm(got - ref) / np.linalg.norm(ref)) <= 1e-13
     T.assemble()
