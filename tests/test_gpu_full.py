@@ -60,3 +60,24 @@ def test_full_size_configured_tolerance(engine, full):
     print("full N=2^20 tol 1e-8: rel.err vs direct (4096 rows) %.3e" % err)
     assert err <= 1e-4
     A.close()
+
+
+def test_published_shape_error_bar(engine):
+    # The reference's published N = 1e6 shape (leaf 100: 244-point leaves,
+    # ranks to 105, streamed M2L blocks, the near field four chunks at a
+    # time), a/R kernel, tolerance 1e-12: error against the FP64 direct sum
+    # on sampled rows.
+    n = 1000000
+    rng = np.random.default_rng(2024)
+    xy = rng.uniform(-1.0, 1.0, size=(n, 2))
+    q = rng.uniform(-1.0, 1.0, size=n)
+    A = engine.AFMM(n, 100, 12, xy, kernel="reg_inv_r")
+    A.assemble()
+    assert A.nLevels == 6 and A.stats()["max_rank"] > 96
+    y = A.computeMatVecProduct(q)
+    rows = np.sort(np.random.default_rng(3).choice(n, 4096, replace=False))
+    ref = A.direct_rows(rows, q)
+    err = float(np.linalg.norm(y[rows] - ref) / np.linalg.norm(ref))
+    print("N=1e6 leaf 100 a/R tol 1e-12: rel.err vs direct (4096 rows) %.3e" % err)
+    assert err <= 1e-7
+    A.close()
