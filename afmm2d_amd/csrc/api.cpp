@@ -752,6 +752,7 @@ int afmm2d_near_field(afmm2d_handle h, const double* b, double* y) {
     }
     AFMM_HIP(hipMemcpyAsync(y, e.io_y.p, nb, hipMemcpyDeviceToHost, s));
     AFMM_HIP(hipStreamSynchronize(s));
+    if (e.profiling) e.prof_collect();
   });
 }
 

@@ -1778,7 +1778,10 @@ void add_device(double* d_y, const double* d_t, int n, hipStream_t s) {
 void near_field_device(Engine& e, const double* d_b, double* d_y, hipStream_t s) {
   e.qs.alloc(sizeof(double) * e.n);
   gather_sorted(e, d_b, e.qs.as<double>(), s);
+  hipEvent_t a = nullptr, b = nullptr;
+  e.prof_begin("p2p", s, &a, &b);
   launch_near(e, s);
+  e.prof_end(s, b);
   const int nleaves = 1 << (2 * e.L);
   k_near_out<<<nleaves, 256, 0, s>>>(e.near_part.as<double>(), e.d_leaf_start.as<int>(),
                                       e.d_leaf_count.as<int>(), e.perm.as<int>(), d_y, nleaves);

@@ -59,6 +59,10 @@ def parse():
                    help="instead of the headline, run the reference's published benchmark "
                         "shapes (leaf 100, a/R kernel, tolerance and size sweeps) and print "
                         "one JSON line per shape")
+    p.add_argument("--near-field", action="store_true",
+                   help="instead of the headline, time the near-field kernel alone "
+                        "(afmm2d_near_field, no assembly) on the same points and print its "
+                        "kernel evaluations per second against the FP64 VALU peak")
     return p.parse_args()
 
 
@@ -260,6 +264,67 @@ def published_shapes(args, torch, E):
         }), flush=True)
 
 
+def near_field_only(args, torch, E):
+    """The near-field kernel alone: each unordered pair of neighbouring (or
+    equal) leaves once, every unordered point pair of the two once (the
+    diagonal included), timed with the engine's HIP events around the kernel
+    on its stream."""
+    xy, q = inputs(args.n, args.seed)
+    A = E.AFMM(args.n, args.leaf, args.tol_pow, xy, kernel=args.kernel)
+    L = A.nLevels
+    nbr, _, _ = A.interactions(L)
+    # per leaf box (tree numbering, as the neighbour lists): first KD-sorted
+    # point and point count
+    rng_box = np.array([A.leaf_range(b) for b in range(4 ** L)], dtype=np.int64)
+    first, cnt = rng_box[:, 0], rng_box[:, 1].astype(np.float64)
+    evals = float(np.sum(cnt * (cnt + 1) / 2))
+    for c in range(8):
+        b = nbr[:, c]
+        m = b > np.arange(len(cnt))
+        evals += float(np.sum(cnt[m] * cnt[b[m]]))
+    for _ in range(args.warmup):
+        y = A.near_field(q)
+    A.reset_kernel_times()
+    A.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y = A.near_field(q)
+    wall = (time.perf_counter() - t0) / args.steps
+    A.set_profiling(False)
+    tot, n_calls = A.kernel_time("p2p")
+    ms = tot / n_calls
+    # the near-field term of 16 sampled rows against the same sum on the
+    # host, from the handle's own getMatrixEntry (KD-sorted indices)
+    perm = A.sorted_Properties
+    inv = np.empty(args.n, dtype=np.int64)
+    inv[perm] = np.arange(args.n)
+    owner = np.empty(args.n, dtype=np.int64)
+    for b in range(4 ** L):
+        owner[first[b]:first[b] + int(cnt[b])] = b
+    worst = 0.0
+    for r in np.random.default_rng(args.seed).choice(args.n, 16, replace=False):
+        i = int(inv[r])
+        leaf = int(owner[i])
+        ref = 0.0
+        for bx in [leaf] + [int(b) for b in nbr[leaf] if b >= 0]:
+            for j in range(int(first[bx]), int(first[bx] + cnt[bx])):
+                ref += A.getMatrixEntry(i, j) * q[perm[j]]
+        worst = max(worst, abs(y[r] - ref) / max(abs(ref), 1e-300))
+    A.close()
+    print(json.dumps({
+        "metric": "near-field kernel evaluations/s (k_p2p alone)",
+        "value": round(evals / (ms * 1e-3), 1), "unit": "evaluations/s",
+        "steps": args.steps, "warmup": args.warmup, "kernel_ms": round(ms, 4),
+        "wall_ms_per_call_pcie_inclusive": round(wall * 1e3, 4), "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "N=%d uniform [-1,1]^2, %s kernel, leaf %d (L=%d), 1 RHS"
+                               % (args.n, args.kernel, args.leaf, L),
+                   "evaluations": evals,
+                   "waves_per_simd": os.environ.get("AFMM2D_P2P_WAVES", "4 (default)")},
+        "max_rel_diff_16_rows_vs_host_sum": worst,
+    }))
+
+
 def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
@@ -288,6 +353,9 @@ def main():
         sys.exit(1)
     if args.published_shapes:
         published_shapes(args, torch, E)
+        return
+    if args.near_field:
+        near_field_only(args, torch, E)
         return
     xy, q = inputs(args.n, args.seed)
 
