@@ -44,7 +44,12 @@ EXPORTS = [
     "afmm2d_direct_rows", "afmm2d_compute_error", "afmm2d_evaluate_error",
     "afmm2d_get_matrix_entry", "afmm2d_kernel_value", "afmm2d_get_stats",
     "afmm2d_set_profiling", "afmm2d_get_kernel_time", "afmm2d_reset_kernel_times",
+    "afmm2d_matvec_ex", "afmm2d_matvec_device_ex", "afmm2d_prepare_precision",
 ]
+
+# afmm2d_precision and the blocked-mode flag (include/afmm2d.h)
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
+MATVEC_BLOCKED = 0x100
 
 
 class AFMM2DError(RuntimeError):
@@ -64,6 +69,7 @@ class Stats(ctypes.Structure):
         ("create_seconds", ctypes.c_double), ("assemble_seconds", ctypes.c_double),
         ("m2l_leaf_bytes", ctypes.c_int64), ("m2l_leaf_columns", ctypes.c_int64),
         ("skeleton_leaf", ctypes.c_int64), ("near_waves", ctypes.c_int64),
+        ("blocked_m2l_coeffs", ctypes.c_int64), ("blocked_basis_coeffs", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -120,6 +126,9 @@ def lib():
         "afmm2d_set_profiling": (i, [_vp, i]),
         "afmm2d_get_kernel_time": (i, [_vp, c, _dp, _lp]),
         "afmm2d_reset_kernel_times": (i, [_vp]),
+        "afmm2d_matvec_ex": (i, [_vp, _dp, ctypes.c_int64, _dp, ctypes.c_int64, i, i]),
+        "afmm2d_matvec_device_ex": (i, [_vp, _vp, ctypes.c_int64, _vp, ctypes.c_int64, i, i, _vp]),
+        "afmm2d_prepare_precision": (i, [_vp, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -143,10 +152,27 @@ def _i(a):
     return a.ctypes.data_as(_ip)
 
 
+def _kparams(kparams):
+    """The 4 kernel parameters the C side reads, zero-padded."""
+    kp = np.zeros(4, dtype=np.float64)
+    if kparams is not None:
+        v = np.asarray(kparams, dtype=np.float64).ravel()
+        if v.size > 4:
+            raise ValueError("at most 4 kernel parameters")
+        kp[: v.size] = v
+    return kp
+
+
+def _mode(precision, blocked):
+    if isinstance(precision, str):
+        precision = PRECISIONS[precision]
+    return int(precision) | (MATVEC_BLOCKED if blocked else 0)
+
+
 def kernel_value(kernel, xi, yi, xj, yj, a=0.0, kparams=None):
     """The built-in kernel on explicit coordinates (host evaluation)."""
     kid = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
-    kp = np.zeros(4) if kparams is None else np.asarray(kparams, dtype=np.float64)
+    kp = _kparams(kparams)
     out = ctypes.c_double()
     _check(lib().afmm2d_kernel_value(kid, _d(kp), a, xi, yi, xj, yj, ctypes.byref(out)))
     return out.value
@@ -175,7 +201,7 @@ class AFMM:
         self.TOL_POW = int(TOL_POW)
         self.kernel = kernel
         kid = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
-        kp = np.zeros(4) if kparams is None else np.asarray(kparams, dtype=np.float64)
+        kp = _kparams(kparams)
         h = _vp()
         _check(lib().afmm2d_create(self.N, self.MinParticlesInLeaf, self.TOL_POW, _d(loc), 0, kid,
                                    _d(kp), int(device), ctypes.byref(h)))
@@ -219,6 +245,32 @@ class AFMM:
     def matvec_device(self, d_b, d_y, stream=None):
         """y = K b on device-resident vectors (device pointers as ints)."""
         _check(lib().afmm2d_matvec_device(self._h, _vp(d_b), _vp(d_y), _vp(stream or 0)))
+
+    def matvec(self, b, precision="fp64", blocked=False):
+        """Y = K B for B of shape (N,) or (N, R) in the caller's order, in the
+        given precision ("fp64", "fp32", "bf16"); blocked=True runs the
+        several-RHS passes also for one FP64 column (afmm2d_matvec_ex)."""
+        b = np.asarray(b, dtype=np.float64)
+        one = b.ndim == 1
+        B = np.asfortranarray(b.reshape(self.N, -1))
+        Y = np.empty_like(B, order="F")
+        _check(lib().afmm2d_matvec_ex(self._h, _d(B), self.N, _d(Y), self.N, B.shape[1],
+                                      _mode(precision, blocked)))
+        return Y[:, 0].copy() if one else Y
+
+    def matvec_device_ex(self, d_b, ldb, d_y, ldy, nrhs, precision="fp64", blocked=False,
+                         stream=None):
+        """Y = K B on device-resident column-major arrays (device pointers as
+        ints), afmm2d_matvec_device_ex."""
+        _check(lib().afmm2d_matvec_device_ex(self._h, _vp(d_b), int(ldb), _vp(d_y), int(ldy),
+                                             int(nrhs), _mode(precision, blocked),
+                                             _vp(stream or 0)))
+
+    def prepare_precision(self, precision):
+        """Builds the factors of a precision now (afmm2d_prepare_precision)."""
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        _check(lib().afmm2d_prepare_precision(self._h, int(precision)))
 
     def evaluateError(self, seed=0):
         """One leaf's relative error vs the direct sum (AFMM2D.cpp:67-74)."""

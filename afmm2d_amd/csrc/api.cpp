@@ -137,7 +137,7 @@ Engine::~Engine() {
 void Engine::prof_begin(const char* name, hipStream_t s, hipEvent_t* a, hipEvent_t* b) {
   *a = *b = nullptr;
   if (!profiling) return;
-  if (profiling == 2 && std::strcmp(name, "m2l") != 0) return;
+  if (profiling == 2 && std::strcmp(name, "m2l") != 0 && std::strcmp(name, "mr_m2l") != 0) return;
   if (ev_used == ev_pool.size()) {
     std::array<hipEvent_t, 2> p;
     AFMM_HIP(hipEventCreate(&p[0]));
@@ -267,6 +267,17 @@ int tune_near_waves(Engine& e, int estimate) {
 struct afmm2d_handle_s {
   std::unique_ptr<Engine> e;
   std::unique_ptr<Engine> e2;
+  // set once every engine has assembled: a failure in the second engine
+  // leaves the handle unassembled, so no matvec launches on a partial one
+  bool assembled = false;
+  // recorded at the end of every call's device work; a call on another
+  // stream first waits for it (the calls share the handle's buffers)
+  hipEvent_t done = nullptr;
+  hipStream_t last = nullptr;
+  bool have_done = false;
+  ~afmm2d_handle_s() {
+    if (done) (void)hipEventDestroy(done);
+  }
 };
 
 namespace {
@@ -289,6 +300,17 @@ bool user_kernel_symmetric(const Engine& e) {
   return true;
 }
 
+// Orders stream s after the handle's previous call (on another stream).
+void begin_on(afmm2d_handle h, hipStream_t s) {
+  if (h->have_done && h->last != s) AFMM_HIP(hipStreamWaitEvent(s, h->done, 0));
+}
+void end_on(afmm2d_handle h, hipStream_t s) {
+  if (!h->done) AFMM_HIP(hipEventCreateWithFlags(&h->done, hipEventDisableTiming));
+  AFMM_HIP(hipEventRecord(h->done, s));
+  h->last = s;
+  h->have_done = true;
+}
+
 // y = the operator applied to b, device arrays in caller order, on stream s.
 void run_matvec(afmm2d_handle h, const double* d_b, double* d_y, hipStream_t s) {
   matvec_device(*h->e, d_b, d_y, s);
@@ -298,6 +320,40 @@ void run_matvec(afmm2d_handle h, const double* d_b, double* d_y, hipStream_t s) 
     matvec_device(e2, d_b, e2.io_y.as<double>(), s);
     add_device(d_y, e2.io_y.as<const double>(), e2.n, s);
   }
+}
+
+constexpr int kModePrecisionMask = 0xFF;
+constexpr int kModeBlocked = 0x100;  // AFMM2D_MATVEC_BLOCKED
+
+// Y = K B for nrhs columns (device, caller order) in the given mode: one FP64
+// column runs the single-RHS passes, anything else the blocked passes in
+// chunks of up to 32 columns.
+void run_matvec_ex(afmm2d_handle h, const double* d_b, int64_t ldb, double* d_y, int64_t ldy,
+                   int nrhs, int mode, hipStream_t s) {
+  const int prec = mode & kModePrecisionMask;
+  if (prec == PREC_F64 && nrhs == 1 && !(mode & kModeBlocked)) {
+    run_matvec(h, d_b, d_y, s);
+    return;
+  }
+  for (int c0 = 0; c0 < nrhs; c0 += kMrMaxCols) {
+    const int nc = std::min(kMrMaxCols, nrhs - c0);
+    const double* b = d_b + (int64_t)c0 * ldb;
+    double* y = d_y + (int64_t)c0 * ldy;
+    mr_matvec(*h->e, b, ldb, y, ldy, nc, prec, s);
+    if (h->e2) {
+      Engine& e2 = *h->e2;
+      e2.io_y.alloc(sizeof(double) * (size_t)e2.n * nc);
+      mr_matvec(e2, b, ldb, e2.io_y.as<double>(), e2.n, nc, prec, s);
+      add_cols_device(y, ldy, e2.io_y.as<const double>(), e2.n, e2.n, nc, s);
+    }
+  }
+}
+
+int check_mode(int mode) {
+  const int prec = mode & kModePrecisionMask;
+  if (prec < 0 || prec >= kPrecisions || (mode & ~(kModePrecisionMask | kModeBlocked)))
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown precision mode");
+  return AFMM2D_OK;
 }
 
 }  // namespace
@@ -509,7 +565,9 @@ int afmm2d_destroy(afmm2d_handle h) {
 
 int afmm2d_assemble(afmm2d_handle h) {
   if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "handle is NULL");
+  if (h->assembled) return AFMM2D_OK;
   return guarded([&] {
+    if (h->have_done) AFMM_HIP(hipEventSynchronize(h->done));
     for (Engine* ep : {h->e.get(), h->e2.get()}) {
       if (!ep) continue;
       Engine& e = *ep;
@@ -524,45 +582,109 @@ int afmm2d_assemble(afmm2d_handle h) {
       }
       e.assemble_seconds = now_s() - t0;
     }
+    h->assembled = true;
   });
 }
 
 int afmm2d_matvec_device(afmm2d_handle h, const double* d_b, double* d_y, void* stream) {
   if (!h || !d_b || !d_y) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (!h->e->assembled) return fail(AFMM2D_ERR_STATE, "matvec before assemble");
+  if (!h->assembled) return fail(AFMM2D_ERR_STATE, "matvec before assemble");
   return guarded([&] {
     Engine& e = *h->e;
     AFMM_HIP(hipSetDevice(e.device));
     hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)0;
+    begin_on(h, s);
     run_matvec(h, d_b, d_y, s);
+    end_on(h, s);
+  });
+}
+
+int afmm2d_matvec_device_ex(afmm2d_handle h, const double* d_b, int64_t ldb, double* d_y,
+                            int64_t ldy, int nrhs, int mode, void* stream) {
+  if (!h || !d_b || !d_y || nrhs <= 0) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "bad argument");
+  if (ldb < h->e->n || ldy < h->e->n)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "leading dimension below N");
+  if (check_mode(mode) != AFMM2D_OK) return AFMM2D_ERR_INVALID_ARGUMENT;
+  if (!h->assembled) return fail(AFMM2D_ERR_STATE, "matvec before assemble");
+  return guarded([&] {
+    Engine& e = *h->e;
+    AFMM_HIP(hipSetDevice(e.device));
+    hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)0;
+    begin_on(h, s);
+    run_matvec_ex(h, d_b, ldb, d_y, ldy, nrhs, mode, s);
+    end_on(h, s);
+  });
+}
+
+int afmm2d_matvec_ex(afmm2d_handle h, const double* b, int64_t ldb, double* y, int64_t ldy,
+                     int nrhs, int mode) {
+  if (!h || !b || !y || nrhs <= 0) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "bad argument");
+  if (ldb < h->e->n || ldy < h->e->n)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "leading dimension below N");
+  if (check_mode(mode) != AFMM2D_OK) return AFMM2D_ERR_INVALID_ARGUMENT;
+  if (!h->assembled) return fail(AFMM2D_ERR_STATE, "matvec before assemble");
+  return guarded([&] {
+    Engine& e = *h->e;
+    AFMM_HIP(hipSetDevice(e.device));
+    const int n = e.n;
+    const size_t col = sizeof(double) * n;
+    // device columns packed with leading dimension n
+    e.io_b.alloc(col * nrhs);
+    e.io_y.alloc(col * nrhs);
+    hipStream_t s = nullptr;  // the null stream plays the caller's stream
+    begin_on(h, s);
+    if (ldb == n) {
+      AFMM_HIP(hipMemcpyAsync(e.io_b.p, b, col * nrhs, hipMemcpyHostToDevice, s));
+    } else {
+      for (int r = 0; r < nrhs; r++)
+        AFMM_HIP(hipMemcpyAsync(e.io_b.as<double>() + (size_t)r * n, b + (size_t)r * ldb, col,
+                                hipMemcpyHostToDevice, s));
+    }
+    const int prec = mode & kModePrecisionMask;
+    if (prec == PREC_F64 && !(mode & kModeBlocked)) {
+      // one column at a time through the single-RHS passes (the reference's
+      // computeMatVecProduct per vector)
+      for (int r = 0; r < nrhs; r++)
+        run_matvec(h, e.io_b.as<double>() + (size_t)r * n, e.io_y.as<double>() + (size_t)r * n, s);
+    } else {
+      run_matvec_ex(h, e.io_b.as<double>(), n, e.io_y.as<double>(), n, nrhs, mode, s);
+    }
+    if (ldy == n) {
+      AFMM_HIP(hipMemcpyAsync(y, e.io_y.p, col * nrhs, hipMemcpyDeviceToHost, s));
+    } else {
+      for (int r = 0; r < nrhs; r++)
+        AFMM_HIP(hipMemcpyAsync(y + (size_t)r * ldy, e.io_y.as<double>() + (size_t)r * n, col,
+                                hipMemcpyDeviceToHost, s));
+    }
+    end_on(h, s);
+    AFMM_HIP(hipStreamSynchronize(s));
+    if (e.profiling) e.prof_collect();
+    e.last_qs.resize(n);
+    e.last_ps.resize(n);
+    for (int i = 0; i < n; i++) {
+      e.last_qs[i] = b[e.h_perm[i]];
+      e.last_ps[i] = y[e.h_perm[i]];
+    }
+    e.have_matvec = true;
   });
 }
 
 int afmm2d_matvec(afmm2d_handle h, const double* b, double* y, int nrhs) {
   if (!h || !b || !y || nrhs <= 0) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "bad argument");
-  if (!h->e->assembled) return fail(AFMM2D_ERR_STATE, "matvec before assemble");
+  return afmm2d_matvec_ex(h, b, h->e->n, y, h->e->n, nrhs, AFMM2D_PRECISION_FP64);
+}
+
+int afmm2d_prepare_precision(afmm2d_handle h, int precision) {
+  if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (precision < 0 || precision >= kPrecisions)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown precision");
+  if (!h->assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
   return guarded([&] {
-    Engine& e = *h->e;
-    AFMM_HIP(hipSetDevice(e.device));
-    const size_t nb = sizeof(double) * e.n;
-    e.io_b.alloc(nb);
-    e.io_y.alloc(nb);
-    hipStream_t s = nullptr;  // the null stream plays the caller's stream
-    for (int r = 0; r < nrhs; r++) {
-      AFMM_HIP(hipMemcpyAsync(e.io_b.p, b + (size_t)r * e.n, nb, hipMemcpyHostToDevice, s));
-      run_matvec(h, e.io_b.as<double>(), e.io_y.as<double>(), s);
-      AFMM_HIP(hipMemcpyAsync(y + (size_t)r * e.n, e.io_y.p, nb, hipMemcpyDeviceToHost, s));
-      AFMM_HIP(hipStreamSynchronize(s));
-      if (e.profiling) e.prof_collect();
-      if (r == 0) {
-        e.last_qs.resize(e.n);
-        e.last_ps.resize(e.n);
-        for (int i = 0; i < e.n; i++) {
-          e.last_qs[i] = b[e.h_perm[i]];
-          e.last_ps[i] = y[e.h_perm[i]];
-        }
-        e.have_matvec = true;
-      }
+    for (Engine* ep : {h->e.get(), h->e2.get()}) {
+      if (!ep) continue;
+      AFMM_HIP(hipSetDevice(ep->device));
+      if (h->have_done) AFMM_HIP(hipEventSynchronize(h->done));
+      mr_prepare(*ep, precision);
     }
   });
 }
@@ -613,7 +735,7 @@ int afmm2d_get_interactions(afmm2d_handle h, int level, int* nbr, int* inner, in
 int afmm2d_get_ranks(afmm2d_handle h, int level, int* ranks) {
   if (!h || !ranks) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
   Engine& e = *h->e;
-  if (!e.assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
+  if (!h->assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
   if (level < 2 || level > e.L) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "level out of range");
   std::memcpy(ranks, e.lv[level].rank.data(), sizeof(int) * e.lv[level].nb);
   return AFMM2D_OK;
@@ -622,7 +744,7 @@ int afmm2d_get_ranks(afmm2d_handle h, int level, int* ranks) {
 int afmm2d_get_skeleton(afmm2d_handle h, int level, int box, int* idx, int capacity, int* count) {
   if (!h || !count) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
   Engine& e = *h->e;
-  if (!e.assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
+  if (!h->assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
   if (level < 2 || level > e.L || box < 0 || box >= e.lv[level].nb)
     return fail(AFMM2D_ERR_INVALID_ARGUMENT, "level/box out of range");
   return guarded([&] {
@@ -638,7 +760,7 @@ int afmm2d_get_l2p(afmm2d_handle h, int level, int box, double* out, int64_t cap
                    int* cols) {
   if (!h || !rows || !cols) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
   Engine& e = *h->e;
-  if (!e.assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
+  if (!h->assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
   if (level < 2 || level > e.L || box < 0 || box >= e.lv[level].nb)
     return fail(AFMM2D_ERR_INVALID_ARGUMENT, "level/box out of range");
   return guarded([&] {
@@ -668,7 +790,7 @@ int afmm2d_get_m2l(afmm2d_handle h, int level, int box, int slot, double* out, i
                    int* rows, int* cols) {
   if (!h || !rows || !cols) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
   Engine& e = *h->e;
-  if (!e.assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
+  if (!h->assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
   if (level < 2 || level > e.L || box < 0 || box >= e.lv[level].nb || slot < 0 || slot >= 40)
     return fail(AFMM2D_ERR_INVALID_ARGUMENT, "level/box/slot out of range");
   const LevelInfo& li = e.lv[level];
@@ -717,6 +839,7 @@ int afmm2d_direct_rows(afmm2d_handle h, int nrows, const int64_t* rows, const do
       srows[r] = inv[rows[r]];
     }
     hipStream_t s = nullptr;  // the null stream plays the caller's stream
+    begin_on(h, s);
     DevBuf d_b, d_qs, d_rows, d_out;
     d_b.alloc(sizeof(double) * e.n);
     d_qs.alloc(sizeof(double) * e.n);
@@ -742,6 +865,7 @@ int afmm2d_near_field(afmm2d_handle h, const double* b, double* y) {
     e.io_b.alloc(nb);
     e.io_y.alloc(nb);
     hipStream_t s = nullptr;
+    begin_on(h, s);
     AFMM_HIP(hipMemcpyAsync(e.io_b.p, b, nb, hipMemcpyHostToDevice, s));
     near_field_device(e, e.io_b.as<double>(), e.io_y.as<double>(), s);
     if (h->e2) {
@@ -751,6 +875,7 @@ int afmm2d_near_field(afmm2d_handle h, const double* b, double* y) {
       add_device(e.io_y.as<double>(), e2.io_y.as<const double>(), e.n, s);
     }
     AFMM_HIP(hipMemcpyAsync(y, e.io_y.p, nb, hipMemcpyDeviceToHost, s));
+    end_on(h, s);
     AFMM_HIP(hipStreamSynchronize(s));
     if (e.profiling) e.prof_collect();
   });
@@ -855,6 +980,8 @@ int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st) {
   st->m2l_leaf_columns = e.assembled ? e.lv[e.L].m2l_cols : 0;
   st->skeleton_leaf = e.assembled ? e.lv[e.L].soff[e.lv[e.L].nb] - e.lv[e.L].soff[0] : 0;
   st->near_waves = e.p2p_waves_per_simd;
+  st->blocked_m2l_coeffs = e.mr_m_coeffs;
+  st->blocked_basis_coeffs = e.mr_v_coeffs;
   return AFMM2D_OK;
 }
 

@@ -1,0 +1,1266 @@
+// blocked.hip -- the blocked matvec: up to 32 right-hand sides per pass, the
+// factors in FP64, FP32 or BF16, every product on the matrix cores (MFMA).
+//
+// The reference applies its operator to one vector at a time
+// (AFMM2D.cpp:46-65: gather, M2M, M2L, L2L, near field, scatter;
+// AFMM2DTree.cpp:1129-1225), each pass a set of small GEMVs.  With R vectors
+// every pass becomes a set of small GEMMs with R columns, and one read of a
+// factor serves all R of them:
+//
+//   M2M (levels L..2)  out_k = V_k^T src_k            src = leaf charges or the
+//                                                     children's multipoles
+//   M2L (all levels)   in_k  = M_k [out_b1; ..; out_bm]   M_k = [M2L(k, b1) .. M2L(k, bm)],
+//                                                     the box's interaction list in
+//                                                     slot order (inner 0..15, outer
+//                                                     0..23), as the reference's ACA
+//                                                     columns (AFMM2DTree.cpp:997-1034)
+//   L2L (levels 2..L-1) in_children += V_k in_k
+//   leaf               pot_k = V_k in_k
+//   near field         near_k = sum over the leaf's neighbours N0..N7 and
+//                      itself of K(leaf, source) q_source (evaluated here)
+//   output             y = pot + near, scattered to the caller's order
+//
+// All five are one kernel each over a list of tasks (MrTask / MrNearTask)
+// with absolute pointers, so one launch can carry the boxes of many problems
+// (the batched entry points use that).  A task is one wave: a block of up to
+// 64 output rows x 16 NCT columns held in MFMA accumulators, the contraction
+// in chunks of 16 (FP64, FP32: four v_mfma_*_16x16x4) or 32 (BF16: one
+// v_mfma_f32_16x16x32_bf16 per half of the split vector), the next chunk's
+// loads issued before the current one is used.
+//
+// MFMA lane layouts (checked on gfx950): lane l = i + 16 g with i = l mod 16,
+// g = l / 16.  A[i][k] and B[k][j] sit in lane i + 16 g / j + 16 g for the
+// k of group g; here lane group g holds k = k0 + 4 g + s in MFMA s (so each
+// lane reads four consecutive k of an A row), or k0 + 8 g + t for BF16.  The
+// 16 x 16 result D[r][j] sits in lane j + 16 g: FP64 r = 4 v + g, FP32 and
+// BF16 r = 4 g + v, for register v.
+//
+// Precisions.  FP64 stores the reference's interpolative basis (L2P with its
+// skeleton rows the exact identity) and K(skeleton_k, skeleton_b) blocks.
+// Rounding those factors to FP32 amplifies: the multipoles of an
+// interpolative basis are large and cancel inside the smooth M2L blocks
+// (measured on the CPU: 4e-6 relative error with FP32 factors at N = 16384,
+// tolerance 1e-12).  FP32 and BF16 therefore store an orthonormal basis of
+// the same operator: bottom-up, E_k = L2P_k (leaves) or diag(R_children)
+// L2P_k, E_k = Q_k R_k (Gram-Schmidt twice, FP64), V_k = Q_k and
+// M2L'(k, b) = R_k M2L(k, b) R_b^T (FP64, then rounded).  The near field
+// of FP32 and BF16 evaluates the kernel in FP32 (FP64 coordinate differences
+// rounded to FP32); BF16 rounds the kernel values and splits the FP32
+// vectors into two BF16 halves (hi + lo), so its error is that of the BF16
+// factors.  FP32 and BF16 accumulate in FP32 on the matrix cores, add the
+// near field's per-source partials with compensated (two-sum) summation,
+// and form the output y = pot + near in FP64.
+#include <algorithm>
+#include <cstring>
+
+#include "engine.hpp"
+#include "near_math.hpp"
+
+namespace afmm {
+
+namespace {
+
+typedef double mr_d4 __attribute__((ext_vector_type(4)));
+typedef float mr_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mr_bf8 __attribute__((ext_vector_type(8)));
+typedef double mr_d2 __attribute__((ext_vector_type(2)));
+typedef uint32_t mr_u4 __attribute__((ext_vector_type(4)));
+
+template <int P>
+struct MrT;
+template <>
+struct MrT<PREC_F64> {
+  using A = double;
+  using V = double;
+  using Acc = mr_d4;
+  static constexpr int KC = 16;
+};
+template <>
+struct MrT<PREC_F32> {
+  using A = float;
+  using V = float;
+  using Acc = mr_f4;
+  static constexpr int KC = 16;
+};
+template <>
+struct MrT<PREC_BF16> {
+  using A = uint16_t;  // bf16 bits
+  using V = float;
+  using Acc = mr_f4;
+  static constexpr int KC = 32;
+};
+
+// Pointers taken from task records are generic; loads through the global
+// address space avoid flat instructions.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gptr(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gptr_w(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
+inline size_t factor_bytes(int prec) { return prec == PREC_F64 ? 8 : prec == PREC_F32 ? 4 : 2; }
+inline size_t vector_bytes(int prec) { return prec == PREC_F64 ? 8 : 4; }
+
+// Output row and column of accumulator register v of the 16 x 16 tile.
+template <int P>
+__device__ __forceinline__ void acc_pos(int i, int g, int v, int& row, int& col) {
+  if constexpr (P == PREC_F64) {
+    row = 4 * v + g;
+    col = i;
+  } else {
+    row = 4 * g + v;
+    col = i;
+  }
+}
+
+__device__ __forceinline__ mr_bf8 bf16_pack(const float (&x)[8]) {
+  mr_bf8 r;
+#pragma unroll
+  for (int t = 0; t < 8; t++) r[t] = (__bf16)x[t];
+  return r;
+}
+
+// x = hi + lo with both halves in BF16: the product with a BF16 factor
+// carries the vector to about 2^-17 relative.
+__device__ __forceinline__ void bf16_split(const float (&x)[8], mr_bf8& hi, mr_bf8& lo) {
+#pragma unroll
+  for (int t = 0; t < 8; t++) {
+    const __bf16 h = (__bf16)x[t];
+    hi[t] = h;
+    lo[t] = (__bf16)(x[t] - (float)h);
+  }
+}
+
+// Workgroup index mapped so that each XCD (workgroups go to the 8 XCDs
+// round robin) takes one contiguous range of tasks: neighbouring boxes share
+// interaction-list vectors, which then meet in the same L2.
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+  const int x = b & 7, idx = b >> 3, q = nblk >> 3, rem = nblk & 7;
+  return x * q + min(x, rem) + idx;
+}
+
+// ---------------------------------------------------------------------------
+// The batched product C (+)= A B over a task list, one task per wave.
+template <int P, int NCT>
+struct GemmFrag {
+  using T = MrT<P>;
+  // FP64/FP32: a[rt][s] is A[row][k0 + 4 g + s]; BF16: a[rt] holds 8 k.
+  typename T::A a[4][P == PREC_BF16 ? 8 : 4];
+  float bf[P == PREC_BF16 ? 8 : 1][NCT];               // BF16: the vector before the split
+  typename T::V b[P == PREC_BF16 ? 1 : 4][NCT];         // FP64/FP32: B[k0 + 4 g + s][j]
+};
+
+template <int P, int NCT, int NRT>
+__device__ __forceinline__ void gemm_load(GemmFrag<P, NCT>& f, const MrTask& d,
+                                          const int (&arow)[4], int k0, int i, int g) {
+  using T = MrT<P>;
+  using AT = typename T::A;
+  using VT = typename T::V;
+  constexpr int RP = 16 * NCT;
+  const auto A = gptr(static_cast<const AT*>(d.A));
+  const auto B = gptr(static_cast<const VT*>(d.B));
+  const auto bidx = gptr(d.bidx);
+  constexpr int KG = P == PREC_BF16 ? 8 : 4;  // k per lane group
+  // a lane's k run starts at k0 + KG g; runs past lda are clamped back (their
+  // B entries are zero: k >= lda >= K)
+  const int ks = min(k0 + KG * g, d.lda - KG);
+#pragma unroll
+  for (int rt = 0; rt < NRT; rt++) {
+    {
+      const auto p = A + (int64_t)arow[rt] * d.lda + ks;
+      if constexpr (P == PREC_F64) {
+        const mr_d2 u = ((const __attribute__((address_space(1))) mr_d2*)p)[0];
+        const mr_d2 v = ((const __attribute__((address_space(1))) mr_d2*)p)[1];
+        f.a[rt][0] = u.x;
+        f.a[rt][1] = u.y;
+        f.a[rt][2] = v.x;
+        f.a[rt][3] = v.y;
+      } else if constexpr (P == PREC_F32) {
+        const mr_f4 u = *((const __attribute__((address_space(1))) mr_f4*)p);
+        f.a[rt][0] = u.x;
+        f.a[rt][1] = u.y;
+        f.a[rt][2] = u.z;
+        f.a[rt][3] = u.w;
+      } else {
+        const mr_u4 u = *((const __attribute__((address_space(1))) mr_u4*)p);
+        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          f.a[rt][2 * t] = (uint16_t)(w[t] & 0xFFFF);
+          f.a[rt][2 * t + 1] = (uint16_t)(w[t] >> 16);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < KG; s++) {
+    const int k = k0 + KG * g + s;
+    const bool ok = k < d.K;
+    const int kc = ok ? k : d.K - 1;
+    const int64_t row = d.bidx ? (int64_t)bidx[d.b0 + kc] : d.b0 + kc;
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      const VT v = B[row * RP + 16 * ct + i];
+      if constexpr (P == PREC_BF16) f.bf[s][ct] = ok ? v : 0.0f;
+      else f.b[s][ct] = ok ? v : (VT)0;
+    }
+  }
+}
+
+template <int P, int NCT, int NRT>
+__device__ __forceinline__ void gemm_mma(const GemmFrag<P, NCT>& f,
+                                         typename MrT<P>::Acc (&acc)[4][NCT]) {
+  if constexpr (P == PREC_BF16) {
+    mr_bf8 bh[NCT], bl[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      float x[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++) x[t] = f.bf[t][ct];
+      bf16_split(x, bh[ct], bl[ct]);
+    }
+#pragma unroll
+    for (int rt = 0; rt < NRT; rt++) {
+      mr_bf8 a;
+#pragma unroll
+      for (int t = 0; t < 8; t++) a[t] = __builtin_bit_cast(__bf16, f.a[rt][t]);
+#pragma unroll
+      for (int ct = 0; ct < NCT; ct++) {
+        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[ct], acc[rt][ct], 0, 0, 0);
+        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl[ct], acc[rt][ct], 0, 0, 0);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int rt = 0; rt < NRT; rt++)
+#pragma unroll
+          for (int ct = 0; ct < NCT; ct++) {
+            if constexpr (P == PREC_F64)
+              acc[rt][ct] =
+                  __builtin_amdgcn_mfma_f64_16x16x4f64(f.a[rt][s], f.b[s][ct], acc[rt][ct], 0, 0, 0);
+            else
+              acc[rt][ct] =
+                  __builtin_amdgcn_mfma_f32_16x16x4f32(f.a[rt][s], f.b[s][ct], acc[rt][ct], 0, 0, 0);
+          }
+  }
+}
+
+template <int P, int NCT, int NRT>
+__global__ __launch_bounds__(256) void k_mr_gemm(const MrTask* __restrict__ tasks, int ntasks,
+                                                 int xcd_map) {
+  using T = MrT<P>;
+  using VT = typename T::V;
+  using Acc = typename T::Acc;
+  constexpr int RP = 16 * NCT;
+  constexpr int KC = T::KC;
+  const int l = threadIdx.x & 63, i = l & 15, g = l >> 4;
+  const int blk = xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int t = __builtin_amdgcn_readfirstlane(blk * 4 + (threadIdx.x >> 6));
+  if (t >= ntasks) return;
+  const MrTask d = tasks[t];
+  const int M = d.M, K = d.K;
+  int arow[4];
+#pragma unroll
+  for (int rt = 0; rt < 4; rt++) arow[rt] = min(16 * rt + i, M - 1);
+  Acc acc[4][NCT];
+#pragma unroll
+  for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = Acc{};
+  if (K > 0) {
+    GemmFrag<P, NCT> f0, f1;
+    gemm_load<P, NCT, NRT>(f0, d, arow, 0, i, g);
+    for (int k0 = 0;; k0 += 2 * KC) {
+      if (k0 + KC < K) gemm_load<P, NCT, NRT>(f1, d, arow, k0 + KC, i, g);
+      gemm_mma<P, NCT, NRT>(f0, acc);
+      if (k0 + KC >= K) break;
+      if (k0 + 2 * KC < K) gemm_load<P, NCT, NRT>(f0, d, arow, k0 + 2 * KC, i, g);
+      gemm_mma<P, NCT, NRT>(f1, acc);
+      if (k0 + 2 * KC >= K) break;
+    }
+  }
+  const auto C = gptr_w(static_cast<VT*>(d.C));
+#pragma unroll
+  for (int rt = 0; rt < NRT; rt++) {
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        int row, col;
+        acc_pos<P>(i, g, v, row, col);
+        row += 16 * rt;
+        if (row < M) {
+          const auto p = C + (d.c0 + row) * RP + 16 * ct + col;
+          const VT x = (VT)acc[rt][ct][v];
+          *p = d.beta ? *p + x : x;
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Near field: one wave per target group, the kernel evaluated on the fly in
+// the MFMA A layout (lane i + 16 g: target row i against sources
+// k0 + 4 g + s, or k0 + 8 g + t for BF16), the sources' charges as B.
+__device__ __forceinline__ void two_sum(float& s, float& c, float x) {
+  const float t = s + x;
+  const float bp = t - s;
+  c += (s - (t - bp)) + (x - bp);
+  s = t;
+}
+
+template <int P, int NCT, int NRT, int KID>
+__global__ __launch_bounds__(256) void k_mr_p2p(const MrNearTask* __restrict__ tasks, int ntasks,
+                                                const MrNearProb* __restrict__ probs,
+                                                const double2* __restrict__ logtab,
+                                                double ln2_hi, double ln2_lo) {
+  using T = MrT<P>;
+  using VT = typename T::V;
+  using Acc = typename T::Acc;
+  constexpr int RP = 16 * NCT;
+  constexpr int KC = T::KC;
+  __shared__ double2 tab[128];
+  if constexpr (P == PREC_F64 && KID == K_LOG) {
+    for (int u = threadIdx.x; u < 128; u += blockDim.x) tab[u] = logtab[u];
+    __syncthreads();
+  }
+  const int l = threadIdx.x & 63, i = l & 15, g = l >> 4;
+  const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (t >= ntasks) return;
+  const MrNearTask& d = tasks[t];
+  const MrNearProb& pr = probs[d.prob];
+  const KParams kp = pr.kp;
+  const auto X = gptr(pr.x);
+  const auto Y = gptr(pr.y);
+  const auto Q = gptr(static_cast<const VT*>(pr.q));
+  const int M = d.M, nsrc = d.nsrc;
+  double xt[4], yt[4];
+#pragma unroll
+  for (int rt = 0; rt < 4; rt++) {
+    const int r = d.ta + min(16 * rt + i, M - 1);
+    xt[rt] = X[r];
+    yt[rt] = Y[r];
+  }
+  const double ninv_p0 = -1.0 / kp.p0, inv_a = 1.0 / kp.a;
+  // FP32 constants: log 0.5 ln 2; Gaussian -log2(e)/p0; a/R 1/a and a
+  const float c0 = KID == K_LOG ? 0.34657359f
+                   : KID == K_GAUSS ? (float)(-1.4426950408889634 / kp.p0)
+                                    : (float)inv_a;
+  const float c1 = (float)kp.a;
+  const double scale = KID == K_LOG ? 0.5 : 1.0;
+  Acc acc[4][NCT];  // FP64: the sum; FP32/BF16: one source's partial
+  Acc sh[4][NCT], sl[4][NCT];
+#pragma unroll
+  for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      acc[rt][ct] = Acc{};
+      sh[rt][ct] = Acc{};
+      sl[rt][ct] = Acc{};
+    }
+  for (int src = 0; src < nsrc; src++) {
+    const int st = __builtin_amdgcn_readfirstlane(d.ss[src]);
+    const int cnt = __builtin_amdgcn_readfirstlane(d.sc[src]);
+    if constexpr (P != PREC_F64) {
+#pragma unroll
+      for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = Acc{};
+    }
+    for (int k0 = 0; k0 < cnt; k0 += KC) {
+      if constexpr (P == PREC_BF16) {
+        float bq[8][NCT];
+        double sx[8], sy[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int k = k0 + 8 * g + u;
+          const bool ok = k < cnt;
+          const int kc = st + (ok ? k : cnt - 1);
+          sx[u] = X[kc];
+          sy[u] = Y[kc];
+#pragma unroll
+          for (int ct = 0; ct < NCT; ct++) {
+            const float v = Q[(int64_t)kc * RP + 16 * ct + i];
+            bq[u][ct] = ok ? v : 0.0f;
+          }
+        }
+        mr_bf8 bh[NCT], bl[NCT];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) {
+          float x[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) x[u] = bq[u][ct];
+          bf16_split(x, bh[ct], bl[ct]);
+        }
+#pragma unroll
+        for (int rt = 0; rt < NRT; rt++) {
+          float kv[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) kv[u] = near_kernel_f32<KID>(xt[rt], yt[rt], sx[u], sy[u], kp, c0, c1);
+          const mr_bf8 a = bf16_pack(kv);
+#pragma unroll
+          for (int ct = 0; ct < NCT; ct++) {
+            acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[ct], acc[rt][ct], 0, 0, 0);
+            acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl[ct], acc[rt][ct], 0, 0, 0);
+          }
+        }
+      } else {
+        VT bq[4][NCT];
+        double sx[4], sy[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          const int k = k0 + 4 * g + s;
+          const bool ok = k < cnt;
+          const int kc = st + (ok ? k : cnt - 1);
+          sx[s] = X[kc];
+          sy[s] = Y[kc];
+#pragma unroll
+          for (int ct = 0; ct < NCT; ct++) {
+            const VT v = Q[(int64_t)kc * RP + 16 * ct + i];
+            bq[s][ct] = ok ? v : (VT)0;
+          }
+        }
+#pragma unroll
+        for (int rt = 0; rt < NRT; rt++) {
+          {
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+              if constexpr (P == PREC_F64) {
+                double kv;
+                if constexpr (KID >= K_USER) {
+                  kv = kernel_xy<KID>(xt[rt], yt[rt], sx[s], sy[s], kp);
+                } else {
+                  const double dx = xt[rt] - sx[s], dy = yt[rt] - sy[s];
+                  kv = scale * near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab,
+                                                ln2_hi, ln2_lo);
+                }
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++)
+                  acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv, bq[s][ct], acc[rt][ct], 0, 0, 0);
+              } else {
+                const float kv = near_kernel_f32<KID>(xt[rt], yt[rt], sx[s], sy[s], kp, c0, c1);
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++)
+                  acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(kv, bq[s][ct], acc[rt][ct], 0, 0, 0);
+              }
+            }
+          }
+        }
+      }
+    }
+    if constexpr (P != PREC_F64) {
+#pragma unroll
+      for (int rt = 0; rt < NRT; rt++)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            float s_ = sh[rt][ct][v], c_ = sl[rt][ct][v];
+            two_sum(s_, c_, acc[rt][ct][v]);
+            sh[rt][ct][v] = s_;
+            sl[rt][ct][v] = c_;
+          }
+    }
+  }
+  const auto out = gptr_w(pr.out);
+#pragma unroll
+  for (int rt = 0; rt < NRT; rt++) {
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        int row, col;
+        acc_pos<P>(i, g, v, row, col);
+        row += 16 * rt;
+        if (row < M) {
+          double val;
+          if constexpr (P == PREC_F64) val = acc[rt][ct][v];
+          else val = (double)sh[rt][ct][v] + (double)sl[rt][ct][v];
+          out[(int64_t)(d.out_row + row) * RP + 16 * ct + col] = val;
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gather: qs[inv_perm[i]][c] = b[i + c ldb] (0 for c >= nrhs).  Lane c of a
+// row of RP threads, RP-column rows written whole, b read along columns.
+// Workgroup w takes caller points [w kMrIoPoints, (w + 1) kMrIoPoints) of
+// d.cnt.
+template <class VT, int NCT>
+__global__ __launch_bounds__(256) void k_mr_gather(MrIoTask d) {
+  constexpr int RP = 16 * NCT, PPI = 256 / RP;
+  const int c = threadIdx.x % RP, pl = threadIdx.x / RP;
+  const int i0 = blockIdx.x * kMrIoPoints, cnt = min(kMrIoPoints, d.cnt - i0);
+  VT* qs = static_cast<VT*>(d.qs);
+  for (int p = pl; p < cnt; p += PPI) {
+    const int64_t i = i0 + p;
+    const double v = c < d.nrhs ? d.b[i + c * d.ldb] : 0.0;
+    qs[(int64_t)d.inv_perm[i] * RP + c] = (VT)v;
+  }
+}
+
+// Output: y[i + c ldy] = pot[inv_perm[i]][c] + near[inv_perm[i]][c] in FP64.
+template <class VT, int NCT>
+__global__ __launch_bounds__(256) void k_mr_out(MrIoTask d) {
+  constexpr int RP = 16 * NCT, PPI = 256 / RP;
+  const int c = threadIdx.x % RP, pl = threadIdx.x / RP;
+  const int i0 = blockIdx.x * kMrIoPoints, cnt = min(kMrIoPoints, d.cnt - i0);
+  const VT* pot = static_cast<const VT*>(d.pot);
+  for (int p = pl; p < cnt; p += PPI) {
+    const int64_t i = i0 + p;
+    const int64_t si = d.inv_perm[i];
+    if (c < d.nrhs) d.y[i + c * d.ldy] = (double)pot[si * RP + c] + d.nearv[si * RP + c];
+  }
+}
+
+__global__ void k_add_cols(double* __restrict__ y, int64_t ldy, const double* __restrict__ t,
+                           int64_t ldt, int n, int nrhs) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * nrhs) return;
+  const int64_t c = e / n, i = e - c * n;
+  y[i + c * ldy] += t[i + c * ldt];
+}
+
+// ---------------------------------------------------------------------------
+// Factor construction.
+struct MrBoxV {
+  int64_t l2poff, rmbase, voff, vtoff;
+  int n, r, vld, vtld;
+};
+
+// FP64 basis: V = the box's L2P with its skeleton rows the exact identity
+// (row rowmap[nc + q] is e_q, stored row t is row rowmap[t]), and V^T.
+__global__ __launch_bounds__(256) void k_mr_build_v64(const MrBoxV* __restrict__ boxes,
+                                                      const double* __restrict__ l2p_rm,
+                                                      const int* __restrict__ rowmap,
+                                                      double* __restrict__ V,
+                                                      double* __restrict__ Vt) {
+  __shared__ int inv_row[1024];
+  const MrBoxV d = boxes[blockIdx.x];
+  const int n = d.n, r = d.r, nc = n - r;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) inv_row[rowmap[d.rmbase + t]] = t;
+  __syncthreads();
+  auto val = [&](int row, int c) -> double {
+    const int t = inv_row[row];
+    return t < nc ? l2p_rm[d.l2poff + (int64_t)t * r + c] : (t - nc == c ? 1.0 : 0.0);
+  };
+  for (int e = threadIdx.x; e < n * d.vld; e += blockDim.x) {
+    const int row = e / d.vld, c = e - row * d.vld;
+    V[d.voff + e] = c < r ? val(row, c) : 0.0;
+  }
+  for (int e = threadIdx.x; e < r * d.vtld; e += blockDim.x) {
+    const int c = e / d.vtld, row = e - c * d.vtld;
+    Vt[d.vtoff + e] = row < n ? val(row, c) : 0.0;
+  }
+}
+
+struct MrBoxM {
+  int64_t soff, moff, coloff;
+  int r, K, mld, pad;
+};
+
+// FP64 M2L row block: M[i][c] = K(skeleton_k[i], point colidx[c]), the same
+// formula as the single-RHS pair blocks.
+template <int KID>
+__global__ __launch_bounds__(256) void k_mr_build_m64(const MrBoxM* __restrict__ boxes,
+                                                      const double* __restrict__ skx,
+                                                      const double* __restrict__ sky,
+                                                      const int* __restrict__ colidx,
+                                                      double* __restrict__ Mout, KParams kp) {
+  const MrBoxM d = boxes[blockIdx.x];
+  const int64_t tot = (int64_t)d.r * d.mld;
+  for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int i = (int)(e / d.mld), c = (int)(e - (int64_t)i * d.mld);
+    double v = 0.0;
+    if (c < d.K) {
+      const int q = colidx[d.coloff + c];
+      v = kernel_xy<KID>(skx[d.soff + i], sky[d.soff + i], skx[q], sky[q], kp);
+    }
+    Mout[d.moff + e] = v;
+  }
+}
+
+struct MrBoxQR {
+  int64_t voff, roff;
+  int64_t croff[4];  // children's R offsets
+  int n, r, vld, nchild;
+  int crank[4];      // children's ranks (their row blocks, in order)
+};
+
+// Orthonormal basis of one box: E = V (leaves) or diag(R_children) V, then
+// E = Q R by classical Gram-Schmidt with one re-orthogonalisation (FP64).
+// Q overwrites Qout (n x r, leading dimension vld), R goes to Rout (r x r).
+__global__ __launch_bounds__(256) void k_mr_qr(const MrBoxQR* __restrict__ boxes,
+                                               const double* __restrict__ V,
+                                               double* __restrict__ Qout,
+                                               double* __restrict__ Rbuf) {
+  __shared__ double vs[1024];
+  __shared__ double hs[4][256];
+  __shared__ double h[256];
+  __shared__ double red[4];
+  const MrBoxQR d = boxes[blockIdx.x];
+  const int n = d.n, r = d.r, ld = d.vld;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const double* Vb = V + d.voff;
+  double* Q = Qout + d.voff;
+  double* R = Rbuf + d.roff;
+  for (int e = tid; e < r * r; e += 256) R[e] = 0.0;
+  // row offsets of the children's blocks
+  int cro[5];
+  cro[0] = 0;
+  for (int u = 0; u < 4; u++) cro[u + 1] = cro[u] + (u < d.nchild ? d.crank[u] : 0);
+  __syncthreads();
+  for (int c = 0; c < r; c++) {
+    // column c of E
+    for (int row = tid; row < n; row += 256) {
+      double v;
+      if (d.nchild == 0) {
+        v = Vb[(int64_t)row * ld + c];
+      } else {
+        int u = 0;
+        while (u < 3 && row >= cro[u + 1]) u++;
+        const int lr = row - cro[u], rc = d.crank[u];
+        const double* Rc = Rbuf + d.croff[u];
+        v = 0.0;
+        for (int p = lr; p < rc; p++) v = fma(Rc[(int64_t)lr * rc + p], Vb[(int64_t)(cro[u] + p) * ld + c], v);
+      }
+      vs[row] = v;
+    }
+    __syncthreads();
+    for (int pass = 0; pass < 2 && c > 0; pass++) {
+      // h = Q[:, :c]^T v: wave w sums rows w, w + 4, ..; lane p columns p, p + 64, ..
+      for (int p = lane; p < c; p += 64) {
+        double s = 0.0;
+        for (int row = w; row < n; row += 4) s = fma(Q[(int64_t)row * ld + p], vs[row], s);
+        hs[w][p] = s;
+      }
+      __syncthreads();
+      for (int p = tid; p < c; p += 256) {
+        const double hp = (hs[0][p] + hs[1][p]) + (hs[2][p] + hs[3][p]);
+        h[p] = hp;
+        R[(int64_t)p * r + c] += hp;
+      }
+      __syncthreads();
+      for (int row = tid; row < n; row += 256) {
+        double v = vs[row];
+        for (int p = 0; p < c; p++) v = fma(-Q[(int64_t)row * ld + p], h[p], v);
+        vs[row] = v;
+      }
+      __syncthreads();
+    }
+    double s = 0.0;
+    for (int row = tid; row < n; row += 256) s = fma(vs[row], vs[row], s);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) red[w] = s;
+    __syncthreads();
+    const double nrm = sqrt((red[0] + red[1]) + (red[2] + red[3]));
+    const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    for (int row = tid; row < n; row += 256) Q[(int64_t)row * ld + c] = vs[row] * inv;
+    if (tid == 0) R[(int64_t)c * r + c] = nrm;
+    __syncthreads();
+  }
+  // padding columns of Q stay zero
+  for (int e = tid; e < n * (ld - r); e += 256) {
+    const int row = e / (ld - r), c = r + e % (ld - r);
+    Q[(int64_t)row * ld + c] = 0.0;
+  }
+}
+
+constexpr int kMaxILBlocks = 27;
+struct MrBoxMP {
+  int64_t moff, roff;
+  int r, K, mld, nblk;
+  int boff[kMaxILBlocks], brank[kMaxILBlocks];
+  int64_t broff[kMaxILBlocks];
+};
+
+// M2L' = R_k M R_b^T block by block (FP64), written in the factor type.
+template <class AT>
+__global__ __launch_bounds__(256) void k_mr_mprime(const MrBoxMP* __restrict__ boxes,
+                                                   const double* __restrict__ M64,
+                                                   const double* __restrict__ Rbuf,
+                                                   AT* __restrict__ Mout) {
+  __shared__ double T[4096];
+  const MrBoxMP& d = boxes[blockIdx.x];
+  const int r = d.r, ld = d.mld;
+  const double* Rk = Rbuf + d.roff;
+  const double* Mk = M64 + d.moff;
+  AT* Ok = Mout + d.moff;
+  // padding columns
+  for (int64_t e = threadIdx.x; e < (int64_t)r * (ld - d.K); e += blockDim.x) {
+    const int i = (int)(e / (ld - d.K)), c = d.K + (int)(e % (ld - d.K));
+    Ok[(int64_t)i * ld + c] = (AT)0.0f;
+  }
+  for (int b = 0; b < d.nblk; b++) {
+    const int rb = d.brank[b], off = d.boff[b];
+    const double* Rb = Rbuf + d.broff[b];
+    const int RI = max(1, min(r, 4096 / max(rb, 1)));  // rows of T per chunk
+    for (int i0 = 0; i0 < r; i0 += RI) {
+      const int ni = min(RI, r - i0);
+      __syncthreads();
+      // T[ii][w] = sum_{p >= i} R_k[i][p] M[p][off + w]
+      for (int e = threadIdx.x; e < ni * rb; e += blockDim.x) {
+        const int ii = e / rb, wv = e - ii * rb, ix = i0 + ii;
+        double s = 0.0;
+        for (int p = ix; p < r; p++) s = fma(Rk[(int64_t)ix * r + p], Mk[(int64_t)p * ld + off + wv], s);
+        T[ii * rb + wv] = s;
+      }
+      __syncthreads();
+      // out[i][off + u] = sum_{w >= u} T[i][w] R_b[u][w]
+      for (int e = threadIdx.x; e < ni * rb; e += blockDim.x) {
+        const int ii = e / rb, u = e - ii * rb;
+        double s = 0.0;
+        for (int wv = u; wv < rb; wv++) s = fma(T[ii * rb + wv], Rb[(int64_t)u * rb + wv], s);
+        Ok[(int64_t)(i0 + ii) * ld + off + u] = (AT)(float)s;
+      }
+    }
+  }
+}
+
+// Q (FP64) -> V and V^T of the factor type.
+template <class AT>
+__global__ __launch_bounds__(256) void k_mr_convert_v(const MrBoxV* __restrict__ boxes,
+                                                      const double* __restrict__ Q,
+                                                      AT* __restrict__ V, AT* __restrict__ Vt) {
+  const MrBoxV d = boxes[blockIdx.x];
+  for (int e = threadIdx.x; e < d.n * d.vld; e += blockDim.x) V[d.voff + e] = (AT)(float)Q[d.voff + e];
+  for (int e = threadIdx.x; e < d.r * d.vtld; e += blockDim.x) {
+    const int c = e / d.vtld, row = e - c * d.vtld;
+    Vt[d.vtoff + e] = row < d.n ? (AT)(float)Q[d.voff + (int64_t)row * d.vld + c] : (AT)0.0f;
+  }
+}
+
+template <class T>
+void upload_vec(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
+  d.alloc(sizeof(T) * std::max<size_t>(h.size(), 1));
+  if (!h.empty()) {
+    AFMM_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+    AFMM_HIP(hipStreamSynchronize(s));
+  }
+}
+
+inline int round8(int v) { return (v + 7) & ~7; }
+
+int64_t level_rbase(const Engine& e, int j, int k) {
+  // where box (j, k)'s rowmap starts: a leaf's at leaf_start, an upper box's
+  // at N + the skeleton offset of its first child
+  return j == e.L ? e.leaf_start[k] : e.n + e.lv[j + 1].soff[4 * (size_t)k];
+}
+
+int slot_box(const Engine& e, int j, int k, int q) {
+  return q < 16 ? e.inner[j][16 * (size_t)k + q] : e.outer[j][24 * (size_t)k + q - 16];
+}
+
+// Layout shared by all precisions: offsets, leading dimensions, the column
+// index of every M2L row block.
+void mr_layout(Engine& e) {
+  if (e.mr_layout) return;
+  const int L = e.L;
+  const int64_t nbox = e.lv[L].gbase + e.lv[L].nb;
+  e.mr_voff.assign(nbox + 1, 0);
+  e.mr_vtoff.assign(nbox + 1, 0);
+  e.mr_moff.assign(nbox + 1, 0);
+  e.mr_coloff.assign(nbox + 1, 0);
+  e.mr_vld.assign(nbox, 8);
+  e.mr_vtld.assign(nbox, 8);
+  e.mr_mld.assign(nbox, 8);
+  e.mr_K.assign(nbox, 0);
+  std::vector<int> colidx;
+  int64_t vo = 0, vto = 0, mo = 0;
+  e.mr_m_coeffs = e.mr_v_coeffs = 0;
+  for (int j = 2; j <= L; j++) {
+    const LevelInfo& li = e.lv[j];
+    for (int k = 0; k < li.nb; k++) {
+      const int64_t g = li.gbase + k;
+      const int r = li.rank[k], n = li.nin[k];
+      int K = 0;
+      e.mr_coloff[g] = (int64_t)colidx.size();
+      for (int q = 0; q < 40; q++) {
+        const int b = slot_box(e, j, k, q);
+        if (b < 0) continue;
+        const int rb = li.rank[b];
+        for (int u = 0; u < rb; u++) colidx.push_back((int)(li.soff[b] + u));
+        K += rb;
+      }
+      e.mr_K[g] = K;
+      e.mr_vld[g] = round8(std::max(r, 1));
+      e.mr_vtld[g] = round8(std::max(n, 1));
+      e.mr_mld[g] = round8(std::max(K, 1));
+      e.mr_voff[g] = vo;
+      e.mr_vtoff[g] = vto;
+      e.mr_moff[g] = mo;
+      vo += (int64_t)n * e.mr_vld[g];
+      vto += (int64_t)r * e.mr_vtld[g];
+      mo += (int64_t)r * e.mr_mld[g];
+      e.mr_m_coeffs += (int64_t)r * K;
+      e.mr_v_coeffs += (int64_t)n * r;
+    }
+  }
+  e.mr_voff[nbox] = vo;
+  e.mr_vtoff[nbox] = vto;
+  e.mr_moff[nbox] = mo;
+  e.mr_coloff[nbox] = (int64_t)colidx.size();
+  e.mr_v_total = vo;
+  e.mr_vt_total = vto;
+  e.mr_m_total = mo;
+  if ((int64_t)colidx.size() > INT32_MAX) throw Error(6, "M2L column index exceeds 2^31 entries");
+  hipStream_t s = e.s_aux;
+  upload_vec(e.mr_colidx, colidx, s);
+  // caller index -> KD-sorted index
+  std::vector<int> ip(e.n);
+  for (int i = 0; i < e.n; i++) ip[e.h_perm[i]] = i;
+  upload_vec(e.inv_perm_d, ip, s);
+  e.mr_layout = true;
+}
+
+void build_f64(Engine& e) {
+  MrFactors& f = e.mrf[PREC_F64];
+  if (f.built) return;
+  mr_layout(e);
+  const int L = e.L;
+  hipStream_t s = e.s_aux;
+  f.V.alloc(sizeof(double) * std::max<int64_t>(e.mr_v_total, 8));
+  f.Vt.alloc(sizeof(double) * std::max<int64_t>(e.mr_vt_total, 8));
+  f.M.alloc(sizeof(double) * std::max<int64_t>(e.mr_m_total, 8));
+  std::vector<MrBoxV> bv;
+  std::vector<MrBoxM> bm;
+  for (int j = 2; j <= L; j++) {
+    const LevelInfo& li = e.lv[j];
+    for (int k = 0; k < li.nb; k++) {
+      const int64_t g = li.gbase + k;
+      const int r = li.rank[k], n = li.nin[k];
+      if (r > 0 && n > 0)
+        bv.push_back(MrBoxV{li.l2poff[k], level_rbase(e, j, k), e.mr_voff[g], e.mr_vtoff[g], n, r,
+                            e.mr_vld[g], e.mr_vtld[g]});
+      if (r > 0)
+        bm.push_back(MrBoxM{li.soff[k], e.mr_moff[g], e.mr_coloff[g], r, e.mr_K[g], e.mr_mld[g], 0});
+    }
+  }
+  // boxes of rank 0 keep zero factors (their tasks read none)
+  AFMM_HIP(hipMemsetAsync(f.V.p, 0, f.V.bytes, s));
+  AFMM_HIP(hipMemsetAsync(f.Vt.p, 0, f.Vt.bytes, s));
+  DevBuf dbv, dbm;
+  upload_vec(dbv, bv, s);
+  upload_vec(dbm, bm, s);
+  if (!bv.empty()) {
+    k_mr_build_v64<<<(unsigned)bv.size(), 256, 0, s>>>(dbv.as<MrBoxV>(), e.l2p_rm.as<double>(),
+                                                       e.rowmap.as<int>(), f.V.as<double>(),
+                                                       f.Vt.as<double>());
+    AFMM_LAUNCH_CHECK();
+  }
+  if (!bm.empty()) {
+    AFMM_DISPATCH_KERNEL(e.kid, KID,
+                         k_mr_build_m64<KID><<<(unsigned)bm.size(), 256, 0, s>>>(
+                             dbm.as<MrBoxM>(), e.skx.as<double>(), e.sky.as<double>(),
+                             e.mr_colidx.as<int>(), f.M.as<double>(), e.kp));
+    AFMM_LAUNCH_CHECK();
+  }
+  AFMM_HIP(hipStreamSynchronize(s));
+  f.built = true;
+}
+
+// The orthonormal basis (R of every box) and the reduced-precision factors.
+void build_reduced(Engine& e, int prec) {
+  MrFactors& f = e.mrf[prec];
+  if (f.built) return;
+  build_f64(e);
+  const MrFactors& f64 = e.mrf[PREC_F64];
+  const int L = e.L;
+  hipStream_t s = e.s_aux;
+  const int64_t nbox = e.lv[L].gbase + e.lv[L].nb;
+  DevBuf Q64;
+  Q64.alloc(sizeof(double) * std::max<int64_t>(e.mr_v_total, 8));
+  AFMM_HIP(hipMemsetAsync(Q64.p, 0, Q64.bytes, s));
+  if (!e.mr_ortho) {
+    e.mr_roff.assign(nbox + 1, 0);
+    int64_t ro = 0;
+    for (int j = 2; j <= L; j++)
+      for (int k = 0; k < e.lv[j].nb; k++) {
+        e.mr_roff[e.lv[j].gbase + k] = ro;
+        ro += (int64_t)e.lv[j].rank[k] * e.lv[j].rank[k];
+      }
+    e.mr_roff[nbox] = ro;
+    e.mr_R.alloc(sizeof(double) * std::max<int64_t>(ro, 1));
+  }
+  // bottom-up: a box's E needs its children's R
+  for (int j = L; j >= 2; j--) {
+    const LevelInfo& li = e.lv[j];
+    std::vector<MrBoxQR> bq;
+    for (int k = 0; k < li.nb; k++) {
+      const int64_t g = li.gbase + k;
+      const int r = li.rank[k], n = li.nin[k];
+      if (r <= 0 || n <= 0) continue;
+      MrBoxQR d{};
+      d.voff = e.mr_voff[g];
+      d.roff = e.mr_roff[g];
+      d.n = n;
+      d.r = r;
+      d.vld = e.mr_vld[g];
+      d.nchild = j == L ? 0 : 4;
+      for (int u = 0; u < 4 && j < L; u++) {
+        const int64_t gc = e.lv[j + 1].gbase + 4 * (int64_t)k + u;
+        d.croff[u] = e.mr_roff[gc];
+        d.crank[u] = e.lv[j + 1].rank[4 * k + u];
+      }
+      bq.push_back(d);
+    }
+    if (bq.empty()) continue;
+    DevBuf dbq;
+    upload_vec(dbq, bq, s);
+    k_mr_qr<<<(unsigned)bq.size(), 256, 0, s>>>(dbq.as<MrBoxQR>(), f64.V.as<double>(),
+                                                Q64.as<double>(), e.mr_R.as<double>());
+    AFMM_LAUNCH_CHECK();
+    AFMM_HIP(hipStreamSynchronize(s));
+  }
+  e.mr_ortho = true;
+  const size_t fb = factor_bytes(prec);
+  f.V.alloc(fb * std::max<int64_t>(e.mr_v_total, 8));
+  f.Vt.alloc(fb * std::max<int64_t>(e.mr_vt_total, 8));
+  f.M.alloc(fb * std::max<int64_t>(e.mr_m_total, 8));
+  AFMM_HIP(hipMemsetAsync(f.V.p, 0, f.V.bytes, s));
+  AFMM_HIP(hipMemsetAsync(f.Vt.p, 0, f.Vt.bytes, s));
+  AFMM_HIP(hipMemsetAsync(f.M.p, 0, f.M.bytes, s));
+  std::vector<MrBoxV> bv;
+  std::vector<MrBoxMP> bm;
+  for (int j = 2; j <= L; j++) {
+    const LevelInfo& li = e.lv[j];
+    for (int k = 0; k < li.nb; k++) {
+      const int64_t g = li.gbase + k;
+      const int r = li.rank[k], n = li.nin[k];
+      if (r > 0 && n > 0)
+        bv.push_back(MrBoxV{0, 0, e.mr_voff[g], e.mr_vtoff[g], n, r, e.mr_vld[g], e.mr_vtld[g]});
+      if (r <= 0) continue;
+      MrBoxMP d{};
+      d.moff = e.mr_moff[g];
+      d.roff = e.mr_roff[g];
+      d.r = r;
+      d.K = e.mr_K[g];
+      d.mld = e.mr_mld[g];
+      int off = 0;
+      for (int q = 0; q < 40; q++) {
+        const int b = slot_box(e, j, k, q);
+        if (b < 0) continue;
+        const int rb = li.rank[b];
+        if (rb > 0) {
+          if (d.nblk >= kMaxILBlocks) throw Error(6, "interaction list longer than 27");
+          d.boff[d.nblk] = off;
+          d.brank[d.nblk] = rb;
+          d.broff[d.nblk] = e.mr_roff[li.gbase + b];
+          d.nblk++;
+        }
+        off += rb;
+      }
+      bm.push_back(d);
+    }
+  }
+  DevBuf dbv, dbm;
+  upload_vec(dbv, bv, s);
+  upload_vec(dbm, bm, s);
+  if (prec == PREC_F32) {
+    if (!bv.empty())
+      k_mr_convert_v<float><<<(unsigned)bv.size(), 256, 0, s>>>(dbv.as<MrBoxV>(), Q64.as<double>(),
+                                                               f.V.as<float>(), f.Vt.as<float>());
+    AFMM_LAUNCH_CHECK();
+    if (!bm.empty())
+      k_mr_mprime<float><<<(unsigned)bm.size(), 256, 0, s>>>(dbm.as<MrBoxMP>(), f64.M.as<double>(),
+                                                            e.mr_R.as<double>(), f.M.as<float>());
+    AFMM_LAUNCH_CHECK();
+  } else {
+    if (!bv.empty())
+      k_mr_convert_v<__bf16><<<(unsigned)bv.size(), 256, 0, s>>>(
+          dbv.as<MrBoxV>(), Q64.as<double>(), f.V.as<__bf16>(), f.Vt.as<__bf16>());
+    AFMM_LAUNCH_CHECK();
+    if (!bm.empty())
+      k_mr_mprime<__bf16><<<(unsigned)bm.size(), 256, 0, s>>>(
+          dbm.as<MrBoxMP>(), f64.M.as<double>(), e.mr_R.as<double>(), f.M.as<__bf16>());
+    AFMM_LAUNCH_CHECK();
+  }
+  AFMM_HIP(hipStreamSynchronize(s));
+  f.built = true;
+}
+
+// ---------------------------------------------------------------------------
+// Plans: the task lists of one (precision, column tiles) pair.
+void add_row_groups(std::vector<MrTask>& out, const void* A, size_t fbytes, int lda, const void* B,
+                    const int* bidx, int64_t b0, void* C, int64_t c0, int M, int K, int beta) {
+  for (int r0 = 0; r0 < M; r0 += 64) {
+    MrTask t;
+    t.A = static_cast<const char*>(A) + (int64_t)r0 * lda * fbytes;
+    t.B = B;
+    t.bidx = bidx;
+    t.C = C;
+    t.b0 = b0;
+    t.c0 = c0 + r0;
+    t.M = std::min(64, M - r0);
+    t.K = K;
+    t.lda = lda;
+    t.beta = beta;
+    out.push_back(t);
+  }
+}
+
+// Orders a task list by row tiles and uploads it.
+template <class T>
+void upload_tasks(MrTaskList& tl, std::vector<T>& tasks, hipStream_t s) {
+  std::stable_sort(tasks.begin(), tasks.end(),
+                   [](const T& a, const T& b) { return (a.M + 15) / 16 < (b.M + 15) / 16; });
+  tl.n = (int)tasks.size();
+  for (int c = 0; c < 5; c++) tl.cls[c] = tl.n;
+  for (int t = tl.n - 1; t >= 0; t--) tl.cls[(tasks[t].M + 15) / 16 - 1] = t;
+  for (int c = 3; c >= 0; c--) tl.cls[c] = std::min(tl.cls[c], tl.cls[c + 1]);
+  upload_vec(tl.buf, tasks, s);
+}
+
+MrPlan& get_plan(Engine& e, int prec, int nct) {
+  for (auto& p : e.mr_plans)
+    if (p->prec == prec && p->nct == nct) return *p;
+  if (prec == PREC_F64) build_f64(e);
+  else build_reduced(e, prec);
+  auto pl = std::make_unique<MrPlan>();
+  MrPlan& p = *pl;
+  p.prec = prec;
+  p.nct = nct;
+  const int L = e.L;
+  const int RP = 16 * nct;
+  const size_t vb = vector_bytes(prec), fb = factor_bytes(prec);
+  const MrFactors& f = e.mrf[prec];
+  hipStream_t s = e.s_aux;
+  const int64_t nsk = std::max<int64_t>(e.skel_total, 1);
+  p.qs.alloc(vb * RP * (size_t)e.n);
+  p.outv.alloc(vb * RP * (size_t)nsk);
+  p.inv.alloc(vb * RP * (size_t)nsk);
+  p.pot.alloc(vb * RP * (size_t)e.n);
+  p.nearv.alloc(sizeof(double) * RP * (size_t)e.n);
+  AFMM_HIP(hipMemsetAsync(p.outv.p, 0, p.outv.bytes, s));
+  AFMM_HIP(hipMemsetAsync(p.inv.p, 0, p.inv.bytes, s));
+  auto fptr = [&](const DevBuf& b, int64_t off) {
+    return static_cast<const void*>(static_cast<const char*>(b.p) + off * fb);
+  };
+  auto vrow = [&](const DevBuf& b) { return b.p; };
+  p.up.resize(L + 1);
+  p.down.resize(L + 1);
+  std::vector<MrTask> m2l, leaf;
+  for (int j = 2; j <= L; j++) {
+    const LevelInfo& li = e.lv[j];
+    std::vector<MrTask> up, down;
+    for (int k = 0; k < li.nb; k++) {
+      const int64_t g = li.gbase + k;
+      const int r = li.rank[k], n = li.nin[k];
+      if (r > 0) {
+        // M2M: out_k = V^T src
+        const void* Bsrc = j == L ? vrow(p.qs) : vrow(p.outv);
+        const int64_t b0 = j == L ? e.leaf_start[k] : e.lv[j + 1].soff[4 * (size_t)k];
+        if (n > 0)
+          add_row_groups(up, fptr(f.Vt, e.mr_vtoff[g]), fb, e.mr_vtld[g], Bsrc, nullptr, b0,
+                         p.outv.p, li.soff[k], r, n, 0);
+        else
+          add_row_groups(up, fptr(f.Vt, e.mr_vtoff[g]), fb, e.mr_vtld[g], Bsrc, nullptr, 0,
+                         p.outv.p, li.soff[k], r, 0, 0);
+        // M2L: in_k = M_k [out_b ...]
+        add_row_groups(m2l, fptr(f.M, e.mr_moff[g]), fb, e.mr_mld[g], vrow(p.outv),
+                       e.mr_colidx.as<int>(), e.mr_coloff[g], p.inv.p, li.soff[k], r, e.mr_K[g], 0);
+      }
+      if (j < L) {
+        // L2L: children's in += V in_k
+        if (r > 0 && n > 0)
+          add_row_groups(down, fptr(f.V, e.mr_voff[g]), fb, e.mr_vld[g], vrow(p.inv), nullptr,
+                         li.soff[k], p.inv.p, e.lv[j + 1].soff[4 * (size_t)k], n, r, 1);
+      } else if (n > 0) {
+        // leaf: pot = V in_k (zero for a rank-0 leaf)
+        add_row_groups(leaf, fptr(f.V, e.mr_voff[g]), fb, e.mr_vld[g], vrow(p.inv), nullptr,
+                       r > 0 ? li.soff[k] : 0, p.pot.p, e.leaf_start[k], n, r, 0);
+      }
+    }
+    upload_tasks(p.up[j], up, s);
+    upload_tasks(p.down[j], down, s);
+  }
+  upload_tasks(p.m2l, m2l, s);
+  upload_tasks(p.leaf, leaf, s);
+  // near field: every leaf's points against its neighbours N0..N7 and itself
+  std::vector<MrNearTask> near;
+  const int nleaf = 1 << (2 * L);
+  for (int k = 0; k < nleaf; k++) {
+    const int st = e.leaf_start[k], n = e.leaf_count[k];
+    for (int r0 = 0; r0 < n; r0 += 64) {
+      MrNearTask t{};
+      t.ta = st + r0;
+      t.M = std::min(64, n - r0);
+      t.out_row = st + r0;
+      for (int q = 0; q <= 8; q++) {
+        const int b = q < 8 ? e.nbr[L][8 * (size_t)k + q] : k;
+        if (b < 0 || e.leaf_count[b] == 0) continue;
+        t.ss[t.nsrc] = e.leaf_start[b];
+        t.sc[t.nsrc] = e.leaf_count[b];
+        t.nsrc++;
+      }
+      near.push_back(t);
+    }
+  }
+  upload_tasks(p.near, near, s);
+  MrNearProb pr{e.x.as<double>(), e.y.as<double>(), p.qs.p, p.nearv.as<double>(), e.kp};
+  upload_vec(p.probs, std::vector<MrNearProb>{pr}, s);
+  AFMM_HIP(hipStreamSynchronize(s));
+  e.mr_plans.push_back(std::move(pl));
+  return *e.mr_plans.back();
+}
+
+template <int P, int NCT>
+void launch_gemm(const MrTaskList& tl, int xcd, hipStream_t s) {
+  for (int c = 0; c < 4; c++) {
+    const int n = tl.cls[c + 1] - tl.cls[c];
+    if (n <= 0) continue;
+    const MrTask* t = tl.buf.as<MrTask>() + tl.cls[c];
+    const unsigned grid = ceil_div(n, 4);
+    switch (c) {
+      case 0: k_mr_gemm<P, NCT, 1><<<grid, 256, 0, s>>>(t, n, xcd); break;
+      case 1: k_mr_gemm<P, NCT, 2><<<grid, 256, 0, s>>>(t, n, xcd); break;
+      case 2: k_mr_gemm<P, NCT, 3><<<grid, 256, 0, s>>>(t, n, xcd); break;
+      default: k_mr_gemm<P, NCT, 4><<<grid, 256, 0, s>>>(t, n, xcd); break;
+    }
+    AFMM_LAUNCH_CHECK();
+  }
+}
+
+void launch_gemm_p(int prec, int nct, const MrTaskList& tl, int xcd, hipStream_t s) {
+  if (nct == 1) {
+    switch (prec) {
+      case PREC_F64: launch_gemm<PREC_F64, 1>(tl, xcd, s); break;
+      case PREC_F32: launch_gemm<PREC_F32, 1>(tl, xcd, s); break;
+      default: launch_gemm<PREC_BF16, 1>(tl, xcd, s); break;
+    }
+  } else {
+    switch (prec) {
+      case PREC_F64: launch_gemm<PREC_F64, 2>(tl, xcd, s); break;
+      case PREC_F32: launch_gemm<PREC_F32, 2>(tl, xcd, s); break;
+      default: launch_gemm<PREC_BF16, 2>(tl, xcd, s); break;
+    }
+  }
+}
+
+template <int P, int NCT>
+void launch_near_t(Engine& e, const MrPlan& p, hipStream_t s) {
+  const MrTaskList& tl = p.near;
+  for (int c = 0; c < 4; c++) {
+    const int n = tl.cls[c + 1] - tl.cls[c];
+    if (n <= 0) continue;
+    const MrNearTask* t = tl.buf.as<MrNearTask>() + tl.cls[c];
+    const unsigned grid = ceil_div(n, 4);
+    const MrNearProb* pr = p.probs.as<MrNearProb>();
+    const double2* lt = e.log_tab.as<double2>();
+#define AFMM_MR_NEAR(NRT)                                                                   \
+  AFMM_DISPATCH_KERNEL(e.kid, KID,                                                          \
+                       k_mr_p2p<P, NCT, NRT, KID><<<grid, 256, 0, s>>>(t, n, pr, lt, e.ln2_hi, \
+                                                                       e.ln2_lo))
+    switch (c) {
+      case 0: AFMM_MR_NEAR(1); break;
+      case 1: AFMM_MR_NEAR(2); break;
+      case 2: AFMM_MR_NEAR(3); break;
+      default: AFMM_MR_NEAR(4); break;
+    }
+#undef AFMM_MR_NEAR
+    AFMM_LAUNCH_CHECK();
+  }
+}
+
+void launch_near_p(Engine& e, const MrPlan& p, hipStream_t s) {
+  if (p.nct == 1) {
+    switch (p.prec) {
+      case PREC_F64: launch_near_t<PREC_F64, 1>(e, p, s); break;
+      case PREC_F32: launch_near_t<PREC_F32, 1>(e, p, s); break;
+      default: launch_near_t<PREC_BF16, 1>(e, p, s); break;
+    }
+  } else {
+    switch (p.prec) {
+      case PREC_F64: launch_near_t<PREC_F64, 2>(e, p, s); break;
+      case PREC_F32: launch_near_t<PREC_F32, 2>(e, p, s); break;
+      default: launch_near_t<PREC_BF16, 2>(e, p, s); break;
+    }
+  }
+}
+
+template <class VT, int NCT>
+void launch_io(bool gather, const MrIoTask& d, hipStream_t s) {
+  if (d.cnt <= 0) return;
+  const unsigned grid = ceil_div(d.cnt, kMrIoPoints);
+  if (gather) k_mr_gather<VT, NCT><<<grid, 256, 0, s>>>(d);
+  else k_mr_out<VT, NCT><<<grid, 256, 0, s>>>(d);
+  AFMM_LAUNCH_CHECK();
+}
+
+void launch_io_p(int prec, int nct, bool gather, const MrIoTask& d, hipStream_t s) {
+  if (prec == PREC_F64) {
+    if (nct == 1) launch_io<double, 1>(gather, d, s);
+    else launch_io<double, 2>(gather, d, s);
+  } else {
+    if (nct == 1) launch_io<float, 1>(gather, d, s);
+    else launch_io<float, 2>(gather, d, s);
+  }
+}
+
+}  // namespace
+
+void mr_prepare(Engine& e, int prec) {
+  if (prec == PREC_F64) build_f64(e);
+  else build_reduced(e, prec);
+}
+
+void mr_matvec(Engine& e, const double* d_b, int64_t ldb, double* d_y, int64_t ldy, int nrhs,
+               int prec, hipStream_t s) {
+  if (nrhs < 1 || nrhs > kMrMaxCols) throw Error(1, "blocked matvec takes 1..32 right-hand sides");
+  const int nct = nrhs <= 16 ? 1 : 2;
+  MrPlan& p = get_plan(e, prec, nct);
+  const int L = e.L;
+  MrIoTask io{};
+  io.b = d_b;
+  io.y = d_y;
+  io.inv_perm = e.inv_perm_d.as<int>();
+  io.qs = p.qs.p;
+  io.pot = p.pot.p;
+  io.nearv = p.nearv.as<double>();
+  io.ldb = ldb;
+  io.ldy = ldy;
+  io.i0 = 0;
+  io.cnt = e.n;
+  io.nrhs = nrhs;
+  hipEvent_t a, b;
+  e.prof_begin("mr_gather", s, &a, &b);
+  launch_io_p(prec, nct, true, io, s);
+  e.prof_end(s, b);
+  // the near field needs only the gathered charges: it runs beside the tree
+  AFMM_HIP(hipEventRecord(e.ev_fork, s));
+  AFMM_HIP(hipStreamWaitEvent(e.s_aux, e.ev_fork, 0));
+  e.prof_begin("mr_p2p", e.s_aux, &a, &b);
+  launch_near_p(e, p, e.s_aux);
+  e.prof_end(e.s_aux, b);
+  AFMM_HIP(hipEventRecord(e.ev_join, e.s_aux));
+  e.prof_begin("mr_m2m", s, &a, &b);
+  for (int j = L; j >= 2; j--) launch_gemm_p(prec, nct, p.up[j], 0, s);
+  e.prof_end(s, b);
+  e.prof_begin("mr_m2l", s, &a, &b);
+  launch_gemm_p(prec, nct, p.m2l, 1, s);
+  e.prof_end(s, b);
+  e.prof_begin("mr_l2l", s, &a, &b);
+  for (int j = 2; j < L; j++) launch_gemm_p(prec, nct, p.down[j], 0, s);
+  launch_gemm_p(prec, nct, p.leaf, 0, s);
+  e.prof_end(s, b);
+  AFMM_HIP(hipStreamWaitEvent(s, e.ev_join, 0));
+  e.prof_begin("mr_out", s, &a, &b);
+  launch_io_p(prec, nct, false, io, s);
+  e.prof_end(s, b);
+}
+
+void add_cols_device(double* d_y, int64_t ldy, const double* d_t, int64_t ldt, int n, int nrhs,
+                     hipStream_t s) {
+  const int64_t tot = (int64_t)n * nrhs;
+  if (tot <= 0) return;
+  k_add_cols<<<ceil_div(tot, 256), 256, 0, s>>>(d_y, ldy, d_t, ldt, n, nrhs);
+  AFMM_LAUNCH_CHECK();
+}
+
+}  // namespace afmm

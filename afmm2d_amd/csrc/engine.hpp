@@ -35,6 +35,7 @@
 
 #include <array>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -48,6 +49,20 @@ struct DevBuf {
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   ~DevBuf() { release(); }
   void release() {
     if (p) (void)hipFree(p);
@@ -122,6 +137,91 @@ constexpr int kILSlots = 27;
 struct KernelTimer {
   double total_ms = 0.0;
   int64_t launches = 0;
+};
+
+// ---- Blocked matvec: several right-hand sides per pass, factors in FP64,
+// FP32 or BF16, products on the matrix cores (blocked.hip).
+enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2 };
+constexpr int kPrecisions = 3;
+// Right-hand sides per pass: 16 (one MFMA column tile) or 32 (two).
+constexpr int kMrMaxCols = 32;
+
+// One product of the blocked passes: C[c0 + i][:] (+)= sum_k A[i][k] B[row(k)][:]
+// for i < M <= 64, k < K.  A is row-major with leading dimension lda (a
+// multiple of 8) in the precision's factor type; B and C are row-major
+// vectors of 16 NCT columns in the precision's vector type (FP64 for FP64,
+// FP32 otherwise); row(k) = bidx ? bidx[b0 + k] : b0 + k.  beta = 1 adds
+// into C.  Pointers are absolute, so tasks of several problems can share a
+// launch.
+struct MrTask {
+  const void* A;
+  const void* B;
+  const int* bidx;
+  void* C;
+  int64_t b0, c0;
+  int M, K, lda, beta;
+};
+
+// Near field of one group of up to 64 target points (KD-sorted indices ta ..
+// ta + M - 1) against up to 9 source ranges (the leaf's neighbours and
+// itself), written to rows out_row .. of the problem's near-field output.
+struct MrNearTask {
+  int ta, M, out_row, nsrc;
+  int ss[9], sc[9];
+  int prob, pad;
+};
+
+// Per-problem data of the near field (coordinates, charges, output).
+struct MrNearProb {
+  const double* x;
+  const double* y;
+  const void* q;  // charges, rows of 16 NCT vector elements, KD order
+  double* out;    // near-field potentials, rows of 16 NCT doubles
+  KParams kp;
+};
+
+// Gather (caller order -> KD order) or final sum and scatter of up to
+// kMrIoPoints caller points of one problem.
+struct MrIoTask {
+  const double* b;        // gather: caller-order charges, column-major, ldb
+  double* y;              // scatter: caller-order output, column-major, ldy
+  const int* inv_perm;    // caller index -> KD-sorted index
+  void* qs;               // KD-order charges (vector type)
+  const void* pot;        // KD-order far-field potentials (vector type)
+  const double* nearv;    // KD-order near-field potentials (FP64)
+  int64_t ldb, ldy;
+  int i0, cnt, nrhs, pad;
+};
+constexpr int kMrIoPoints = 256;
+
+// The factors of one precision.  V: each box's basis, nin x rank row-major
+// (leading dimension rank rounded up to 8); Vt: its transpose, rank x nin
+// (leading dimension nin rounded up to 8); M: each box's M2L row block,
+// rank x (sum of its interaction-list boxes' ranks), row-major (leading
+// dimension rounded up to 8), the column blocks in interaction-slot order.
+// FP64 uses the reference's interpolative basis; FP32 and BF16 an
+// orthonormal basis of the same operator (Q R = L2P, M2L' = R_k M2L R_b^T).
+struct MrFactors {
+  bool built = false;
+  DevBuf V, Vt, M;
+};
+
+// A task list on the device, ordered by row tiles (16-row blocks of output,
+// 1..4): tasks [cls[c], cls[c + 1]) have c + 1 row tiles, one launch each.
+struct MrTaskList {
+  DevBuf buf;
+  int n = 0;
+  int cls[5] = {0, 0, 0, 0, 0};
+};
+
+// Task lists and vectors of the blocked matvec for one (precision, column
+// tiles) pair.
+struct MrPlan {
+  int prec = -1, nct = 0;
+  DevBuf qs, outv, inv, pot, nearv;
+  std::vector<MrTaskList> up, down;  // per level (index j)
+  MrTaskList m2l, leaf, near;
+  DevBuf probs;
 };
 
 struct Engine {
@@ -208,6 +308,22 @@ struct Engine {
   bool have_matvec = false;
   std::vector<double> last_qs, last_ps;  // host copies for compute_error
 
+  // blocked matvec: layout shared by the precisions (element offsets per box
+  // gid), the interaction-list column index of every M2L row block, the
+  // factors per precision and the plans
+  bool mr_layout = false;
+  std::vector<int64_t> mr_voff, mr_vtoff, mr_moff, mr_coloff;  // per gid
+  std::vector<int> mr_vld, mr_vtld, mr_mld, mr_K;              // per gid
+  int64_t mr_v_total = 0, mr_vt_total = 0, mr_m_total = 0;     // elements incl. padding
+  int64_t mr_m_coeffs = 0, mr_v_coeffs = 0;                    // sum rank K, sum nin rank
+  DevBuf mr_colidx, inv_perm_d;
+  MrFactors mrf[kPrecisions];
+  std::vector<std::unique_ptr<MrPlan>> mr_plans;
+  // orthonormal basis (FP32 / BF16): R of every box, rank x rank row-major
+  std::vector<int64_t> mr_roff;
+  DevBuf mr_R;
+  bool mr_ortho = false;
+
   // profiling
   int profiling = 0;  // 0 off, 1 every phase, 2 the leaf-level M2L only
   std::map<std::string, KernelTimer> timers;
@@ -242,5 +358,16 @@ void add_device(double* d_y, const double* d_t, int n, hipStream_t s);
 void direct_rows_device(Engine& e, int nrows, const int* d_rows, const double* d_qs, double* d_out,
                         hipStream_t s);
 void gather_sorted(Engine& e, const double* d_b, double* d_qs, hipStream_t s);
+
+// blocked.hip
+// Builds the factors of precision prec (and the FP64 layout they share).
+void mr_prepare(Engine& e, int prec);
+// y[:, c] = K b[:, c] for c < nrhs (nrhs <= kMrMaxCols), device arrays in
+// caller order, column-major with leading dimensions ldb and ldy, on stream s.
+void mr_matvec(Engine& e, const double* d_b, int64_t ldb, double* d_y, int64_t ldy, int nrhs,
+               int prec, hipStream_t s);
+// y[:, c] += t[:, c]
+void add_cols_device(double* d_y, int64_t ldy, const double* d_t, int64_t ldt, int n, int nrhs,
+                     hipStream_t s);
 
 }  // namespace afmm

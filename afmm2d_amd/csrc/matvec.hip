@@ -23,6 +23,7 @@
 #include <cstdlib>
 
 #include "engine.hpp"
+#include "near_math.hpp"
 
 namespace afmm {
 
@@ -818,85 +819,6 @@ __global__ __launch_bounds__(64 * kTreeWaves) void k_down_tree(TreeArgs a) {
       if (t < nc) dst[rm[t]] = acc;
     }
     __syncthreads();
-  }
-}
-
-// log(x) for normal x > 0 by Tang's table method (ACM TOMS 16(4), 1990):
-// x = 2^e m, m in [1,2); j = the top 7 bits of m's fraction;
-// c_j = 1 + (j + 1/2)/128, t = (RN(1/c_j), -log RN(1/c_j));
-// log x = e ln2 + t.y + log1p(m t.x - 1), |m t.x - 1| < 2^-8, log1p by its
-// Taylor polynomial to degree 6 (truncation < 4e-18).
-__device__ __forceinline__ double log_tang(double x, const double2* __restrict__ tab,
-                                           double ln2_hi, double ln2_lo) {
-  const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
-  const int ex = (int)(bits >> 52) - 1023;
-  const int j = (int)((bits >> 45) & 127);
-  const double m = __longlong_as_double((long long)((bits & 0x000FFFFFFFFFFFFFull) |
-                                                    0x3FF0000000000000ull));
-  const double2 t = tab[j];
-  const double r = fma(m, t.x, -1.0);
-  double p = fma(r, -1.0 / 6.0, 1.0 / 5.0);
-  p = fma(r, p, -0.25);
-  p = fma(r, p, 1.0 / 3.0);
-  p = fma(r, p, -0.5);
-  p = r * p;
-  p = fma(r, p, r);  // log1p(r)
-  const double e = (double)ex;
-  return fma(e, ln2_hi, t.y) + fma(e, ln2_lo, p);
-}
-
-// 1/sqrt(x) for normal x > 0: the hardware estimate y0 = (1 + d)/sqrt(x)
-// (v_rsq_f64) refined by one second-order Newton step.  With
-// e = 1 - x y0^2 = -2d - d^2, sqrt(x) y0 (1 + e/2 + 3e^2/8) = 1 + O(d^3);
-// what remains is the rounding of x y0 in e and of the last FMA, about one
-// ulp.  x = 0 gives +inf from the estimate and NaN after the step: callers
-// select the diagonal value separately.
-__device__ __forceinline__ double rsqrt_refined(double x) {
-  const double y0 = __builtin_amdgcn_rsq(x);
-  const double e = fma(-(x * y0), y0, 1.0);
-  return fma(y0 * e, fma(e, 0.375, 0.5), y0);
-}
-
-// The kernel from the squared distance for the near field.  Same values as
-// kernel_r2 to a few ulp, without its branches and correctly rounded
-// divisions and square roots: the log by Tang's table, the 1/R kernels
-// through rsqrt_refined (one estimate, no division), the Gaussian with the
-// reciprocal of its width taken once (ninv_p0 = -1/p0).
-template <int KID>
-__device__ __forceinline__ double near_kernel(double r2, const KParams& kp, double ninv_p0,
-                                              double inv_a, const double2* tab, double ln2_hi,
-                                              double ln2_lo) {
-  if constexpr (KID == K_LOG) {
-    // 2 x the log kernel; the factor 1/2 is applied to the sums.  The log is
-    // taken of max(r2, tiny) and the diagonal value selected after, so no
-    // branch splits the wave
-    const double lg = log_tang(fmax(r2, 1e-300), tab, ln2_hi, ln2_lo);
-    return r2 == 0.0 ? 0.0 : lg;
-  } else if constexpr (KID == K_INV_R) {
-    const double rs = rsqrt_refined(r2);
-    return r2 == 0.0 ? 0.0 : rs;
-  } else if constexpr (KID == K_GAUSS) {
-    return exp(r2 * ninv_p0);
-  } else {
-    // R < a ? R/a : a/R = min(R/a, a/R) with R = r2 / sqrt(r2); r2 is kept
-    // off zero inside the square root only, so R = 0 gives R/a = 0 exactly
-    const double rs = rsqrt_refined(fmax(r2, 1e-300));
-    const double r = r2 * rs;
-    return fmin(r * inv_a, kp.a * rs);
-  }
-}
-
-// K(target, source) for the near field: the user kernel takes the points,
-// the built-in ones the squared distance.
-template <int KID>
-__device__ __forceinline__ double pair_value(double xi, double yi, double sx, double sy,
-                                             const KParams& kp, double ninv_p0, double inv_a,
-                                             const double2* tab, double ln2_hi, double ln2_lo) {
-  if constexpr (KID >= K_USER) {
-    return kernel_xy<KID>(xi, yi, sx, sy, kp);
-  } else {
-    const double dx = xi - sx, dy = yi - sy;
-    return near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab, ln2_hi, ln2_lo);
   }
 }
 

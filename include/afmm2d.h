@@ -4,8 +4,10 @@
  * Plain pointers and sizes only.  Every entry point returns an afmm2d_status;
  * afmm2d_last_error() gives a message for the calling thread's last failure.
  * Handles are not thread-safe: one caller thread per handle.  Host-pointer
- * entry points synchronise before returning; afmm2d_matvec_device is ordered
- * on the stream it is given.
+ * entry points synchronise before returning; the device entry points are
+ * ordered on the stream they are given.  Calls on one handle share its
+ * buffers, so each call's device work is also ordered after the previous
+ * call's, whatever stream that one used.
  *
  * Each declaration names the reference interface it replaces (file:line in
  * the vaishna77/AFMM2D repository).
@@ -49,6 +51,26 @@ typedef enum {
   AFMM2D_KERNEL_USER_ANTI = 6  /* (K(x,y) - K(y,x)) / 2 of the user kernel */
 } afmm2d_kernel_id;
 
+/* Precision of the stored factors and of the arithmetic of a matvec
+ * (afmm2d_matvec_ex, afmm2d_matvec_device_ex).  The reference is FP64 only
+ * (definitions.hpp:5-7).  FP64 applies the reference's own factors.  FP32 and
+ * BF16 apply an orthonormal basis of the same operator, built from the FP64
+ * factors at first use (or by afmm2d_prepare_precision): E_k = Q_k R_k per
+ * box, M2L blocks R_k M2L R_b^T, rounded to the precision, products
+ * accumulated in FP32, the near field evaluated in FP32, y formed in FP64.
+ * Expect a few 1e-7 (FP32) and 1e-3 .. 1e-2 (BF16) relative error on top of
+ * the FP64 operator's. */
+typedef enum {
+  AFMM2D_PRECISION_FP64 = 0,
+  AFMM2D_PRECISION_FP32 = 1,
+  AFMM2D_PRECISION_BF16 = 2
+} afmm2d_precision;
+
+/* Mode flag for afmm2d_matvec_ex / _device_ex: run the blocked passes (the
+ * several-RHS kernels on the matrix cores) also for one FP64 right-hand side,
+ * which otherwise runs the single-RHS passes. */
+#define AFMM2D_MATVEC_BLOCKED 0x100
+
 typedef enum {
   AFMM2D_LOC_INTERLEAVED = 0,  /* x0,y0,x1,y1,... (N x 2 row-major) */
   AFMM2D_LOC_COLUMN_MAJOR = 1  /* all x then all y (Eigen::MatrixXd N x 2) */
@@ -75,6 +97,9 @@ typedef struct {
   int64_t m2l_leaf_columns;  /* their partial columns (rank_k + rank_b per pair) */
   int64_t skeleton_leaf;     /* sum of the leaf level's ranks */
   int64_t near_waves;        /* near-field waves per SIMD used beside the M2L */
+  int64_t blocked_m2l_coeffs;   /* blocked passes: M2L row-block coefficients, sum rank_k K_k
+                                   (each interaction block stored for both of its boxes) */
+  int64_t blocked_basis_coeffs; /* blocked passes: basis coefficients, sum nin_k rank_k */
 } afmm2d_stats;
 
 /* Message of the calling thread's last failure ("" if none). */
@@ -105,6 +130,27 @@ int afmm2d_matvec(afmm2d_handle h, const double* b, double* y, int nrhs);
 /* Same on device-resident vectors (one RHS each call), ordered on 'stream'
  * (a hipStream_t; NULL = the null stream).  Nothing is copied to the host. */
 int afmm2d_matvec_device(afmm2d_handle h, const double* d_b, double* d_y, void* stream);
+
+/* AFMM::computeMatVecProduct (AFMM2D.cpp:46-65) for nrhs right-hand sides at
+ * once: Y = K B with B and Y column-major (caller point order) with leading
+ * dimensions ldb, ldy >= N.  mode = an afmm2d_precision, optionally
+ * | AFMM2D_MATVEC_BLOCKED.  FP64 without the flag applies the single-RHS
+ * passes to each column; otherwise the blocked passes run, up to 32 columns
+ * per pass, each stored factor read once per pass for all its columns.
+ * Host arrays; returns after the result is in y. */
+int afmm2d_matvec_ex(afmm2d_handle h, const double* b, int64_t ldb, double* y, int64_t ldy,
+                     int nrhs, int mode);
+
+/* Same on device-resident B and Y, ordered on 'stream' (a hipStream_t; NULL =
+ * the null stream).  One FP64 column without AFMM2D_MATVEC_BLOCKED is
+ * afmm2d_matvec_device. */
+int afmm2d_matvec_device_ex(afmm2d_handle h, const double* d_b, int64_t ldb, double* d_y,
+                            int64_t ldy, int nrhs, int mode, void* stream);
+
+/* Builds the factors of one precision now rather than at the first matvec
+ * that uses it (FP32/BF16: the orthonormal basis; any: the blocked layout).
+ * No reference counterpart: the reference has one precision. */
+int afmm2d_prepare_precision(afmm2d_handle h, int precision);
 
 /* nLevels (AFMM2D.hpp:20). */
 int afmm2d_levels(afmm2d_handle h, int* levels);
