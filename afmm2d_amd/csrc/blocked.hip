@@ -51,6 +51,7 @@
 // near field's per-source partials with compensated (two-sum) summation,
 // and form the output y = pot + near in FP64.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "engine.hpp"
@@ -89,6 +90,30 @@ struct MrT<PREC_BF16> {
   using Acc = mr_f4;
   static constexpr int KC = 32;
 };
+
+// Tiled layout of a factor matrix of mbox rows and kp columns (kp a multiple
+// of the column group KG: 4 for FP64 and FP32, 8 for BF16): chunks of
+// KC = 4 KG columns one after another; within a chunk its 16-row tiles in
+// row order; within a tile, column group g by column group, the KG elements
+// of each row (FP64: first the elements 0-1 of every (g, row), then 2-3).  A
+// load instruction of the products, lane i + 16 g taking 16 bytes of row i
+// of group g, reads one contiguous KiB (less for a tile of fewer rows), and a
+// box's matrix streams in order chunk by chunk.
+template <int P>
+__host__ __device__ inline int64_t tile_off(int row, int k, int mbox, int kp) {
+  constexpr int KG = P == PREC_BF16 ? 8 : 4, KC = 4 * KG;
+  const int c0 = k - k % KC;
+  const int kc = kp - c0 < KC ? kp - c0 : KC;
+  const int ng = kc / KG;
+  const int kk = k - c0, g = kk / KG, e = kk - g * KG;
+  const int rt = row >> 4, i = row & 15;
+  const int rows = mbox - 16 * rt < 16 ? mbox - 16 * rt : 16;
+  const int64_t base = (int64_t)c0 * mbox + (int64_t)kc * 16 * rt;
+  if (P == PREC_F64) return base + ((int64_t)((e >> 1) * ng + g) * rows + i) * 2 + (e & 1);
+  return base + ((int64_t)g * rows + i) * KG + e;
+}
+inline int col_group(int prec) { return prec == PREC_BF16 ? 8 : 4; }
+constexpr int kMrCtrStride = 32;  // ints between work counters (128 bytes)
 
 // Pointers taken from task records are generic; loads through the global
 // address space avoid flat instructions.
@@ -153,45 +178,50 @@ struct GemmFrag {
   typename T::V b[P == PREC_BF16 ? 1 : 4][NCT];         // FP64/FP32: B[k0 + 4 g + s][j]
 };
 
-template <int P, int NCT, int NRT>
+template <int P, int NCT, int NRT, int IM>
 __device__ __forceinline__ void gemm_load(GemmFrag<P, NCT>& f, const MrTask& d,
-                                          const int (&arow)[4], int k0, int i, int g) {
+                                          int k0, int i, int g, const int* sidx) {
   using T = MrT<P>;
   using AT = typename T::A;
   using VT = typename T::V;
   constexpr int RP = 16 * NCT;
+  constexpr int KG = P == PREC_BF16 ? 8 : 4;  // k per lane group
+  constexpr int KC = 4 * KG;
   const auto A = gptr(static_cast<const AT*>(d.A));
   const auto B = gptr(static_cast<const VT*>(d.B));
   const auto bidx = gptr(d.bidx);
-  constexpr int KG = P == PREC_BF16 ? 8 : 4;  // k per lane group
-  // a lane's k run starts at k0 + KG g; runs past lda are clamped back (their
-  // B entries are zero: k >= lda >= K)
-  const int ks = min(k0 + KG * g, d.lda - KG);
+  // the chunk's tiles (tile_off): a lane group past the chunk's last column
+  // group reads the last one again (its B entries are zero: k >= kp >= K)
+  const int kc = min(KC, d.kp - k0);
+  const int ng = kc / KG;
+  const int gg = min(g, ng - 1);
+  const int64_t cbase = (int64_t)k0 * d.mbox;
 #pragma unroll
   for (int rt = 0; rt < NRT; rt++) {
-    {
-      const auto p = A + (int64_t)arow[rt] * d.lda + ks;
-      if constexpr (P == PREC_F64) {
-        const mr_d2 u = ((const __attribute__((address_space(1))) mr_d2*)p)[0];
-        const mr_d2 v = ((const __attribute__((address_space(1))) mr_d2*)p)[1];
-        f.a[rt][0] = u.x;
-        f.a[rt][1] = u.y;
-        f.a[rt][2] = v.x;
-        f.a[rt][3] = v.y;
-      } else if constexpr (P == PREC_F32) {
-        const mr_f4 u = *((const __attribute__((address_space(1))) mr_f4*)p);
-        f.a[rt][0] = u.x;
-        f.a[rt][1] = u.y;
-        f.a[rt][2] = u.z;
-        f.a[rt][3] = u.w;
-      } else {
-        const mr_u4 u = *((const __attribute__((address_space(1))) mr_u4*)p);
-        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+    const int rtg = d.rt0 + rt;
+    const int rows = min(16, d.mbox - 16 * rtg);
+    const int ii = min(i, rows - 1);
+    const int64_t tb = cbase + (int64_t)kc * 16 * rtg;
+    if constexpr (P == PREC_F64) {
+      const mr_d2 u = *((const __attribute__((address_space(1))) mr_d2*)(A + tb + ((int64_t)gg * rows + ii) * 2));
+      const mr_d2 v = *((const __attribute__((address_space(1))) mr_d2*)(A + tb + ((int64_t)(ng + gg) * rows + ii) * 2));
+      f.a[rt][0] = u.x;
+      f.a[rt][1] = u.y;
+      f.a[rt][2] = v.x;
+      f.a[rt][3] = v.y;
+    } else if constexpr (P == PREC_F32) {
+      const mr_f4 u = *((const __attribute__((address_space(1))) mr_f4*)(A + tb + ((int64_t)gg * rows + ii) * 4));
+      f.a[rt][0] = u.x;
+      f.a[rt][1] = u.y;
+      f.a[rt][2] = u.z;
+      f.a[rt][3] = u.w;
+    } else {
+      const mr_u4 u = *((const __attribute__((address_space(1))) mr_u4*)(A + tb + ((int64_t)gg * rows + ii) * 8));
+      const uint32_t w[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-          f.a[rt][2 * t] = (uint16_t)(w[t] & 0xFFFF);
-          f.a[rt][2 * t + 1] = (uint16_t)(w[t] >> 16);
-        }
+      for (int t = 0; t < 4; t++) {
+        f.a[rt][2 * t] = (uint16_t)(w[t] & 0xFFFF);
+        f.a[rt][2 * t + 1] = (uint16_t)(w[t] >> 16);
       }
     }
   }
@@ -199,8 +229,11 @@ __device__ __forceinline__ void gemm_load(GemmFrag<P, NCT>& f, const MrTask& d,
   for (int s = 0; s < KG; s++) {
     const int k = k0 + KG * g + s;
     const bool ok = k < d.K;
-    const int kc = ok ? k : d.K - 1;
-    const int64_t row = d.bidx ? (int64_t)bidx[d.b0 + kc] : d.b0 + kc;
+    const int kc2 = ok ? k : d.K - 1;
+    int64_t row;
+    if constexpr (IM == 0) row = d.b0 + kc2;
+    else if constexpr (IM == 1) row = sidx[kc2];
+    else row = bidx[d.b0 + kc2];
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) {
       const VT v = B[row * RP + 16 * ct + i];
@@ -250,38 +283,59 @@ __device__ __forceinline__ void gemm_mma(const GemmFrag<P, NCT>& f,
   }
 }
 
-template <int P, int NCT, int NRT>
+// IM: how B rows are found: 0 contiguous (b0 + k), 1 through the task's index
+// list staged in LDS first (dynamic LDS: kmax ints per wave), 2 through the
+// index list in global memory.  Staging the indices takes the index load off
+// the path of every chunk: the vector loads of the chunks in flight then
+// depend on nothing.  Chunks go through a ring of three register sets, two
+// chunks' loads in flight while one is multiplied.
+template <int P, int NCT, int NRT, int IM, int D>
 __global__ __launch_bounds__(256) void k_mr_gemm(const MrTask* __restrict__ tasks, int ntasks,
-                                                 int xcd_map) {
+                                                 int xcd_map, int kmax) {
   using T = MrT<P>;
   using VT = typename T::V;
   using Acc = typename T::Acc;
   constexpr int RP = 16 * NCT;
   constexpr int KC = T::KC;
+  extern __shared__ int lds_idx[];
   const int l = threadIdx.x & 63, i = l & 15, g = l >> 4;
   const int blk = xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-  const int t = __builtin_amdgcn_readfirstlane(blk * 4 + (threadIdx.x >> 6));
+  const int w = threadIdx.x >> 6;
+  const int t = __builtin_amdgcn_readfirstlane(blk * 4 + w);
   if (t >= ntasks) return;
   const MrTask d = tasks[t];
   const int M = d.M, K = d.K;
-  int arow[4];
-#pragma unroll
-  for (int rt = 0; rt < 4; rt++) arow[rt] = min(16 * rt + i, M - 1);
+  int* sidx = lds_idx + w * kmax;
+  if constexpr (IM == 1) {
+    const auto bidx = gptr(d.bidx);
+    for (int k = l; k < K; k += 64) sidx[k] = bidx[d.b0 + k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
   Acc acc[4][NCT];
 #pragma unroll
   for (int rt = 0; rt < 4; rt++)
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = Acc{};
-  if (K > 0) {
-    GemmFrag<P, NCT> f0, f1;
-    gemm_load<P, NCT, NRT>(f0, d, arow, 0, i, g);
-    for (int k0 = 0;; k0 += 2 * KC) {
-      if (k0 + KC < K) gemm_load<P, NCT, NRT>(f1, d, arow, k0 + KC, i, g);
-      gemm_mma<P, NCT, NRT>(f0, acc);
-      if (k0 + KC >= K) break;
-      if (k0 + 2 * KC < K) gemm_load<P, NCT, NRT>(f0, d, arow, k0 + 2 * KC, i, g);
-      gemm_mma<P, NCT, NRT>(f1, acc);
-      if (k0 + 2 * KC >= K) break;
+  const int nch = (K + KC - 1) / KC;
+  if (nch > 0) {
+    // ring of D register sets: chunk c + D - 1 is loaded while chunk c is
+    // multiplied (static indices: the ring position is a compile-time
+    // constant in the unrolled loop)
+    GemmFrag<P, NCT> f[D];
+#pragma unroll
+    for (int u = 0; u < D - 1; u++)
+      if (u < nch) gemm_load<P, NCT, NRT, IM>(f[u], d, u * KC, i, g, sidx);
+    for (int c = 0; c < nch; c += D) {
+#pragma unroll
+      for (int u = 0; u < D; u++) {
+        if (c + u < nch) {
+          if (c + u + D - 1 < nch)
+            gemm_load<P, NCT, NRT, IM>(f[(u + D - 1) % D], d, (c + u + D - 1) * KC, i, g, sidx);
+          gemm_mma<P, NCT, NRT>(f[u], acc);
+        }
+      }
     }
   }
   const auto C = gptr_w(static_cast<VT*>(d.C));
@@ -315,24 +369,15 @@ __device__ __forceinline__ void two_sum(float& s, float& c, float x) {
 }
 
 template <int P, int NCT, int NRT, int KID>
-__global__ __launch_bounds__(256) void k_mr_p2p(const MrNearTask* __restrict__ tasks, int ntasks,
-                                                const MrNearProb* __restrict__ probs,
-                                                const double2* __restrict__ logtab,
-                                                double ln2_hi, double ln2_lo) {
+__device__ __forceinline__ void mr_near_task(const MrNearTask& d,
+                                             const MrNearProb* __restrict__ probs,
+                                             const double2* tab, double ln2_hi, double ln2_lo) {
   using T = MrT<P>;
   using VT = typename T::V;
   using Acc = typename T::Acc;
   constexpr int RP = 16 * NCT;
   constexpr int KC = T::KC;
-  __shared__ double2 tab[128];
-  if constexpr (P == PREC_F64 && KID == K_LOG) {
-    for (int u = threadIdx.x; u < 128; u += blockDim.x) tab[u] = logtab[u];
-    __syncthreads();
-  }
   const int l = threadIdx.x & 63, i = l & 15, g = l >> 4;
-  const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (t >= ntasks) return;
-  const MrNearTask& d = tasks[t];
   const MrNearProb& pr = probs[d.prob];
   const KParams kp = pr.kp;
   const auto X = gptr(pr.x);
@@ -487,6 +532,35 @@ __global__ __launch_bounds__(256) void k_mr_p2p(const MrNearTask* __restrict__ t
   }
 }
 
+// The near field's launch: a persistent grid of a few waves per SIMD (the
+// memory-bound tree passes run beside it) taking tasks from a counter, one
+// atomic per wave and task.
+template <int P, int NCT, int NRT, int KID>
+__global__ __launch_bounds__(256) void k_mr_p2p(const MrNearTask* __restrict__ tasks, int ntasks,
+                                                const MrNearProb* __restrict__ probs,
+                                                const double2* __restrict__ logtab,
+                                                double ln2_hi, double ln2_lo,
+                                                int* __restrict__ ctr) {
+  __shared__ double2 tab[128];
+  if constexpr (P == PREC_F64 && KID == K_LOG) {
+    for (int u = threadIdx.x; u < 128; u += blockDim.x) tab[u] = logtab[u];
+    __syncthreads();
+  }
+  const int l = threadIdx.x & 63;
+  for (;;) {
+    int t = 0;
+    if (l == 0) t = atomicAdd(ctr, 1);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= ntasks) break;
+    mr_near_task<P, NCT, NRT, KID>(tasks[t], probs, tab, ln2_hi, ln2_lo);
+  }
+}
+
+__global__ void k_mr_zero(int* __restrict__ v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // Gather: qs[inv_perm[i]][c] = b[i + c ldb] (0 for c >= nrhs).  Lane c of a
 // row of RP threads, RP-column rows written whole, b read along columns.
@@ -534,29 +608,52 @@ struct MrBoxV {
   int n, r, vld, vtld;
 };
 
-// FP64 basis: V = the box's L2P with its skeleton rows the exact identity
-// (row rowmap[nc + q] is e_q, stored row t is row rowmap[t]), and V^T.
+// FP64 basis, row-major: V = the box's L2P with its skeleton rows the exact
+// identity (row rowmap[nc + q] is e_q, stored row t is row rowmap[t]).
 __global__ __launch_bounds__(256) void k_mr_build_v64(const MrBoxV* __restrict__ boxes,
                                                       const double* __restrict__ l2p_rm,
                                                       const int* __restrict__ rowmap,
-                                                      double* __restrict__ V,
-                                                      double* __restrict__ Vt) {
+                                                      double* __restrict__ V) {
   __shared__ int inv_row[1024];
   const MrBoxV d = boxes[blockIdx.x];
   const int n = d.n, r = d.r, nc = n - r;
   for (int t = threadIdx.x; t < n; t += blockDim.x) inv_row[rowmap[d.rmbase + t]] = t;
   __syncthreads();
-  auto val = [&](int row, int c) -> double {
-    const int t = inv_row[row];
-    return t < nc ? l2p_rm[d.l2poff + (int64_t)t * r + c] : (t - nc == c ? 1.0 : 0.0);
-  };
   for (int e = threadIdx.x; e < n * d.vld; e += blockDim.x) {
     const int row = e / d.vld, c = e - row * d.vld;
-    V[d.voff + e] = c < r ? val(row, c) : 0.0;
+    double v = 0.0;
+    if (c < r) {
+      const int t = inv_row[row];
+      v = t < nc ? l2p_rm[d.l2poff + (int64_t)t * r + c] : (t - nc == c ? 1.0 : 0.0);
+    }
+    V[d.voff + e] = v;
   }
-  for (int e = threadIdx.x; e < r * d.vtld; e += blockDim.x) {
-    const int c = e / d.vtld, row = e - c * d.vtld;
-    Vt[d.vtoff + e] = row < n ? val(row, c) : 0.0;
+}
+
+// A row-major FP64 matrix (M x K, leading dimension ld; trans: the matrix is
+// the transpose of the one stored) into the tiled layout of precision P.
+struct MrBoxRT {
+  int64_t src, dst;
+  int M, K, ld, trans;
+};
+template <int P>
+__global__ __launch_bounds__(256) void k_mr_retile(const MrBoxRT* __restrict__ boxes,
+                                                   const double* __restrict__ src,
+                                                   typename MrT<P>::A* __restrict__ dst) {
+  using AT = typename MrT<P>::A;
+  const MrBoxRT d = boxes[blockIdx.x];
+  const int kg = P == PREC_BF16 ? 8 : 4;
+  const int kp = (d.K + kg - 1) / kg * kg;
+  const int64_t tot = (int64_t)d.M * kp;
+  for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int row = (int)(e / kp), k = (int)(e - (int64_t)row * kp);
+    double v = 0.0;
+    if (k < d.K) v = d.trans ? src[d.src + (int64_t)k * d.ld + row] : src[d.src + (int64_t)row * d.ld + k];
+    AT x;
+    if constexpr (P == PREC_F64) x = v;
+    else if constexpr (P == PREC_F32) x = (float)v;
+    else x = __builtin_bit_cast(uint16_t, (__bf16)(float)v);
+    dst[d.dst + tile_off<P>(row, k, d.M, kp)] = x;
   }
 }
 
@@ -680,22 +777,33 @@ struct MrBoxMP {
   int64_t broff[kMaxILBlocks];
 };
 
-// M2L' = R_k M R_b^T block by block (FP64), written in the factor type.
-template <class AT>
+// M2L' = R_k M R_b^T block by block (FP64), written in the tiled layout of
+// precision P (r x K matrix, columns up to kp zero).
+template <int P>
 __global__ __launch_bounds__(256) void k_mr_mprime(const MrBoxMP* __restrict__ boxes,
                                                    const double* __restrict__ M64,
                                                    const double* __restrict__ Rbuf,
-                                                   AT* __restrict__ Mout) {
+                                                   typename MrT<P>::A* __restrict__ Mout) {
+  using AT = typename MrT<P>::A;
   __shared__ double T[4096];
   const MrBoxMP& d = boxes[blockIdx.x];
   const int r = d.r, ld = d.mld;
+  const int kg = P == PREC_BF16 ? 8 : 4;
+  const int kp = (d.K + kg - 1) / kg * kg;
   const double* Rk = Rbuf + d.roff;
   const double* Mk = M64 + d.moff;
   AT* Ok = Mout + d.moff;
+  auto put = [&](int i, int c, double v) {
+    AT x;
+    if constexpr (P == PREC_F64) x = v;
+    else if constexpr (P == PREC_F32) x = (float)v;
+    else x = __builtin_bit_cast(uint16_t, (__bf16)(float)v);
+    Ok[tile_off<P>(i, c, r, kp)] = x;
+  };
   // padding columns
-  for (int64_t e = threadIdx.x; e < (int64_t)r * (ld - d.K); e += blockDim.x) {
-    const int i = (int)(e / (ld - d.K)), c = d.K + (int)(e % (ld - d.K));
-    Ok[(int64_t)i * ld + c] = (AT)0.0f;
+  for (int e = threadIdx.x; e < r * (kp - d.K); e += blockDim.x) {
+    const int i = e / (kp - d.K), c = d.K + e % (kp - d.K);
+    put(i, c, 0.0);
   }
   for (int b = 0; b < d.nblk; b++) {
     const int rb = d.brank[b], off = d.boff[b];
@@ -717,22 +825,9 @@ __global__ __launch_bounds__(256) void k_mr_mprime(const MrBoxMP* __restrict__ b
         const int ii = e / rb, u = e - ii * rb;
         double s = 0.0;
         for (int wv = u; wv < rb; wv++) s = fma(T[ii * rb + wv], Rb[(int64_t)u * rb + wv], s);
-        Ok[(int64_t)(i0 + ii) * ld + off + u] = (AT)(float)s;
+        put(i0 + ii, off + u, s);
       }
     }
-  }
-}
-
-// Q (FP64) -> V and V^T of the factor type.
-template <class AT>
-__global__ __launch_bounds__(256) void k_mr_convert_v(const MrBoxV* __restrict__ boxes,
-                                                      const double* __restrict__ Q,
-                                                      AT* __restrict__ V, AT* __restrict__ Vt) {
-  const MrBoxV d = boxes[blockIdx.x];
-  for (int e = threadIdx.x; e < d.n * d.vld; e += blockDim.x) V[d.voff + e] = (AT)(float)Q[d.voff + e];
-  for (int e = threadIdx.x; e < d.r * d.vtld; e += blockDim.x) {
-    const int c = e / d.vtld, row = e - c * d.vtld;
-    Vt[d.vtoff + e] = row < d.n ? (AT)(float)Q[d.voff + (int64_t)row * d.vld + c] : (AT)0.0f;
   }
 }
 
@@ -819,15 +914,15 @@ void mr_layout(Engine& e) {
   e.mr_layout = true;
 }
 
-void build_f64(Engine& e) {
-  MrFactors& f = e.mrf[PREC_F64];
-  if (f.built) return;
+// Row-major FP64 sources of the factors: V (nin x rank, leading dimension
+// vld) and M (rank x K, leading dimension mld) of every box.
+void build_rm64(Engine& e, DevBuf& Vrm, DevBuf& Mrm) {
   mr_layout(e);
   const int L = e.L;
   hipStream_t s = e.s_aux;
-  f.V.alloc(sizeof(double) * std::max<int64_t>(e.mr_v_total, 8));
-  f.Vt.alloc(sizeof(double) * std::max<int64_t>(e.mr_vt_total, 8));
-  f.M.alloc(sizeof(double) * std::max<int64_t>(e.mr_m_total, 8));
+  Vrm.alloc(sizeof(double) * std::max<int64_t>(e.mr_v_total, 8));
+  Mrm.alloc(sizeof(double) * std::max<int64_t>(e.mr_m_total, 8));
+  AFMM_HIP(hipMemsetAsync(Vrm.p, 0, Vrm.bytes, s));
   std::vector<MrBoxV> bv;
   std::vector<MrBoxM> bm;
   for (int j = 2; j <= L; j++) {
@@ -842,23 +937,96 @@ void build_f64(Engine& e) {
         bm.push_back(MrBoxM{li.soff[k], e.mr_moff[g], e.mr_coloff[g], r, e.mr_K[g], e.mr_mld[g], 0});
     }
   }
-  // boxes of rank 0 keep zero factors (their tasks read none)
-  AFMM_HIP(hipMemsetAsync(f.V.p, 0, f.V.bytes, s));
-  AFMM_HIP(hipMemsetAsync(f.Vt.p, 0, f.Vt.bytes, s));
   DevBuf dbv, dbm;
   upload_vec(dbv, bv, s);
   upload_vec(dbm, bm, s);
   if (!bv.empty()) {
     k_mr_build_v64<<<(unsigned)bv.size(), 256, 0, s>>>(dbv.as<MrBoxV>(), e.l2p_rm.as<double>(),
-                                                       e.rowmap.as<int>(), f.V.as<double>(),
-                                                       f.Vt.as<double>());
+                                                       e.rowmap.as<int>(), Vrm.as<double>());
     AFMM_LAUNCH_CHECK();
   }
   if (!bm.empty()) {
     AFMM_DISPATCH_KERNEL(e.kid, KID,
                          k_mr_build_m64<KID><<<(unsigned)bm.size(), 256, 0, s>>>(
                              dbm.as<MrBoxM>(), e.skx.as<double>(), e.sky.as<double>(),
-                             e.mr_colidx.as<int>(), f.M.as<double>(), e.kp));
+                             e.mr_colidx.as<int>(), Mrm.as<double>(), e.kp));
+    AFMM_LAUNCH_CHECK();
+  }
+  AFMM_HIP(hipStreamSynchronize(s));
+}
+
+// The tiled V and Vt of precision prec from a row-major FP64 basis (Vrm's
+// layout), into f.
+void retile_basis(Engine& e, int prec, const DevBuf& Vsrc, MrFactors& f) {
+  const int L = e.L;
+  hipStream_t s = e.s_aux;
+  std::vector<MrBoxRT> bv, bt;
+  for (int j = 2; j <= L; j++) {
+    const LevelInfo& li = e.lv[j];
+    for (int k = 0; k < li.nb; k++) {
+      const int64_t g = li.gbase + k;
+      const int r = li.rank[k], n = li.nin[k];
+      if (r <= 0 || n <= 0) continue;
+      bv.push_back(MrBoxRT{e.mr_voff[g], e.mr_voff[g], n, r, e.mr_vld[g], 0});
+      bt.push_back(MrBoxRT{e.mr_voff[g], e.mr_vtoff[g], r, n, e.mr_vld[g], 1});
+    }
+  }
+  if (bv.empty()) return;
+  DevBuf dbv, dbt;
+  upload_vec(dbv, bv, s);
+  upload_vec(dbt, bt, s);
+  const unsigned nb = (unsigned)bv.size();
+#define AFMM_MR_RETILE(P)                                                                       \
+  k_mr_retile<P><<<nb, 256, 0, s>>>(dbv.as<MrBoxRT>(), Vsrc.as<double>(),                      \
+                                    f.V.as<typename MrT<P>::A>());                              \
+  AFMM_LAUNCH_CHECK();                                                                          \
+  k_mr_retile<P><<<nb, 256, 0, s>>>(dbt.as<MrBoxRT>(), Vsrc.as<double>(),                      \
+                                    f.Vt.as<typename MrT<P>::A>());                             \
+  AFMM_LAUNCH_CHECK();
+  switch (prec) {
+    case PREC_F64: AFMM_MR_RETILE(PREC_F64) break;
+    case PREC_F32: AFMM_MR_RETILE(PREC_F32) break;
+    default: AFMM_MR_RETILE(PREC_BF16) break;
+  }
+#undef AFMM_MR_RETILE
+  AFMM_HIP(hipStreamSynchronize(s));
+}
+
+void alloc_factors(Engine& e, int prec, MrFactors& f) {
+  const size_t fb = factor_bytes(prec);
+  hipStream_t s = e.s_aux;
+  f.V.alloc(fb * std::max<int64_t>(e.mr_v_total, 8));
+  f.Vt.alloc(fb * std::max<int64_t>(e.mr_vt_total, 8));
+  f.M.alloc(fb * std::max<int64_t>(e.mr_m_total, 8));
+  // boxes of rank 0 keep zero factors (their tasks read none)
+  AFMM_HIP(hipMemsetAsync(f.V.p, 0, f.V.bytes, s));
+  AFMM_HIP(hipMemsetAsync(f.Vt.p, 0, f.Vt.bytes, s));
+  AFMM_HIP(hipMemsetAsync(f.M.p, 0, f.M.bytes, s));
+}
+
+void build_f64(Engine& e) {
+  MrFactors& f = e.mrf[PREC_F64];
+  if (f.built) return;
+  DevBuf Vrm, Mrm;
+  build_rm64(e, Vrm, Mrm);
+  alloc_factors(e, PREC_F64, f);
+  retile_basis(e, PREC_F64, Vrm, f);
+  const int L = e.L;
+  hipStream_t s = e.s_aux;
+  std::vector<MrBoxRT> bm;
+  for (int j = 2; j <= L; j++) {
+    const LevelInfo& li = e.lv[j];
+    for (int k = 0; k < li.nb; k++) {
+      const int64_t g = li.gbase + k;
+      if (li.rank[k] > 0)
+        bm.push_back(MrBoxRT{e.mr_moff[g], e.mr_moff[g], li.rank[k], e.mr_K[g], e.mr_mld[g], 0});
+    }
+  }
+  if (!bm.empty()) {
+    DevBuf dbm;
+    upload_vec(dbm, bm, s);
+    k_mr_retile<PREC_F64><<<(unsigned)bm.size(), 256, 0, s>>>(dbm.as<MrBoxRT>(), Mrm.as<double>(),
+                                                             f.M.as<double>());
     AFMM_LAUNCH_CHECK();
   }
   AFMM_HIP(hipStreamSynchronize(s));
@@ -869,8 +1037,8 @@ void build_f64(Engine& e) {
 void build_reduced(Engine& e, int prec) {
   MrFactors& f = e.mrf[prec];
   if (f.built) return;
-  build_f64(e);
-  const MrFactors& f64 = e.mrf[PREC_F64];
+  DevBuf Vrm, Mrm;
+  build_rm64(e, Vrm, Mrm);
   const int L = e.L;
   hipStream_t s = e.s_aux;
   const int64_t nbox = e.lv[L].gbase + e.lv[L].nb;
@@ -913,28 +1081,20 @@ void build_reduced(Engine& e, int prec) {
     if (bq.empty()) continue;
     DevBuf dbq;
     upload_vec(dbq, bq, s);
-    k_mr_qr<<<(unsigned)bq.size(), 256, 0, s>>>(dbq.as<MrBoxQR>(), f64.V.as<double>(),
+    k_mr_qr<<<(unsigned)bq.size(), 256, 0, s>>>(dbq.as<MrBoxQR>(), Vrm.as<double>(),
                                                 Q64.as<double>(), e.mr_R.as<double>());
     AFMM_LAUNCH_CHECK();
     AFMM_HIP(hipStreamSynchronize(s));
   }
   e.mr_ortho = true;
-  const size_t fb = factor_bytes(prec);
-  f.V.alloc(fb * std::max<int64_t>(e.mr_v_total, 8));
-  f.Vt.alloc(fb * std::max<int64_t>(e.mr_vt_total, 8));
-  f.M.alloc(fb * std::max<int64_t>(e.mr_m_total, 8));
-  AFMM_HIP(hipMemsetAsync(f.V.p, 0, f.V.bytes, s));
-  AFMM_HIP(hipMemsetAsync(f.Vt.p, 0, f.Vt.bytes, s));
-  AFMM_HIP(hipMemsetAsync(f.M.p, 0, f.M.bytes, s));
-  std::vector<MrBoxV> bv;
+  alloc_factors(e, prec, f);
+  retile_basis(e, prec, Q64, f);
   std::vector<MrBoxMP> bm;
   for (int j = 2; j <= L; j++) {
     const LevelInfo& li = e.lv[j];
     for (int k = 0; k < li.nb; k++) {
       const int64_t g = li.gbase + k;
-      const int r = li.rank[k], n = li.nin[k];
-      if (r > 0 && n > 0)
-        bv.push_back(MrBoxV{0, 0, e.mr_voff[g], e.mr_vtoff[g], n, r, e.mr_vld[g], e.mr_vtld[g]});
+      const int r = li.rank[k];
       if (r <= 0) continue;
       MrBoxMP d{};
       d.moff = e.mr_moff[g];
@@ -959,26 +1119,16 @@ void build_reduced(Engine& e, int prec) {
       bm.push_back(d);
     }
   }
-  DevBuf dbv, dbm;
-  upload_vec(dbv, bv, s);
-  upload_vec(dbm, bm, s);
-  if (prec == PREC_F32) {
-    if (!bv.empty())
-      k_mr_convert_v<float><<<(unsigned)bv.size(), 256, 0, s>>>(dbv.as<MrBoxV>(), Q64.as<double>(),
-                                                               f.V.as<float>(), f.Vt.as<float>());
-    AFMM_LAUNCH_CHECK();
-    if (!bm.empty())
-      k_mr_mprime<float><<<(unsigned)bm.size(), 256, 0, s>>>(dbm.as<MrBoxMP>(), f64.M.as<double>(),
-                                                            e.mr_R.as<double>(), f.M.as<float>());
-    AFMM_LAUNCH_CHECK();
-  } else {
-    if (!bv.empty())
-      k_mr_convert_v<__bf16><<<(unsigned)bv.size(), 256, 0, s>>>(
-          dbv.as<MrBoxV>(), Q64.as<double>(), f.V.as<__bf16>(), f.Vt.as<__bf16>());
-    AFMM_LAUNCH_CHECK();
-    if (!bm.empty())
-      k_mr_mprime<__bf16><<<(unsigned)bm.size(), 256, 0, s>>>(
-          dbm.as<MrBoxMP>(), f64.M.as<double>(), e.mr_R.as<double>(), f.M.as<__bf16>());
+  if (!bm.empty()) {
+    DevBuf dbm;
+    upload_vec(dbm, bm, s);
+    const unsigned nb = (unsigned)bm.size();
+    if (prec == PREC_F32)
+      k_mr_mprime<PREC_F32><<<nb, 256, 0, s>>>(dbm.as<MrBoxMP>(), Mrm.as<double>(),
+                                              e.mr_R.as<double>(), f.M.as<float>());
+    else
+      k_mr_mprime<PREC_BF16><<<nb, 256, 0, s>>>(dbm.as<MrBoxMP>(), Mrm.as<double>(),
+                                               e.mr_R.as<double>(), f.M.as<uint16_t>());
     AFMM_LAUNCH_CHECK();
   }
   AFMM_HIP(hipStreamSynchronize(s));
@@ -987,23 +1137,60 @@ void build_reduced(Engine& e, int prec) {
 
 // ---------------------------------------------------------------------------
 // Plans: the task lists of one (precision, column tiles) pair.
-void add_row_groups(std::vector<MrTask>& out, const void* A, size_t fbytes, int lda, const void* B,
-                    const int* bidx, int64_t b0, void* C, int64_t c0, int M, int K, int beta) {
-  for (int r0 = 0; r0 < M; r0 += 64) {
+
+// Rows per task of the M2L and the other passes (AFMM2D_MR_ROWS_M2L,
+// AFMM2D_MR_ROWS: 16, 32, 48 or 64), the XCD mapping of the M2L launch
+// (AFMM2D_MR_XCD), the near field's waves per SIMD (AFMM2D_MR_NEAR_WAVES)
+// and the chunk ring depth of the M2L (AFMM2D_MR_DEPTH: 3, 4, 6), read once.
+struct MrKnobs {
+  int rows_m2l = 64, rows = 16, xcd = 0, near_waves = 2, depth = 3;
+  MrKnobs() {
+    auto rd = [](const char* n, int d) {
+      const char* v = std::getenv(n);
+      return v ? std::atoi(v) : d;
+    };
+    rows_m2l = std::max(16, std::min(64, rd("AFMM2D_MR_ROWS_M2L", 64))) / 16 * 16;
+    rows = std::max(16, std::min(64, rd("AFMM2D_MR_ROWS", 16))) / 16 * 16;
+    xcd = rd("AFMM2D_MR_XCD", 0);
+    near_waves = std::max(1, std::min(16, rd("AFMM2D_MR_NEAR_WAVES", 2)));
+    depth = rd("AFMM2D_MR_DEPTH", 3);
+  }
+};
+const MrKnobs& knobs() {
+  static MrKnobs k;
+  return k;
+}
+
+void add_row_groups(std::vector<MrTask>& out, const void* A, int kg, const void* B,
+                    const int* bidx, int64_t b0, void* C, int64_t c0, int M, int K, int beta,
+                    int rows = 0) {
+  if (rows <= 0) rows = knobs().rows;
+  // the same number of rows in every group of the box (none smaller than
+  // needed): ceil(M / ceil(M / rows)) rounded up to 16
+  const int ng = (M + rows - 1) / rows;
+  const int gr = ng > 0 ? ((M + ng - 1) / ng + 15) / 16 * 16 : rows;
+  for (int r0 = 0; r0 < M; r0 += gr) {
     MrTask t;
-    t.A = static_cast<const char*>(A) + (int64_t)r0 * lda * fbytes;
+    t.A = A;
     t.B = B;
     t.bidx = bidx;
     t.C = C;
     t.b0 = b0;
     t.c0 = c0 + r0;
-    t.M = std::min(64, M - r0);
+    t.M = std::min(gr, M - r0);
     t.K = K;
-    t.lda = lda;
+    t.kp = (K + kg - 1) / kg * kg;
+    t.mbox = M;
+    t.rt0 = r0 / 16;
     t.beta = beta;
     out.push_back(t);
   }
 }
+
+inline int task_k(const MrTask& t) { return t.K; }
+inline int task_k(const MrNearTask&) { return 0; }
+inline bool task_indexed(const MrTask& t) { return t.bidx != nullptr; }
+inline bool task_indexed(const MrNearTask&) { return false; }
 
 // Orders a task list by row tiles and uploads it.
 template <class T>
@@ -1011,6 +1198,12 @@ void upload_tasks(MrTaskList& tl, std::vector<T>& tasks, hipStream_t s) {
   std::stable_sort(tasks.begin(), tasks.end(),
                    [](const T& a, const T& b) { return (a.M + 15) / 16 < (b.M + 15) / 16; });
   tl.n = (int)tasks.size();
+  for (int c = 0; c < 4; c++) tl.kmax[c] = 0;
+  tl.indexed = false;
+  for (const T& t : tasks) {
+    tl.kmax[(t.M + 15) / 16 - 1] = std::max(tl.kmax[(t.M + 15) / 16 - 1], task_k(t));
+    tl.indexed = tl.indexed || task_indexed(t);
+  }
   for (int c = 0; c < 5; c++) tl.cls[c] = tl.n;
   for (int t = tl.n - 1; t >= 0; t--) tl.cls[(tasks[t].M + 15) / 16 - 1] = t;
   for (int c = 3; c >= 0; c--) tl.cls[c] = std::min(tl.cls[c], tl.cls[c + 1]);
@@ -1029,6 +1222,7 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
   const int L = e.L;
   const int RP = 16 * nct;
   const size_t vb = vector_bytes(prec), fb = factor_bytes(prec);
+  const int kg = col_group(prec);
   const MrFactors& f = e.mrf[prec];
   hipStream_t s = e.s_aux;
   const int64_t nsk = std::max<int64_t>(e.skel_total, 1);
@@ -1057,23 +1251,23 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
         const void* Bsrc = j == L ? vrow(p.qs) : vrow(p.outv);
         const int64_t b0 = j == L ? e.leaf_start[k] : e.lv[j + 1].soff[4 * (size_t)k];
         if (n > 0)
-          add_row_groups(up, fptr(f.Vt, e.mr_vtoff[g]), fb, e.mr_vtld[g], Bsrc, nullptr, b0,
-                         p.outv.p, li.soff[k], r, n, 0);
+          add_row_groups(up, fptr(f.Vt, e.mr_vtoff[g]), kg, Bsrc, nullptr, b0, p.outv.p,
+                         li.soff[k], r, n, 0);
         else
-          add_row_groups(up, fptr(f.Vt, e.mr_vtoff[g]), fb, e.mr_vtld[g], Bsrc, nullptr, 0,
-                         p.outv.p, li.soff[k], r, 0, 0);
+          add_row_groups(up, fptr(f.Vt, e.mr_vtoff[g]), kg, Bsrc, nullptr, 0, p.outv.p,
+                         li.soff[k], r, 0, 0);
         // M2L: in_k = M_k [out_b ...]
-        add_row_groups(m2l, fptr(f.M, e.mr_moff[g]), fb, e.mr_mld[g], vrow(p.outv),
-                       e.mr_colidx.as<int>(), e.mr_coloff[g], p.inv.p, li.soff[k], r, e.mr_K[g], 0);
+        add_row_groups(m2l, fptr(f.M, e.mr_moff[g]), kg, vrow(p.outv), e.mr_colidx.as<int>(),
+                       e.mr_coloff[g], p.inv.p, li.soff[k], r, e.mr_K[g], 0, knobs().rows_m2l);
       }
       if (j < L) {
         // L2L: children's in += V in_k
         if (r > 0 && n > 0)
-          add_row_groups(down, fptr(f.V, e.mr_voff[g]), fb, e.mr_vld[g], vrow(p.inv), nullptr,
-                         li.soff[k], p.inv.p, e.lv[j + 1].soff[4 * (size_t)k], n, r, 1);
+          add_row_groups(down, fptr(f.V, e.mr_voff[g]), kg, vrow(p.inv), nullptr, li.soff[k],
+                         p.inv.p, e.lv[j + 1].soff[4 * (size_t)k], n, r, 1);
       } else if (n > 0) {
         // leaf: pot = V in_k (zero for a rank-0 leaf)
-        add_row_groups(leaf, fptr(f.V, e.mr_voff[g]), fb, e.mr_vld[g], vrow(p.inv), nullptr,
+        add_row_groups(leaf, fptr(f.V, e.mr_voff[g]), kg, vrow(p.inv), nullptr,
                        r > 0 ? li.soff[k] : 0, p.pot.p, e.leaf_start[k], n, r, 0);
       }
     }
@@ -1103,6 +1297,7 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
     }
   }
   upload_tasks(p.near, near, s);
+  p.near_ctr.alloc(sizeof(int) * 4 * kMrCtrStride);
   MrNearProb pr{e.x.as<double>(), e.y.as<double>(), p.qs.p, p.nearv.as<double>(), e.kp};
   upload_vec(p.probs, std::vector<MrNearProb>{pr}, s);
   AFMM_HIP(hipStreamSynchronize(s));
@@ -1110,35 +1305,54 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
   return *e.mr_plans.back();
 }
 
+template <int P, int NCT, int IM, int D>
+void launch_gemm_d(const MrTask* t, int n, int c, int xcd, int kmax, size_t shm, hipStream_t s) {
+  const unsigned grid = ceil_div(n, 4);
+  switch (c) {
+    case 0: k_mr_gemm<P, NCT, 1, IM, D><<<grid, 256, shm, s>>>(t, n, xcd, kmax); break;
+    case 1: k_mr_gemm<P, NCT, 2, IM, D><<<grid, 256, shm, s>>>(t, n, xcd, kmax); break;
+    case 2: k_mr_gemm<P, NCT, 3, IM, D><<<grid, 256, shm, s>>>(t, n, xcd, kmax); break;
+    default: k_mr_gemm<P, NCT, 4, IM, D><<<grid, 256, shm, s>>>(t, n, xcd, kmax); break;
+  }
+  AFMM_LAUNCH_CHECK();
+}
+
+// Indices staged in LDS up to this many bytes per workgroup (4 waves).
+constexpr size_t kMrIdxLds = 48 * 1024;
+
+// depth: register sets of the chunk ring (3, 4 or 6)
 template <int P, int NCT>
-void launch_gemm(const MrTaskList& tl, int xcd, hipStream_t s) {
+void launch_gemm(const MrTaskList& tl, int xcd, int depth, hipStream_t s) {
   for (int c = 0; c < 4; c++) {
     const int n = tl.cls[c + 1] - tl.cls[c];
     if (n <= 0) continue;
     const MrTask* t = tl.buf.as<MrTask>() + tl.cls[c];
-    const unsigned grid = ceil_div(n, 4);
-    switch (c) {
-      case 0: k_mr_gemm<P, NCT, 1><<<grid, 256, 0, s>>>(t, n, xcd); break;
-      case 1: k_mr_gemm<P, NCT, 2><<<grid, 256, 0, s>>>(t, n, xcd); break;
-      case 2: k_mr_gemm<P, NCT, 3><<<grid, 256, 0, s>>>(t, n, xcd); break;
-      default: k_mr_gemm<P, NCT, 4><<<grid, 256, 0, s>>>(t, n, xcd); break;
+    if (!tl.indexed) {
+      launch_gemm_d<P, NCT, 0, 3>(t, n, c, xcd, 0, 0, s);
+    } else {
+      const int kmax = std::max(1, tl.kmax[c]);
+      const size_t shm = sizeof(int) * 4 * (size_t)kmax;
+      if (shm > kMrIdxLds) launch_gemm_d<P, NCT, 2, 3>(t, n, c, xcd, 0, 0, s);
+      else if (depth >= 6) launch_gemm_d<P, NCT, 1, 6>(t, n, c, xcd, kmax, shm, s);
+      else if (depth == 4) launch_gemm_d<P, NCT, 1, 4>(t, n, c, xcd, kmax, shm, s);
+      else launch_gemm_d<P, NCT, 1, 3>(t, n, c, xcd, kmax, shm, s);
     }
-    AFMM_LAUNCH_CHECK();
   }
 }
 
 void launch_gemm_p(int prec, int nct, const MrTaskList& tl, int xcd, hipStream_t s) {
+  const int depth = knobs().depth;
   if (nct == 1) {
     switch (prec) {
-      case PREC_F64: launch_gemm<PREC_F64, 1>(tl, xcd, s); break;
-      case PREC_F32: launch_gemm<PREC_F32, 1>(tl, xcd, s); break;
-      default: launch_gemm<PREC_BF16, 1>(tl, xcd, s); break;
+      case PREC_F64: launch_gemm<PREC_F64, 1>(tl, xcd, depth, s); break;
+      case PREC_F32: launch_gemm<PREC_F32, 1>(tl, xcd, depth, s); break;
+      default: launch_gemm<PREC_BF16, 1>(tl, xcd, depth, s); break;
     }
   } else {
     switch (prec) {
-      case PREC_F64: launch_gemm<PREC_F64, 2>(tl, xcd, s); break;
-      case PREC_F32: launch_gemm<PREC_F32, 2>(tl, xcd, s); break;
-      default: launch_gemm<PREC_BF16, 2>(tl, xcd, s); break;
+      case PREC_F64: launch_gemm<PREC_F64, 2>(tl, xcd, depth, s); break;
+      case PREC_F32: launch_gemm<PREC_F32, 2>(tl, xcd, depth, s); break;
+      default: launch_gemm<PREC_BF16, 2>(tl, xcd, depth, s); break;
     }
   }
 }
@@ -1146,17 +1360,23 @@ void launch_gemm_p(int prec, int nct, const MrTaskList& tl, int xcd, hipStream_t
 template <int P, int NCT>
 void launch_near_t(Engine& e, const MrPlan& p, hipStream_t s) {
   const MrTaskList& tl = p.near;
+  int* ctr = p.near_ctr.as<int>();
+  k_mr_zero<<<1, 4 * kMrCtrStride, 0, s>>>(ctr, 4 * kMrCtrStride);
+  AFMM_LAUNCH_CHECK();
   for (int c = 0; c < 4; c++) {
     const int n = tl.cls[c + 1] - tl.cls[c];
     if (n <= 0) continue;
     const MrNearTask* t = tl.buf.as<MrNearTask>() + tl.cls[c];
-    const unsigned grid = ceil_div(n, 4);
+    // waves per SIMD of the persistent grid (4 waves per workgroup)
+    const int64_t slots = (int64_t)e.num_cus * knobs().near_waves;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 4), slots));
     const MrNearProb* pr = p.probs.as<MrNearProb>();
     const double2* lt = e.log_tab.as<double2>();
+    int* cc = ctr + c * kMrCtrStride;
 #define AFMM_MR_NEAR(NRT)                                                                   \
   AFMM_DISPATCH_KERNEL(e.kid, KID,                                                          \
                        k_mr_p2p<P, NCT, NRT, KID><<<grid, 256, 0, s>>>(t, n, pr, lt, e.ln2_hi, \
-                                                                       e.ln2_lo))
+                                                                       e.ln2_lo, cc))
     switch (c) {
       case 0: AFMM_MR_NEAR(1); break;
       case 1: AFMM_MR_NEAR(2); break;
@@ -1233,23 +1453,32 @@ void mr_matvec(Engine& e, const double* d_b, int64_t ldb, double* d_y, int64_t l
   launch_io_p(prec, nct, true, io, s);
   e.prof_end(s, b);
   // the near field needs only the gathered charges: it runs beside the tree
-  AFMM_HIP(hipEventRecord(e.ev_fork, s));
-  AFMM_HIP(hipStreamWaitEvent(e.s_aux, e.ev_fork, 0));
-  e.prof_begin("mr_p2p", e.s_aux, &a, &b);
-  launch_near_p(e, p, e.s_aux);
-  e.prof_end(e.s_aux, b);
-  AFMM_HIP(hipEventRecord(e.ev_join, e.s_aux));
+  // (AFMM2D_NEAR_OVERLAP=0: after it, on the caller's stream)
+  if (e.near_overlap) {
+    AFMM_HIP(hipEventRecord(e.ev_fork, s));
+    AFMM_HIP(hipStreamWaitEvent(e.s_aux, e.ev_fork, 0));
+    e.prof_begin("mr_p2p", e.s_aux, &a, &b);
+    launch_near_p(e, p, e.s_aux);
+    e.prof_end(e.s_aux, b);
+    AFMM_HIP(hipEventRecord(e.ev_join, e.s_aux));
+  }
   e.prof_begin("mr_m2m", s, &a, &b);
   for (int j = L; j >= 2; j--) launch_gemm_p(prec, nct, p.up[j], 0, s);
   e.prof_end(s, b);
   e.prof_begin("mr_m2l", s, &a, &b);
-  launch_gemm_p(prec, nct, p.m2l, 1, s);
+  launch_gemm_p(prec, nct, p.m2l, knobs().xcd, s);
   e.prof_end(s, b);
   e.prof_begin("mr_l2l", s, &a, &b);
   for (int j = 2; j < L; j++) launch_gemm_p(prec, nct, p.down[j], 0, s);
   launch_gemm_p(prec, nct, p.leaf, 0, s);
   e.prof_end(s, b);
-  AFMM_HIP(hipStreamWaitEvent(s, e.ev_join, 0));
+  if (e.near_overlap) {
+    AFMM_HIP(hipStreamWaitEvent(s, e.ev_join, 0));
+  } else {
+    e.prof_begin("mr_p2p", s, &a, &b);
+    launch_near_p(e, p, s);
+    e.prof_end(s, b);
+  }
   e.prof_begin("mr_out", s, &a, &b);
   launch_io_p(prec, nct, false, io, s);
   e.prof_end(s, b);

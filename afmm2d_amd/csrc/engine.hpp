@@ -146,20 +146,21 @@ constexpr int kPrecisions = 3;
 // Right-hand sides per pass: 16 (one MFMA column tile) or 32 (two).
 constexpr int kMrMaxCols = 32;
 
-// One product of the blocked passes: C[c0 + i][:] (+)= sum_k A[i][k] B[row(k)][:]
-// for i < M <= 64, k < K.  A is row-major with leading dimension lda (a
-// multiple of 8) in the precision's factor type; B and C are row-major
-// vectors of 16 NCT columns in the precision's vector type (FP64 for FP64,
-// FP32 otherwise); row(k) = bidx ? bidx[b0 + k] : b0 + k.  beta = 1 adds
-// into C.  Pointers are absolute, so tasks of several problems can share a
-// launch.
+// One product of the blocked passes: C[c0 + i][:] (+)= sum_k A[16 rt0 + i][k]
+// B[row(k)][:] for i < M <= 64, k < K.  A is a box's factor matrix of mbox
+// rows and kp columns (K rounded up to the precision's column group) in the
+// tiled layout of blocked.hip, in the precision's factor type; B and C are
+// row-major vectors of 16 NCT columns in the precision's vector type (FP64
+// for FP64, FP32 otherwise); row(k) = bidx ? bidx[b0 + k] : b0 + k.  beta = 1
+// adds into C.  Pointers are absolute, so tasks of several problems can share
+// a launch.
 struct MrTask {
   const void* A;
   const void* B;
   const int* bidx;
   void* C;
   int64_t b0, c0;
-  int M, K, lda, beta;
+  int M, K, kp, mbox, rt0, beta;
 };
 
 // Near field of one group of up to 64 target points (KD-sorted indices ta ..
@@ -194,11 +195,11 @@ struct MrIoTask {
 };
 constexpr int kMrIoPoints = 256;
 
-// The factors of one precision.  V: each box's basis, nin x rank row-major
-// (leading dimension rank rounded up to 8); Vt: its transpose, rank x nin
-// (leading dimension nin rounded up to 8); M: each box's M2L row block,
-// rank x (sum of its interaction-list boxes' ranks), row-major (leading
-// dimension rounded up to 8), the column blocks in interaction-slot order.
+// The factors of one precision, each box's matrix in the tiled layout of
+// blocked.hip within a slot of rows x (columns rounded up to 8) elements.
+// V: the box's basis, nin x rank; Vt: its transpose, rank x nin; M: the
+// box's M2L row block, rank x (sum of its interaction-list boxes' ranks), the
+// column blocks in interaction-slot order.
 // FP64 uses the reference's interpolative basis; FP32 and BF16 an
 // orthonormal basis of the same operator (Q R = L2P, M2L' = R_k M2L R_b^T).
 struct MrFactors {
@@ -212,6 +213,8 @@ struct MrTaskList {
   DevBuf buf;
   int n = 0;
   int cls[5] = {0, 0, 0, 0, 0};
+  int kmax[4] = {0, 0, 0, 0};  // largest K per class
+  bool indexed = false;        // B rows through an index list
 };
 
 // Task lists and vectors of the blocked matvec for one (precision, column
@@ -222,6 +225,7 @@ struct MrPlan {
   std::vector<MrTaskList> up, down;  // per level (index j)
   MrTaskList m2l, leaf, near;
   DevBuf probs;
+  DevBuf near_ctr;  // work counters of the near field's persistent grid
 };
 
 struct Engine {
