@@ -45,6 +45,7 @@ EXPORTS = [
     "afmm2d_get_matrix_entry", "afmm2d_kernel_value", "afmm2d_get_stats",
     "afmm2d_set_profiling", "afmm2d_get_kernel_time", "afmm2d_reset_kernel_times",
     "afmm2d_matvec_ex", "afmm2d_matvec_device_ex", "afmm2d_prepare_precision",
+    "afmm2d_direct_rows_device_noalloc",
 ]
 
 # afmm2d_precision and the blocked-mode flag (include/afmm2d.h)
@@ -129,6 +130,7 @@ def lib():
         "afmm2d_matvec_ex": (i, [_vp, _dp, ctypes.c_int64, _dp, ctypes.c_int64, i, i]),
         "afmm2d_matvec_device_ex": (i, [_vp, _vp, ctypes.c_int64, _vp, ctypes.c_int64, i, i, _vp]),
         "afmm2d_prepare_precision": (i, [_vp, i]),
+        "afmm2d_direct_rows_device_noalloc": (i, [_vp, i, _vp, _vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -305,7 +305,6 @@ void begin_on(afmm2d_handle h, hipStream_t s) {
   if (h->have_done && h->last != s) AFMM_HIP(hipStreamWaitEvent(s, h->done, 0));
 }
 void end_on(afmm2d_handle h, hipStream_t s) {
-  if (!h->done) AFMM_HIP(hipEventCreateWithFlags(&h->done, hipEventDisableTiming));
   AFMM_HIP(hipEventRecord(h->done, s));
   h->last = s;
   h->have_done = true;
@@ -537,6 +536,13 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
     AFMM_HIP(hipMemcpyAsync(e->d_leaf_nbr.p, e->nbr[L].data(), sizeof(int) * 8 * (size_t)nleaf,
                             hipMemcpyHostToDevice, s));
     AFMM_HIP(hipStreamSynchronize(s));
+    {
+      // caller index -> KD-sorted index, on the device
+      std::vector<int> ip(n);
+      for (int i = 0; i < n; i++) ip[e->h_perm[i]] = i;
+      e->inv_perm_d.alloc(sizeof(int) * (size_t)n);
+      AFMM_HIP(hipMemcpy(e->inv_perm_d.p, ip.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+    }
     e->create_seconds = now_s() - t0;
     return e;
   };
@@ -546,7 +552,12 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
       e->kid = K_USER_SYM;
       e2 = make_engine(K_USER_ANTI);
     }
-    *out = new afmm2d_handle_s{std::move(e), std::move(e2)};
+    auto* hh = new afmm2d_handle_s{std::move(e), std::move(e2)};
+    if (hipEventCreateWithFlags(&hh->done, hipEventDisableTiming) != hipSuccess) {
+      delete hh;
+      throw Error(AFMM2D_ERR_HIP, "hipEventCreateWithFlags failed");
+    }
+    *out = hh;
   });
 }
 
@@ -853,6 +864,22 @@ int afmm2d_direct_rows(afmm2d_handle h, int nrows, const int64_t* rows, const do
     if (nrows > 0)
       AFMM_HIP(hipMemcpyAsync(out, d_out.p, sizeof(double) * nrows, hipMemcpyDeviceToHost, s));
     AFMM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int afmm2d_direct_rows_device_noalloc(afmm2d_handle h, int nrows, const int64_t* d_rows,
+                                      const double* d_b, double* d_out, void* stream) {
+  if (!h || nrows < 0 || (nrows > 0 && (!d_rows || !d_b || !d_out)))
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "bad argument");
+  if ((int64_t)nrows > kNoAllocScratchDoubles)
+    return fail(AFMM2D_ERR_UNSUPPORTED, "more rows than the static scratch holds (2^20)");
+  return guarded([&] {
+    Engine& e = *h->e;
+    AFMM_HIP(hipSetDevice(e.device));
+    hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)0;
+    begin_on(h, s);
+    direct_rows_noalloc(e, nrows, d_rows, d_b, d_out, s);
+    end_on(h, s);
   });
 }
 

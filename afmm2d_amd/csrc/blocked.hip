@@ -907,10 +907,6 @@ void mr_layout(Engine& e) {
   if ((int64_t)colidx.size() > INT32_MAX) throw Error(6, "M2L column index exceeds 2^31 entries");
   hipStream_t s = e.s_aux;
   upload_vec(e.mr_colidx, colidx, s);
-  // caller index -> KD-sorted index
-  std::vector<int> ip(e.n);
-  for (int i = 0; i < e.n; i++) ip[e.h_perm[i]] = i;
-  upload_vec(e.inv_perm_d, ip, s);
   e.mr_layout = true;
 }
 

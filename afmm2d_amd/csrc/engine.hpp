@@ -361,6 +361,11 @@ void near_field_device(Engine& e, const double* d_b, double* d_y, hipStream_t s)
 void add_device(double* d_y, const double* d_t, int n, hipStream_t s);
 void direct_rows_device(Engine& e, int nrows, const int* d_rows, const double* d_qs, double* d_out,
                         hipStream_t s);
+// Direct-sum rows of caller indices d_rows against caller-order charges d_b
+// (device arrays), the partial sums in a static device array: no allocation.
+constexpr int64_t kNoAllocScratchDoubles = (int64_t)1 << 20;  // 8 MiB
+void direct_rows_noalloc(Engine& e, int nrows, const int64_t* d_rows, const double* d_b,
+                         double* d_out, hipStream_t s);
 void gather_sorted(Engine& e, const double* d_b, double* d_qs, hipStream_t s);
 
 // blocked.hip

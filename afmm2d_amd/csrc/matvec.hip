@@ -21,6 +21,7 @@
 // downward pass adds each box's partials in its interaction-slot order, so
 // results do not depend on scheduling.
 #include <cstdlib>
+#include <mutex>
 
 #include "engine.hpp"
 #include "near_math.hpp"
@@ -1195,30 +1196,38 @@ __global__ __launch_bounds__(64 * WPB) void k_leaf(const double* __restrict__ l2
 }
 
 // Dense direct sum over all points for a set of rows, split over column
-// chunks into partial sums: part[chunk * nrows + row].
-template <int KID>
+// chunks into partial sums: part[chunk * nrows + row].  CALLER = false: rows
+// are KD-sorted indices (rows32) and q the KD-order charges; CALLER = true:
+// rows are caller indices (rows64, mapped through inv_perm) and q the
+// caller-order charges, read as q[perm[j]], the same values in the same order.
+template <int KID, bool CALLER>
 __global__ __launch_bounds__(256) void k_direct(const double* __restrict__ x,
                                                 const double* __restrict__ y,
-                                                const double* __restrict__ qs,
-                                                const int* __restrict__ rows, int nrows, int n,
+                                                const double* __restrict__ q,
+                                                const int* __restrict__ rows32,
+                                                const int64_t* __restrict__ rows64,
+                                                const int* __restrict__ perm,
+                                                const int* __restrict__ inv_perm, int nrows, int n,
                                                 int chunk, double* __restrict__ part, KParams kp) {
   __shared__ double sx[256], sy[256], sq[256];
   const int rr = blockIdx.x * 256 + threadIdx.x;
   const int c0 = blockIdx.y * chunk;
   const int c1 = min(n, c0 + chunk);
-  const int row = rr < nrows ? rows[rr] : 0;
+  int row = 0;
+  if (rr < nrows) row = CALLER ? inv_perm[rows64[rr]] : rows32[rr];
   const double xi = x[row], yi = y[row];
   double acc = 0.0;
   for (int cb = c0; cb < c1; cb += 256) {
     const int cn = min(256, c1 - cb);
     __syncthreads();
     if ((int)threadIdx.x < cn) {
-      sx[threadIdx.x] = x[cb + threadIdx.x];
-      sy[threadIdx.x] = y[cb + threadIdx.x];
-      sq[threadIdx.x] = qs[cb + threadIdx.x];
+      const int j = cb + threadIdx.x;
+      sx[threadIdx.x] = x[j];
+      sy[threadIdx.x] = y[j];
+      sq[threadIdx.x] = CALLER ? q[perm[j]] : q[j];
     }
     __syncthreads();
-    for (int q = 0; q < cn; q++) acc += kernel_xy<KID>(xi, yi, sx[q], sy[q], kp) * sq[q];
+    for (int t = 0; t < cn; t++) acc += kernel_xy<KID>(xi, yi, sx[t], sy[t], kp) * sq[t];
   }
   if (rr < nrows) part[(int64_t)blockIdx.y * nrows + rr] = acc;
 }
@@ -1710,25 +1719,87 @@ void near_field_device(Engine& e, const double* d_b, double* d_y, hipStream_t s)
   AFMM_LAUNCH_CHECK();
 }
 
+namespace {
+
+// Column chunks of the direct sum for nrows rows: enough workgroups to fill
+// the device (nrows * nchunk <= 2^20 for every nrows <= 2^20).
+int direct_chunks(int n, int nrows, int& chunk) {
+  const int rb = (int)ceil_div(nrows, 256);
+  int nchunk = std::max(1, std::min(4096 / rb, (int)ceil_div(n, 4096)));
+  chunk = (int)ceil_div(n, nchunk);
+  return (int)ceil_div(n, chunk);
+}
+
+}  // namespace
+
 void direct_rows_device(Engine& e, int nrows, const int* d_rows, const double* d_qs,
                         double* d_out, hipStream_t s) {
   if (nrows <= 0) return;
-  const int rb = (int)ceil_div(nrows, 256);
-  // enough workgroups to fill the device
-  int nchunk = std::max(1, std::min(4096 / rb, (int)ceil_div(e.n, 4096)));
-  const int chunk = (int)ceil_div(e.n, nchunk);
-  nchunk = (int)ceil_div(e.n, chunk);
+  int chunk = 0;
+  const int nchunk = direct_chunks(e.n, nrows, chunk);
   DevBuf part;
   part.alloc(sizeof(double) * (size_t)nrows * nchunk);
-  dim3 grid(rb, nchunk);
+  dim3 grid(ceil_div(nrows, 256), nchunk);
   AFMM_DISPATCH_KERNEL(e.kid_full, KID,
-                       k_direct<KID><<<grid, 256, 0, s>>>(e.x.as<double>(), e.y.as<double>(), d_qs,
-                                                          d_rows, nrows, e.n, chunk,
-                                                          part.as<double>(), e.kp));
+                       k_direct<KID, false><<<grid, 256, 0, s>>>(
+                           e.x.as<double>(), e.y.as<double>(), d_qs, d_rows, nullptr, nullptr,
+                           nullptr, nrows, e.n, chunk, part.as<double>(), e.kp));
   AFMM_LAUNCH_CHECK();
   k_sum_parts<<<ceil_div(nrows, 256), 256, 0, s>>>(part.as<double>(), nrows, nchunk, d_out);
   AFMM_LAUNCH_CHECK();
   AFMM_HIP(hipStreamSynchronize(s));
+}
+
+// ---- Direct-sum rows without allocation.
+//
+// The partial sums live in a file-scope device array (each GPU holds its own
+// copy until the process ends).  Its device address comes from
+// hipGetSymbolAddress, once per device with that device current; the array's
+// name is never used as an address in host code.  Every call on a GPU shares
+// that GPU's array: calls that overlap on one GPU (two streams, two threads)
+// race.  Nothing in it needs to start at zero: each call writes every partial
+// it reads.
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};  // device -> address of its scratch
+
+double* scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) throw Error(6, "device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return g_scratch_ptr[device];
+  int cur = -1;
+  AFMM_HIP(hipGetDevice(&cur));
+  if (cur != device) AFMM_HIP(hipSetDevice(device));
+  double* p = nullptr;
+  const hipError_t st = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+  if (cur != device) (void)hipSetDevice(cur);
+  if (st != hipSuccess || !p)
+    throw Error(5, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(st));
+  g_scratch_ptr[device] = p;
+  return p;
+}
+}  // namespace
+
+void direct_rows_noalloc(Engine& e, int nrows, const int64_t* d_rows, const double* d_b,
+                         double* d_out, hipStream_t s) {
+  if (nrows <= 0) return;
+  int chunk = 0;
+  const int nchunk = direct_chunks(e.n, nrows, chunk);
+  if ((int64_t)nrows * nchunk > kNoAllocScratchDoubles)
+    throw Error(6, "direct rows: partial sums exceed the static scratch");
+  double* part = scratch_ptr(e.device);
+  dim3 grid(ceil_div(nrows, 256), nchunk);
+  AFMM_DISPATCH_KERNEL(e.kid_full, KID,
+                       k_direct<KID, true><<<grid, 256, 0, s>>>(
+                           e.x.as<double>(), e.y.as<double>(), d_b, nullptr, d_rows,
+                           e.perm.as<int>(), e.inv_perm_d.as<int>(), nrows, e.n, chunk, part,
+                           e.kp));
+  AFMM_LAUNCH_CHECK();
+  k_sum_parts<<<ceil_div(nrows, 256), 256, 0, s>>>(part, nrows, nchunk, d_out);
+  AFMM_LAUNCH_CHECK();
 }
 
 }  // namespace afmm

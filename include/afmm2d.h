@@ -207,6 +207,18 @@ int afmm2d_direct(afmm2d_handle h, const double* b, double* y);
 int afmm2d_direct_rows(afmm2d_handle h, int nrows, const int64_t* rows, const double* b,
                        double* out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): afmm2d_direct_rows on device arrays without any allocation.
+ * d_rows: nrows caller indices (int64); d_b: N charges in caller order;
+ * d_out: nrows results; ordered on 'stream'.  The partial sums go to a
+ * static device buffer of 2^20 doubles (8 MiB per GPU, held until the
+ * process ends) shared by every call on that GPU: calls that overlap on one
+ * GPU race.  nrows > 2^20 is AFMM2D_ERR_UNSUPPORTED.  Results are
+ * bit-identical to afmm2d_direct_rows. */
+int afmm2d_direct_rows_device_noalloc(afmm2d_handle h, int nrows, const int64_t* d_rows,
+                                      const double* d_b, double* d_out, void* stream);
+
 /* FMM2DTree::compute_error (AFMM2DTree.cpp:1227-1236): relative 2-norm error
  * of leaf 'box' against the direct sum, for the last matvec's first RHS. */
 int afmm2d_compute_error(afmm2d_handle h, int box, double* err);
