@@ -11,6 +11,15 @@ gives about 3e-6 error at this size, so the headline run uses the smallest
 TOL_POW that meets the error bar (12); the run at the configured tolerance
 (TOL_POW = 8) is reported beside it under config["at_tol_pow_8"].
 
+Beside the headline (the single-RHS FP64 passes), the line reports the
+blocked passes on the same operator (16 and 32 right-hand sides per pass,
+factors in FP64, FP32 and BF16) under "blocked": point.RHS/s, the error of
+the first column, and the roofline of their M2L launch.
+
+Timing uses HIP events of the runtime libafmm2d.so links, recorded on the
+stream the kernels run on (afmm2d_amd.hip); device buffers come from the
+same runtime.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
@@ -49,7 +58,9 @@ def parse():
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
-    p.add_argument("--cpu-reps", type=int, default=3)
+    p.add_argument("--cpu-reps", type=int, default=5)
+    p.add_argument("--blocked-steps", type=int, default=10,
+                   help="timed passes per blocked variant (0: skip the blocked variants)")
     p.add_argument("--no-side-run", action="store_true",
                    help="skip the run at the configured tolerance (profiling runs)")
     p.add_argument("--calib-copy-mb", type=int, default=1024,
@@ -94,63 +105,128 @@ def measured_traffic(kernel, alg_bytes):
     return None, None
 
 
-def calib_copy(mb, torch):
+def calib_copy(mb, H):
     """A plain copy of `mb` MiB (read mb, write mb) from a source that an
     equally large fill has pushed out of the 256 MB last-level cache: the
     reference point for the HBM byte counters."""
     if mb <= 0:
         return
-    n = mb * (1 << 20) // 8
-    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
-    flush = torch.empty(n, dtype=torch.float64, device="cuda:0")
-    flush.fill_(2.0)
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    del src, flush, dst
+    nb = mb * (1 << 20)
+    src, flush, dst = H.DeviceArray(nb), H.DeviceArray(nb), H.DeviceArray(nb)
+    src.fill_zero()
+    flush.fill_zero()
+    dst.copy_from(src)
+    H.synchronize()
+    for a in (src, flush, dst):
+        a.free()
 
 
-def run_gpu(A, q, steps, warmup, torch):
+def timed(H, fn, steps, warmup):
+    """ms per call of fn over `steps` calls after `warmup`, with HIP events on
+    the null stream (the stream fn's kernels run on), device synchronised on
+    both sides; and the host wall clock per call."""
+    for _ in range(warmup):
+        fn()
+    H.synchronize()
+    e0, e1 = H.Event(), H.Event()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    e1.synchronize()
+    H.synchronize()
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    ms = e0.elapsed_ms(e1) / steps
+    e0.destroy()
+    e1.destroy()
+    return ms, wall
+
+
+def run_gpu(A, q, steps, warmup, H):
     """Times `steps` matvecs on device-resident q/y; returns ms/step and
     per-kernel averages from the engine's HIP events."""
-    dev = torch.device("cuda:0")
-    dq = torch.from_numpy(q).to(dev)
-    dy = torch.empty_like(dq)
-    stream = torch.cuda.current_stream().cuda_stream
+    dq = H.DeviceArray.from_numpy(q)
+    dy = H.DeviceArray(q.nbytes)
     for _ in range(warmup):
-        A.matvec_device(dq.data_ptr(), dy.data_ptr(), stream)
-    torch.cuda.synchronize()
+        A.matvec_device(dq.ptr, dy.ptr)
     # timed region: only the leaf-level M2L launch carries an event pair (on
     # the stream it runs on), for the roofline; everything else is untouched
     A.reset_kernel_times()
     A.set_profiling(True, m2l_only=True)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record()
-    for _ in range(steps):
-        A.matvec_device(dq.data_ptr(), dy.data_ptr(), stream)
-    ev1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    ms, wall = timed(H, lambda: A.matvec_device(dq.ptr, dy.ptr), steps, 0)
     A.set_profiling(False)
-    ms = ev0.elapsed_time(ev1) / steps
     tot, cnt = A.kernel_time("m2l")
     m2l_ms = tot / cnt if cnt else 0.0
     # phase breakdown from a separate pass with every phase timed
     A.reset_kernel_times()
     A.set_profiling(True)
     for _ in range(min(steps, 10)):
-        A.matvec_device(dq.data_ptr(), dy.data_ptr(), stream)
-    torch.cuda.synchronize()
+        A.matvec_device(dq.ptr, dy.ptr)
+    H.synchronize()
     A.set_profiling(False)
     kern = {}
     for name in ("m2m", "m2m_upper", "m2l", "m2l_upper", "l2l", "p2p", "l2p_out"):
         tot, cnt = A.kernel_time(name)
         kern[name] = tot / cnt if cnt else 0.0
     kern["m2l_timed_region"] = m2l_ms
-    return ms, wall / steps * 1e3, kern, dy.cpu().numpy()
+    y = dy.download(np.empty_like(q))
+    dq.free()
+    dy.free()
+    return ms, wall, kern, y
+
+
+def run_blocked(A, Q, steps, rows, ref_rows, H):
+    """The blocked passes on the same operator: 16 and 32 columns of Q per
+    pass in FP64, FP32 and BF16.  For each: ms per pass, point.RHS/s, the
+    relative error of column 0 on the sampled rows, the M2L launch's time
+    (HIP events on its stream) and roofline."""
+    n = A.N
+    st = A.stats()
+    dB = H.DeviceArray.from_numpy(np.asfortranarray(Q))
+    dY = H.DeviceArray(Q.nbytes)
+    out = {}
+    for prec, fb, vb in (("fp64", 8, 8), ("fp32", 4, 4), ("bf16", 2, 4)):
+        t0 = time.perf_counter()
+        A.prepare_precision(prec)
+        t_prep = time.perf_counter() - t0
+        st = A.stats()
+        for R in (16, 32):
+            fn = lambda: A.matvec_device_ex(dB.ptr, n, dY.ptr, n, R, prec)
+            A.reset_kernel_times()
+            A.set_profiling(True, m2l_only=True)
+            ms, _ = timed(H, fn, steps, 3)
+            A.set_profiling(False)
+            tot, cnt = A.kernel_time("mr_m2l")
+            m2l_ms = tot / cnt if cnt else 0.0
+            A.reset_kernel_times()
+            A.set_profiling(True)
+            for _ in range(3):
+                fn()
+            H.synchronize()
+            A.set_profiling(False)
+            phases = {}
+            for name in ("mr_gather", "mr_p2p", "mr_m2m", "mr_m2l", "mr_l2l", "mr_out"):
+                tot, cnt = A.kernel_time(name)
+                phases[name[3:]] = round(tot / cnt, 4) if cnt else 0.0
+            Y = dY.download(np.empty((n, R), order="F"))
+            err = float(np.linalg.norm(Y[rows, 0] - ref_rows) / np.linalg.norm(ref_rows))
+            rp = 16 if R <= 16 else 32
+            # algorithmic bytes of the M2L launch: each box's M2L row block
+            # once, the multipoles read once, the locals written once
+            alg = st["blocked_m2l_coeffs"] * fb + 2 * st["skeleton_total"] * rp * vb
+            ach = alg / (m2l_ms * 1e-3) / 1e9 if m2l_ms > 0 else 0.0
+            out["%s_r%d" % (prec, R)] = {
+                "value": round(n * R / (ms * 1e-3), 1), "ms_per_pass": round(ms, 4),
+                "rel_err_col0": err, "prepare_s": round(t_prep, 3),
+                "phase_ms": phases,
+                "m2l_roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                                 "algorithmic_bytes": int(alg), "avg_ms": round(m2l_ms, 4)},
+            }
+    dB.free()
+    dY.free()
+    return out
 
 
 def rel_err_rows(A, q, y, nrows, seed):
@@ -162,7 +238,8 @@ def rel_err_rows(A, q, y, nrows, seed):
 
 def cpu_baseline(xy, q, leaf, tol_pow, threads, reps, kernel_id=0):
     """The oracle's OpenMP FP64 matvec (reference pragma placement) on the
-    same workload, timed with its own wall clock around the matvec only."""
+    same workload, timed with its own wall clock around the matvec only:
+    median of `reps` after one warm-up."""
     import oracle
     os.environ["OMP_NUM_THREADS"] = str(threads)
     T = oracle.Tree(xy, leaf, tol_pow, kernel_id)
@@ -197,13 +274,11 @@ PUBLISHED = [
 ]
 
 
-def published_shapes(args, torch, E):
+def published_shapes(args, H, E):
     """Every shape of the published table on the GPU: create, assembly,
     the host-vector matvec (what the reference's driver times, PCIe copies
     included) and the device-resident matvec, with the relative error on
     sampled rows against the direct sum."""
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream().cuda_stream
     leaf = 100
     for n, tp, pub_asm, pub_mv in PUBLISHED:
         print("published shape N=%d TOL_POW=%d" % (n, tp), file=sys.stderr, flush=True)
@@ -221,25 +296,14 @@ def published_shapes(args, torch, E):
             y = A.computeMatVecProduct(q)
             if i:
                 host.append(time.perf_counter() - t0)
-        dq = torch.from_numpy(q).to(dev)
-        dy = torch.empty_like(dq)
-        for _ in range(5):
-            A.matvec_device(dq.data_ptr(), dy.data_ptr(), stream)
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        steps = 20
-        torch.cuda.synchronize()
-        ev0.record()
-        for _ in range(steps):
-            A.matvec_device(dq.data_ptr(), dy.data_ptr(), stream)
-        ev1.record()
-        torch.cuda.synchronize()
-        ms = ev0.elapsed_time(ev1) / steps
+        dq = H.DeviceArray.from_numpy(q)
+        dy = H.DeviceArray(q.nbytes)
+        ms, _ = timed(H, lambda: A.matvec_device(dq.ptr, dy.ptr), 20, 5)
         A.reset_kernel_times()
         A.set_profiling(True)
         for _ in range(5):
-            A.matvec_device(dq.data_ptr(), dy.data_ptr(), stream)
-        torch.cuda.synchronize()
+            A.matvec_device(dq.ptr, dy.ptr)
+        H.synchronize()
         A.set_profiling(False)
         kern = {}
         for name in ("m2m", "m2m_upper", "m2l", "m2l_upper", "l2l", "p2p", "l2p_out"):
@@ -247,7 +311,9 @@ def published_shapes(args, torch, E):
             kern[name] = round(tot / cnt, 4) if cnt else 0.0
         err, _, _ = rel_err_rows(A, q, y, min(args.check_rows, n), args.seed)
         A.close()
-        del A, dq, dy
+        dq.free()
+        dy.free()
+        del A
         t_host = statistics.median(host)
         print(json.dumps({
             "n": n, "leaf": leaf, "tol_pow": tp, "kernel": "reg_inv_r",
@@ -264,7 +330,7 @@ def published_shapes(args, torch, E):
         }), flush=True)
 
 
-def near_field_only(args, torch, E):
+def near_field_only(args, H, E):
     """The near-field kernel alone: each unordered pair of neighbouring (or
     equal) leaves once, every unordered point pair of the two once (the
     diagonal included), timed with the engine's HIP events around the kernel
@@ -326,14 +392,33 @@ def near_field_only(args, torch, E):
 
 
 def cpu_model():
+    """The host CPU as lscpu and /proc/cpuinfo name it (model name, family,
+    model, stepping, sockets, cores per socket)."""
+    info = {}
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
         for line in out.splitlines():
-            if line.startswith("Model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                info[k.strip()] = v.strip()
     except Exception:
         pass
-    return "unknown"
+    name = info.get("Model name", "")
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                full = line.split(":", 1)[1].strip()
+                if len(full) > len(name):
+                    name = full
+                break
+    except Exception:
+        pass
+    parts = [name or "unknown"]
+    for k in ("CPU family", "Model", "Stepping", "Socket(s)", "Core(s) per socket",
+              "Thread(s) per core", "CPU max MHz"):
+        if k in info:
+            parts.append("%s %s" % (k, info[k]))
+    return "; ".join(parts)
 
 
 def main():
@@ -342,24 +427,27 @@ def main():
         print("bench.py: this engine runs on one GPU; a multi-GPU run is not part of this "
               "build (--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    # The CPU baseline's OpenMP pool is set up before torch/HIP start.
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    import torch
+    # The CPU baseline's OpenMP pool: the host cores allotted to this GPU
+    # (OMP_NUM_THREADS where the environment sets it, else every core), set
+    # before the HIP runtime starts its threads.
+    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     import afmm2d_amd as E
+    from afmm2d_amd import hip as H
 
-    if not torch.cuda.is_available():
+    if H.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     if args.published_shapes:
-        published_shapes(args, torch, E)
+        published_shapes(args, H, E)
         return
     if args.near_field:
-        near_field_only(args, torch, E)
+        near_field_only(args, H, E)
         return
     xy, q = inputs(args.n, args.seed)
 
     # headline: smallest TOL_POW meeting the 1e-8 error bar
+    print("bench: headline N=%d tol 1e-%d" % (args.n, args.tol_pow), file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     A = E.AFMM(args.n, args.leaf, args.tol_pow, xy, kernel=args.kernel)
     t_create = time.perf_counter() - t0
@@ -367,18 +455,26 @@ def main():
     A.assemble()
     t_asm = time.perf_counter() - t0
     st = A.stats()
-    calib_copy(args.calib_copy_mb, torch)
-    ms, wall_ms, kern, y = run_gpu(A, q, args.steps, args.warmup, torch)
+    calib_copy(args.calib_copy_mb, H)
+    ms, wall_ms, kern, y = run_gpu(A, q, args.steps, args.warmup, H)
     err, rows, ref_rows = rel_err_rows(A, q, y, args.check_rows, args.seed)
+    blocked = None
+    if args.blocked_steps > 0:
+        print("bench: blocked variants", file=sys.stderr, flush=True)
+        Qb = np.random.default_rng(args.seed + 1).uniform(-1.0, 1.0, size=(args.n, 32))
+        Qb[:, 0] = q
+        blocked = run_blocked(A, Qb, args.blocked_steps, rows, ref_rows, H)
+    near_evals = near_field_evaluations(A)
     A.close()
     del A
 
     # configured tolerance, reported beside the headline
     side = None
     if not args.no_side_run:
+        print("bench: side run tol 1e-%d" % args.config_tol_pow, file=sys.stderr, flush=True)
         Ac = E.AFMM(args.n, args.leaf, args.config_tol_pow, xy, kernel=args.kernel)
         Ac.assemble()
-        ms_c, _, kern_c, y_c = run_gpu(Ac, q, max(20, args.steps // 2), args.warmup, torch)
+        ms_c, _, kern_c, y_c = run_gpu(Ac, q, max(20, args.steps // 2), args.warmup, H)
         err_c = float(np.linalg.norm(y_c[rows] - ref_rows) / np.linalg.norm(ref_rows))
         st_c = Ac.stats()
         Ac.close()
@@ -387,8 +483,11 @@ def main():
                 "rel_err": err_c, "max_rank": st_c["max_rank"],
                 "kernel_avg_ms": {k: round(v, 4) for k, v in kern_c.items()}}
 
-    # roofline of the dominant kernel
-    dom = max((k for k in kern if k != "m2l_timed_region"), key=lambda k: kern[k])
+    # roofline of the dominant kernel of the critical path: the near field
+    # ("p2p") runs on the handle's side stream beside the upward pass, M2L
+    # and downward pass, which form the critical path
+    crit = {k: v for k, v in kern.items() if k not in ("m2l_timed_region", "p2p")}
+    dom = max(crit, key=lambda k: crit[k])
     # M2L pair kernel over the leaf level's pairs (the "m2l" launch; the
     # upper levels' pairs are a separate, smaller launch), algorithmic bytes
     # per launch: every stored symmetric block once, the partial vectors it
@@ -403,9 +502,16 @@ def main():
             "algorithmic_bytes": int(m2l_alg), "avg_ms": round(m2l_ms, 4)}
     if traffic is not None:
         roof["traffic_source"] = traffic_src
+    # the whole matvec against HBM, and the near field against the FP64 VALU
+    # (26.9 FP64 FLOP per log-kernel evaluation, from the SQ_INSTS_VALU_FLOPS
+    # counters of profiles/p2p_near_field_alone_fp64_valu.md)
+    whole = st["matvec_bytes"] / (ms * 1e-3) / 1e9
+    p2p_ms = kern["p2p"]
+    p2p_tf = near_evals * 26.9 / (p2p_ms * 1e-3) / 1e12 if p2p_ms > 0 else 0.0
 
     cpu = None
     if not args.no_cpu:
+        print("bench: CPU baseline, %d threads" % threads, file=sys.stderr, flush=True)
         t_cpu, t_cpu_asm, y_cpu = cpu_baseline(xy, q, args.leaf, args.tol_pow, threads,
                                                args.cpu_reps, E.KERNELS[args.kernel])
         cpu = {"value": round(args.n / t_cpu, 1), "unit": UNIT, "cores": threads,
@@ -415,6 +521,7 @@ def main():
                          "assembly %.1f s untimed" % (args.n, args.tol_pow, args.cpu_reps,
                                                       t_cpu_asm),
                "ms_per_matvec": round(t_cpu * 1e3, 2), "cpu_model": cpu_model(),
+               "host_cpus_visible": os.cpu_count(),
                "gpu_vs_cpu_rel_diff": float(np.linalg.norm(y - y_cpu) / np.linalg.norm(y_cpu))}
 
     value = args.n / (ms * 1e-3)
@@ -441,6 +548,7 @@ def main():
             "wall_ms_per_step": round(wall_ms, 4),
             "kernel_avg_ms": {k: round(v, 4) for k, v in kern.items()},
             "dominant_kernel": dom,
+            "side_stream_kernel": "p2p",
             "m2l_bytes": st["m2l_bytes"], "l2p_bytes": st["l2p_bytes"],
             "near_waves_per_simd": st["near_waves"],
             "matvec_bytes": st["matvec_bytes"],
@@ -448,9 +556,33 @@ def main():
             "at_tol_pow_%d" % args.config_tol_pow: side,
         },
         "roofline": roof,
+        "roofline_matvec": {"bound": "hbm", "achieved": round(whole, 1), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(whole / HBM_PEAK_GBS, 4),
+                            "algorithmic_bytes": st["matvec_bytes"]},
+        "roofline_p2p": {"kernel": "p2p", "bound": "valu_fp64", "achieved": round(p2p_tf, 2),
+                         "peak": 78.6, "unit": "TFLOP/s", "frac": round(p2p_tf / 78.6, 4),
+                         "evaluations": near_evals, "flop_per_evaluation": 26.9,
+                         "avg_ms": round(p2p_ms, 4),
+                         "note": "beside the M2L on the side stream"},
+        "blocked": blocked,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
+
+
+def near_field_evaluations(A):
+    """Kernel evaluations of the symmetric near field: every unordered pair
+    of neighbouring leaves once, a leaf with itself once, the diagonal
+    included."""
+    L = A.nLevels
+    nbr, _, _ = A.interactions(L)
+    cnt = np.array([A.leaf_range(b)[1] for b in range(4 ** L)], dtype=np.float64)
+    evals = float(np.sum(cnt * (cnt + 1) / 2))
+    for c in range(8):
+        b = nbr[:, c]
+        m = b > np.arange(len(cnt))
+        evals += float(np.sum(cnt[m] * cnt[b[m]]))
+    return evals
 
 
 if __name__ == "__main__":
