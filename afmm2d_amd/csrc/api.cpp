@@ -310,8 +310,27 @@ void end_on(afmm2d_handle h, hipStream_t s) {
   h->have_done = true;
 }
 
+// Ranks the single-RHS M2L pair kernels hold (their largest tile class).
+constexpr int kSingleMaxRank = 256;
+
+bool single_ok(afmm2d_handle h) {
+  return h->e->max_rank <= kSingleMaxRank && (!h->e2 || h->e2->max_rank <= kSingleMaxRank);
+}
+
 // y = the operator applied to b, device arrays in caller order, on stream s.
+// Operators with ranks beyond the single-RHS kernels' run the FP64 blocked
+// passes for the one column.
 void run_matvec(afmm2d_handle h, const double* d_b, double* d_y, hipStream_t s) {
+  if (!single_ok(h)) {
+    mr_matvec(*h->e, d_b, h->e->n, d_y, h->e->n, 1, PREC_F64, s);
+    if (h->e2) {
+      Engine& e2 = *h->e2;
+      e2.io_y.alloc(sizeof(double) * e2.n);
+      mr_matvec(e2, d_b, e2.n, e2.io_y.as<double>(), e2.n, 1, PREC_F64, s);
+      add_device(d_y, e2.io_y.as<const double>(), e2.n, s);
+    }
+    return;
+  }
   matvec_device(*h->e, d_b, d_y, s);
   if (h->e2) {
     Engine& e2 = *h->e2;
@@ -589,7 +608,8 @@ int afmm2d_assemble(afmm2d_handle h) {
       e.assembled = true;
       if (!e.p2p_waves_fixed) {
         e.p2p_waves_per_simd = near_waves_per_simd(e);
-        if (e.near_tune) e.p2p_waves_per_simd = tune_near_waves(e, e.p2p_waves_per_simd);
+        if (e.near_tune && e.max_rank <= kSingleMaxRank)
+          e.p2p_waves_per_simd = tune_near_waves(e, e.p2p_waves_per_simd);
       }
       e.assemble_seconds = now_s() - t0;
     }

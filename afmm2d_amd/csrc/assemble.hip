@@ -808,8 +808,8 @@ void assemble_tree(Engine& e) {
   }
   e.h_red_start[nbox] = (int)e.h_red_pair.size();
   e.m2l_cols = part;
-  // the streamed pair kernel holds up to 256 rows per block
-  if (e.max_rank > 256) throw Error(6, "ranks above 256 are not supported by the M2L kernel");
+  // the single-RHS pair kernels hold up to 256 rows per block; operators
+  // with larger ranks run their matvecs through the blocked passes
   upload(e.slot_off, slot_off, s);
   upload(e.pair_k, e.h_pair_k, s);
   upload(e.pair_b, e.h_pair_b, s);

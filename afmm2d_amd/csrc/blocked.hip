@@ -614,7 +614,7 @@ __global__ __launch_bounds__(256) void k_mr_build_v64(const MrBoxV* __restrict__
                                                       const double* __restrict__ l2p_rm,
                                                       const int* __restrict__ rowmap,
                                                       double* __restrict__ V) {
-  __shared__ int inv_row[1024];
+  __shared__ int inv_row[4096];  // rows: leaf points (<= 512) or 4 children's ranks (<= 4 x 1024)
   const MrBoxV d = boxes[blockIdx.x];
   const int n = d.n, r = d.r, nc = n - r;
   for (int t = threadIdx.x; t < n; t += blockDim.x) inv_row[rowmap[d.rmbase + t]] = t;
@@ -663,7 +663,8 @@ struct MrBoxM {
 };
 
 // FP64 M2L row block: M[i][c] = K(skeleton_k[i], point colidx[c]), the same
-// formula as the single-RHS pair blocks.
+// formula as the single-RHS pair blocks, written in the FP64 tiled layout
+// (columns up to K rounded to 4, the padding zero).
 template <int KID>
 __global__ __launch_bounds__(256) void k_mr_build_m64(const MrBoxM* __restrict__ boxes,
                                                       const double* __restrict__ skx,
@@ -671,15 +672,16 @@ __global__ __launch_bounds__(256) void k_mr_build_m64(const MrBoxM* __restrict__
                                                       const int* __restrict__ colidx,
                                                       double* __restrict__ Mout, KParams kp) {
   const MrBoxM d = boxes[blockIdx.x];
-  const int64_t tot = (int64_t)d.r * d.mld;
+  const int kp4 = (d.K + 3) / 4 * 4;
+  const int64_t tot = (int64_t)d.r * kp4;
   for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
-    const int i = (int)(e / d.mld), c = (int)(e - (int64_t)i * d.mld);
+    const int i = (int)(e / kp4), c = (int)(e - (int64_t)i * kp4);
     double v = 0.0;
     if (c < d.K) {
       const int q = colidx[d.coloff + c];
       v = kernel_xy<KID>(skx[d.soff + i], sky[d.soff + i], skx[q], sky[q], kp);
     }
-    Mout[d.moff + e] = v;
+    Mout[d.moff + tile_off<PREC_F64>(i, c, d.r, kp4)] = v;
   }
 }
 
@@ -693,13 +695,15 @@ struct MrBoxQR {
 // Orthonormal basis of one box: E = V (leaves) or diag(R_children) V, then
 // E = Q R by classical Gram-Schmidt with one re-orthogonalisation (FP64).
 // Q overwrites Qout (n x r, leading dimension vld), R goes to Rout (r x r).
+// Dynamic LDS: vs[nmax], hs[4][rmax], h[rmax] doubles.
 __global__ __launch_bounds__(256) void k_mr_qr(const MrBoxQR* __restrict__ boxes,
                                                const double* __restrict__ V,
                                                double* __restrict__ Qout,
-                                               double* __restrict__ Rbuf) {
-  __shared__ double vs[1024];
-  __shared__ double hs[4][256];
-  __shared__ double h[256];
+                                               double* __restrict__ Rbuf, int nmax, int rmax) {
+  extern __shared__ double qr_lds[];
+  double* vs = qr_lds;
+  double* hs0 = qr_lds + nmax;
+  double* h = hs0 + 4 * rmax;
   __shared__ double red[4];
   const MrBoxQR d = boxes[blockIdx.x];
   const int n = d.n, r = d.r, ld = d.vld;
@@ -735,11 +739,11 @@ __global__ __launch_bounds__(256) void k_mr_qr(const MrBoxQR* __restrict__ boxes
       for (int p = lane; p < c; p += 64) {
         double s = 0.0;
         for (int row = w; row < n; row += 4) s = fma(Q[(int64_t)row * ld + p], vs[row], s);
-        hs[w][p] = s;
+        hs0[w * rmax + p] = s;
       }
       __syncthreads();
       for (int p = tid; p < c; p += 256) {
-        const double hp = (hs[0][p] + hs[1][p]) + (hs[2][p] + hs[3][p]);
+        const double hp = (hs0[p] + hs0[rmax + p]) + (hs0[2 * rmax + p] + hs0[3 * rmax + p]);
         h[p] = hp;
         R[(int64_t)p * r + c] += hp;
       }
@@ -787,11 +791,12 @@ __global__ __launch_bounds__(256) void k_mr_mprime(const MrBoxMP* __restrict__ b
   using AT = typename MrT<P>::A;
   __shared__ double T[4096];
   const MrBoxMP& d = boxes[blockIdx.x];
-  const int r = d.r, ld = d.mld;
+  const int r = d.r;
   const int kg = P == PREC_BF16 ? 8 : 4;
   const int kp = (d.K + kg - 1) / kg * kg;
   const double* Rk = Rbuf + d.roff;
-  const double* Mk = M64 + d.moff;
+  const double* Mk = M64 + d.moff;  // FP64 tiled layout (columns to K rounded to 4)
+  const int kp4 = (d.K + 3) / 4 * 4;
   AT* Ok = Mout + d.moff;
   auto put = [&](int i, int c, double v) {
     AT x;
@@ -816,7 +821,8 @@ __global__ __launch_bounds__(256) void k_mr_mprime(const MrBoxMP* __restrict__ b
       for (int e = threadIdx.x; e < ni * rb; e += blockDim.x) {
         const int ii = e / rb, wv = e - ii * rb, ix = i0 + ii;
         double s = 0.0;
-        for (int p = ix; p < r; p++) s = fma(Rk[(int64_t)ix * r + p], Mk[(int64_t)p * ld + off + wv], s);
+        for (int p = ix; p < r; p++)
+          s = fma(Rk[(int64_t)ix * r + p], Mk[tile_off<PREC_F64>(p, off + wv, r, kp4)], s);
         T[ii * rb + wv] = s;
       }
       __syncthreads();
@@ -910,17 +916,15 @@ void mr_layout(Engine& e) {
   e.mr_layout = true;
 }
 
-// Row-major FP64 sources of the factors: V (nin x rank, leading dimension
-// vld) and M (rank x K, leading dimension mld) of every box.
-void build_rm64(Engine& e, DevBuf& Vrm, DevBuf& Mrm) {
+// The row-major FP64 basis V (nin x rank, leading dimension vld) of every
+// box, the source of the tiled bases.
+void build_vrm64(Engine& e, DevBuf& Vrm) {
   mr_layout(e);
   const int L = e.L;
   hipStream_t s = e.s_aux;
   Vrm.alloc(sizeof(double) * std::max<int64_t>(e.mr_v_total, 8));
-  Mrm.alloc(sizeof(double) * std::max<int64_t>(e.mr_m_total, 8));
   AFMM_HIP(hipMemsetAsync(Vrm.p, 0, Vrm.bytes, s));
   std::vector<MrBoxV> bv;
-  std::vector<MrBoxM> bm;
   for (int j = 2; j <= L; j++) {
     const LevelInfo& li = e.lv[j];
     for (int k = 0; k < li.nb; k++) {
@@ -929,23 +933,13 @@ void build_rm64(Engine& e, DevBuf& Vrm, DevBuf& Mrm) {
       if (r > 0 && n > 0)
         bv.push_back(MrBoxV{li.l2poff[k], level_rbase(e, j, k), e.mr_voff[g], e.mr_vtoff[g], n, r,
                             e.mr_vld[g], e.mr_vtld[g]});
-      if (r > 0)
-        bm.push_back(MrBoxM{li.soff[k], e.mr_moff[g], e.mr_coloff[g], r, e.mr_K[g], e.mr_mld[g], 0});
     }
   }
-  DevBuf dbv, dbm;
+  DevBuf dbv;
   upload_vec(dbv, bv, s);
-  upload_vec(dbm, bm, s);
   if (!bv.empty()) {
     k_mr_build_v64<<<(unsigned)bv.size(), 256, 0, s>>>(dbv.as<MrBoxV>(), e.l2p_rm.as<double>(),
                                                        e.rowmap.as<int>(), Vrm.as<double>());
-    AFMM_LAUNCH_CHECK();
-  }
-  if (!bm.empty()) {
-    AFMM_DISPATCH_KERNEL(e.kid, KID,
-                         k_mr_build_m64<KID><<<(unsigned)bm.size(), 256, 0, s>>>(
-                             dbm.as<MrBoxM>(), e.skx.as<double>(), e.sky.as<double>(),
-                             e.mr_colidx.as<int>(), Mrm.as<double>(), e.kp));
     AFMM_LAUNCH_CHECK();
   }
   AFMM_HIP(hipStreamSynchronize(s));
@@ -1003,26 +997,31 @@ void alloc_factors(Engine& e, int prec, MrFactors& f) {
 void build_f64(Engine& e) {
   MrFactors& f = e.mrf[PREC_F64];
   if (f.built) return;
-  DevBuf Vrm, Mrm;
-  build_rm64(e, Vrm, Mrm);
-  alloc_factors(e, PREC_F64, f);
-  retile_basis(e, PREC_F64, Vrm, f);
+  {
+    DevBuf Vrm;
+    build_vrm64(e, Vrm);
+    alloc_factors(e, PREC_F64, f);
+    retile_basis(e, PREC_F64, Vrm, f);
+  }
   const int L = e.L;
   hipStream_t s = e.s_aux;
-  std::vector<MrBoxRT> bm;
+  std::vector<MrBoxM> bm;
   for (int j = 2; j <= L; j++) {
     const LevelInfo& li = e.lv[j];
     for (int k = 0; k < li.nb; k++) {
       const int64_t g = li.gbase + k;
       if (li.rank[k] > 0)
-        bm.push_back(MrBoxRT{e.mr_moff[g], e.mr_moff[g], li.rank[k], e.mr_K[g], e.mr_mld[g], 0});
+        bm.push_back(MrBoxM{li.soff[k], e.mr_moff[g], e.mr_coloff[g], li.rank[k], e.mr_K[g],
+                            e.mr_mld[g], 0});
     }
   }
   if (!bm.empty()) {
     DevBuf dbm;
     upload_vec(dbm, bm, s);
-    k_mr_retile<PREC_F64><<<(unsigned)bm.size(), 256, 0, s>>>(dbm.as<MrBoxRT>(), Mrm.as<double>(),
-                                                             f.M.as<double>());
+    AFMM_DISPATCH_KERNEL(e.kid, KID,
+                         k_mr_build_m64<KID><<<(unsigned)bm.size(), 256, 0, s>>>(
+                             dbm.as<MrBoxM>(), e.skx.as<double>(), e.sky.as<double>(),
+                             e.mr_colidx.as<int>(), f.M.as<double>(), e.kp));
     AFMM_LAUNCH_CHECK();
   }
   AFMM_HIP(hipStreamSynchronize(s));
@@ -1033,8 +1032,9 @@ void build_f64(Engine& e) {
 void build_reduced(Engine& e, int prec) {
   MrFactors& f = e.mrf[prec];
   if (f.built) return;
-  DevBuf Vrm, Mrm;
-  build_rm64(e, Vrm, Mrm);
+  build_f64(e);  // the FP64 M2L row blocks are the transform's source
+  DevBuf Vrm;
+  build_vrm64(e, Vrm);
   const int L = e.L;
   hipStream_t s = e.s_aux;
   const int64_t nbox = e.lv[L].gbase + e.lv[L].nb;
@@ -1075,10 +1075,20 @@ void build_reduced(Engine& e, int prec) {
       bq.push_back(d);
     }
     if (bq.empty()) continue;
+    int nmax = 1, rmax = 1;
+    for (const MrBoxQR& d : bq) {
+      nmax = std::max(nmax, d.n);
+      rmax = std::max(rmax, d.r);
+    }
+    const size_t shm = sizeof(double) * ((size_t)nmax + 5 * (size_t)rmax);
+    if (shm > (size_t)e.max_lds)
+      throw Error(6, "a box too large for the orthonormal basis (" + std::to_string(nmax) +
+                         " rows, rank " + std::to_string(rmax) + ")");
     DevBuf dbq;
     upload_vec(dbq, bq, s);
-    k_mr_qr<<<(unsigned)bq.size(), 256, 0, s>>>(dbq.as<MrBoxQR>(), Vrm.as<double>(),
-                                                Q64.as<double>(), e.mr_R.as<double>());
+    k_mr_qr<<<(unsigned)bq.size(), 256, shm, s>>>(dbq.as<MrBoxQR>(), Vrm.as<double>(),
+                                                  Q64.as<double>(), e.mr_R.as<double>(), nmax,
+                                                  rmax);
     AFMM_LAUNCH_CHECK();
     AFMM_HIP(hipStreamSynchronize(s));
   }
@@ -1120,10 +1130,10 @@ void build_reduced(Engine& e, int prec) {
     upload_vec(dbm, bm, s);
     const unsigned nb = (unsigned)bm.size();
     if (prec == PREC_F32)
-      k_mr_mprime<PREC_F32><<<nb, 256, 0, s>>>(dbm.as<MrBoxMP>(), Mrm.as<double>(),
+      k_mr_mprime<PREC_F32><<<nb, 256, 0, s>>>(dbm.as<MrBoxMP>(), e.mrf[PREC_F64].M.as<double>(),
                                               e.mr_R.as<double>(), f.M.as<float>());
     else
-      k_mr_mprime<PREC_BF16><<<nb, 256, 0, s>>>(dbm.as<MrBoxMP>(), Mrm.as<double>(),
+      k_mr_mprime<PREC_BF16><<<nb, 256, 0, s>>>(dbm.as<MrBoxMP>(), e.mrf[PREC_F64].M.as<double>(),
                                                e.mr_R.as<double>(), f.M.as<uint16_t>());
     AFMM_LAUNCH_CHECK();
   }

@@ -70,11 +70,57 @@ def parse():
                    help="instead of the headline, run the reference's published benchmark "
                         "shapes (leaf 100, a/R kernel, tolerance and size sweeps) and print "
                         "one JSON line per shape")
+    p.add_argument("--config4", action="store_true",
+                   help="instead of the headline, the precision workload: N = 2^22 points "
+                        "from 16 clipped Gaussians (sigma 0.05), 1/r kernel, leaf 64, ACA tol "
+                        "1e-8, 32 right-hand sides in FP64, FP32 and BF16 (one JSON line)")
     p.add_argument("--near-field", action="store_true",
                    help="instead of the headline, time the near-field kernel alone "
                         "(afmm2d_near_field, no assembly) on the same points and print its "
                         "kernel evaluations per second against the FP64 VALU peak")
     return p.parse_args()
+
+
+def mixture_inputs(n, seed, clusters=16, sigma=0.05):
+    """Points from a mixture of isotropic Gaussians (centres uniform in
+    [-1,1]^2, standard deviation sigma, clipped to the box); U(-1,1) charges."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-1.0, 1.0, size=(clusters, 2))
+    lab = rng.integers(0, clusters, size=n)
+    xy = np.clip(centres[lab] + sigma * rng.standard_normal((n, 2)), -1.0, 1.0)
+    return xy, rng.uniform(-1.0, 1.0, size=n)
+
+
+def config4(args, H, E):
+    """The precision workload at full size, one JSON line: the blocked passes
+    with 32 right-hand sides in each precision, their M2L rooflines and their
+    errors against the FP64 direct sum on sampled rows of the first column."""
+    n, R = 1 << 22, 32
+    xy, q = mixture_inputs(n, args.seed)
+    print("bench: config 4 N=%d" % n, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    A = E.AFMM(n, 64, 8, xy, kernel="inv_r")
+    A.assemble()
+    t_asm = time.perf_counter() - t0
+    st = A.stats()
+    Q = np.random.default_rng(args.seed + 1).uniform(-1.0, 1.0, size=(n, R))
+    Q[:, 0] = q
+    rows = np.sort(np.random.default_rng(args.seed + 7).choice(n, min(args.check_rows, n),
+                                                                 replace=False))
+    ref = A.direct_rows(rows, q)
+    res = run_blocked(A, Q, max(3, args.blocked_steps // 2), rows, ref, H, cols=(R,))
+    A.close()
+    best = max(res.values(), key=lambda v: v["value"])
+    print(json.dumps({
+        "metric": "FMM matvec point\u00b7RHS/s, precision workload",
+        "value": best["value"], "unit": UNIT, "n_gpus": 1, "dtype": "mixed",
+        "data": "synthetic",
+        "config": {"workload": "N=%d, 16 clipped Gaussians (sigma 0.05), 1/r kernel, leaf 64 "
+                               "(L=%d), ACA tol 1e-8, %d RHS" % (n, st["levels"], R),
+                   "max_rank": st["max_rank"], "assemble_s": round(t_asm, 3),
+                   "m2l_bytes_fp64_once": st["m2l_bytes"], "rel_err_rows": int(len(rows))},
+        "by_precision": res,
+    }))
 
 
 def inputs(n, seed):
@@ -176,7 +222,7 @@ def run_gpu(A, q, steps, warmup, H):
     return ms, wall, kern, y
 
 
-def run_blocked(A, Q, steps, rows, ref_rows, H):
+def run_blocked(A, Q, steps, rows, ref_rows, H, cols=(16, 32)):
     """The blocked passes on the same operator: 16 and 32 columns of Q per
     pass in FP64, FP32 and BF16.  For each: ms per pass, point.RHS/s, the
     relative error of column 0 on the sampled rows, the M2L launch's time
@@ -191,7 +237,7 @@ def run_blocked(A, Q, steps, rows, ref_rows, H):
         A.prepare_precision(prec)
         t_prep = time.perf_counter() - t0
         st = A.stats()
-        for R in (16, 32):
+        for R in cols:
             fn = lambda: A.matvec_device_ex(dB.ptr, n, dY.ptr, n, R, prec)
             A.reset_kernel_times()
             A.set_profiling(True, m2l_only=True)
@@ -443,6 +489,9 @@ def main():
         return
     if args.near_field:
         near_field_only(args, H, E)
+        return
+    if args.config4:
+        config4(args, H, E)
         return
     xy, q = inputs(args.n, args.seed)
 

@@ -30,3 +30,15 @@ def uniform_points(n, seed):
     import numpy as np
     rng = np.random.default_rng(seed)
     return rng.uniform(-1.0, 1.0, size=(n, 2)), rng.uniform(-1.0, 1.0, size=n)
+
+
+def mixture_points(n, seed, clusters=16, sigma=0.05):
+    """Points from a mixture of isotropic Gaussians: centres uniform in
+    [-1,1]^2, standard deviation sigma, clipped to the box; and U(-1,1)
+    charges."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-1.0, 1.0, size=(clusters, 2))
+    lab = rng.integers(0, clusters, size=n)
+    xy = np.clip(centres[lab] + sigma * rng.standard_normal((n, 2)), -1.0, 1.0)
+    return xy, rng.uniform(-1.0, 1.0, size=n)
