@@ -46,6 +46,8 @@ EXPORTS = [
     "afmm2d_set_profiling", "afmm2d_get_kernel_time", "afmm2d_reset_kernel_times",
     "afmm2d_matvec_ex", "afmm2d_matvec_device_ex", "afmm2d_prepare_precision",
     "afmm2d_direct_rows_device_noalloc",
+    "afmm2d_batch_assemble", "afmm2d_batch_create", "afmm2d_batch_matvec", "afmm2d_batch_tune",
+    "afmm2d_batch_capture", "afmm2d_batch_get_info", "afmm2d_batch_destroy",
 ]
 
 # afmm2d_precision and the blocked-mode flag (include/afmm2d.h)
@@ -82,6 +84,15 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _lp = ctypes.POINTER(ctypes.c_int64)
 _vp = ctypes.c_void_p
+
+
+class BatchInfo(ctypes.Structure):
+    """afmm2d_batch_info (include/afmm2d.h)."""
+    _fields_ = [(n, ctypes.c_int) for n in ("launches", "depth", "xcd", "near_waves", "rows_m2l",
+                                             "captured", "tuned", "nproblems")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 def library_path():
@@ -131,6 +142,14 @@ def lib():
         "afmm2d_matvec_device_ex": (i, [_vp, _vp, ctypes.c_int64, _vp, ctypes.c_int64, i, i, _vp]),
         "afmm2d_prepare_precision": (i, [_vp, i]),
         "afmm2d_direct_rows_device_noalloc": (i, [_vp, i, _vp, _vp, _vp, _vp]),
+        "afmm2d_batch_assemble": (i, [ctypes.POINTER(_vp), i]),
+        "afmm2d_batch_create": (i, [ctypes.POINTER(_vp), i, ctypes.POINTER(_vp), _lp,
+                                    ctypes.POINTER(_vp), _lp, i, i, ctypes.POINTER(_vp)]),
+        "afmm2d_batch_matvec": (i, [_vp, _vp]),
+        "afmm2d_batch_tune": (i, [_vp, _vp, _dp]),
+        "afmm2d_batch_capture": (i, [_vp, i]),
+        "afmm2d_batch_get_info": (i, [_vp, ctypes.POINTER(BatchInfo)]),
+        "afmm2d_batch_destroy": (i, [_vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -400,3 +419,62 @@ class AFMM:
 
     def reset_kernel_times(self):
         _check(lib().afmm2d_reset_kernel_times(self._h))
+
+
+def batch_assemble(problems):
+    """Assembles every AFMM of the list (afmm2d_batch_assemble)."""
+    hs = (_vp * len(problems))(*[p._h for p in problems])
+    _check(lib().afmm2d_batch_assemble(hs, len(problems)))
+
+
+class Batch:
+    """Several problems' blocked matvecs as one (afmm2d_batch_*).
+
+    problems: assembled AFMM objects; d_b, d_y: device pointers (ints) of
+    each problem's charges and outputs, column-major with leading dimensions
+    ldb, ldy (default N of each problem); nrhs columns (<= 32)."""
+
+    def __init__(self, problems, d_b, d_y, nrhs, precision="fp64", ldb=None, ldy=None):
+        n = len(problems)
+        self._problems = list(problems)  # the handles outlive the batch
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        ldb = [p.N for p in problems] if ldb is None else list(ldb)
+        ldy = [p.N for p in problems] if ldy is None else list(ldy)
+        hs = (_vp * n)(*[p._h for p in problems])
+        bs = (_vp * n)(*d_b)
+        ys = (_vp * n)(*d_y)
+        lb = np.asarray(ldb, dtype=np.int64)
+        ly = np.asarray(ldy, dtype=np.int64)
+        h = _vp()
+        _check(lib().afmm2d_batch_create(hs, n, bs, lb.ctypes.data_as(_lp), ys,
+                                         ly.ctypes.data_as(_lp), int(nrhs), int(precision),
+                                         ctypes.byref(h)))
+        self._b = h
+
+    def close(self):
+        if getattr(self, "_b", None) and self._b.value:
+            lib().afmm2d_batch_destroy(self._b)
+            self._b = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def matvec(self, stream=None):
+        _check(lib().afmm2d_batch_matvec(self._b, _vp(stream or 0)))
+
+    def tune(self, stream=None):
+        ms = ctypes.c_double()
+        _check(lib().afmm2d_batch_tune(self._b, _vp(stream or 0), ctypes.byref(ms)))
+        return ms.value
+
+    def capture(self, on=True):
+        _check(lib().afmm2d_batch_capture(self._b, 1 if on else 0))
+
+    def info(self):
+        inf = BatchInfo()
+        _check(lib().afmm2d_batch_get_info(self._b, ctypes.byref(inf)))
+        return inf.as_dict()

@@ -720,6 +720,89 @@ int afmm2d_prepare_precision(afmm2d_handle h, int precision) {
   });
 }
 
+struct afmm2d_batch_s {
+  MrBatch* b = nullptr;
+  ~afmm2d_batch_s() { mr_batch_destroy(b); }
+};
+
+int afmm2d_batch_assemble(const afmm2d_handle* handles, int count) {
+  if (!handles || count < 0) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "bad argument");
+  for (int i = 0; i < count; i++) {
+    if (!handles[i]) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL handle in batch");
+    const int st = afmm2d_assemble(handles[i]);
+    if (st != AFMM2D_OK) return st;
+  }
+  return AFMM2D_OK;
+}
+
+int afmm2d_batch_create(const afmm2d_handle* handles, int count, const double* const* d_b,
+                        const int64_t* ldb, double* const* d_y, const int64_t* ldy, int nrhs,
+                        int precision, afmm2d_batch* out) {
+  if (!out) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL output");
+  *out = nullptr;
+  if (!handles || count <= 0 || !d_b || !ldb || !d_y || !ldy || nrhs < 1 || nrhs > kMrMaxCols)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "bad argument");
+  if (precision < 0 || precision >= kPrecisions)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown precision");
+  for (int i = 0; i < count; i++) {
+    afmm2d_handle h = handles[i];
+    if (!h || !d_b[i] || !d_y[i]) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL in batch");
+    if (!h->assembled) return fail(AFMM2D_ERR_STATE, "batch of an unassembled handle");
+    if (h->e2) return fail(AFMM2D_ERR_UNSUPPORTED, "batch of a non-symmetric user kernel");
+    if (ldb[i] < h->e->n || ldy[i] < h->e->n)
+      return fail(AFMM2D_ERR_INVALID_ARGUMENT, "leading dimension below N");
+  }
+  return guarded([&] {
+    std::vector<Engine*> eng;
+    for (int i = 0; i < count; i++) {
+      afmm2d_handle h = handles[i];
+      AFMM_HIP(hipSetDevice(h->e->device));
+      if (h->have_done) AFMM_HIP(hipEventSynchronize(h->done));
+      eng.push_back(h->e.get());
+    }
+    auto bb = std::make_unique<afmm2d_batch_s>();
+    bb->b = mr_batch_create(eng, d_b, ldb, d_y, ldy, nrhs, precision);
+    *out = bb.release();
+  });
+}
+
+int afmm2d_batch_matvec(afmm2d_batch b, void* stream) {
+  if (!b) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL batch");
+  return guarded([&] { mr_batch_run(b->b, stream ? (hipStream_t)stream : (hipStream_t)0); });
+}
+
+int afmm2d_batch_tune(afmm2d_batch b, void* stream, double* ms_per_run) {
+  if (!b) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL batch");
+  return guarded([&] {
+    const double ms = mr_batch_tune(b->b, stream ? (hipStream_t)stream : (hipStream_t)0);
+    if (ms_per_run) *ms_per_run = ms;
+  });
+}
+
+int afmm2d_batch_capture(afmm2d_batch b, int on) {
+  if (!b) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL batch");
+  return guarded([&] { mr_batch_graph(b->b, on != 0); });
+}
+
+int afmm2d_batch_get_info(afmm2d_batch b, afmm2d_batch_info* info) {
+  if (!b || !info) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  const MrBatchInfo in = mr_batch_info(b->b);
+  info->launches = in.launches;
+  info->depth = in.depth;
+  info->xcd = in.xcd;
+  info->near_waves = in.near_waves;
+  info->rows_m2l = in.rows_m2l;
+  info->captured = in.captured;
+  info->tuned = in.tuned;
+  info->nproblems = in.nproblems;
+  return AFMM2D_OK;
+}
+
+int afmm2d_batch_destroy(afmm2d_batch b) {
+  if (!b) return AFMM2D_OK;
+  return guarded([&] { delete b; });
+}
+
 int afmm2d_levels(afmm2d_handle h, int* levels) {
   if (!h || !levels) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
   *levels = h->e->L;

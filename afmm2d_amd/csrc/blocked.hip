@@ -51,6 +51,13 @@
 // near field's per-source partials with compensated (two-sum) summation,
 // and form the output y = pot + near in FP64.
 #include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
 #include <cstdlib>
 #include <cstring>
 
@@ -564,13 +571,13 @@ __global__ void k_mr_zero(int* __restrict__ v, int n) {
 // ---------------------------------------------------------------------------
 // Gather: qs[inv_perm[i]][c] = b[i + c ldb] (0 for c >= nrhs).  Lane c of a
 // row of RP threads, RP-column rows written whole, b read along columns.
-// Workgroup w takes caller points [w kMrIoPoints, (w + 1) kMrIoPoints) of
-// d.cnt.
+// Workgroup blk of a problem takes its caller points
+// [blk kMrIoPoints, (blk + 1) kMrIoPoints) of d.cnt.
 template <class VT, int NCT>
-__global__ __launch_bounds__(256) void k_mr_gather(MrIoTask d) {
+__device__ __forceinline__ void mr_gather_block(const MrIoTask& d, int blk) {
   constexpr int RP = 16 * NCT, PPI = 256 / RP;
   const int c = threadIdx.x % RP, pl = threadIdx.x / RP;
-  const int i0 = blockIdx.x * kMrIoPoints, cnt = min(kMrIoPoints, d.cnt - i0);
+  const int i0 = blk * kMrIoPoints, cnt = min(kMrIoPoints, d.cnt - i0);
   VT* qs = static_cast<VT*>(d.qs);
   for (int p = pl; p < cnt; p += PPI) {
     const int64_t i = i0 + p;
@@ -581,16 +588,44 @@ __global__ __launch_bounds__(256) void k_mr_gather(MrIoTask d) {
 
 // Output: y[i + c ldy] = pot[inv_perm[i]][c] + near[inv_perm[i]][c] in FP64.
 template <class VT, int NCT>
-__global__ __launch_bounds__(256) void k_mr_out(MrIoTask d) {
+__device__ __forceinline__ void mr_out_block(const MrIoTask& d, int blk) {
   constexpr int RP = 16 * NCT, PPI = 256 / RP;
   const int c = threadIdx.x % RP, pl = threadIdx.x / RP;
-  const int i0 = blockIdx.x * kMrIoPoints, cnt = min(kMrIoPoints, d.cnt - i0);
+  const int i0 = blk * kMrIoPoints, cnt = min(kMrIoPoints, d.cnt - i0);
   const VT* pot = static_cast<const VT*>(d.pot);
   for (int p = pl; p < cnt; p += PPI) {
     const int64_t i = i0 + p;
     const int64_t si = d.inv_perm[i];
     if (c < d.nrhs) d.y[i + c * d.ldy] = (double)pot[si * RP + c] + d.nearv[si * RP + c];
   }
+}
+
+template <class VT, int NCT>
+__global__ __launch_bounds__(256) void k_mr_gather(MrIoTask d) {
+  mr_gather_block<VT, NCT>(d, blockIdx.x);
+}
+
+template <class VT, int NCT>
+__global__ __launch_bounds__(256) void k_mr_out(MrIoTask d) {
+  mr_out_block<VT, NCT>(d, blockIdx.x);
+}
+
+// Several problems in one launch: problem t owns workgroups
+// [tasks[t].i0, tasks[t + 1].i0); workgroup w finds its problem by binary
+// search (the last t with tasks[t].i0 <= w).
+template <class VT, int NCT, bool GATHER>
+__global__ __launch_bounds__(256) void k_mr_io_multi(const MrIoTask* __restrict__ tasks,
+                                                     int ntasks) {
+  const int w = blockIdx.x;
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].i0 <= w) lo = mid;
+    else hi = mid - 1;
+  }
+  const MrIoTask d = tasks[lo];
+  if constexpr (GATHER) mr_gather_block<VT, NCT>(d, w - d.i0);
+  else mr_out_block<VT, NCT>(d, w - d.i0);
 }
 
 __global__ void k_add_cols(double* __restrict__ y, int64_t ldy, const double* __restrict__ t,
@@ -1167,6 +1202,19 @@ const MrKnobs& knobs() {
   return k;
 }
 
+}  // namespace
+
+MrTune mr_default_tune() {
+  MrTune t;
+  t.depth = knobs().depth;
+  t.xcd = knobs().xcd;
+  t.near_waves = knobs().near_waves;
+  t.rows_m2l = knobs().rows_m2l;
+  return t;
+}
+
+namespace {
+
 void add_row_groups(std::vector<MrTask>& out, const void* A, int kg, const void* B,
                     const int* bidx, int64_t b0, void* C, int64_t c0, int M, int K, int beta,
                     int rows = 0) {
@@ -1216,6 +1264,25 @@ void upload_tasks(MrTaskList& tl, std::vector<T>& tasks, hipStream_t s) {
   upload_vec(tl.buf, tasks, s);
 }
 
+// M2L tasks of a plan, rows rows per task: in_k = M_k [out_b ...] for every
+// box of rank > 0, levels 2..L in order.
+void m2l_tasks(const Engine& e, const MrPlan& p, int rows, std::vector<MrTask>& out) {
+  const MrFactors& f = e.mrf[p.prec];
+  const size_t fb = factor_bytes(p.prec);
+  const int kg = col_group(p.prec);
+  for (int j = 2; j <= e.L; j++) {
+    const LevelInfo& li = e.lv[j];
+    for (int k = 0; k < li.nb; k++) {
+      const int64_t g = li.gbase + k;
+      const int r = li.rank[k];
+      if (r <= 0) continue;
+      add_row_groups(out, static_cast<const char*>(f.M.p) + e.mr_moff[g] * fb, kg, p.outv.p,
+                     e.mr_colidx.as<int>(), e.mr_coloff[g], p.inv.p, li.soff[k], r, e.mr_K[g], 0,
+                     rows);
+    }
+  }
+}
+
 MrPlan& get_plan(Engine& e, int prec, int nct) {
   for (auto& p : e.mr_plans)
     if (p->prec == prec && p->nct == nct) return *p;
@@ -1245,6 +1312,8 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
   auto vrow = [&](const DevBuf& b) { return b.p; };
   p.up.resize(L + 1);
   p.down.resize(L + 1);
+  p.h_up.assign(L + 1, {});
+  p.h_down.assign(L + 1, {});
   std::vector<MrTask> m2l, leaf;
   for (int j = 2; j <= L; j++) {
     const LevelInfo& li = e.lv[j];
@@ -1262,9 +1331,6 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
         else
           add_row_groups(up, fptr(f.Vt, e.mr_vtoff[g]), kg, Bsrc, nullptr, 0, p.outv.p,
                          li.soff[k], r, 0, 0);
-        // M2L: in_k = M_k [out_b ...]
-        add_row_groups(m2l, fptr(f.M, e.mr_moff[g]), kg, vrow(p.outv), e.mr_colidx.as<int>(),
-                       e.mr_coloff[g], p.inv.p, li.soff[k], r, e.mr_K[g], 0, knobs().rows_m2l);
       }
       if (j < L) {
         // L2L: children's in += V in_k
@@ -1279,9 +1345,14 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
     }
     upload_tasks(p.up[j], up, s);
     upload_tasks(p.down[j], down, s);
+    p.h_up[j] = std::move(up);
+    p.h_down[j] = std::move(down);
   }
+  m2l_tasks(e, p, mr_default_tune().rows_m2l, m2l);
   upload_tasks(p.m2l, m2l, s);
   upload_tasks(p.leaf, leaf, s);
+  p.h_m2l = std::move(m2l);
+  p.h_leaf = std::move(leaf);
   // near field: every leaf's points against its neighbours N0..N7 and itself
   std::vector<MrNearTask> near;
   const int nleaf = 1 << (2 * L);
@@ -1303,6 +1374,7 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
     }
   }
   upload_tasks(p.near, near, s);
+  p.h_near = std::move(near);
   p.near_ctr.alloc(sizeof(int) * 4 * kMrCtrStride);
   MrNearProb pr{e.x.as<double>(), e.y.as<double>(), p.qs.p, p.nearv.as<double>(), e.kp};
   upload_vec(p.probs, std::vector<MrNearProb>{pr}, s);
@@ -1346,8 +1418,7 @@ void launch_gemm(const MrTaskList& tl, int xcd, int depth, hipStream_t s) {
   }
 }
 
-void launch_gemm_p(int prec, int nct, const MrTaskList& tl, int xcd, hipStream_t s) {
-  const int depth = knobs().depth;
+void launch_gemm_p(int prec, int nct, const MrTaskList& tl, int xcd, int depth, hipStream_t s) {
   if (nct == 1) {
     switch (prec) {
       case PREC_F64: launch_gemm<PREC_F64, 1>(tl, xcd, depth, s); break;
@@ -1363,10 +1434,21 @@ void launch_gemm_p(int prec, int nct, const MrTaskList& tl, int xcd, hipStream_t
   }
 }
 
+// The near field of a task list (one kernel id): one persistent launch per
+// row-tile class, ctr holding 4 counters kMrCtrStride apart.
+struct NearLaunch {
+  const MrTaskList* tl;
+  const MrNearProb* probs;
+  const double2* logtab;
+  double ln2_hi, ln2_lo;
+  int* ctr;
+  int kid, num_cus, near_waves;
+};
+
 template <int P, int NCT>
-void launch_near_t(Engine& e, const MrPlan& p, hipStream_t s) {
-  const MrTaskList& tl = p.near;
-  int* ctr = p.near_ctr.as<int>();
+void launch_near_t(const NearLaunch& nl, hipStream_t s) {
+  const MrTaskList& tl = *nl.tl;
+  int* ctr = nl.ctr;
   k_mr_zero<<<1, 4 * kMrCtrStride, 0, s>>>(ctr, 4 * kMrCtrStride);
   AFMM_LAUNCH_CHECK();
   for (int c = 0; c < 4; c++) {
@@ -1374,15 +1456,13 @@ void launch_near_t(Engine& e, const MrPlan& p, hipStream_t s) {
     if (n <= 0) continue;
     const MrNearTask* t = tl.buf.as<MrNearTask>() + tl.cls[c];
     // waves per SIMD of the persistent grid (4 waves per workgroup)
-    const int64_t slots = (int64_t)e.num_cus * knobs().near_waves;
+    const int64_t slots = (int64_t)nl.num_cus * nl.near_waves;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 4), slots));
-    const MrNearProb* pr = p.probs.as<MrNearProb>();
-    const double2* lt = e.log_tab.as<double2>();
     int* cc = ctr + c * kMrCtrStride;
-#define AFMM_MR_NEAR(NRT)                                                                   \
-  AFMM_DISPATCH_KERNEL(e.kid, KID,                                                          \
-                       k_mr_p2p<P, NCT, NRT, KID><<<grid, 256, 0, s>>>(t, n, pr, lt, e.ln2_hi, \
-                                                                       e.ln2_lo, cc))
+#define AFMM_MR_NEAR(NRT)                                                                        \
+  AFMM_DISPATCH_KERNEL(nl.kid, KID,                                                              \
+                       k_mr_p2p<P, NCT, NRT, KID><<<grid, 256, 0, s>>>(t, n, nl.probs, nl.logtab, \
+                                                                       nl.ln2_hi, nl.ln2_lo, cc))
     switch (c) {
       case 0: AFMM_MR_NEAR(1); break;
       case 1: AFMM_MR_NEAR(2); break;
@@ -1394,20 +1474,26 @@ void launch_near_t(Engine& e, const MrPlan& p, hipStream_t s) {
   }
 }
 
-void launch_near_p(Engine& e, const MrPlan& p, hipStream_t s) {
-  if (p.nct == 1) {
-    switch (p.prec) {
-      case PREC_F64: launch_near_t<PREC_F64, 1>(e, p, s); break;
-      case PREC_F32: launch_near_t<PREC_F32, 1>(e, p, s); break;
-      default: launch_near_t<PREC_BF16, 1>(e, p, s); break;
+void launch_near_p(int prec, int nct, const NearLaunch& nl, hipStream_t s) {
+  if (nct == 1) {
+    switch (prec) {
+      case PREC_F64: launch_near_t<PREC_F64, 1>(nl, s); break;
+      case PREC_F32: launch_near_t<PREC_F32, 1>(nl, s); break;
+      default: launch_near_t<PREC_BF16, 1>(nl, s); break;
     }
   } else {
-    switch (p.prec) {
-      case PREC_F64: launch_near_t<PREC_F64, 2>(e, p, s); break;
-      case PREC_F32: launch_near_t<PREC_F32, 2>(e, p, s); break;
-      default: launch_near_t<PREC_BF16, 2>(e, p, s); break;
+    switch (prec) {
+      case PREC_F64: launch_near_t<PREC_F64, 2>(nl, s); break;
+      case PREC_F32: launch_near_t<PREC_F32, 2>(nl, s); break;
+      default: launch_near_t<PREC_BF16, 2>(nl, s); break;
     }
   }
+}
+
+void launch_near_plan(Engine& e, const MrPlan& p, int near_waves, hipStream_t s) {
+  NearLaunch nl{&p.near, p.probs.as<MrNearProb>(), e.log_tab.as<double2>(), e.ln2_hi, e.ln2_lo,
+                p.near_ctr.as<int>(), e.kid, e.num_cus, near_waves};
+  launch_near_p(p.prec, p.nct, nl, s);
 }
 
 template <class VT, int NCT>
@@ -1429,6 +1515,32 @@ void launch_io_p(int prec, int nct, bool gather, const MrIoTask& d, hipStream_t 
   }
 }
 
+template <class VT, int NCT>
+void launch_io_multi_t(bool gather, const MrIoTask* t, int n, int blocks, hipStream_t s) {
+  if (gather) k_mr_io_multi<VT, NCT, true><<<blocks, 256, 0, s>>>(t, n);
+  else k_mr_io_multi<VT, NCT, false><<<blocks, 256, 0, s>>>(t, n);
+  AFMM_LAUNCH_CHECK();
+}
+
+void launch_io_multi(int prec, int nct, bool gather, const MrIoTask* t, int n, int blocks,
+                     hipStream_t s) {
+  if (blocks <= 0) return;
+  if (prec == PREC_F64) {
+    if (nct == 1) launch_io_multi_t<double, 1>(gather, t, n, blocks, s);
+    else launch_io_multi_t<double, 2>(gather, t, n, blocks, s);
+  } else {
+    if (nct == 1) launch_io_multi_t<float, 1>(gather, t, n, blocks, s);
+    else launch_io_multi_t<float, 2>(gather, t, n, blocks, s);
+  }
+}
+
+// Launches one task list makes (one per non-empty row-tile class).
+int list_launches(const MrTaskList& tl) {
+  int n = 0;
+  for (int c = 0; c < 4; c++) n += tl.cls[c + 1] > tl.cls[c];
+  return n;
+}
+
 }  // namespace
 
 void mr_prepare(Engine& e, int prec) {
@@ -1442,6 +1554,7 @@ void mr_matvec(Engine& e, const double* d_b, int64_t ldb, double* d_y, int64_t l
   const int nct = nrhs <= 16 ? 1 : 2;
   MrPlan& p = get_plan(e, prec, nct);
   const int L = e.L;
+  const MrTune tn = mr_default_tune();
   MrIoTask io{};
   io.b = d_b;
   io.y = d_y;
@@ -1464,25 +1577,25 @@ void mr_matvec(Engine& e, const double* d_b, int64_t ldb, double* d_y, int64_t l
     AFMM_HIP(hipEventRecord(e.ev_fork, s));
     AFMM_HIP(hipStreamWaitEvent(e.s_aux, e.ev_fork, 0));
     e.prof_begin("mr_p2p", e.s_aux, &a, &b);
-    launch_near_p(e, p, e.s_aux);
+    launch_near_plan(e, p, tn.near_waves, e.s_aux);
     e.prof_end(e.s_aux, b);
     AFMM_HIP(hipEventRecord(e.ev_join, e.s_aux));
   }
   e.prof_begin("mr_m2m", s, &a, &b);
-  for (int j = L; j >= 2; j--) launch_gemm_p(prec, nct, p.up[j], 0, s);
+  for (int j = L; j >= 2; j--) launch_gemm_p(prec, nct, p.up[j], 0, 3, s);
   e.prof_end(s, b);
   e.prof_begin("mr_m2l", s, &a, &b);
-  launch_gemm_p(prec, nct, p.m2l, knobs().xcd, s);
+  launch_gemm_p(prec, nct, p.m2l, tn.xcd, tn.depth, s);
   e.prof_end(s, b);
   e.prof_begin("mr_l2l", s, &a, &b);
-  for (int j = 2; j < L; j++) launch_gemm_p(prec, nct, p.down[j], 0, s);
-  launch_gemm_p(prec, nct, p.leaf, 0, s);
+  for (int j = 2; j < L; j++) launch_gemm_p(prec, nct, p.down[j], 0, 3, s);
+  launch_gemm_p(prec, nct, p.leaf, 0, 3, s);
   e.prof_end(s, b);
   if (e.near_overlap) {
     AFMM_HIP(hipStreamWaitEvent(s, e.ev_join, 0));
   } else {
     e.prof_begin("mr_p2p", s, &a, &b);
-    launch_near_p(e, p, s);
+    launch_near_plan(e, p, tn.near_waves, s);
     e.prof_end(s, b);
   }
   e.prof_begin("mr_out", s, &a, &b);
@@ -1496,6 +1609,366 @@ void add_cols_device(double* d_y, int64_t ldy, const double* d_t, int64_t ldt, i
   if (tot <= 0) return;
   k_add_cols<<<ceil_div(tot, 256), 256, 0, s>>>(d_y, ldy, d_t, ldt, n, nrhs);
   AFMM_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Batches of problems.
+
+struct MrBatch {
+  std::vector<Engine*> eng;
+  std::vector<MrPlan*> plans;
+  int prec = 0, nct = 1, nrhs = 1, device = 0;
+  std::vector<MrTaskList> up, down;  // by step: up aligned at the leaves, down from level 2
+  MrTaskList m2l;
+  int m2l_rows = 0;
+  struct NearGroup {
+    int kid = 0;
+    const Engine* first = nullptr;  // log table owner
+    MrTaskList tl;
+    DevBuf probs;
+  };
+  std::vector<NearGroup> near;
+  DevBuf ctr;                   // 4 counters per near group
+  DevBuf io;                    // MrIoTask per problem, i0 = first workgroup
+  int nio = 0, io_blocks = 0;
+  hipStream_t side = nullptr;   // near field, beside the tree passes
+  hipStream_t cap = nullptr;    // graph capture
+  hipEvent_t fork = nullptr, join = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  MrTune tune;
+  bool tuned = false;
+  std::string key;
+  ~MrBatch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (side) (void)hipStreamDestroy(side);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+};
+
+namespace {
+
+// Tuned parameters per "architecture|shape class", for the process and the
+// file AFMM2D_TUNE_CACHE names (lines "key depth xcd near_waves rows_m2l").
+std::mutex g_tune_mu;
+std::map<std::string, MrTune> g_tune_cache;
+bool g_tune_loaded = false;
+
+const char* tune_file() {
+  const char* f = std::getenv("AFMM2D_TUNE_CACHE");
+  return f && *f ? f : nullptr;
+}
+
+void load_tune_cache() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const char* f = tune_file();
+  if (!f) return;
+  std::ifstream in(f);
+  std::string key;
+  MrTune t;
+  while (in >> key >> t.depth >> t.xcd >> t.near_waves >> t.rows_m2l) g_tune_cache[key] = t;
+}
+
+void store_tune(const std::string& key, const MrTune& t) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = t;
+  if (const char* f = tune_file()) {
+    std::ofstream out(f, std::ios::app);
+    out << key << ' ' << t.depth << ' ' << t.xcd << ' ' << t.near_waves << ' ' << t.rows_m2l
+        << '\n';
+  }
+}
+
+// Shape class: precision, column tiles, kernel ids present, log2 of the
+// total points and of the number of problems.
+std::string batch_key(const MrBatch& b) {
+  hipDeviceProp_t prop;
+  AFMM_HIP(hipGetDeviceProperties(&prop, b.device));
+  std::string arch = prop.gcnArchName;
+  const size_t colon = arch.find(':');
+  if (colon != std::string::npos) arch.resize(colon);
+  int64_t pts = 0;
+  unsigned kids = 0;
+  for (const Engine* e : b.eng) {
+    pts += e->n;
+    kids |= 1u << e->kid;
+  }
+  const int lp = (int)std::lround(std::log2((double)std::max<int64_t>(pts, 1)));
+  const int lb = (int)std::lround(std::log2((double)b.eng.size()));
+  std::ostringstream k;
+  k << arch << "|p" << b.prec << "|c" << b.nct << "|k" << kids << "|n" << lp << "|b" << lb;
+  return k.str();
+}
+
+// Rebuilds the batch's M2L list with rows rows per task.
+void batch_m2l(MrBatch& b, int rows, hipStream_t s) {
+  if (rows == b.m2l_rows) return;
+  std::vector<MrTask> all;
+  for (size_t i = 0; i < b.eng.size(); i++) m2l_tasks(*b.eng[i], *b.plans[i], rows, all);
+  AFMM_HIP(hipStreamSynchronize(s));
+  upload_tasks(b.m2l, all, s);
+  AFMM_HIP(hipStreamSynchronize(s));
+  b.m2l_rows = rows;
+}
+
+void batch_enqueue(MrBatch& b, hipStream_t s) {
+  const MrIoTask* io = b.io.as<MrIoTask>();
+  launch_io_multi(b.prec, b.nct, true, io, b.nio, b.io_blocks, s);
+  AFMM_HIP(hipEventRecord(b.fork, s));
+  AFMM_HIP(hipStreamWaitEvent(b.side, b.fork, 0));
+  for (size_t g = 0; g < b.near.size(); g++) {
+    const MrBatch::NearGroup& ng = b.near[g];
+    NearLaunch nl{&ng.tl, ng.probs.as<MrNearProb>(), ng.first->log_tab.as<double2>(),
+                  ng.first->ln2_hi, ng.first->ln2_lo,
+                  b.ctr.as<int>() + 4 * kMrCtrStride * g, ng.kid, ng.first->num_cus,
+                  b.tune.near_waves};
+    launch_near_p(b.prec, b.nct, nl, b.side);
+  }
+  AFMM_HIP(hipEventRecord(b.join, b.side));
+  for (const MrTaskList& tl : b.up) launch_gemm_p(b.prec, b.nct, tl, 0, 3, s);
+  launch_gemm_p(b.prec, b.nct, b.m2l, b.tune.xcd, b.tune.depth, s);
+  for (const MrTaskList& tl : b.down) launch_gemm_p(b.prec, b.nct, tl, 0, 3, s);
+  AFMM_HIP(hipStreamWaitEvent(s, b.join, 0));
+  launch_io_multi(b.prec, b.nct, false, io, b.nio, b.io_blocks, s);
+}
+
+void drop_graph(MrBatch& b) {
+  if (b.exec) (void)hipGraphExecDestroy(b.exec);
+  if (b.graph) (void)hipGraphDestroy(b.graph);
+  b.exec = nullptr;
+  b.graph = nullptr;
+}
+
+}  // namespace
+
+MrBatch* mr_batch_create(const std::vector<Engine*>& eng, const double* const* d_b,
+                         const int64_t* ldb, double* const* d_y, const int64_t* ldy, int nrhs,
+                         int prec) {
+  if (eng.empty()) throw Error(1, "batch without problems");
+  if (nrhs < 1 || nrhs > kMrMaxCols) throw Error(1, "batched matvec takes 1..32 right-hand sides");
+  auto bp = std::make_unique<MrBatch>();
+  MrBatch& b = *bp;
+  b.eng = eng;
+  b.prec = prec;
+  b.nrhs = nrhs;
+  b.nct = nrhs <= 16 ? 1 : 2;
+  b.device = eng[0]->device;
+  int maxL = 0;
+  for (size_t i = 0; i < eng.size(); i++) {
+    const Engine& e = *eng[i];
+    if (e.device != b.device) throw Error(1, "batch problems on different devices");
+    if (ldb[i] < e.n || ldy[i] < e.n) throw Error(1, "leading dimension below N");
+    maxL = std::max(maxL, e.L);
+  }
+  AFMM_HIP(hipSetDevice(b.device));
+  for (Engine* e : eng) b.plans.push_back(&get_plan(*e, prec, b.nct));
+  AFMM_HIP(hipStreamCreateWithFlags(&b.side, hipStreamNonBlocking));
+  AFMM_HIP(hipStreamCreateWithFlags(&b.cap, hipStreamNonBlocking));
+  AFMM_HIP(hipEventCreateWithFlags(&b.fork, hipEventDisableTiming));
+  AFMM_HIP(hipEventCreateWithFlags(&b.join, hipEventDisableTiming));
+  hipStream_t s = b.side;
+  // upward step u: level L - u of every problem with L - u >= 2; downward
+  // step d: level 2 + d (< L) and, at d = L - 2, the leaf pass
+  const int nsteps = maxL - 1;
+  std::vector<std::vector<MrTask>> up(nsteps), down(nsteps);
+  for (size_t i = 0; i < eng.size(); i++) {
+    const Engine& e = *eng[i];
+    const MrPlan& p = *b.plans[i];
+    for (int j = e.L; j >= 2; j--) {
+      auto& v = up[e.L - j];
+      v.insert(v.end(), p.h_up[j].begin(), p.h_up[j].end());
+    }
+    for (int j = 2; j < e.L; j++) {
+      auto& v = down[j - 2];
+      v.insert(v.end(), p.h_down[j].begin(), p.h_down[j].end());
+    }
+    auto& v = down[e.L - 2];
+    v.insert(v.end(), p.h_leaf.begin(), p.h_leaf.end());
+  }
+  b.up.resize(nsteps);
+  b.down.resize(nsteps);
+  for (int u = 0; u < nsteps; u++) {
+    upload_tasks(b.up[u], up[u], s);
+    upload_tasks(b.down[u], down[u], s);
+  }
+  // near field: one group per kernel id, problems indexed within the group
+  for (int kid = 0; kid < 32; kid++) {
+    std::vector<MrNearTask> tasks;
+    std::vector<MrNearProb> probs;
+    const Engine* first = nullptr;
+    for (size_t i = 0; i < eng.size(); i++) {
+      const Engine& e = *eng[i];
+      if (e.kid != kid) continue;
+      if (!first) first = &e;
+      const MrPlan& p = *b.plans[i];
+      const int pi = (int)probs.size();
+      probs.push_back(MrNearProb{e.x.as<double>(), e.y.as<double>(), p.qs.p, p.nearv.as<double>(),
+                                 e.kp});
+      for (MrNearTask t : p.h_near) {
+        t.prob = pi;
+        tasks.push_back(t);
+      }
+    }
+    if (!first) continue;
+    b.near.emplace_back();
+    MrBatch::NearGroup& ng = b.near.back();
+    ng.kid = kid;
+    ng.first = first;
+    upload_tasks(ng.tl, tasks, s);
+    upload_vec(ng.probs, probs, s);
+  }
+  b.ctr.alloc(sizeof(int) * 4 * kMrCtrStride * std::max<size_t>(b.near.size(), 1));
+  // gather / output: problem i owns workgroups [i0, i0 + ceil(n / kMrIoPoints))
+  std::vector<MrIoTask> io;
+  int blocks = 0;
+  for (size_t i = 0; i < eng.size(); i++) {
+    const Engine& e = *eng[i];
+    const MrPlan& p = *b.plans[i];
+    MrIoTask t{};
+    t.b = d_b[i];
+    t.y = d_y[i];
+    t.inv_perm = e.inv_perm_d.as<int>();
+    t.qs = p.qs.p;
+    t.pot = p.pot.p;
+    t.nearv = p.nearv.as<double>();
+    t.ldb = ldb[i];
+    t.ldy = ldy[i];
+    t.i0 = blocks;
+    t.cnt = e.n;
+    t.nrhs = nrhs;
+    io.push_back(t);
+    const int64_t nb = blocks + (int64_t)ceil_div(e.n, kMrIoPoints);
+    if (nb > INT32_MAX) throw Error(1, "batch too large");
+    blocks = (int)nb;
+  }
+  b.nio = (int)io.size();
+  b.io_blocks = blocks;
+  upload_vec(b.io, io, s);
+  // launch parameters: the cached tuning of this shape class, else defaults
+  b.key = batch_key(b);
+  b.tune = mr_default_tune();
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    load_tune_cache();
+    auto it = g_tune_cache.find(b.key);
+    if (it != g_tune_cache.end()) {
+      b.tune = it->second;
+      b.tuned = true;
+    }
+  }
+  batch_m2l(b, b.tune.rows_m2l, s);
+  AFMM_HIP(hipStreamSynchronize(s));
+  return bp.release();
+}
+
+void mr_batch_destroy(MrBatch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  (void)hipStreamSynchronize(b->side);
+  (void)hipStreamSynchronize(b->cap);
+  delete b;
+}
+
+void mr_batch_run(MrBatch* b, hipStream_t s) {
+  AFMM_HIP(hipSetDevice(b->device));
+  if (b->exec) {
+    AFMM_HIP(hipGraphLaunch(b->exec, s));
+  } else {
+    batch_enqueue(*b, s);
+  }
+}
+
+void mr_batch_graph(MrBatch* b, bool on) {
+  AFMM_HIP(hipSetDevice(b->device));
+  AFMM_HIP(hipStreamSynchronize(b->cap));
+  drop_graph(*b);
+  if (!on) return;
+  AFMM_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+  try {
+    batch_enqueue(*b, b->cap);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(b->cap, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  AFMM_HIP(hipStreamEndCapture(b->cap, &b->graph));
+  AFMM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+}
+
+double mr_batch_tune(MrBatch* b, hipStream_t s) {
+  AFMM_HIP(hipSetDevice(b->device));
+  const bool had_graph = b->exec != nullptr;
+  drop_graph(*b);
+  std::vector<MrTune> cand;
+  for (int rows : {32, 64})
+    for (int depth : {3, 4})
+      for (int xcd : {0, 1})
+        for (int waves : {1, 2, 4}) {
+          MrTune t;
+          t.rows_m2l = rows;
+          t.depth = depth;
+          t.xcd = xcd;
+          t.near_waves = waves;
+          cand.push_back(t);
+        }
+  hipEvent_t e0, e1;
+  AFMM_HIP(hipEventCreate(&e0));
+  AFMM_HIP(hipEventCreate(&e1));
+  constexpr int kReps = 3;
+  double best_ms = 1e30;
+  MrTune best = b->tune;
+  try {
+    for (const MrTune& t : cand) {
+      b->tune = t;
+      batch_m2l(*b, t.rows_m2l, s);
+      batch_enqueue(*b, s);  // warm-up
+      AFMM_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < kReps; r++) batch_enqueue(*b, s);
+      AFMM_HIP(hipEventRecord(e1, s));
+      AFMM_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      AFMM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms / kReps < best_ms) {
+        best_ms = ms / kReps;
+        best = t;
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->tune = best;
+  batch_m2l(*b, best.rows_m2l, s);
+  b->tuned = true;
+  store_tune(b->key, best);
+  if (had_graph) mr_batch_graph(b, true);
+  return best_ms;
+}
+
+MrBatchInfo mr_batch_info(const MrBatch* b) {
+  MrBatchInfo in{};
+  int n = b->io_blocks > 0 ? 2 : 0;
+  for (const MrTaskList& tl : b->up) n += list_launches(tl);
+  for (const MrTaskList& tl : b->down) n += list_launches(tl);
+  n += list_launches(b->m2l);
+  for (const MrBatch::NearGroup& g : b->near) n += 1 + list_launches(g.tl);
+  in.launches = n;
+  in.depth = b->tune.depth;
+  in.xcd = b->tune.xcd;
+  in.near_waves = b->tune.near_waves;
+  in.rows_m2l = b->tune.rows_m2l;
+  in.captured = b->exec != nullptr;
+  in.tuned = b->tuned;
+  in.nproblems = (int)b->eng.size();
+  return in;
 }
 
 }  // namespace afmm

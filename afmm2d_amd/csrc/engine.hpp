@@ -226,7 +226,51 @@ struct MrPlan {
   MrTaskList m2l, leaf, near;
   DevBuf probs;
   DevBuf near_ctr;  // work counters of the near field's persistent grid
+  // host copies of the task lists (multi-problem batches concatenate them)
+  std::vector<std::vector<MrTask>> h_up, h_down;
+  std::vector<MrTask> h_m2l, h_leaf;
+  std::vector<MrNearTask> h_near;
 };
+
+// Launch parameters of the blocked passes that do not change the task
+// lists: the M2L's chunk ring depth (3, 4, 6), XCD mapping and rows per
+// task (16, 32, 64), the near field's waves per SIMD.  Defaults from the
+// environment (AFMM2D_MR_DEPTH, AFMM2D_MR_XCD, AFMM2D_MR_ROWS_M2L,
+// AFMM2D_MR_NEAR_WAVES).
+struct MrTune {
+  int depth = 3, xcd = 0, near_waves = 2, rows_m2l = 64;
+};
+MrTune mr_default_tune();
+
+// Several problems' blocked matvecs as one: per pass one launch over the
+// concatenated task lists of all problems (the upward pass aligned at the
+// leaves, the downward pass from level 2, the near field one persistent
+// launch per kernel id), optionally captured into one hipGraph and
+// replayed.  Charges and outputs are fixed device arrays given at creation
+// (caller order, column-major, nrhs columns).  The batch works in the
+// problems' own blocked workspaces: a batch run and a call on one of its
+// problems must not overlap.
+struct MrBatch;
+struct Engine;
+MrBatch* mr_batch_create(const std::vector<Engine*>& eng, const double* const* d_b,
+                         const int64_t* ldb, double* const* d_y, const int64_t* ldy, int nrhs,
+                         int prec);
+void mr_batch_destroy(MrBatch* b);
+// One batched matvec on stream s: the captured graph if there is one, the
+// eager launches otherwise.
+void mr_batch_run(MrBatch* b, hipStream_t s);
+// Times the candidate launch parameters on this batch (eager runs on s) and
+// keeps the fastest; results are cached per GPU architecture and shape
+// class for the process (and in the file AFMM2D_TUNE_CACHE names, if set).
+// A captured graph is captured again with the new parameters.  Returns the
+// time per run of the chosen parameters (ms).
+double mr_batch_tune(MrBatch* b, hipStream_t s);
+// Captures the batched matvec into a graph (on = true) or drops it.
+void mr_batch_graph(MrBatch* b, bool on);
+struct MrBatchInfo {
+  int launches, depth, xcd, near_waves, rows_m2l, captured, tuned, nproblems;
+};
+MrBatchInfo mr_batch_info(const MrBatch* b);
 
 struct Engine {
   // problem

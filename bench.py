@@ -74,6 +74,11 @@ def parse():
                    help="instead of the headline, the precision workload: N = 2^22 points "
                         "from 16 clipped Gaussians (sigma 0.05), 1/r kernel, leaf 64, ACA tol "
                         "1e-8, 32 right-hand sides in FP64, FP32 and BF16 (one JSON line)")
+    p.add_argument("--batch", action="store_true",
+                   help="instead of the headline, the batching workload: 256 problems with N "
+                        "log-uniform in [2048, 65536], kernels log r / 1/r / exp(-r^2/0.01), "
+                        "8 RHS ~ N(0,1), FP64 factors, one captured graph with tuned "
+                        "parameters (one JSON line)")
     p.add_argument("--near-field", action="store_true",
                    help="instead of the headline, time the near-field kernel alone "
                         "(afmm2d_near_field, no assembly) on the same points and print its "
@@ -120,6 +125,91 @@ def config4(args, H, E):
                    "max_rank": st["max_rank"], "assemble_s": round(t_asm, 3),
                    "m2l_bytes_fp64_once": st["m2l_bytes"], "rel_err_rows": int(len(rows))},
         "by_precision": res,
+    }))
+
+
+def batch_config(args, H, E):
+    """The batching workload, one JSON line: 256 problems applied by one
+    batched matvec (eager with default parameters, eager tuned, captured graph
+    tuned) beside the same problems' blocked matvecs one after another; the
+    error of column 0 of 8 sampled problems against the FP64 direct sum."""
+    count, R, tol_pow = 256, 8, 8
+    rng = np.random.default_rng(args.seed)
+    kernels = (("log", None), ("inv_r", None), ("gaussian", [0.01, 0.0, 0.0, 0.0]))
+    print("bench: batch of %d problems" % count, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    probs, names = [], []
+    for _ in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(2048), np.log(65536)))))
+        name, kp = kernels[int(rng.integers(0, 3))]
+        xy = rng.uniform(-1.0, 1.0, size=(n, 2))
+        probs.append(E.AFMM(n, 64, tol_pow, xy, kernel=name, kparams=kp))
+        names.append(name)
+    E.batch_assemble(probs)
+    t_asm = time.perf_counter() - t0
+    total = sum(p.N for p in probs)
+    Q = [np.asfortranarray(rng.standard_normal((p.N, R))) for p in probs]
+    dB = [H.DeviceArray.from_numpy(q) for q in Q]
+    dY = [H.DeviceArray(q.nbytes) for q in Q]
+    steps = max(5, args.steps // 2)
+
+    def one_by_one():
+        for p, b, y in zip(probs, dB, dY):
+            p.matvec_device_ex(b.ptr, p.N, y.ptr, p.N, R, "fp64", blocked=True)
+
+    print("bench: batch timing", file=sys.stderr, flush=True)
+    ms_loop, wall_loop = timed(H, one_by_one, steps, 2)
+    B = E.Batch(probs, [b.ptr for b in dB], [y.ptr for y in dY], R, "fp64")
+    ms_eager0, _ = timed(H, B.matvec, steps, 2)
+    t0 = time.perf_counter()
+    tune_ms = B.tune()
+    t_tune = time.perf_counter() - t0
+    ms_eager, wall_eager = timed(H, B.matvec, steps, 2)
+    B.capture(True)
+    ms_graph, wall_graph = timed(H, B.matvec, steps, 2)
+    info = B.info()
+    # errors of 8 sampled problems (column 0) against the FP64 direct sum
+    errs = []
+    for i in np.linspace(0, count - 1, 8).astype(int):
+        p = probs[i]
+        y = dY[i].download(np.empty((p.N, R), order="F"))
+        rows = np.sort(np.random.default_rng(i).choice(p.N, min(1024, p.N), replace=False))
+        ref = p.direct_rows(rows, Q[i][:, 0])
+        errs.append({"n": p.N, "kernel": names[i],
+                     "rel_err": float(np.linalg.norm(y[rows, 0] - ref) / np.linalg.norm(ref))})
+    # whole-batch algorithmic bytes: every problem's M2L row blocks once, its
+    # basis twice (upward V^T, downward V), the charges read and outputs
+    # written once
+    alg = 0
+    for p in probs:
+        st = p.stats()
+        alg += 8 * (st["blocked_m2l_coeffs"] + 2 * st["blocked_basis_coeffs"]) + 16 * p.N * R
+    ach = alg / (ms_graph * 1e-3) / 1e9
+    B.close()
+    for a in dB + dY:
+        a.free()
+    for p in probs:
+        p.close()
+    print(json.dumps({
+        "metric": "FMM batched matvec point\u00b7RHS/s, 256 problems",
+        "value": round(total * R / (ms_graph * 1e-3), 1), "unit": UNIT, "n_gpus": 1,
+        "steps": steps, "ms_per_step": round(ms_graph, 4), "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "256 problems, N log-uniform in [2048, 65536] (total %d), "
+                               "uniform [-1,1]^2, kernels log r / 1/r / exp(-r^2/0.01), leaf 64, "
+                               "ACA tol 1e-%d, %d RHS ~ N(0,1), FP64" % (total, tol_pow, R),
+                   "total_points": total, "assemble_s": round(t_asm, 2),
+                   "tune_s": round(t_tune, 2), "tuned_ms": round(tune_ms, 4), "batch": info},
+        "ms_per_run": {"graph_tuned": round(ms_graph, 4), "eager_tuned": round(ms_eager, 4),
+                       "eager_default": round(ms_eager0, 4),
+                       "one_by_one": round(ms_loop, 4)},
+        "wall_ms_per_run": {"graph_tuned": round(wall_graph, 4),
+                            "eager_tuned": round(wall_eager, 4),
+                            "one_by_one": round(wall_loop, 4)},
+        "roofline_matvec": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                            "algorithmic_bytes": int(alg)},
+        "errors": errs,
     }))
 
 
@@ -492,6 +582,9 @@ def main():
         return
     if args.config4:
         config4(args, H, E)
+        return
+    if args.batch:
+        batch_config(args, H, E)
         return
     xy, q = inputs(args.n, args.seed)
 

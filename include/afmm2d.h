@@ -152,6 +152,62 @@ int afmm2d_matvec_device_ex(afmm2d_handle h, const double* d_b, int64_t ldb, dou
  * No reference counterpart: the reference has one precision. */
 int afmm2d_prepare_precision(afmm2d_handle h, int precision);
 
+/* ---- Batches: many problems' matvecs as one ----------------------------
+ * The reference has one AFMM object per problem (AFMM2D.hpp:14-33) and
+ * applies them one after another; a batch applies the blocked matvec
+ * (afmm2d_matvec_device_ex with AFMM2D_MATVEC_BLOCKED) of several handles at
+ * once: one launch per pass over the concatenated work of all problems
+ * (upward pass aligned at the leaves, downward pass from level 2, the near
+ * field one persistent launch per kernel id), optionally captured into one
+ * hipGraph.  Each problem's result equals its own blocked matvec bit for
+ * bit.  Handles of non-symmetric user kernels are refused
+ * (AFMM2D_ERR_UNSUPPORTED).  A batch works in its handles' blocked
+ * workspaces: a batch matvec and a call on one of its handles must not
+ * overlap (order them on one stream, or synchronize). */
+typedef struct afmm2d_batch_s* afmm2d_batch;
+
+typedef struct {
+  int launches;    /* kernel launches per batch matvec */
+  int depth;       /* M2L chunk ring depth */
+  int xcd;         /* M2L XCD-aware task order (0/1) */
+  int near_waves;  /* near-field persistent waves per SIMD */
+  int rows_m2l;    /* M2L rows per task */
+  int captured;    /* a graph is captured and replayed */
+  int tuned;       /* parameters from the autotuner (this batch or its cache) */
+  int nproblems;
+} afmm2d_batch_info;
+
+/* afmm2d_assemble (AFMM::assemble, AFMM2D.cpp:41-44) of every handle. */
+int afmm2d_batch_assemble(const afmm2d_handle* handles, int count);
+
+/* A batch over count assembled handles.  d_b[i], d_y[i]: device arrays of
+ * problem i (caller order, column-major, leading dimensions ldb[i], ldy[i]
+ * >= N_i, nrhs <= 32 columns), fixed for the batch's life.  precision: an
+ * afmm2d_precision.  Parameters come from the autotuner's cache for this
+ * GPU and shape class when it holds them (AFMM2D_TUNE_CACHE names a file
+ * that keeps it across processes), else from the defaults. */
+int afmm2d_batch_create(const afmm2d_handle* handles, int count, const double* const* d_b,
+                        const int64_t* ldb, double* const* d_y, const int64_t* ldy, int nrhs,
+                        int precision, afmm2d_batch* out);
+
+/* Y_i = K_i B_i for every problem, ordered on 'stream' (NULL = null stream):
+ * the captured graph when there is one, else eager launches. */
+int afmm2d_batch_matvec(afmm2d_batch b, void* stream);
+
+/* Times the candidate launch parameters (M2L rows per task 32/64, ring depth
+ * 3/4, XCD order 0/1, near-field waves 1/2/4) on this batch with eager runs
+ * on 'stream', keeps the fastest and caches it for the shape class.  A
+ * captured graph is captured again.  ms_per_run (may be NULL): the time per
+ * batch matvec of the chosen parameters. */
+int afmm2d_batch_tune(afmm2d_batch b, void* stream, double* ms_per_run);
+
+/* on = 1: captures the batch matvec into a hipGraph replayed by every later
+ * afmm2d_batch_matvec; on = 0: drops it. */
+int afmm2d_batch_capture(afmm2d_batch b, int on);
+
+int afmm2d_batch_get_info(afmm2d_batch b, afmm2d_batch_info* info);
+int afmm2d_batch_destroy(afmm2d_batch b);
+
 /* nLevels (AFMM2D.hpp:20). */
 int afmm2d_levels(afmm2d_handle h, int* levels);
 
