@@ -48,7 +48,13 @@ EXPORTS = [
     "afmm2d_direct_rows_device_noalloc",
     "afmm2d_batch_assemble", "afmm2d_batch_create", "afmm2d_batch_matvec", "afmm2d_batch_tune",
     "afmm2d_batch_capture", "afmm2d_batch_get_info", "afmm2d_batch_destroy",
+    "afmm2d_rank_model_create", "afmm2d_rank_model_destroy", "afmm2d_rank_model_get_params",
+    "afmm2d_rank_model_set_params", "afmm2d_rank_model_predict", "afmm2d_rank_model_gradient",
+    "afmm2d_rank_model_train", "afmm2d_set_rank_model", "afmm2d_get_rank_samples",
 ]
+
+RANK_FEATURES = 16
+RANK_PARAMS = 9473
 
 # afmm2d_precision and the blocked-mode flag (include/afmm2d.h)
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
@@ -73,6 +79,9 @@ class Stats(ctypes.Structure):
         ("m2l_leaf_bytes", ctypes.c_int64), ("m2l_leaf_columns", ctypes.c_int64),
         ("skeleton_leaf", ctypes.c_int64), ("near_waves", ctypes.c_int64),
         ("blocked_m2l_coeffs", ctypes.c_int64), ("blocked_basis_coeffs", ctypes.c_int64),
+        ("rank_boxes", ctypes.c_int64), ("rank_fallbacks", ctypes.c_int64),
+        ("aca_scratch_peak", ctypes.c_int64), ("aca_launches", ctypes.c_int64),
+        ("aca_seconds", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -84,6 +93,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _lp = ctypes.POINTER(ctypes.c_int64)
 _vp = ctypes.c_void_p
+_fp = ctypes.POINTER(ctypes.c_float)
 
 
 class BatchInfo(ctypes.Structure):
@@ -150,6 +160,16 @@ def lib():
         "afmm2d_batch_capture": (i, [_vp, i]),
         "afmm2d_batch_get_info": (i, [_vp, ctypes.POINTER(BatchInfo)]),
         "afmm2d_batch_destroy": (i, [_vp]),
+        "afmm2d_rank_model_create": (i, [i, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+        "afmm2d_rank_model_destroy": (i, [_vp]),
+        "afmm2d_rank_model_get_params": (i, [_vp, _fp]),
+        "afmm2d_rank_model_set_params": (i, [_vp, _fp]),
+        "afmm2d_rank_model_predict": (i, [_vp, _fp, ctypes.c_int64, _fp]),
+        "afmm2d_rank_model_gradient": (i, [_vp, _fp, _fp, i, _fp, _dp]),
+        "afmm2d_rank_model_train": (i, [_vp, _fp, _fp, ctypes.c_int64, i, i, ctypes.c_float,
+                                        ctypes.c_uint64, _dp]),
+        "afmm2d_set_rank_model": (i, [_vp, _vp, ctypes.c_float]),
+        "afmm2d_get_rank_samples": (i, [_vp, _fp, _ip, ctypes.c_int64, _lp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -401,6 +421,23 @@ class AFMM:
                                         _d(out)))
         return out
 
+    def set_rank_model(self, model, margin=0.5):
+        """Assemble with ACA capacities from model (None: the plain ACA)."""
+        _check(lib().afmm2d_set_rank_model(self._h, model._m if model is not None else None,
+                                           float(margin)))
+        self._rank_model = model  # the model outlives the assembly
+
+    def rank_samples(self):
+        """(features (n, 16) float32, ranks (n,) int32): one per box of
+        levels L..2, from the assembly."""
+        cnt = ctypes.c_int64()
+        _check(lib().afmm2d_get_rank_samples(self._h, None, None, 0, ctypes.byref(cnt)))
+        X = np.zeros((cnt.value, RANK_FEATURES), dtype=np.float32)
+        r = np.zeros(cnt.value, dtype=np.int32)
+        _check(lib().afmm2d_get_rank_samples(self._h, X.ctypes.data_as(_fp), _i(r), cnt.value,
+                                             ctypes.byref(cnt)))
+        return X, r
+
     def stats(self):
         st = Stats()
         _check(lib().afmm2d_get_stats(self._h, ctypes.byref(st)))
@@ -478,3 +515,66 @@ class Batch:
         inf = BatchInfo()
         _check(lib().afmm2d_batch_get_info(self._b, ctypes.byref(inf)))
         return inf.as_dict()
+
+
+class RankModel:
+    """The learned rank predictor (afmm2d_rank_model_*): a 16-64-64-64-1
+    FP32 MLP trained and evaluated by the library's kernels on the GPU."""
+
+    def __init__(self, seed=0, device=-1):
+        h = _vp()
+        _check(lib().afmm2d_rank_model_create(int(device), int(seed), ctypes.byref(h)))
+        self._m = h
+
+    def close(self):
+        if getattr(self, "_m", None) and self._m.value:
+            lib().afmm2d_rank_model_destroy(self._m)
+            self._m = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(RANK_PARAMS, dtype=np.float32)
+        _check(lib().afmm2d_rank_model_get_params(self._m, p.ctypes.data_as(_fp)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == RANK_PARAMS
+        _check(lib().afmm2d_rank_model_set_params(self._m, p.ctypes.data_as(_fp)))
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, RANK_FEATURES)
+        out = np.zeros(len(X), dtype=np.float32)
+        _check(lib().afmm2d_rank_model_predict(self._m, X.ctypes.data_as(_fp), len(X),
+                                               out.ctypes.data_as(_fp)))
+        return out
+
+    def gradient(self, X, y):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, RANK_FEATURES)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(RANK_PARAMS, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().afmm2d_rank_model_gradient(self._m, X.ctypes.data_as(_fp),
+                                                y.ctypes.data_as(_fp), len(X),
+                                                g.ctypes.data_as(_fp), ctypes.byref(loss)))
+        return loss.value, g
+
+    def train(self, X, y, epochs=10, batch=1024, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, RANK_FEATURES)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().afmm2d_rank_model_train(self._m, X.ctypes.data_as(_fp),
+                                             y.ctypes.data_as(_fp), len(X), int(epochs),
+                                             int(batch), float(lr), int(seed),
+                                             ctypes.byref(loss)))
+        return loss.value
+
+
+def rank_target(ranks):
+    """The rank model's training target for ACA ranks: log2(1 + rank)."""
+    return np.log2(1.0 + np.asarray(ranks, dtype=np.float64)).astype(np.float32)

@@ -21,7 +21,8 @@ LIB = os.path.join(HERE, "libafmm2d.so")
 EXAMPLE_SRC = os.path.join(HERE, "examples", "test_afmm2d.cpp")
 EXAMPLE_BIN = os.path.join(HERE, "examples", "test_afmm2d")
 
-SOURCES = ["api.cpp", "kd_sort.hip", "assemble.hip", "matvec.hip", "blocked.hip"]
+SOURCES = ["api.cpp", "kd_sort.hip", "assemble.hip", "matvec.hip", "blocked.hip",
+           "rank_model.hip"]
 DEFAULT_USER_KERNEL = os.path.join(HERE, "user_kernels", "anisotropic_log.h")
 
 

@@ -4,6 +4,7 @@
 // this file only moves data and builds index tables.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -803,6 +804,88 @@ int afmm2d_batch_destroy(afmm2d_batch b) {
   return guarded([&] { delete b; });
 }
 
+struct afmm2d_rank_model_s {
+  std::unique_ptr<RankModel> m;
+};
+
+int afmm2d_rank_model_create(int device, uint64_t seed, afmm2d_rank_model* out) {
+  if (!out) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL output");
+  *out = nullptr;
+  return guarded([&] {
+    int dev = device;
+    if (dev < 0) AFMM_HIP(hipGetDevice(&dev));
+    auto m = std::make_unique<afmm2d_rank_model_s>();
+    m->m.reset(rank_model_create(dev, seed));
+    *out = m.release();
+  });
+}
+
+int afmm2d_rank_model_destroy(afmm2d_rank_model m) {
+  if (!m) return AFMM2D_OK;
+  return guarded([&] { delete m; });
+}
+
+int afmm2d_rank_model_get_params(afmm2d_rank_model m, float* params) {
+  if (!m || !params) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  return guarded([&] { rank_model_get_params(*m->m, params); });
+}
+
+int afmm2d_rank_model_set_params(afmm2d_rank_model m, const float* params) {
+  if (!m || !params) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  return guarded([&] { rank_model_set_params(*m->m, params); });
+}
+
+int afmm2d_rank_model_predict(afmm2d_rank_model m, const float* x, int64_t n, float* out) {
+  if (!m || n < 0 || (n > 0 && (!x || !out))) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "bad argument");
+  return guarded([&] { rank_model_predict(*m->m, x, n, out); });
+}
+
+int afmm2d_rank_model_gradient(afmm2d_rank_model m, const float* x, const float* y, int n,
+                               float* grad, double* loss) {
+  if (!m || !x || !y || !grad || n <= 0) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "bad argument");
+  return guarded([&] {
+    const double l = rank_model_gradient(*m->m, x, y, n, grad);
+    if (loss) *loss = l;
+  });
+}
+
+int afmm2d_rank_model_train(afmm2d_rank_model m, const float* x, const float* y, int64_t n,
+                            int epochs, int batch, float lr, uint64_t seed, double* loss) {
+  if (!m || !x || !y || n <= 0 || epochs <= 0 || batch <= 0 || !(lr > 0.0f))
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "bad argument");
+  return guarded([&] {
+    const double l = rank_model_train(*m->m, x, y, n, epochs, batch, lr, seed);
+    if (loss) *loss = l;
+  });
+}
+
+int afmm2d_set_rank_model(afmm2d_handle h, afmm2d_rank_model m, float margin) {
+  if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (!std::isfinite(margin)) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "margin not finite");
+  if (h->assembled) return fail(AFMM2D_ERR_STATE, "rank model set after assembly");
+  if (m && m->m->device != h->e->device)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "rank model on another device");
+  for (Engine* e : {h->e.get(), h->e2.get()}) {
+    if (!e) continue;
+    e->rank_model = m ? m->m.get() : nullptr;
+    e->rank_margin = margin;
+  }
+  return AFMM2D_OK;
+}
+
+int afmm2d_get_rank_samples(afmm2d_handle h, float* features, int* ranks, int64_t capacity,
+                            int64_t* count) {
+  if (!h || !count) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!h->assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
+  const Engine& e = *h->e;
+  const int64_t n = (int64_t)e.rank_label.size();
+  *count = n;
+  if (capacity < n) return AFMM2D_OK;
+  if (features) std::memcpy(features, e.rank_feat.data(), sizeof(float) * kRmIn * (size_t)n);
+  if (ranks) std::memcpy(ranks, e.rank_label.data(), sizeof(int) * (size_t)n);
+  return AFMM2D_OK;
+}
+
 int afmm2d_levels(afmm2d_handle h, int* levels) {
   if (!h || !levels) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
   *levels = h->e->L;
@@ -1112,6 +1195,11 @@ int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st) {
   st->near_waves = e.p2p_waves_per_simd;
   st->blocked_m2l_coeffs = e.mr_m_coeffs;
   st->blocked_basis_coeffs = e.mr_v_coeffs;
+  st->rank_boxes = e.rank_boxes;
+  st->rank_fallbacks = e.rank_fallbacks;
+  st->aca_scratch_peak = e.aca_scratch_peak;
+  st->aca_launches = e.aca_launches;
+  st->aca_seconds = e.aca_seconds;
   return AFMM2D_OK;
 }
 

@@ -30,6 +30,8 @@
 // B and +-B^T (the kernel is symmetric, or the antisymmetric part of one).
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
+#include <cmath>
 #include <numeric>
 #include <unordered_map>
 
@@ -101,6 +103,11 @@ struct AcaArgs {
   double* pcy;
   double tol;
   KParams kp;
+  // learned capacities: per batch entry the rank the scratch holds (null:
+  // min(rows, columns)); overflow[box] = 1 when a box reached it before its
+  // stopping test passed (its result is then discarded and recomputed)
+  const int* cap;
+  int* overflow;
 };
 
 template <int KID, int NT>
@@ -125,9 +132,10 @@ __global__ __launch_bounds__(NT) void k_aca(AcaArgs a) {
     if (tid == 0) a.rank[b] = 0;
     return;
   }
+  const int cmax = a.cap ? min(a.cap[blockIdx.x], mn) : mn;
   double* U = a.scr + a.scr_off[blockIdx.x];
-  double* V = U + (int64_t)n1 * mn;
-  double* cx = V + (int64_t)n2 * mn;
+  double* V = U + (int64_t)n1 * cmax;
+  double* cx = V + (int64_t)n2 * cmax;
   double* cy = cx + n2;
   const int r0 = a.rstart[b];
   if (tid == 0) {
@@ -201,8 +209,13 @@ __global__ __launch_bounds__(NT) void k_aca(AcaArgs a) {
   Cand nr = block_argmax(cr, s4);
   istar = nr.i;
   double normS = 0.0, prev = DBL_MAX;
+  bool capped = false;
 
   while (fabs(prev - normS) >= a.tol * prev && rank < mn && istar >= 0) {
+    if (rank >= cmax) {  // the ACA goes on beyond the capacity held
+      capped = true;
+      break;
+    }
     const int t = rank;
     for (int l = tid; l < t; l += NT) coefU[l] = U[(int64_t)l * n1 + istar];
     __syncthreads();
@@ -280,7 +293,10 @@ __global__ __launch_bounds__(NT) void k_aca(AcaArgs a) {
     istar = nr.i;
     __syncthreads();
   }
-  if (tid == 0) a.rank[b] = rank;
+  if (tid == 0) {
+    a.rank[b] = rank;
+    if (a.overflow) a.overflow[b] = capped;
+  }
 }
 
 struct IdArgs {
@@ -448,10 +464,69 @@ void upload(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
   }
 }
 
+double now_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 size_t free_device_bytes() {
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)8 << 30;
   return fr;
+}
+
+struct BBox {
+  double x0 = DBL_MAX, y0 = DBL_MAX, x1 = -DBL_MAX, y1 = -DBL_MAX;
+  void add(const BBox& o) {
+    x0 = std::min(x0, o.x0);
+    y0 = std::min(y0, o.y0);
+    x1 = std::max(x1, o.x1);
+    y1 = std::max(y1, o.y1);
+  }
+  bool empty() const { return x0 > x1; }
+};
+
+// The rank model's inputs for box k of level j (kRmIn floats): level, the
+// half-width of the box's point bounding box (log2), its centre relative to
+// the domain's, the kernel (one-hot: log, 1/r, Gaussian, a/r, user), log10
+// of the tolerance, the ACA block's rows and columns (log2), the share of
+// interaction slots present, the kernel's length parameter (log10) and
+// whether the box is a leaf.
+void rank_features(const Engine& e, int j, int n1, int n2, int nil, const BBox& bb,
+                   const BBox& dom, float* f) {
+  for (int u = 0; u < kRmIn; u++) f[u] = 0.0f;
+  const double dcx = 0.5 * (dom.x0 + dom.x1), dcy = 0.5 * (dom.y0 + dom.y1);
+  const double dhw = std::max(1e-300, 0.5 * std::max(dom.x1 - dom.x0, dom.y1 - dom.y0));
+  double hw = 0.0, cx = dcx, cy = dcy;
+  if (!bb.empty()) {
+    hw = 0.5 * std::max(bb.x1 - bb.x0, bb.y1 - bb.y0);
+    cx = 0.5 * (bb.x0 + bb.x1);
+    cy = 0.5 * (bb.y0 + bb.y1);
+  }
+  f[0] = (float)(j / 10.0);
+  f[1] = (float)(std::log2(std::max(hw, 1e-12)) / 10.0);
+  f[2] = (float)((cx - dcx) / dhw);
+  f[3] = (float)((cy - dcy) / dhw);
+  const int k = e.kid_full >= K_USER ? 4 : e.kid_full;
+  f[4 + k] = 1.0f;
+  f[9] = (float)(-e.tol_pow / 10.0);
+  f[10] = (float)(std::log2(1.0 + n1) / 10.0);
+  f[11] = (float)(std::log2(1.0 + n2) / 16.0);
+  f[12] = (float)(nil / 40.0);
+  if (e.kid_full == K_GAUSS) f[13] = (float)(std::log10(e.kp.p0) / 4.0);
+  if (e.kid_full == K_REG_INV_R) f[13] = (float)(std::log10(e.kp.a) / 4.0);
+  f[14] = j == e.L ? 1.0f : 0.0f;
+}
+
+// ACA capacity from a prediction of log2(1 + rank): 2^(pred + margin) - 1
+// rounded up to a bucket (multiples of 8 to 64, of 16 to 128, of 32 beyond),
+// at least 4, at most min(rows, columns).
+int rank_capacity(float pred, float margin, int mn) {
+  const double r = std::exp2(std::min(20.0, (double)pred + margin)) - 1.0;
+  int c = std::max(4, (int)std::ceil(r));
+  if (c <= 64) c = (c + 7) / 8 * 8;
+  else if (c <= 128) c = (c + 15) / 16 * 16;
+  else c = (c + 31) / 32 * 32;
+  return std::min(c, mn);
 }
 
 }  // namespace
@@ -466,7 +541,25 @@ void assemble_tree(Engine& e) {
   std::vector<int64_t> nrows(L + 1, 0);  // source rows of each level
   std::vector<std::vector<int64_t>> sloc(L + 2);
   DevBuf d_rstart, d_n1, d_n2, d_il, d_boxes, d_scroff, d_scr, d_pivoff, d_rank, d_prow, d_pcx,
-      d_pcy, d_sloc, d_l2poff, d_moff, d_mscr;
+      d_pcy, d_sloc, d_l2poff, d_moff, d_mscr, d_cap, d_over, d_feat, d_pred;
+  // point bounding boxes: leaves from their points, parents from children
+  std::vector<BBox> bbox(1 << (2 * L));
+  BBox dom;
+  for (int k = 0; k < (1 << (2 * L)); k++) {
+    for (int i = e.leaf_start[k]; i < e.leaf_start[k] + e.leaf_count[k]; i++) {
+      const double px = e.h_xy[2 * (size_t)i], py = e.h_xy[2 * (size_t)i + 1];
+      bbox[k].x0 = std::min(bbox[k].x0, px);
+      bbox[k].x1 = std::max(bbox[k].x1, px);
+      bbox[k].y0 = std::min(bbox[k].y0, py);
+      bbox[k].y1 = std::max(bbox[k].y1, py);
+    }
+    dom.add(bbox[k]);
+  }
+  e.rank_feat.clear();
+  e.rank_label.clear();
+  e.rank_boxes = e.rank_fallbacks = e.aca_scratch_peak = 0;
+  e.aca_launches = 0;
+  e.aca_seconds = 0.0;
 
   for (int j = L; j >= 2; j--) {
     LevelInfo& li = e.lv[j];
@@ -532,46 +625,108 @@ void assemble_tree(Engine& e) {
     d_pcx.alloc(sizeof(double) * npiv);
     d_pcy.alloc(sizeof(double) * npiv);
 
+    // bounding boxes of this level (children -> parents above the leaves)
+    if (j < L) {
+      std::vector<BBox> up(nb);
+      for (int k = 0; k < nb; k++)
+        for (int c = 0; c < 4; c++) up[k].add(bbox[4 * k + c]);
+      bbox.swap(up);
+    }
+    const size_t f0 = e.rank_feat.size();
+    e.rank_feat.resize(f0 + (size_t)nb * kRmIn);
+    for (int k = 0; k < nb; k++) {
+      int nil = 0;
+      for (int q = 0; q < 40; q++) nil += il[40 * k + q] >= 0;
+      rank_features(e, j, n1[k], n2[k], nil, bbox[k], dom, &e.rank_feat[f0 + (size_t)k * kRmIn]);
+    }
+    const double t_aca = now_seconds();
+    // capacities: min(rows, columns), or the rank model's bucketed prediction
+    std::vector<int> cap(nb);
+    for (int k = 0; k < nb; k++) cap[k] = std::max(0, std::min(n1[k], n2[k]));
+    std::vector<int> order(nb);
+    std::iota(order.begin(), order.end(), 0);
+    const bool learned = e.rank_model != nullptr;
+    if (learned) {
+      d_feat.alloc(sizeof(float) * (size_t)nb * kRmIn);
+      d_pred.alloc(sizeof(float) * (size_t)nb);
+      AFMM_HIP(hipMemcpyAsync(d_feat.p, &e.rank_feat[f0], sizeof(float) * (size_t)nb * kRmIn,
+                              hipMemcpyHostToDevice, s));
+      rank_model_predict_device(*e.rank_model, d_feat.as<float>(), nb, d_pred.as<float>(), s);
+      std::vector<float> pred(nb);
+      AFMM_HIP(hipMemcpyAsync(pred.data(), d_pred.p, sizeof(float) * nb, hipMemcpyDeviceToHost, s));
+      AFMM_HIP(hipStreamSynchronize(s));
+      for (int k = 0; k < nb; k++)
+        if (cap[k] > 0) cap[k] = rank_capacity(pred[k], e.rank_margin, cap[k]);
+      // buckets: boxes of equal capacity run side by side
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cap[a] < cap[b]; });
+      d_over.alloc(sizeof(int) * nb);
+      AFMM_HIP(hipMemsetAsync(d_over.p, 0, sizeof(int) * nb, s));
+    }
     // ACA in batches bounded by a scratch budget of at most 16 GiB, one
     // buffer reused by every batch and level: a level's whole scratch at once
     // (24-55 GB at N = 1e6) made assembly take 4-5 s instead of ~1 s in some
     // runs; 4 GiB batches left too few boxes per launch at the leaf level.
     const size_t budget = std::min<size_t>((size_t)16 << 30,
                                            std::max<size_t>((size_t)1 << 30, free_device_bytes() / 4));
-    int k0 = 0;
-    while (k0 < nb) {
-      std::vector<int> boxes;
-      std::vector<int64_t> soffs;
-      int64_t tot = 0;
-      int k = k0;
-      for (; k < nb; k++) {
-        int mn = std::max(0, std::min(n1[k], n2[k]));
-        int64_t need = (int64_t)(n1[k] + n2[k]) * mn + 2 * (int64_t)n2[k];
-        need = (need + 31) & ~(int64_t)31;
-        if (!boxes.empty() && (size_t)(tot + need) * sizeof(double) > budget) break;
-        boxes.push_back(k);
-        soffs.push_back(tot);
-        tot += need;
+    // boxes: the order to run; capped: hold cap[] (else full capacity)
+    auto run_aca = [&](const std::vector<int>& boxes_in, bool capped) {
+      size_t b0 = 0;
+      while (b0 < boxes_in.size()) {
+        std::vector<int> boxes, caps;
+        std::vector<int64_t> soffs;
+        int64_t tot = 0;
+        size_t q = b0;
+        for (; q < boxes_in.size(); q++) {
+          const int k = boxes_in[q];
+          const int mn = std::max(0, std::min(n1[k], n2[k]));
+          const int c = capped ? cap[k] : mn;
+          int64_t need = (int64_t)(n1[k] + n2[k]) * c + 2 * (int64_t)n2[k];
+          need = (need + 31) & ~(int64_t)31;
+          if (!boxes.empty() && (size_t)(tot + need) * sizeof(double) > budget) break;
+          boxes.push_back(k);
+          caps.push_back(c);
+          soffs.push_back(tot);
+          tot += need;
+        }
+        b0 = q;
+        upload(d_boxes, boxes, s);
+        upload(d_scroff, soffs, s);
+        if (capped) upload(d_cap, caps, s);
+        d_scr.alloc(sizeof(double) * std::max<int64_t>(tot, 1));
+        e.aca_scratch_peak = std::max<int64_t>(e.aca_scratch_peak, 8 * tot);
+        AcaArgs a{src_x, src_y, d_rstart.as<int>(), d_n1.as<int>(), d_n2.as<int>(), d_il.as<int>(),
+                  d_boxes.as<int>(), d_scroff.as<int64_t>(), d_scr.as<double>(),
+                  d_pivoff.as<int64_t>(), d_rank.as<int>(), d_prow.as<int>(), d_pcx.as<double>(),
+                  d_pcy.as<double>(), e.tol, e.kp, capped ? d_cap.as<int>() : nullptr,
+                  capped ? d_over.as<int>() : nullptr};
+        if (aca_nt == 1024) {
+          AFMM_DISPATCH_KERNEL(e.kid, KID, k_aca<KID, 1024><<<(unsigned)boxes.size(), 1024, 0, s>>>(a));
+        } else {
+          AFMM_DISPATCH_KERNEL(e.kid, KID,
+                               k_aca<KID, ACA_THREADS><<<(unsigned)boxes.size(), ACA_THREADS, 0, s>>>(a));
+        }
+        AFMM_LAUNCH_CHECK();
+        e.aca_launches++;
       }
-      k0 = k;
-      upload(d_boxes, boxes, s);
-      upload(d_scroff, soffs, s);
-      d_scr.alloc(sizeof(double) * std::max<int64_t>(tot, 1));
-      AcaArgs a{src_x, src_y, d_rstart.as<int>(), d_n1.as<int>(), d_n2.as<int>(), d_il.as<int>(),
-                d_boxes.as<int>(), d_scroff.as<int64_t>(), d_scr.as<double>(),
-                d_pivoff.as<int64_t>(), d_rank.as<int>(), d_prow.as<int>(), d_pcx.as<double>(),
-                d_pcy.as<double>(), e.tol, e.kp};
-      if (aca_nt == 1024) {
-        AFMM_DISPATCH_KERNEL(e.kid, KID, k_aca<KID, 1024><<<(unsigned)boxes.size(), 1024, 0, s>>>(a));
-      } else {
-        AFMM_DISPATCH_KERNEL(e.kid, KID,
-                             k_aca<KID, ACA_THREADS><<<(unsigned)boxes.size(), ACA_THREADS, 0, s>>>(a));
-      }
-      AFMM_LAUNCH_CHECK();
+    };
+    run_aca(order, learned);
+    if (learned) {
+      // boxes whose ACA went beyond the predicted capacity run again in full
+      std::vector<int> over(nb);
+      AFMM_HIP(hipMemcpyAsync(over.data(), d_over.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+      AFMM_HIP(hipStreamSynchronize(s));
+      std::vector<int> redo;
+      for (int k = 0; k < nb; k++)
+        if (over[k]) redo.push_back(k);
+      e.rank_fallbacks += (int64_t)redo.size();
+      run_aca(redo, false);
     }
+    for (int k = 0; k < nb; k++) e.rank_boxes += std::min(n1[k], n2[k]) > 0;
     li.rank.resize(nb);
     AFMM_HIP(hipMemcpyAsync(li.rank.data(), d_rank.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
     AFMM_HIP(hipStreamSynchronize(s));
+    e.aca_seconds += now_seconds() - t_aca;
+    e.rank_label.insert(e.rank_label.end(), li.rank.begin(), li.rank.end());
     li.nin = n1;
     // local skeleton offsets and L2P offsets
     sloc[j].assign(nb + 1, 0);

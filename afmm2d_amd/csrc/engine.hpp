@@ -272,6 +272,37 @@ struct MrBatchInfo {
 };
 MrBatchInfo mr_batch_info(const MrBatch* b);
 
+// Learned rank predictor: a 16-64-64-64-1 FP32 MLP (ReLU hidden layers)
+// predicting log2(1 + ACA rank) of a box; trained and evaluated by the
+// library's own kernels (rank_model.hip).
+constexpr int kRmIn = 16, kRmHid = 64;
+constexpr int kRmParams = kRmHid * kRmIn + kRmHid + 2 * (kRmHid * kRmHid + kRmHid) + kRmHid + 1;
+struct RankModel {
+  int device = 0;
+  hipStream_t s = nullptr;
+  DevBuf params, grads, adam_m, adam_v;
+  DevBuf act[4], out, yb, dout, dact[2], loss_part, wpart;  // per-batch activations
+  int cap = 0;                                               // samples they hold
+  int64_t step = 0;                                          // Adam steps taken
+  RankModel() = default;
+  RankModel(const RankModel&) = delete;
+  RankModel& operator=(const RankModel&) = delete;
+  ~RankModel();
+};
+RankModel* rank_model_create(int device, uint64_t seed);
+void rank_model_set_params(RankModel& m, const float* p);
+void rank_model_get_params(RankModel& m, float* p);
+// outputs of n samples (n x kRmIn row-major), device arrays, on stream s
+void rank_model_predict_device(RankModel& m, const float* d_x, int64_t n, float* d_out,
+                               hipStream_t s);
+void rank_model_predict(RankModel& m, const float* x, int64_t n, float* out);
+// mean squared error of n host samples and its gradient (kRmParams floats)
+double rank_model_gradient(RankModel& m, const float* x, const float* y, int n, float* grad);
+// Adam over shuffled minibatches (a partial last batch is dropped); returns
+// the last epoch's mean squared error
+double rank_model_train(RankModel& m, const float* x, const float* y, int64_t n, int epochs,
+                        int batch, float lr, uint64_t seed);
+
 struct Engine {
   // problem
   int n = 0, leaf = 0, tol_pow = 0, L = 0, kid = 0, device = 0;
@@ -281,6 +312,16 @@ struct Engine {
   int kid_full = 0;
   double tol = 0.0;
   KParams kp{0.01, 0.0};
+
+  // learned ACA capacities (null: every box reserves min(rows, columns))
+  RankModel* rank_model = nullptr;
+  float rank_margin = 0.5f;  // log2 units added to the prediction
+  // assembly record: per box (levels L..2) its features and ACA rank
+  std::vector<float> rank_feat;
+  std::vector<int> rank_label;
+  int64_t rank_boxes = 0, rank_fallbacks = 0, aca_scratch_peak = 0;
+  int aca_launches = 0;
+  double aca_seconds = 0.0;
 
   // s_aux: set-up work and the near field; s_tree: the upper tree in the
   // schedules that run it beside the leaf-level M2L (created on first use)

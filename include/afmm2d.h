@@ -100,6 +100,11 @@ typedef struct {
   int64_t blocked_m2l_coeffs;   /* blocked passes: M2L row-block coefficients, sum rank_k K_k
                                    (each interaction block stored for both of its boxes) */
   int64_t blocked_basis_coeffs; /* blocked passes: basis coefficients, sum nin_k rank_k */
+  int64_t rank_boxes;           /* boxes with an ACA block (min(rows, columns) > 0) */
+  int64_t rank_fallbacks;       /* of them, rerun at full capacity (learned capacities) */
+  int64_t aca_scratch_peak;     /* largest ACA scratch of one launch (bytes) */
+  int64_t aca_launches;         /* ACA launches of the assembly */
+  double aca_seconds;           /* ACA phase of the assembly (incl. capacity prediction) */
 } afmm2d_stats;
 
 /* Message of the calling thread's last failure ("" if none). */
@@ -207,6 +212,52 @@ int afmm2d_batch_capture(afmm2d_batch b, int on);
 
 int afmm2d_batch_get_info(afmm2d_batch b, afmm2d_batch_info* info);
 int afmm2d_batch_destroy(afmm2d_batch b);
+
+/* ---- Learned ACA capacities -------------------------------------------
+ * The reference's ACA (ACA.cpp:27-133, called per box at
+ * AFMM2DTree.cpp:1009-1041) holds min(rows, columns) factor columns per box.
+ * A rank model -- a 16-64-64-64-1 FP32 MLP with ReLU hidden layers, trained
+ * and evaluated by the library's own kernels -- predicts log2(1 + rank) of
+ * each box from AFMM2D_RANK_FEATURES inputs (level, point bounding-box
+ * half-width and centre, kernel, tolerance, block shape; see
+ * afmm2d_get_rank_samples).  With a model set, assembly sizes each box's ACA
+ * scratch to its bucketed prediction and groups boxes of equal capacity;
+ * the ACA still decides every rank, and a box that reaches its capacity
+ * before its stopping test passes runs again at full capacity, so ranks,
+ * skeletons and results equal the plain assembly's.  No reference
+ * counterpart. */
+#define AFMM2D_RANK_FEATURES 16
+#define AFMM2D_RANK_PARAMS 9473
+typedef struct afmm2d_rank_model_s* afmm2d_rank_model;
+
+/* He-uniform weights from seed, zero biases, on device (< 0: current). */
+int afmm2d_rank_model_create(int device, uint64_t seed, afmm2d_rank_model* out);
+int afmm2d_rank_model_destroy(afmm2d_rank_model m);
+/* AFMM2D_RANK_PARAMS floats: W1 (64x16 row-major), b1, W2 (64x64), b2, W3,
+ * b3, W4 (1x64), b4.  Setting them clears the optimizer state. */
+int afmm2d_rank_model_get_params(afmm2d_rank_model m, float* params);
+int afmm2d_rank_model_set_params(afmm2d_rank_model m, const float* params);
+/* out[i] = prediction for features x[i * AFMM2D_RANK_FEATURES ...]. */
+int afmm2d_rank_model_predict(afmm2d_rank_model m, const float* x, int64_t n, float* out);
+/* Mean squared error of (x, y) and its gradient (AFMM2D_RANK_PARAMS floats). */
+int afmm2d_rank_model_gradient(afmm2d_rank_model m, const float* x, const float* y, int n,
+                               float* grad, double* loss);
+/* Adam (beta 0.9, 0.999, eps 1e-8) over minibatches of a seeded shuffle,
+ * epochs passes; loss: the last epoch's mean squared error (may be NULL). */
+int afmm2d_rank_model_train(afmm2d_rank_model m, const float* x, const float* y, int64_t n,
+                            int epochs, int batch, float lr, uint64_t seed, double* loss);
+
+/* Assemble h with capacities from model m (NULL: the plain ACA), each
+ * 2^(prediction + margin) - 1 bucketed.  The model must outlive the
+ * assembly; models and handles are not thread-safe. */
+int afmm2d_set_rank_model(afmm2d_handle h, afmm2d_rank_model m, float margin);
+
+/* After assembly: one sample per box of levels L..2 (level L first, boxes in
+ * quad-tree order): its AFMM2D_RANK_FEATURES features and its ACA rank.
+ * count = the number of samples; nothing is written when capacity is
+ * smaller (features and ranks may be NULL to query the count). */
+int afmm2d_get_rank_samples(afmm2d_handle h, float* features, int* ranks, int64_t capacity,
+                            int64_t* count);
 
 /* nLevels (AFMM2D.hpp:20). */
 int afmm2d_levels(afmm2d_handle h, int* levels);
