@@ -51,6 +51,7 @@ EXPORTS = [
     "afmm2d_rank_model_create", "afmm2d_rank_model_destroy", "afmm2d_rank_model_get_params",
     "afmm2d_rank_model_set_params", "afmm2d_rank_model_predict", "afmm2d_rank_model_gradient",
     "afmm2d_rank_model_train", "afmm2d_set_rank_model", "afmm2d_get_rank_samples",
+    "afmm2d_set_m2l_factoring", "afmm2d_get_m2l_storage",
 ]
 
 RANK_FEATURES = 16
@@ -82,6 +83,8 @@ class Stats(ctypes.Structure):
         ("rank_boxes", ctypes.c_int64), ("rank_fallbacks", ctypes.c_int64),
         ("aca_scratch_peak", ctypes.c_int64), ("aca_launches", ctypes.c_int64),
         ("aca_seconds", ctypes.c_double),
+        ("m2l_factored_pairs", ctypes.c_int64), ("m2l_factored_bytes", ctypes.c_int64),
+        ("m2l_factored_dense_bytes", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -170,6 +173,8 @@ def lib():
                                         ctypes.c_uint64, _dp]),
         "afmm2d_set_rank_model": (i, [_vp, _vp, ctypes.c_float]),
         "afmm2d_get_rank_samples": (i, [_vp, _fp, _ip, ctypes.c_int64, _lp]),
+        "afmm2d_set_m2l_factoring": (i, [_vp, i]),
+        "afmm2d_get_m2l_storage": (i, [_vp, i, i, i, _ip, _ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -391,6 +396,17 @@ class AFMM:
         _check(st)
         return out[: rows.value * cols.value].reshape(rows.value, cols.value).copy()
 
+    def m2l_storage(self, level, box, slot):
+        """(factor rank or -1 for a dense block, whether the stored block is
+        K(skeleton_box, skeleton_slot)), or None for an empty slot."""
+        p, first = ctypes.c_int(), ctypes.c_int()
+        st = lib().afmm2d_get_m2l_storage(self._h, level, box, slot, ctypes.byref(p),
+                                          ctypes.byref(first))
+        if st == ERR_NOT_FOUND:
+            return None
+        _check(st)
+        return p.value, bool(first.value)
+
     def getMatrixEntry(self, i, j):
         """kernel::getMatrixEntry on KD-sorted indices (kernel.hpp:23-26)."""
         v = ctypes.c_double()
@@ -420,6 +436,10 @@ class AFMM:
         _check(lib().afmm2d_direct_rows(self._h, rows.shape[0], rows.ctypes.data_as(_lp), _d(b),
                                         _d(out)))
         return out
+
+    def set_m2l_factoring(self, on=True):
+        """Store compressible M2L blocks as ACA factors (before assemble)."""
+        _check(lib().afmm2d_set_m2l_factoring(self._h, 1 if on else 0))
 
     def set_rank_model(self, model, margin=0.5):
         """Assemble with ACA capacities from model (None: the plain ACA)."""

@@ -859,6 +859,14 @@ int afmm2d_rank_model_train(afmm2d_rank_model m, const float* x, const float* y,
   });
 }
 
+int afmm2d_set_m2l_factoring(afmm2d_handle h, int on) {
+  if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (h->assembled) return fail(AFMM2D_ERR_STATE, "M2L factoring set after assembly");
+  for (Engine* e : {h->e.get(), h->e2.get()})
+    if (e) e->m2l_factor = on != 0;
+  return AFMM2D_OK;
+}
+
 int afmm2d_set_rank_model(afmm2d_handle h, afmm2d_rank_model m, float margin) {
   if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL handle");
   if (!std::isfinite(margin)) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "margin not finite");
@@ -983,6 +991,26 @@ int afmm2d_get_l2p(afmm2d_handle h, int level, int box, double* out, int64_t cap
   });
 }
 
+int afmm2d_get_m2l_storage(afmm2d_handle h, int level, int box, int slot, int* factor_rank,
+                           int* box_first) {
+  if (!h || !factor_rank || !box_first) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  Engine& e = *h->e;
+  if (!h->assembled) return fail(AFMM2D_ERR_STATE, "not assembled");
+  if (level < 2 || level > e.L || box < 0 || box >= e.lv[level].nb || slot < 0 || slot >= 40)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "level/box/slot out of range");
+  const LevelInfo& li = e.lv[level];
+  auto slot_box = [&](int s) {
+    return s < 16 ? e.inner[level][16 * (size_t)box + s] : e.outer[level][24 * (size_t)box + s - 16];
+  };
+  if (slot_box(slot) < 0) return fail(AFMM2D_ERR_NOT_FOUND, "empty interaction slot");
+  int t = e.h_red_start[li.gbase + box];
+  for (int q = 0; q < slot; q++) t += slot_box(q) >= 0;
+  const int p = e.h_red_pair[t];
+  *factor_rank = e.h_pair_fact.empty() ? -1 : e.h_pair_fact[p];
+  *box_first = e.h_pair_k[p] == (int)(li.gbase + box);
+  return AFMM2D_OK;
+}
+
 int afmm2d_get_m2l(afmm2d_handle h, int level, int box, int slot, double* out, int64_t capacity,
                    int* rows, int* cols) {
   if (!h || !rows || !cols) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -1010,11 +1038,29 @@ int afmm2d_get_m2l(afmm2d_handle h, int level, int box, int slot, double* out, i
     // first box's rank rounded up to even as leading dimension
     const bool first = e.h_pair_k[p] == (int)(li.gbase + box);
     const int rows_lo = first ? r : rc, cols_lo = first ? rc : r;
-    const int ld = rows_lo + (rows_lo & 1);
+    int ld = rows_lo + (rows_lo & 1);
     std::vector<double> blk((size_t)ld * cols_lo);
-    if (!blk.empty())
+    const int fp = e.h_pair_fact.empty() ? -1 : e.h_pair_fact[p];
+    if (fp >= 0) {
+      // stored as U (rows_lo x fp) and V (cols_lo x fp): the block is U V^T
+      ld = rows_lo;
+      std::vector<double> uv((size_t)fp * (rows_lo + cols_lo));
+      if (!uv.empty())
+        AFMM_HIP(hipMemcpy(uv.data(), e.m2lf.as<double>() + e.h_pair_off[p],
+                           sizeof(double) * uv.size(), hipMemcpyDeviceToHost));
+      const double* U = uv.data();
+      const double* V = U + (size_t)fp * rows_lo;
+      std::fill(blk.begin(), blk.end(), 0.0);
+      for (int c = 0; c < cols_lo; c++)
+        for (int i = 0; i < rows_lo; i++) {
+          double acc = 0.0;
+          for (int l = 0; l < fp; l++) acc += U[i + (size_t)l * rows_lo] * V[c + (size_t)l * cols_lo];
+          blk[(size_t)c * ld + i] = acc;
+        }
+    } else if (!blk.empty()) {
       AFMM_HIP(hipMemcpy(blk.data(), e.m2l.as<double>() + e.h_pair_off[p],
                          sizeof(double) * blk.size(), hipMemcpyDeviceToHost));
+    }
     for (int i = 0; i < r; i++)
       for (int c = 0; c < rc; c++)
         out[(size_t)i * rc + c] = first ? blk[(size_t)c * ld + i] : blk[(size_t)i * ld + c];
@@ -1200,6 +1246,9 @@ int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st) {
   st->aca_scratch_peak = e.aca_scratch_peak;
   st->aca_launches = e.aca_launches;
   st->aca_seconds = e.aca_seconds;
+  st->m2l_factored_pairs = e.m2lf_pairs;
+  st->m2l_factored_bytes = 8 * e.m2lf_total;
+  st->m2l_factored_dense_bytes = 8 * e.m2lf_dense;
   return AFMM2D_OK;
 }
 

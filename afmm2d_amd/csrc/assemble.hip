@@ -443,6 +443,7 @@ __global__ __launch_bounds__(64) void k_m2l_pair_build(
     const int64_t* __restrict__ soff, const int* __restrict__ pk, const int* __restrict__ pb,
     const int64_t* __restrict__ poff, double* __restrict__ m2l, KParams kp) {
   const int64_t p = blockIdx.x;
+  if (poff[p] < 0) return;  // stored as ACA factors
   const int k = pk[p], b = pb[p];
   const int rk = rank[k], rb = rank[b];
   const int64_t sk = soff[k], sb = soff[b];
@@ -527,6 +528,129 @@ int rank_capacity(float pred, float margin, int mn) {
   else if (c <= 128) c = (c + 15) / 16 * 16;
   else c = (c + 31) / 32 * 32;
   return std::min(c, mn);
+}
+
+// Factors of pair entry e of an ACA batch (its U: n1 x p at src, its V:
+// n2 x p at src + n1 cap) to dst (U then V, column-major).
+struct FactCopy {
+  int64_t src, dst;
+  int n1, n2, p, cap;
+};
+
+__global__ __launch_bounds__(256) void k_copy_factors(const FactCopy* __restrict__ fc,
+                                                      const double* __restrict__ scr,
+                                                      double* __restrict__ dst) {
+  const FactCopy f = fc[blockIdx.x];
+  const int64_t nu = (int64_t)f.n1 * f.p, nv = (int64_t)f.n2 * f.p;
+  for (int64_t t = threadIdx.x; t < nu; t += blockDim.x) dst[f.dst + t] = scr[f.src + t];
+  const double* v = scr + f.src + (int64_t)f.n1 * f.cap;
+  for (int64_t t = threadIdx.x; t < nv; t += blockDim.x) dst[f.dst + nu + t] = v[t];
+}
+
+// Per-pair ACA of the M2L blocks K(skeleton_first, skeleton_second): the
+// box ACA kernel run on the global skeleton points, one entry per pair
+// whose only interaction slot is a column entry for the second box.
+// p[i] = the factorisation's rank for pairs stored factored (p (r1 + r2) <
+// r1 r2 and p <= 256), -1 otherwise; foff[i] = their offsets in e.m2lf.
+void pair_aca(Engine& e, const std::vector<int>& first, const std::vector<int>& second,
+              const std::vector<int>& rank_g, const std::vector<int64_t>& soff_g,
+              std::vector<int>& p, std::vector<int64_t>& foff) {
+  hipStream_t s = e.s_aux;
+  const int P = (int)first.size();
+  p.assign(P, -1);
+  foff.assign(P, -1);
+  if (P == 0) return;
+  if (e.skel_total > INT32_MAX) throw Error(6, "skeleton count exceeds 2^31");
+  std::vector<int> rstart(2 * (size_t)P), n1(2 * (size_t)P), n2(2 * (size_t)P, 0),
+      il(40 * (size_t)P, -1);
+  std::vector<int64_t> pivoff(P + 1, 0);
+  for (int i = 0; i < P; i++) {
+    rstart[i] = (int)soff_g[first[i]];
+    n1[i] = rank_g[first[i]];
+    rstart[P + i] = (int)soff_g[second[i]];
+    n1[P + i] = rank_g[second[i]];
+    n2[i] = rank_g[second[i]];
+    il[40 * (size_t)i] = P + i;
+    pivoff[i + 1] = pivoff[i] + std::min(n1[i], n2[i]);
+  }
+  DevBuf d_rstart, d_n1, d_n2, d_il, d_pivoff, d_rank, d_prow, d_pcx, d_pcy, d_boxes, d_scroff,
+      d_scr, d_fc;
+  upload(d_rstart, rstart, s);
+  upload(d_n1, n1, s);
+  upload(d_n2, n2, s);
+  upload(d_il, il, s);
+  upload(d_pivoff, pivoff, s);
+  const int64_t npiv = std::max<int64_t>(pivoff[P], 1);
+  d_rank.alloc(sizeof(int) * P);
+  d_prow.alloc(sizeof(int) * npiv);
+  d_pcx.alloc(sizeof(double) * npiv);
+  d_pcy.alloc(sizeof(double) * npiv);
+  const size_t budget = std::min<size_t>((size_t)4 << 30,
+                                         std::max<size_t>((size_t)1 << 28, free_device_bytes() / 8));
+  std::vector<DevBuf> parts;           // factors of each batch
+  std::vector<int64_t> part_base;      // their offsets in m2lf
+  int64_t ftot = 0;
+  int i0 = 0;
+  while (i0 < P) {
+    std::vector<int> boxes;
+    std::vector<int64_t> soffs;
+    int64_t tot = 0;
+    int i = i0;
+    for (; i < P; i++) {
+      const int mn = std::min(n1[i], n2[i]);
+      int64_t need = (int64_t)(n1[i] + n2[i]) * mn + 2 * (int64_t)n2[i];
+      need = (need + 31) & ~(int64_t)31;
+      if (!boxes.empty() && (size_t)(tot + need) * sizeof(double) > budget) break;
+      boxes.push_back(i);
+      soffs.push_back(tot);
+      tot += need;
+    }
+    i0 = i;
+    upload(d_boxes, boxes, s);
+    upload(d_scroff, soffs, s);
+    d_scr.alloc(sizeof(double) * std::max<int64_t>(tot, 1));
+    AcaArgs a{e.skx.as<double>(), e.sky.as<double>(), d_rstart.as<int>(), d_n1.as<int>(),
+              d_n2.as<int>(), d_il.as<int>(), d_boxes.as<int>(), d_scroff.as<int64_t>(),
+              d_scr.as<double>(), d_pivoff.as<int64_t>(), d_rank.as<int>(), d_prow.as<int>(),
+              d_pcx.as<double>(), d_pcy.as<double>(), e.tol, e.kp, nullptr, nullptr};
+    AFMM_DISPATCH_KERNEL(e.kid, KID,
+                         k_aca<KID, ACA_THREADS><<<(unsigned)boxes.size(), ACA_THREADS, 0, s>>>(a));
+    AFMM_LAUNCH_CHECK();
+    std::vector<int> rk(P);
+    AFMM_HIP(hipMemcpyAsync(rk.data(), d_rank.p, sizeof(int) * P, hipMemcpyDeviceToHost, s));
+    AFMM_HIP(hipStreamSynchronize(s));
+    std::vector<FactCopy> fc;
+    int64_t local = 0;
+    for (size_t u = 0; u < boxes.size(); u++) {
+      const int q = boxes[u];
+      const int r1 = n1[q], r2 = n2[q], pr = rk[q];
+      if (pr > 256 || (int64_t)pr * (r1 + r2) >= (int64_t)r1 * r2) continue;
+      p[q] = pr;
+      foff[q] = ftot + local;
+      fc.push_back(FactCopy{soffs[u], local, r1, r2, pr, std::min(r1, r2)});
+      local += (int64_t)pr * (r1 + r2);
+    }
+    if (!fc.empty()) {
+      parts.emplace_back();
+      parts.back().alloc(sizeof(double) * std::max<int64_t>(local, 1));
+      part_base.push_back(ftot);
+      upload(d_fc, fc, s);
+      k_copy_factors<<<(unsigned)fc.size(), 256, 0, s>>>(d_fc.as<FactCopy>(), d_scr.as<double>(),
+                                                         parts.back().as<double>());
+      AFMM_LAUNCH_CHECK();
+      AFMM_HIP(hipStreamSynchronize(s));  // the next batch reuses the scratch
+      ftot += local;
+    }
+  }
+  e.m2lf.alloc(sizeof(double) * std::max<int64_t>(ftot, 1));
+  for (size_t u = 0; u < parts.size(); u++) {
+    const int64_t n = (u + 1 < part_base.size() ? part_base[u + 1] : ftot) - part_base[u];
+    if (n > 0)
+      AFMM_HIP(hipMemcpyAsync(e.m2lf.as<double>() + part_base[u], parts[u].p, sizeof(double) * n,
+                              hipMemcpyDeviceToDevice, s));
+  }
+  AFMM_HIP(hipStreamSynchronize(s));
+  e.m2lf_total = ftot;
 }
 
 }  // namespace
@@ -902,6 +1026,30 @@ void assemble_tree(Engine& e) {
       }
     }
   }
+  // ACA-factored pairs (opt-in; symmetric kernels: the pair's other
+  // direction applies V U^T)
+  std::vector<int> fact_p(cand.size(), -1);
+  std::vector<int64_t> fact_off(cand.size(), -1);
+  e.m2lf_pairs = e.m2lf_total = e.m2lf_dense = 0;
+  e.m2lf.release();
+  if (e.m2l_factor && e.kid != K_USER_ANTI && !cand.empty()) {
+    std::vector<int> fi(cand.size()), se(cand.size());
+    for (size_t i = 0; i < cand.size(); i++) {
+      fi[i] = cand[i].first;
+      se[i] = cand[i].second;
+    }
+    pair_aca(e, fi, se, rank_g, soff_g, fact_p, fact_off);
+    for (size_t i = 0; i < cand.size(); i++) {
+      if (fact_p[i] < 0) continue;
+      const int r1 = rank_g[cand[i].first], r2 = rank_g[cand[i].second];
+      const int64_t dense = (int64_t)r1 * r2, stored = (int64_t)fact_p[i] * (r1 + r2);
+      cand[i].cls = kM2LFactored;
+      e.m2lf_pairs++;
+      e.m2lf_dense += dense;
+      e.m2l_total += stored - dense;
+      if (cand[i].group == 1) e.lv[L].m2l_coeffs += stored - dense;  // the leaf launch's bytes
+    }
+  }
   std::vector<int> order(cand.size());
   for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int u, int v) {
@@ -909,7 +1057,9 @@ void assemble_tree(Engine& e) {
                                           : cand[u].cls < cand[v].cls;
   });
   for (int g = 0; g < 2; g++)
-    for (int c = 0; c <= kM2LAllClasses; c++) e.m2l_cls_begin[g][c] = e.m2l_cls_end[g][c] = 0;
+    for (int c = 0; c <= kM2LFactored; c++) e.m2l_cls_begin[g][c] = e.m2l_cls_end[g][c] = 0;
+  e.h_pair_fact.clear();
+  std::vector<int64_t> build_off;  // per pair: dense block offset, -1 when factored
   for (size_t i = 0; i < order.size(); i++) {
     const Cand& cd = cand[order[i]];
     const int qidx = (int)i;
@@ -921,7 +1071,15 @@ void assemble_tree(Engine& e) {
     pair_of[((int64_t)lo << 32) | hi] = qidx;
     e.h_pair_k.push_back(cd.first);
     e.h_pair_b.push_back(cd.second);
+    const int fp = fact_p[order[i]];
+    e.h_pair_fact.push_back(fp);
+    if (fp >= 0) {
+      e.h_pair_off.push_back(fact_off[order[i]]);
+      build_off.push_back(-1);
+      continue;
+    }
     e.h_pair_off.push_back(off);
+    build_off.push_back(off);
     const int ld = rf + (rf & 1);
     off += (int64_t)ld * rs;  // ld even: every block and column 16-byte aligned
     e.pair_max_ld = std::max(e.pair_max_ld, ld);
@@ -968,13 +1126,14 @@ void assemble_tree(Engine& e) {
   upload(e.slot_off, slot_off, s);
   upload(e.pair_k, e.h_pair_k, s);
   upload(e.pair_b, e.h_pair_b, s);
-  upload(e.pair_off, e.h_pair_off, s);
+  upload(e.pair_off, build_off, s);
   {
     std::vector<PairDesc> pd(e.npairs);
     for (int64_t q = 0; q < e.npairs; q++) {
       const int gk = e.h_pair_k[q], gbx = e.h_pair_b[q];
-      pd[q] = PairDesc{rank_g[gk], rank_g[gbx], part_k[q], rank_g[gk] + (rank_g[gk] & 1),
-                       e.h_pair_off[q], soff_g[gk], soff_g[gbx], part_b[q], 0};
+      const int ld = e.h_pair_fact[q] >= 0 ? e.h_pair_fact[q] : rank_g[gk] + (rank_g[gk] & 1);
+      pd[q] = PairDesc{rank_g[gk], rank_g[gbx], part_k[q], ld, e.h_pair_off[q], soff_g[gk],
+                       soff_g[gbx], part_b[q], 0};
     }
     upload(e.pair_desc, pd, s);
   }

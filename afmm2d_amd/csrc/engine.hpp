@@ -109,6 +109,8 @@ constexpr int kM2LTileCols[kM2LClasses] = {4, 5, 6, 4, 5, 6, 8};
 constexpr int kM2LStreamClasses = 6;
 constexpr int kM2LStreamRows[kM2LStreamClasses] = {4, 6, 8, 10, 12, 16};
 constexpr int kM2LAllClasses = kM2LClasses + kM2LStreamClasses;
+// Pairs stored as ACA factors U V^T (opt-in), after the dense classes.
+constexpr int kM2LFactored = kM2LAllClasses + 1;
 
 inline int m2l_tile_class(int ld, int cols) {
   const int rt = (ld + 15) / 16, ct = (cols + 7) / 8;
@@ -363,7 +365,15 @@ struct Engine {
   std::vector<int64_t> h_pair_off;        // element offset of each block
   // pairs of group g (0: levels 2..L-1, 1: level L) in tile class c:
   // [m2l_cls_begin[g][c], m2l_cls_end[g][c])
-  int64_t m2l_cls_begin[2][kM2LAllClasses + 1] = {}, m2l_cls_end[2][kM2LAllClasses + 1] = {};
+  int64_t m2l_cls_begin[2][kM2LFactored + 1] = {}, m2l_cls_end[2][kM2LFactored + 1] = {};
+  // ACA-factored M2L pairs (m2l_factor set before assembly): a pair whose
+  // block has an ACA factorisation U V^T of rank p with p (r1 + r2) < r1 r2
+  // is stored in m2lf as U (r1 x p) then V (r2 x p), column-major; its
+  // PairDesc holds p in ld and the offset into m2lf in off
+  bool m2l_factor = false;
+  DevBuf m2lf;
+  std::vector<int> h_pair_fact;  // per pair: p, or -1 for a dense block
+  int64_t m2lf_pairs = 0, m2lf_total = 0, m2lf_dense = 0;  // pairs, factor / dense coefficients
   std::vector<int> h_red_start;           // per box gid (nbox + 1): first entry of h_red_pair
   std::vector<int> h_red_pair;            // per present slot, slot order: pair index
   DevBuf pair_k, pair_b, pair_off, m2l;

@@ -431,6 +431,47 @@ __device__ __forceinline__ void m2l_pair(const double* __restrict__ m2l, const P
   }
 }
 
+// M2L of a pair stored as ACA factors B = U V^T (U: rk x p, V: rb x p,
+// column-major at d.off): t1 = V^T out_b and t2 = U^T out_k (lane c per
+// column c), then B out_b = U t1 and B^T out_k = V t2 (lane i per row i),
+// written where the dense kernels write them.  One wave per pair; p <= 256.
+__global__ __launch_bounds__(64) void k_m2l_fact(const double* __restrict__ F,
+                                                 const PairDesc* __restrict__ pd,
+                                                 const double* __restrict__ outv,
+                                                 double* __restrict__ part, int npairs,
+                                                 double tsign) {
+  __shared__ double t1[256], t2[256];
+  const int q = blockIdx.x;
+  if (q >= npairs) return;
+  const PairDesc d = pd[q];
+  const int rk = d.rk, rb = d.rb, p = d.ld;
+  const double* U = F + d.off;
+  const double* V = U + (int64_t)rk * p;
+  const double* xk = outv + d.sk;
+  const double* xb = outv + d.sb;
+  const int l = threadIdx.x;
+  for (int c = l; c < p; c += 64) {
+    const double* Vc = V + (int64_t)c * rb;
+    const double* Uc = U + (int64_t)c * rk;
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = 0; i < rb; i++) s1 = fma(Vc[i], xb[i], s1);
+    for (int i = 0; i < rk; i++) s2 = fma(Uc[i], xk[i], s2);
+    t1[c] = s1;
+    t2[c] = s2;
+  }
+  __syncthreads();
+  for (int i = l; i < rk; i += 64) {
+    double acc = 0.0;
+    for (int c = 0; c < p; c++) acc = fma(U[i + (int64_t)c * rk], t1[c], acc);
+    part[d.part_k + i] = acc;
+  }
+  for (int i = l; i < rb; i += 64) {
+    double acc = 0.0;
+    for (int c = 0; c < p; c++) acc = fma(V[i + (int64_t)c * rb], t2[c], acc);
+    part[d.part_b + i] = tsign * acc;
+  }
+}
+
 template <int MAXRT, int MAXCT>
 __global__ __launch_bounds__(64) void k_m2l_reg(const double* __restrict__ m2l,
                                                 const PairDesc* __restrict__ pd,
@@ -1466,6 +1507,13 @@ void launch_m2l(Engine& e, int g, hipStream_t s) {
       case 12: k_m2l_stream<16, 1><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
       default: throw Error(6, "M2L block taller than 256 rows");
     }
+    AFMM_LAUNCH_CHECK();
+  }
+  const int64_t f0 = e.m2l_cls_begin[g][kM2LFactored], f1 = e.m2l_cls_end[g][kM2LFactored];
+  if (f1 > f0) {
+    const double ts = e.kid == K_USER_ANTI ? -1.0 : 1.0;
+    k_m2l_fact<<<(unsigned)(f1 - f0), 64, 0, s>>>(e.m2lf.as<double>(), pd0 + f0, outv, part,
+                                                  (int)(f1 - f0), ts);
     AFMM_LAUNCH_CHECK();
   }
 }

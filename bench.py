@@ -79,6 +79,12 @@ def parse():
                         "log-uniform in [2048, 65536], kernels log r / 1/r / exp(-r^2/0.01), "
                         "8 RHS ~ N(0,1), FP64 factors, one captured graph with tuned "
                         "parameters (one JSON line)")
+    p.add_argument("--rank-model", action="store_true",
+                   help="instead of the headline, the learned-rank workload: train the rank "
+                        "model on the ACA ranks of 200 seeded N = 65536 trees, then assemble "
+                        "N = 2^23 (log kernel, ACA tol 1e-8) with and without it and time "
+                        "16-RHS matvecs of both (one JSON line)")
+    p.add_argument("--train-trees", type=int, default=200)
     p.add_argument("--near-field", action="store_true",
                    help="instead of the headline, time the near-field kernel alone "
                         "(afmm2d_near_field, no assembly) on the same points and print its "
@@ -210,6 +216,99 @@ def batch_config(args, H, E):
                             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                             "algorithmic_bytes": int(alg)},
         "errors": errs,
+    }))
+
+
+def rank_model_ab(args, H, E):
+    """The learned-rank workload, one JSON line.  Training set: the per-box
+    ACA ranks of seeded N = 65536 trees over kernels (log r, 1/r,
+    exp(-r^2/0.01)), tolerances (1e-6 .. 1e-12) and point sets (uniform, 16
+    clipped Gaussians).  A/B: N = 2^23 uniform points, log kernel, leaf 64,
+    ACA tol 1e-8, assembled plain and with learned capacities; ranks must
+    agree; 16-RHS blocked matvecs (FP64, FP32) of both are timed."""
+    print("bench: rank model training set (%d trees)" % args.train_trees, file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
+    Xs, rs = [], []
+    kernels = (("log", None), ("inv_r", None), ("gaussian", [0.01, 0.0, 0.0, 0.0]))
+    for i in range(args.train_trees):
+        kname, kp = kernels[i % 3]
+        tol = (6, 8, 10, 12)[(i // 3) % 4]
+        if (i // 12) % 2 == 0:
+            xy = np.random.default_rng(5000 + i).uniform(-1.0, 1.0, size=(65536, 2))
+        else:
+            xy, _ = mixture_inputs(65536, 5000 + i)
+        T = E.AFMM(65536, 64, tol, xy, kernel=kname, kparams=kp)
+        T.assemble()
+        X, r = T.rank_samples()
+        Xs.append(X)
+        rs.append(r)
+        T.close()
+    X = np.concatenate(Xs)
+    r = np.concatenate(rs)
+    t_collect = time.perf_counter() - t0
+    y = E.rank_target(r)
+    m = E.RankModel(seed=args.seed)
+    t0 = time.perf_counter()
+    loss = m.train(X, y, epochs=20, batch=1024, lr=2e-3, seed=args.seed)
+    t_train = time.perf_counter() - t0
+    print("bench: trained on %d samples, loss %.4f" % (len(y), loss), file=sys.stderr, flush=True)
+
+    n, R, margin = 1 << 23, 16, 0.5
+    xy, q = inputs(n, args.seed)
+    rows = np.sort(np.random.default_rng(args.seed + 7).choice(n, 2048, replace=False))
+    Q = np.random.default_rng(args.seed + 1).uniform(-1.0, 1.0, size=(n, R))
+    Q[:, 0] = q
+    dB = H.DeviceArray.from_numpy(np.asfortranarray(Q))
+    dY = H.DeviceArray(Q.nbytes)
+    out, ranks = {}, {}
+    ref = None
+    for arm in ("plain", "learned"):
+        print("bench: N=%d assembly (%s)" % (n, arm), file=sys.stderr, flush=True)
+        A = E.AFMM(n, 64, 8, xy)
+        if arm == "learned":
+            A.set_rank_model(m, margin)
+        t0 = time.perf_counter()
+        A.assemble()
+        t_asm = time.perf_counter() - t0
+        st = A.stats()
+        ranks[arm] = [A.ranks(j) for j in range(2, A.nLevels + 1)]
+        if ref is None:
+            ref = A.direct_rows(rows, q)
+        res = {"assemble_s": round(t_asm, 3), "aca_s": round(st["aca_seconds"], 3),
+               "aca_scratch_peak_bytes": st["aca_scratch_peak"],
+               "aca_launches": st["aca_launches"], "boxes": st["rank_boxes"],
+               "fallbacks": st["rank_fallbacks"], "max_rank": st["max_rank"]}
+        if arm == "learned":
+            Xb, rb = A.rank_samples()
+            pred = m.predict(Xb)
+            res["hit_rate"] = round(1.0 - st["rank_fallbacks"] / max(1, st["rank_boxes"]), 5)
+            res["pred_abs_log2_err_mean"] = round(float(np.mean(np.abs(pred - E.rank_target(rb)))), 4)
+        for prec in ("fp64", "fp32"):
+            A.prepare_precision(prec)
+            fn = lambda: A.matvec_device_ex(dB.ptr, n, dY.ptr, n, R, prec)
+            ms, _ = timed(H, fn, max(3, args.blocked_steps // 2), 2)
+            Y = dY.download(np.empty((n, R), order="F"))
+            err = float(np.linalg.norm(Y[rows, 0] - ref) / np.linalg.norm(ref))
+            res[prec] = {"value": round(n * R / (ms * 1e-3), 1), "ms_per_pass": round(ms, 4),
+                         "rel_err_col0": err}
+        out[arm] = res
+        A.close()
+        del A
+    same = all(np.array_equal(a, b) for a, b in zip(ranks["plain"], ranks["learned"]))
+    dB.free()
+    dY.free()
+    m.close()
+    print(json.dumps({
+        "metric": "FMM matvec point\u00b7RHS/s with learned ACA capacities",
+        "value": out["learned"]["fp32"]["value"], "unit": UNIT, "n_gpus": 1, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "N=%d uniform [-1,1]^2, log kernel, leaf 64, ACA tol 1e-8, %d RHS; "
+                               "rank model trained on %d N=65536 trees" % (n, R, args.train_trees),
+                   "train_samples": int(len(y)), "train_loss": round(loss, 5),
+                   "collect_s": round(t_collect, 2), "train_s": round(t_train, 2),
+                   "margin_log2": margin, "ranks_identical": bool(same)},
+        "plain": out["plain"], "learned": out["learned"],
     }))
 
 
@@ -585,6 +684,9 @@ def main():
         return
     if args.batch:
         batch_config(args, H, E)
+        return
+    if args.rank_model:
+        rank_model_ab(args, H, E)
         return
     xy, q = inputs(args.n, args.seed)
 

@@ -105,6 +105,9 @@ typedef struct {
   int64_t aca_scratch_peak;     /* largest ACA scratch of one launch (bytes) */
   int64_t aca_launches;         /* ACA launches of the assembly */
   double aca_seconds;           /* ACA phase of the assembly (incl. capacity prediction) */
+  int64_t m2l_factored_pairs;   /* interaction pairs stored as ACA factors U V^T */
+  int64_t m2l_factored_bytes;   /* their factor bytes (included in m2l_bytes) */
+  int64_t m2l_factored_dense_bytes; /* the bytes their dense blocks would take */
 } afmm2d_stats;
 
 /* Message of the calling thread's last failure ("" if none). */
@@ -212,6 +215,26 @@ int afmm2d_batch_capture(afmm2d_batch b, int on);
 
 int afmm2d_batch_get_info(afmm2d_batch b, afmm2d_batch_info* info);
 int afmm2d_batch_destroy(afmm2d_batch b);
+
+/* ACA-factored M2L (set before afmm2d_assemble; on = 0 by default).  The
+ * reference builds every M2L block K(skeleton_k, skeleton_kIL) densely
+ * (AFMM2DTree.cpp:1070-1097) and applies it as one GEMV (1156-1178).  With
+ * factoring on, assembly also runs the ACA (ACA.cpp:27-133, same tolerance
+ * and stopping rule) on each interaction pair's block; a block whose rank-p
+ * factorisation U V^T is smaller (p (r_k + r_b) < r_k r_b) is stored as U
+ * and V and applied as U (V^T out_b) and V (U^T out_k) by the single-RHS
+ * matvec.  Each factored block adds an error of about the ACA tolerance.
+ * The blocked passes (AFMM2D_MATVEC_BLOCKED, FP32/BF16) keep dense blocks.
+ * Symmetric kernels only (no effect on a non-symmetric user kernel's
+ * antisymmetric part).  No reference counterpart. */
+int afmm2d_set_m2l_factoring(afmm2d_handle h, int on);
+
+/* How the pair of (level, box, slot) is stored: factor_rank = p for ACA
+ * factors (-1: a dense block); box_first = 1 when the stored block (or
+ * U V^T) is K(skeleton_box, skeleton_slot), 0 when it is the transpose.
+ * AFMM2D_ERR_NOT_FOUND for an empty slot. */
+int afmm2d_get_m2l_storage(afmm2d_handle h, int level, int box, int slot, int* factor_rank,
+                           int* box_first);
 
 /* ---- Learned ACA capacities -------------------------------------------
  * The reference's ACA (ACA.cpp:27-133, called per box at
