@@ -22,7 +22,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("AFMM2D_LIB") or os.path.join(_HERE, "libafmm2d.so")
 
 KERNELS = {"log": 0, "inv_r": 1, "gaussian": 2, "reg_inv_r": 3, "user": 4, "user_sym": 5,
-           "user_anti": 6}
+           "user_anti": 6, "user_symmetric": 7}
 
 OK = 0
 ERR_INVALID_ARGUMENT = 1
@@ -51,7 +51,7 @@ EXPORTS = [
     "afmm2d_rank_model_create", "afmm2d_rank_model_destroy", "afmm2d_rank_model_get_params",
     "afmm2d_rank_model_set_params", "afmm2d_rank_model_predict", "afmm2d_rank_model_gradient",
     "afmm2d_rank_model_train", "afmm2d_set_rank_model", "afmm2d_get_rank_samples",
-    "afmm2d_set_m2l_factoring", "afmm2d_get_m2l_storage",
+    "afmm2d_set_m2l_factoring", "afmm2d_get_m2l_storage", "afmm2d_check_symmetry",
 ]
 
 RANK_FEATURES = 16
@@ -175,6 +175,7 @@ def lib():
         "afmm2d_get_rank_samples": (i, [_vp, _fp, _ip, ctypes.c_int64, _lp]),
         "afmm2d_set_m2l_factoring": (i, [_vp, i]),
         "afmm2d_get_m2l_storage": (i, [_vp, i, i, i, _ip, _ip]),
+        "afmm2d_check_symmetry": (i, [_vp, _ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -436,6 +437,12 @@ class AFMM:
         _check(lib().afmm2d_direct_rows(self._h, rows.shape[0], rows.ctypes.data_as(_lp), _d(b),
                                         _d(out)))
         return out
+
+    def check_symmetry(self):
+        """False when K(i, j) != K(j, i) on a sample of this point set's pairs."""
+        v = ctypes.c_int()
+        _check(lib().afmm2d_check_symmetry(self._h, ctypes.byref(v)))
+        return bool(v.value)
 
     def set_m2l_factoring(self, on=True):
         """Store compressible M2L blocks as ACA factors (before assemble)."""

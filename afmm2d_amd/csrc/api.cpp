@@ -401,8 +401,11 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
   *out = nullptr;
   if (!loc) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "loc is NULL");
   if (n <= 1 || leaf <= 0) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "need N > 1 and leaf > 0");
-  if (kernel_id < 0 || kernel_id >= kNumKernels)
+  if ((kernel_id < 0 || kernel_id >= kNumKernels) && kernel_id != AFMM2D_KERNEL_USER_SYMMETRIC)
     return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown kernel id");
+  // the user kernel declared symmetric runs as one operator, as it is
+  const bool declared_symmetric = kernel_id == AFMM2D_KERNEL_USER_SYMMETRIC;
+  if (declared_symmetric) kernel_id = K_USER;
   if (layout != AFMM2D_LOC_INTERLEAVED && layout != AFMM2D_LOC_COLUMN_MAJOR)
     return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown location layout");
   if (tol_pow < 0 || tol_pow > 300) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "tol_pow out of range");
@@ -568,7 +571,12 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
   };
     auto e = make_engine(kernel_id);
     std::unique_ptr<Engine> e2;
-    if (kernel_id == K_USER && !user_kernel_symmetric(*e)) {
+    if (kernel_id == K_USER && !declared_symmetric) {
+      // not declared symmetric: the operator is its symmetric part plus its
+      // antisymmetric part, whatever a sample of entries suggests (a kernel
+      // asymmetric on a small region passes any sample).  A sampled check
+      // that finds K(i, j) != K(j, i) is kept as a diagnostic for callers
+      // that declared symmetry wrongly: see afmm2d_check_symmetry.
       e->kid = K_USER_SYM;
       e2 = make_engine(K_USER_ANTI);
     }
@@ -856,6 +864,13 @@ int afmm2d_rank_model_train(afmm2d_rank_model m, const float* x, const float* y,
   return guarded([&] {
     const double l = rank_model_train(*m->m, x, y, n, epochs, batch, lr, seed);
     if (loss) *loss = l;
+  });
+}
+
+int afmm2d_check_symmetry(afmm2d_handle h, int* symmetric) {
+  if (!h || !symmetric) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
+  return guarded([&] {
+    *symmetric = h->e->kid_full != K_USER || user_kernel_symmetric(*h->e) ? 1 : 0;
   });
 }
 

@@ -431,35 +431,53 @@ __device__ __forceinline__ void m2l_pair(const double* __restrict__ m2l, const P
   }
 }
 
-// M2L of a pair stored as ACA factors B = U V^T (U: rk x p, V: rb x p,
-// column-major at d.off): t1 = V^T out_b and t2 = U^T out_k (lane c per
-// column c), then B out_b = U t1 and B^T out_k = V t2 (lane i per row i),
-// written where the dense kernels write them.  One wave per pair; p <= 256.
-__global__ __launch_bounds__(64) void k_m2l_fact(const double* __restrict__ F,
-                                                 const PairDesc* __restrict__ pd,
-                                                 const double* __restrict__ outv,
-                                                 double* __restrict__ part, int npairs,
-                                                 double tsign) {
-  __shared__ double t1[256], t2[256];
-  const int q = blockIdx.x;
-  if (q >= npairs) return;
+// M2L of pairs stored as ACA factors B = U V^T (U: rk x p, V: rb x p,
+// column-major at d.off), one wave per pair, four pairs per workgroup.
+// Stage 1: t1 = V^T out_b and t2 = U^T out_k, four lanes per column (each
+// summing every fourth row, then a two-step shuffle sum), 16 columns per
+// pass.  Stage 2: B out_b = U t1 and B^T out_k = V t2, one lane per row,
+// written where the dense pair kernels write them.  p <= 256.
+constexpr int kFactLanes = 4;
+__global__ __launch_bounds__(256) void k_m2l_fact(const double* __restrict__ F,
+                                                  const PairDesc* __restrict__ pd,
+                                                  const double* __restrict__ outv,
+                                                  double* __restrict__ part, int npairs,
+                                                  double tsign) {
+  __shared__ double tsh[4][2][256];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int q = blockIdx.x * 4 + w;
+  if (q >= npairs) return;  // whole waves: no barrier below spans waves
   const PairDesc d = pd[q];
   const int rk = d.rk, rb = d.rb, p = d.ld;
   const double* U = F + d.off;
   const double* V = U + (int64_t)rk * p;
   const double* xk = outv + d.sk;
   const double* xb = outv + d.sb;
-  const int l = threadIdx.x;
-  for (int c = l; c < p; c += 64) {
-    const double* Vc = V + (int64_t)c * rb;
-    const double* Uc = U + (int64_t)c * rk;
+  double* t1 = tsh[w][0];
+  double* t2 = tsh[w][1];
+  const int gi = l % kFactLanes, col = l / kFactLanes;
+  for (int c0 = 0; c0 < p; c0 += 64 / kFactLanes) {
+    const int c = c0 + col;
     double s1 = 0.0, s2 = 0.0;
-    for (int i = 0; i < rb; i++) s1 = fma(Vc[i], xb[i], s1);
-    for (int i = 0; i < rk; i++) s2 = fma(Uc[i], xk[i], s2);
-    t1[c] = s1;
-    t2[c] = s2;
+    if (c < p) {
+      const double* Vc = V + (int64_t)c * rb;
+      const double* Uc = U + (int64_t)c * rk;
+      for (int i = gi; i < rb; i += kFactLanes) s1 = fma(Vc[i], xb[i], s1);
+      for (int i = gi; i < rk; i += kFactLanes) s2 = fma(Uc[i], xk[i], s2);
+    }
+#pragma unroll
+    for (int o = kFactLanes / 2; o > 0; o >>= 1) {
+      s1 += __shfl_xor(s1, o, 64);
+      s2 += __shfl_xor(s2, o, 64);
+    }
+    if (gi == 0 && c < p) {
+      t1[c] = s1;
+      t2[c] = s2;
+    }
   }
-  __syncthreads();
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   for (int i = l; i < rk; i += 64) {
     double acc = 0.0;
     for (int c = 0; c < p; c++) acc = fma(U[i + (int64_t)c * rk], t1[c], acc);
@@ -1512,8 +1530,8 @@ void launch_m2l(Engine& e, int g, hipStream_t s) {
   const int64_t f0 = e.m2l_cls_begin[g][kM2LFactored], f1 = e.m2l_cls_end[g][kM2LFactored];
   if (f1 > f0) {
     const double ts = e.kid == K_USER_ANTI ? -1.0 : 1.0;
-    k_m2l_fact<<<(unsigned)(f1 - f0), 64, 0, s>>>(e.m2lf.as<double>(), pd0 + f0, outv, part,
-                                                  (int)(f1 - f0), ts);
+    k_m2l_fact<<<(unsigned)ceil_div(f1 - f0, 4), 256, 0, s>>>(e.m2lf.as<double>(), pd0 + f0,
+                                                              outv, part, (int)(f1 - f0), ts);
     AFMM_LAUNCH_CHECK();
   }
 }

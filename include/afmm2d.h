@@ -45,10 +45,13 @@ typedef enum {
                                   afmm2d_user_kernel(xi, yi, xj, yj, kparams, a), chosen at build
                                   time (the reference's userkernel::getMatrixEntry, kernel.hpp:58);
                                   default afmm2d_amd/user_kernels/anisotropic_log.h.  May be
-                                  non-symmetric: create samples K(i,j) against K(j,i) and, if they
-                                  differ, runs the operator as the sum of the two below */
+                                  non-symmetric: the operator runs as the sum of the two below
+                                  (declare a symmetric kernel with AFMM2D_KERNEL_USER_SYMMETRIC) */
   AFMM2D_KERNEL_USER_SYM = 5,  /* (K(x,y) + K(y,x)) / 2 of the user kernel */
-  AFMM2D_KERNEL_USER_ANTI = 6  /* (K(x,y) - K(y,x)) / 2 of the user kernel */
+  AFMM2D_KERNEL_USER_ANTI = 6, /* (K(x,y) - K(y,x)) / 2 of the user kernel */
+  AFMM2D_KERNEL_USER_SYMMETRIC = 7 /* the user kernel, declared symmetric by the caller: one
+                                      operator (M2L blocks applied both ways, the near field's
+                                      K(j,i) = K(i,j)); afmm2d_check_symmetry samples the claim */
 } afmm2d_kernel_id;
 
 /* Precision of the stored factors and of the arithmetic of a matvec
@@ -228,6 +231,11 @@ int afmm2d_batch_destroy(afmm2d_batch b);
  * Symmetric kernels only (no effect on a non-symmetric user kernel's
  * antisymmetric part).  No reference counterpart. */
 int afmm2d_set_m2l_factoring(afmm2d_handle h, int on);
+
+/* symmetric = 0 when K(i, j) != K(j, i) on one of 4096 sampled pairs of the
+ * handle's points (a diagnostic for AFMM2D_KERNEL_USER_SYMMETRIC; 1 for the
+ * built-in kernels). */
+int afmm2d_check_symmetry(afmm2d_handle h, int* symmetric);
 
 /* How the pair of (level, box, slot) is stored: factor_rank = p for ACA
  * factors (-1: a dense block); box_first = 1 when the stored block (or

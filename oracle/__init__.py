@@ -55,6 +55,7 @@ def lib():
         L.or_near_field.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.or_set_parallel_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_set_exact_skeleton_rows.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_set_aca_wave_sums.argtypes = [ctypes.c_int]
         L.or_levels.restype = ctypes.c_int
         L.or_levels.argtypes = [ctypes.c_void_p]
         L.or_kernel_a.restype = ctypes.c_double
@@ -272,3 +273,9 @@ class Tree:
         if not ok:
             return None
         return out[: r * rank_il].reshape(r, rank_il)
+
+
+def set_aca_wave_sums(on):
+    """The ACA's dot products in the engine's wave order (True) or in
+    ascending order as the reference (False), for every tree."""
+    lib().or_set_aca_wave_sums(1 if on else 0)

@@ -293,6 +293,36 @@ static int argmax_allowed(const double* v, const unsigned char* used, int n) {
 
 typedef double (*aca_fn)(const void* ctx, int i, int j);
 
+/* Summation order of the ACA's dot products (the norm estimate's terms).
+ * 0: ascending index (this restatement's order for ACA.cpp's dot products).
+ * 1: the engine's order: 64 partial sums, partial l adding the products of
+ * indices l, l + 64, l + 128, ... in ascending order, then combined by the
+ * butterfly s_l + s_(l xor d) for d = 32, 16, ..., 1 (a wave's shuffle
+ * reduction).  Set for the whole library (not thread-safe). */
+static int g_aca_wave_sums = 0;
+
+void or_set_aca_wave_sums(int on) { g_aca_wave_sums = on; }
+
+static double aca_dot(const double* a, const double* b, int n) {
+  if (!g_aca_wave_sums) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+  }
+  double part[64];
+  for (int l = 0; l < 64; l++) {
+    double s = 0.0;
+    for (int i = l; i < n; i += 64) s += a[i] * b[i];
+    part[l] = s;
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    double nxt[64];
+    for (int l = 0; l < 64; l++) nxt[l] = part[l] + part[l ^ d];
+    memcpy(part, nxt, sizeof(part));
+  }
+  return part[0];
+}
+
 /* Returns rank; U (n1 x mn, column l at U + l*n1) and V (n2 x mn) are scratch. */
 static int aca_core(aca_fn K, const void* ctx, int n1, int n2, double tol, int* row_piv,
                     int* col_piv, double* U, double* V, double* row, unsigned char* used_r,
@@ -342,17 +372,12 @@ static int aca_core(aca_fn K, const void* ctx, int n1, int n2, double tol, int* 
     used_c[jstar] = 1;
     used_r[istar] = 1;
     if (rank != 2) prev = normS;
-    double nu = 0.0, nv = 0.0;
-    for (int i = 0; i < n1; i++) nu += u[i] * u[i];
-    for (int j = 0; j < n2; j++) nv += v[j] * v[j];
+    double nu = aca_dot(u, u, n1), nv = aca_dot(v, v, n2);
     double p = sqrt(nu) * sqrt(nv);
     double s = normS * normS + p * p;
     for (int q = 0; q < rank; q++) {
-      double du = 0.0, dv = 0.0;
-      const double* Uq = U + (size_t)q * n1;
-      const double* Vq = V + (size_t)q * n2;
-      for (int i = 0; i < n1; i++) du += Uq[i] * u[i];
-      for (int j = 0; j < n2; j++) dv += Vq[j] * v[j];
+      const double du = aca_dot(U + (size_t)q * n1, u, n1);
+      const double dv = aca_dot(V + (size_t)q * n2, v, n2);
       s += 2.0 * fabs(du * dv);
     }
     normS = sqrt(s);

@@ -71,6 +71,12 @@ void or_set_parallel_mode(or_tree* t, int mode);
  * or_assemble. */
 void or_set_exact_skeleton_rows(or_tree* t, int on);
 
+/* The ACA's dot-product summation order for the whole library: 0 ascending
+ * (the default), 1 the engine's wave order (64
+ * strided partial sums, then a butterfly).  Test infrastructure: explains
+ * rank differences that come from rounding of the norm estimate only. */
+void or_set_aca_wave_sums(int on);
+
 /* Accessors. */
 int or_levels(const or_tree* t);
 double or_kernel_a(const or_tree* t);

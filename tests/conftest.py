@@ -42,3 +42,11 @@ def mixture_points(n, seed, clusters=16, sigma=0.05):
     lab = rng.integers(0, clusters, size=n)
     xy = np.clip(centres[lab] + sigma * rng.standard_normal((n, 2)), -1.0, 1.0)
     return xy, rng.uniform(-1.0, 1.0, size=n)
+
+
+def oracle_kernel(name):
+    """The oracle's kernel id for an engine kernel name (a user kernel
+    declared symmetric is the user kernel itself)."""
+    import afmm2d_amd
+    kid = afmm2d_amd.KERNELS[name]
+    return 4 if kid == 7 else kid

@@ -64,7 +64,7 @@ def _create(engine, n, leaf, tol_pow, loc, kid, layout=0):
 def test_create_rejects_bad_arguments(engine):
     loc = np.zeros((4096, 2))
     assert _create(engine, 4096, 64, 8, None, 0)[0] == engine.ERR_INVALID_ARGUMENT
-    assert _create(engine, 4096, 64, 8, loc, 7)[0] == engine.ERR_INVALID_ARGUMENT
+    assert _create(engine, 4096, 64, 8, loc, 8)[0] == engine.ERR_INVALID_ARGUMENT
     assert _create(engine, 4096, 64, 8, loc, 0, layout=5)[0] == engine.ERR_INVALID_ARGUMENT
     assert _create(engine, 4096, 0, 8, loc, 0)[0] == engine.ERR_INVALID_ARGUMENT
     assert _create(engine, 4096, 64, -1, loc, 0)[0] == engine.ERR_INVALID_ARGUMENT

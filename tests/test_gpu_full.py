@@ -35,21 +35,47 @@ def test_full_size_matches_oracle_and_direct_sum(engine, oracle_mod, full):
     inv[perm] = np.arange(n)
     d_cpu = T.direct_rows(inv[rows].astype(np.int32), q[perm])
     assert float(np.linalg.norm(yd[rows] - d_cpu) / np.linalg.norm(d_cpu)) <= 1e-13
-    # and the whole operator against the oracle's
+    # and the whole operator against the oracle's.  The ACA's stopping test
+    # compares norm estimates built from dot products; the oracle sums them
+    # in ascending order, the engine in a wave's order, so a borderline
+    # |prev - normS| >= tol prev can go either way on a few boxes
     T.assemble()
-    for j in range(2, 8):
-        mism = int((A.ranks(j) != T.ranks(j)).sum())
-        assert mism <= 4 ** j // 1000, (j, mism)
+    mism = [int((A.ranks(j) != T.ranks(j)).sum()) for j in range(2, 8)]
+    print("full N=2^20 tol 1e-12: rank mismatches per level 2..7 (ascending sums): %s" % mism)
+    assert sum(mism) <= 4, mism   # observed: [0, 0, 0, 0, 0, 2]
     yo = T.matvec(q)
     d = float(np.linalg.norm(y - yo) / np.linalg.norm(yo))
     print("full N=2^20: GPU vs oracle rel.diff %.3e" % d)
     assert d <= 1e-11
     T.close()
+    # with the engine's summation order the oracle's ranks move towards the
+    # engine's; what is left comes from the kernel values themselves (the
+    # device log differs from the host's in the last bit on some entries)
+    T2 = wave_order_oracle(oracle_mod, xy, 12)
+    mism2 = [int((A.ranks(j) != T2.ranks(j)).sum()) for j in range(2, 8)]
+    d2 = float(np.linalg.norm(y - T2.matvec(q)) / np.linalg.norm(yo))
+    print("full N=2^20 tol 1e-12: rank mismatches per level 2..7 (wave-order sums): %s, "
+          "rel.diff %.3e" % (mism2, d2))
+    assert sum(mism2) <= sum(mism)
+    assert d2 <= 1e-11
+    T2.close()
     A.close()
 
 
-def test_full_size_configured_tolerance(engine, full):
-    # the configured ACA tolerance 1e-8 gives ~4e-6: reported, bounded here
+def wave_order_oracle(oracle_mod, xy, tol_pow):
+    oracle_mod.set_aca_wave_sums(True)
+    try:
+        T = oracle_mod.Tree(xy, 64, tol_pow)
+        T.assemble()
+    finally:
+        oracle_mod.set_aca_wave_sums(False)
+    return T
+
+
+def test_full_size_configured_tolerance(engine, oracle_mod, full):
+    # the configured ACA tolerance 1e-8 gives ~4e-6 (the reference's method
+    # at that tolerance): reported and bounded against the direct sum; the
+    # operator itself against the oracle's at the same tolerance
     n, xy, q = full
     A = engine.AFMM(n, 64, 8, xy)
     A.assemble()
@@ -59,6 +85,25 @@ def test_full_size_configured_tolerance(engine, full):
     err = float(np.linalg.norm(y[rows] - ref) / np.linalg.norm(ref))
     print("full N=2^20 tol 1e-8: rel.err vs direct (4096 rows) %.3e" % err)
     assert err <= 1e-4
+    T = oracle_mod.Tree(xy, 64, 8)
+    T.assemble()
+    mism = [int((A.ranks(j) != T.ranks(j)).sum()) for j in range(2, 8)]
+    print("full N=2^20 tol 1e-8: rank mismatches per level 2..7 (ascending sums): %s" % mism)
+    assert sum(mism) <= 8, mism
+    yo = T.matvec(q)
+    d = float(np.linalg.norm(y - yo) / np.linalg.norm(yo))
+    print("full N=2^20 tol 1e-8: GPU vs oracle rel.diff %.3e" % d)
+    # a differing rank moves the result by about the ACA error
+    assert d <= (1e-11 if sum(mism) == 0 else 1e-6)
+    T.close()
+    T2 = wave_order_oracle(oracle_mod, xy, 8)
+    mism2 = [int((A.ranks(j) != T2.ranks(j)).sum()) for j in range(2, 8)]
+    d2 = float(np.linalg.norm(y - T2.matvec(q)) / np.linalg.norm(yo))
+    print("full N=2^20 tol 1e-8: rank mismatches per level 2..7 (wave-order sums): %s, "
+          "rel.diff %.3e" % (mism2, d2))
+    assert sum(mism2) <= sum(mism)
+    assert d2 <= (1e-11 if sum(mism2) == 0 else 1e-6)
+    T2.close()
     A.close()
 
 

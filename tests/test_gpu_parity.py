@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from conftest import uniform_points
+from conftest import uniform_points, oracle_kernel
 
 pytestmark = pytest.mark.gpu
 
@@ -40,10 +40,10 @@ def test_topology_matches_oracle(engine, oracle_mod):
     (8192, 64, 10, "inv_r"), (8192, 64, 10, "reg_inv_r"), (8192, 64, 6, "gaussian"),
     # ranks above 64: the streamed M2L classes (leaf level and upper levels)
     (10000, 100, 12, "reg_inv_r"), (5000, 100, 14, "reg_inv_r"),
-    # the compiled-in user kernel (anisotropic log)
-    (8192, 64, 10, "user"), (20000, 64, 12, "user")])
+    # the compiled-in user kernel (anisotropic log), declared symmetric
+    (8192, 64, 10, "user_symmetric"), (20000, 64, 12, "user_symmetric")])
 def test_assembly_and_matvec_match_oracle(engine, oracle_mod, n, leaf, tol_pow, kernel):
-    kid = engine.KERNELS[kernel]
+    kid = oracle_kernel(kernel)
     xy, q = uniform_points(n, 11 + tol_pow)
     T = oracle_mod.Tree(xy, leaf, tol_pow, kid)
     T.assemble()
@@ -107,14 +107,14 @@ def test_direct_sum_matches_oracle(engine, oracle_mod):
     (5000, 100, "inv_r", 0),       # 312: 512
     (5000, 100, "reg_inv_r", 16),  # duplicated points: R = 0 off the diagonal
     (6000, 64, "inv_r", 16),
-    (7000, 100, "user", 0),        # user kernel: padding repeats a real point
-    (6000, 64, "user", 16),
+    (7000, 100, "user_symmetric", 0),  # user kernel: padding repeats a real point
+    (6000, 64, "user_symmetric", 16),
 ])
 def test_near_field_matches_oracle(engine, oracle_mod, n, leaf, kernel, quant):
     """The near-field kernel alone (AFMM2DTree.cpp:1199-1213) against the
     oracle's: the rsqrt-based 1/R kernels and the table log agree to a few
     ulp per entry, so the sums agree to 1e-13 relative."""
-    kid = engine.KERNELS[kernel]
+    kid = oracle_kernel(kernel)
     xy, q = uniform_points(n, 40 + n % 97)
     if quant:
         xy = np.round(xy * quant) / quant
@@ -132,7 +132,7 @@ def test_near_field_matches_oracle(engine, oracle_mod, n, leaf, kernel, quant):
     (14336, 200, "log", 4),        # 224 points per leaf: four 64-point chunks
     (14336, 200, "reg_inv_r", 4),
     (7680, 400, "inv_r", 8),       # 480: eight chunks, two groups of four
-    (14336, 200, "user", 4),
+    (14336, 200, "user_symmetric", 4),
     (20000, 300, "log", 5),        # 312: chunks two at a time, then one
     (100000, 100, "reg_inv_r", 7),  # 390
 ])
@@ -142,7 +142,7 @@ def test_near_field_long_leaves_one_wave_per_pair(engine, oracle_mod, monkeypatc
     # chunk of a leaf: four at a time when every chunk is full, else two at
     # a time and one; near field and matvec against the oracle.
     monkeypatch.setenv("AFMM2D_P2P_WAVES", "0")
-    kid = engine.KERNELS[kernel]
+    kid = oracle_kernel(kernel)
     xy, q = uniform_points(n, 5 + n % 89)
     T = oracle_mod.Tree(xy, leaf, 10, kid)
     a = engine.AFMM(n, leaf, 10, xy, kernel=kernel)
@@ -164,7 +164,7 @@ def test_user_kernel_parameters_reach_the_device(engine, oracle_mod):
     xy, q = uniform_points(8192, 77)
     kp = [3.0, 0.0, 0.0, 0.0]
     T = oracle_mod.Tree(xy, 64, 10, 4, kparams=kp)
-    a = engine.AFMM(8192, 64, 10, xy, kernel="user", kparams=kp)
+    a = engine.AFMM(8192, 64, 10, xy, kernel="user_symmetric", kparams=kp)
     nf, nf_ref = a.near_field(q), T.near_field(q)
     assert np.linalg.norm(nf - nf_ref) <= 1e-13 * np.linalg.norm(nf_ref)
     T.assemble()
@@ -172,7 +172,7 @@ def test_user_kernel_parameters_reach_the_device(engine, oracle_mod):
     y, yo = a.computeMatVecProduct(q), T.matvec(q)
     assert np.linalg.norm(y - yo) <= 1e-11 * np.linalg.norm(yo)
     # a different g is a different operator
-    a2 = engine.AFMM(8192, 64, 10, xy, kernel="user")
+    a2 = engine.AFMM(8192, 64, 10, xy, kernel="user_symmetric")
     assert np.linalg.norm(a2.near_field(q) - nf) > 1e-3 * np.linalg.norm(nf)
     a.close()
     a2.close()
@@ -216,3 +216,28 @@ def test_nonsymmetric_user_kernel(engine, oracle_mod, tol_pow):
     assert a.getMatrixEntry(0, 1) != a.getMatrixEntry(1, 0)
     a.close()
     T4.close()
+
+
+def test_user_kernel_symmetry_is_declared(engine, oracle_mod):
+    """An undeclared user kernel runs as its symmetric plus antisymmetric
+    parts, whatever a sample suggests; the declared-symmetric id runs it as
+    one operator.  For a symmetric kernel both give the oracle's operator;
+    afmm2d_check_symmetry tells the two kernels apart."""
+    n = 12000
+    xy, q = uniform_points(n, 123)
+    T = oracle_mod.Tree(xy, 64, 10, 4)
+    T.assemble()
+    yo = T.matvec(q)
+    split = engine.AFMM(n, 64, 10, xy, kernel="user")
+    decl = engine.AFMM(n, 64, 10, xy, kernel="user_symmetric")
+    split.assemble()
+    decl.assemble()
+    assert split.check_symmetry() and decl.check_symmetry()
+    for a in (split, decl):
+        y = a.computeMatVecProduct(q)
+        assert np.linalg.norm(y - yo) <= 1e-11 * np.linalg.norm(yo)
+    asym = engine.AFMM(n, 64, 10, xy, kernel="user_symmetric", kparams=[0.0, 0.25, 0, 0])
+    assert not asym.check_symmetry()
+    for a in (split, decl, asym):
+        a.close()
+    T.close()
