@@ -52,6 +52,7 @@ EXPORTS = [
     "afmm2d_rank_model_set_params", "afmm2d_rank_model_predict", "afmm2d_rank_model_gradient",
     "afmm2d_rank_model_train", "afmm2d_set_rank_model", "afmm2d_get_rank_samples",
     "afmm2d_set_m2l_factoring", "afmm2d_get_m2l_storage", "afmm2d_check_symmetry",
+    "afmm2d_release_scratch",
 ]
 
 RANK_FEATURES = 16
@@ -176,6 +177,7 @@ def lib():
         "afmm2d_set_m2l_factoring": (i, [_vp, i]),
         "afmm2d_get_m2l_storage": (i, [_vp, i, i, i, _ip, _ip]),
         "afmm2d_check_symmetry": (i, [_vp, _ip]),
+        "afmm2d_release_scratch": (i, [i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -483,6 +485,11 @@ class AFMM:
 
     def reset_kernel_times(self):
         _check(lib().afmm2d_reset_kernel_times(self._h))
+
+
+def release_scratch(device=-1):
+    """Frees the ACA scratch the library keeps between assemblies."""
+    _check(lib().afmm2d_release_scratch(int(device)))
 
 
 def batch_assemble(problems):

@@ -626,6 +626,10 @@ int afmm2d_assemble(afmm2d_handle h) {
   });
 }
 
+int afmm2d_release_scratch(int device) {
+  return guarded([&] { release_scratch(device); });
+}
+
 int afmm2d_matvec_device(afmm2d_handle h, const double* d_b, double* d_y, void* stream) {
   if (!h || !d_b || !d_y) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL argument");
   if (!h->assembled) return fail(AFMM2D_ERR_STATE, "matvec before assemble");

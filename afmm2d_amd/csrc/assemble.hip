@@ -32,6 +32,9 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <unordered_map>
 
@@ -475,6 +478,41 @@ size_t free_device_bytes() {
   return fr;
 }
 
+// The ACA scratch (up to 16 GiB) is kept per device between assemblies and
+// only grows: freeing and allocating buffers of this size in a loop made
+// later assemblies wait seconds in the allocator (on the driver's clearing
+// of the freed memory) at random.  afmm2d_release_scratch frees it.
+// AFMM2D_SCRATCH_POOL=0: a buffer per assembly, freed at its end.
+std::mutex g_pool_mu;
+std::map<int, DevBuf> g_pool;
+// assemblies share the pooled scratch: one at a time per process
+std::mutex g_assembly_mu;
+
+bool pool_enabled() {
+  static const bool on = [] {
+    const char* v = std::getenv("AFMM2D_SCRATCH_POOL");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+
+struct Scratch {
+  DevBuf own;  // when the pool is off
+  int device;
+  double* get(size_t doubles, int64_t& peak) {
+    const size_t bytes = sizeof(double) * std::max<size_t>(doubles, 1);
+    if (!pool_enabled()) {
+      own.alloc(bytes);
+      return own.as<double>();
+    }
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    DevBuf& b = g_pool[device];
+    b.alloc(bytes);  // grows only
+    peak = std::max<int64_t>(peak, (int64_t)b.bytes);
+    return b.as<double>();
+  }
+};
+
 struct BBox {
   double x0 = DBL_MAX, y0 = DBL_MAX, x1 = -DBL_MAX, y1 = -DBL_MAX;
   void add(const BBox& o) {
@@ -574,7 +612,9 @@ void pair_aca(Engine& e, const std::vector<int>& first, const std::vector<int>& 
     pivoff[i + 1] = pivoff[i] + std::min(n1[i], n2[i]);
   }
   DevBuf d_rstart, d_n1, d_n2, d_il, d_pivoff, d_rank, d_prow, d_pcx, d_pcy, d_boxes, d_scroff,
-      d_scr, d_fc;
+      d_fc;
+  Scratch scr{DevBuf{}, e.device};
+  int64_t peak_unused = 0;
   upload(d_rstart, rstart, s);
   upload(d_n1, n1, s);
   upload(d_n2, n2, s);
@@ -608,10 +648,10 @@ void pair_aca(Engine& e, const std::vector<int>& first, const std::vector<int>& 
     i0 = i;
     upload(d_boxes, boxes, s);
     upload(d_scroff, soffs, s);
-    d_scr.alloc(sizeof(double) * std::max<int64_t>(tot, 1));
+    double* d_scr = scr.get((size_t)tot, peak_unused);
     AcaArgs a{e.skx.as<double>(), e.sky.as<double>(), d_rstart.as<int>(), d_n1.as<int>(),
               d_n2.as<int>(), d_il.as<int>(), d_boxes.as<int>(), d_scroff.as<int64_t>(),
-              d_scr.as<double>(), d_pivoff.as<int64_t>(), d_rank.as<int>(), d_prow.as<int>(),
+              d_scr, d_pivoff.as<int64_t>(), d_rank.as<int>(), d_prow.as<int>(),
               d_pcx.as<double>(), d_pcy.as<double>(), e.tol, e.kp, nullptr, nullptr};
     AFMM_DISPATCH_KERNEL(e.kid, KID,
                          k_aca<KID, ACA_THREADS><<<(unsigned)boxes.size(), ACA_THREADS, 0, s>>>(a));
@@ -635,7 +675,7 @@ void pair_aca(Engine& e, const std::vector<int>& first, const std::vector<int>& 
       parts.back().alloc(sizeof(double) * std::max<int64_t>(local, 1));
       part_base.push_back(ftot);
       upload(d_fc, fc, s);
-      k_copy_factors<<<(unsigned)fc.size(), 256, 0, s>>>(d_fc.as<FactCopy>(), d_scr.as<double>(),
+      k_copy_factors<<<(unsigned)fc.size(), 256, 0, s>>>(d_fc.as<FactCopy>(), d_scr,
                                                          parts.back().as<double>());
       AFMM_LAUNCH_CHECK();
       AFMM_HIP(hipStreamSynchronize(s));  // the next batch reuses the scratch
@@ -656,6 +696,7 @@ void pair_aca(Engine& e, const std::vector<int>& first, const std::vector<int>& 
 }  // namespace
 
 void assemble_tree(Engine& e) {
+  std::lock_guard<std::mutex> assembly_lock(g_assembly_mu);
   const int L = e.L;
   hipStream_t s = e.s_aux;
   e.lv.assign(L + 1, LevelInfo{});
@@ -664,7 +705,8 @@ void assemble_tree(Engine& e) {
       lrow(L + 1);
   std::vector<int64_t> nrows(L + 1, 0);  // source rows of each level
   std::vector<std::vector<int64_t>> sloc(L + 2);
-  DevBuf d_rstart, d_n1, d_n2, d_il, d_boxes, d_scroff, d_scr, d_pivoff, d_rank, d_prow, d_pcx,
+  Scratch scr{DevBuf{}, e.device};
+  DevBuf d_rstart, d_n1, d_n2, d_il, d_boxes, d_scroff, d_pivoff, d_rank, d_prow, d_pcx,
       d_pcy, d_sloc, d_l2poff, d_moff, d_mscr, d_cap, d_over, d_feat, d_pred;
   // point bounding boxes: leaves from their points, parents from children
   std::vector<BBox> bbox(1 << (2 * L));
@@ -816,10 +858,11 @@ void assemble_tree(Engine& e) {
         upload(d_boxes, boxes, s);
         upload(d_scroff, soffs, s);
         if (capped) upload(d_cap, caps, s);
-        d_scr.alloc(sizeof(double) * std::max<int64_t>(tot, 1));
+        int64_t held = 0;
+        double* d_scr = scr.get((size_t)tot, held);
         e.aca_scratch_peak = std::max<int64_t>(e.aca_scratch_peak, 8 * tot);
         AcaArgs a{src_x, src_y, d_rstart.as<int>(), d_n1.as<int>(), d_n2.as<int>(), d_il.as<int>(),
-                  d_boxes.as<int>(), d_scroff.as<int64_t>(), d_scr.as<double>(),
+                  d_boxes.as<int>(), d_scroff.as<int64_t>(), d_scr,
                   d_pivoff.as<int64_t>(), d_rank.as<int>(), d_prow.as<int>(), d_pcx.as<double>(),
                   d_pcy.as<double>(), e.tol, e.kp, capped ? d_cap.as<int>() : nullptr,
                   capped ? d_over.as<int>() : nullptr};
@@ -951,7 +994,7 @@ void assemble_tree(Engine& e) {
     lskx[j].release(); lsky[j].release(); lskg[j].release();
   }
   for (int j = 0; j <= L; j++) { lrm[j].release(); lcm[j].release(); lrow[j].release(); }
-  d_scr.release();
+  scr.own.release();
   d_mscr.release();
 
   // Per-box arrays over all levels.
@@ -1156,6 +1199,12 @@ void assemble_tree(Engine& e) {
   AFMM_HIP(hipMemsetAsync(e.outv.p, 0, e.outv.bytes, s));
   AFMM_HIP(hipMemsetAsync(e.inv.p, 0, e.inv.bytes, s));
   AFMM_HIP(hipStreamSynchronize(s));
+}
+
+void release_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  if (device < 0) g_pool.clear();
+  else g_pool.erase(device);
 }
 
 }  // namespace afmm

@@ -465,6 +465,8 @@ void gather_sorted(Engine& e, const double* d_b, double* d_qs, hipStream_t s);
 
 // blocked.hip
 // Builds the factors of precision prec (and the FP64 layout they share).
+// Frees the ACA scratch kept between assemblies on device (< 0: all).
+void release_scratch(int device);
 void mr_prepare(Engine& e, int prec);
 // y[:, c] = K b[:, c] for c < nrhs (nrhs <= kMrMaxCols), device arrays in
 // caller order, column-major with leading dimensions ldb and ldy, on stream s.

@@ -130,8 +130,16 @@ int afmm2d_create(int n, int min_particles_in_leaf, int tol_pow, const double* l
 int afmm2d_destroy(afmm2d_handle h);
 
 /* Replaces AFMM::assemble (AFMM2D.cpp:41-44): per-box ACA skeletons and L2P
- * (AFMM2DTree.cpp:991-1068), M2L blocks (1070-1097). */
+ * (AFMM2DTree.cpp:991-1068), M2L blocks (1070-1097).  The ACA's scratch
+ * (up to 16 GiB, or a quarter of the free memory) stays allocated on the
+ * device after the assembly for the next one (freeing and reallocating it
+ * made assemblies stall in the allocator); afmm2d_release_scratch frees it.
+ * Assemblies in one process run one at a time. */
 int afmm2d_assemble(afmm2d_handle h);
+
+/* Frees the ACA scratch kept on device (< 0: every device).  No reference
+ * counterpart. */
+int afmm2d_release_scratch(int device);
 
 /* Replaces AFMM::computeMatVecProduct (AFMM2D.cpp:46-65): y = K b for nrhs
  * right-hand sides, host arrays in the caller's point order, column-major
