@@ -85,6 +85,11 @@ def parse():
                         "N = 2^23 (log kernel, ACA tol 1e-8) with and without it and time "
                         "16-RHS matvecs of both (one JSON line)")
     p.add_argument("--train-trees", type=int, default=200)
+    p.add_argument("--assembly-repeat", type=int, default=0,
+                   help="instead of the headline, create/assemble/matvec/destroy an N = 2^20 "
+                        "handle (log kernel, tol 1e-12) this many times, with an N = 2^22 "
+                        "handle (tol 1e-10) every second time, and print the times (one "
+                        "JSON line)")
     p.add_argument("--near-field", action="store_true",
                    help="instead of the headline, time the near-field kernel alone "
                         "(afmm2d_near_field, no assembly) on the same points and print its "
@@ -310,6 +315,41 @@ def rank_model_ab(args, H, E):
                    "margin_log2": margin, "ranks_identical": bool(same)},
         "plain": out["plain"], "learned": out["learned"],
     }))
+
+
+def assembly_repeat(args, H, E):
+    """Assembly times over repeated create/assemble/destroy cycles in one
+    process (allocator behaviour after large frees)."""
+    rng = np.random.default_rng(args.seed)
+    xy = rng.uniform(-1.0, 1.0, size=(1 << 20, 2))
+    xy2 = rng.uniform(-1.0, 1.0, size=(1 << 22, 2))
+    q = rng.uniform(-1.0, 1.0, size=1 << 20)
+    Q = np.asfortranarray(rng.uniform(-1.0, 1.0, size=(1 << 20, 16)))
+    rows = []
+    for it in range(args.assembly_repeat):
+        t0 = time.perf_counter()
+        A = E.AFMM(1 << 20, 64, 12, xy)
+        t1 = time.perf_counter()
+        A.assemble()
+        t2 = time.perf_counter()
+        st = A.stats()
+        A.computeMatVecProduct(q)
+        A.matvec(Q, "fp32")
+        t3 = time.perf_counter()
+        r = {"create_s": round(t1 - t0, 3), "assemble_s": round(t2 - t1, 3),
+             "aca_s": round(st["aca_seconds"], 3), "first_matvecs_s": round(t3 - t2, 3)}
+        if it % 2 == 1:
+            B = E.AFMM(1 << 22, 64, 10, xy2)
+            tb = time.perf_counter()
+            B.assemble()
+            r["assemble_2e22_s"] = round(time.perf_counter() - tb, 3)
+            B.close()
+        A.close()
+        print("bench: iteration %d %s" % (it, r), file=sys.stderr, flush=True)
+        rows.append(r)
+    print(json.dumps({"metric": "assembly seconds, repeated handles", "unit": "s",
+                      "scratch_pool": os.environ.get("AFMM2D_SCRATCH_POOL", "1") != "0",
+                      "iterations": rows}))
 
 
 def inputs(n, seed):
@@ -687,6 +727,9 @@ def main():
         return
     if args.rank_model:
         rank_model_ab(args, H, E)
+        return
+    if args.assembly_repeat > 0:
+        assembly_repeat(args, H, E)
         return
     xy, q = inputs(args.n, args.seed)
 
