@@ -85,7 +85,7 @@ class Stats(ctypes.Structure):
         ("aca_scratch_peak", ctypes.c_int64), ("aca_launches", ctypes.c_int64),
         ("aca_seconds", ctypes.c_double),
         ("m2l_factored_pairs", ctypes.c_int64), ("m2l_factored_bytes", ctypes.c_int64),
-        ("m2l_factored_dense_bytes", ctypes.c_int64),
+        ("m2l_factored_dense_bytes", ctypes.c_int64), ("blocked_m2l_coeffs_bf16", ctypes.c_int64),
     ]
 
     def as_dict(self):

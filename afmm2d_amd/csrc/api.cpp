@@ -1258,7 +1258,8 @@ int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st) {
   st->m2l_leaf_columns = e.assembled ? e.lv[e.L].m2l_cols : 0;
   st->skeleton_leaf = e.assembled ? e.lv[e.L].soff[e.lv[e.L].nb] - e.lv[e.L].soff[0] : 0;
   st->near_waves = e.p2p_waves_per_simd;
-  st->blocked_m2l_coeffs = e.mr_m_coeffs;
+  st->blocked_m2l_coeffs = e.mr_pm_total;
+  st->blocked_m2l_coeffs_bf16 = e.mr_m_coeffs;
   st->blocked_basis_coeffs = e.mr_v_coeffs;
   st->rank_boxes = e.rank_boxes;
   st->rank_fallbacks = e.rank_fallbacks;

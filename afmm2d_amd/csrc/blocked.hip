@@ -120,6 +120,13 @@ __host__ __device__ inline int64_t tile_off(int row, int k, int mbox, int kp) {
   return base + ((int64_t)g * rows + i) * KG + e;
 }
 inline int col_group(int prec) { return prec == PREC_BF16 ? 8 : 4; }
+// FP64 stores every interaction block once (pair-stored M2L, read forward by
+// one box and transposed by the other); FP32 and BF16 store each box's row
+// block (every block twice), whose 16-byte loads both directions share.
+// Either way each direction reads its block from memory, so the traffic is
+// the same; FP64 halves its M2L memory at equal speed, FP32's transposed
+// 4-byte reads measured 2-8% slower than its row blocks.
+inline bool pair_m2l(int prec) { return prec == PREC_F64; }
 constexpr int kMrCtrStride = 32;  // ints between work counters (128 bytes)
 
 // Pointers taken from task records are generic; loads through the global
@@ -176,13 +183,20 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
 
 // ---------------------------------------------------------------------------
 // The batched product C (+)= A B over a task list, one task per wave.
+// A chunk's operands as loaded.  Nothing touches the loaded registers until
+// the chunk is multiplied (masks, signs and BF16 unpacking are applied in
+// gemm_mma): any instruction on them at load time makes the wave wait for
+// the load there, and the chunks meant to be in flight would not be.
 template <int P, int NCT>
 struct GemmFrag {
   using T = MrT<P>;
-  // FP64/FP32: a[rt][s] is A[row][k0 + 4 g + s]; BF16: a[rt] holds 8 k.
-  typename T::A a[4][P == PREC_BF16 ? 8 : 4];
+  // FP64/FP32: a[rt][s] is A[row][k0 + 4 g + s]
+  typename T::A a[P == PREC_BF16 ? 1 : 4][4];
+  mr_u4 au[P == PREC_BF16 ? 4 : 1];                     // BF16: 8 k of row tile rt
   float bf[P == PREC_BF16 ? 8 : 1][NCT];               // BF16: the vector before the split
   typename T::V b[P == PREC_BF16 ? 1 : 4][NCT];         // FP64/FP32: B[k0 + 4 g + s][j]
+  uint32_t bmask;  // bit s (t for BF16): B row valid, else it counts as zero
+  uint32_t neg;    // FP64/FP32: sign bit applied to a (transposed antisymmetric blocks)
 };
 
 template <int P, int NCT, int NRT, int IM>
@@ -199,6 +213,8 @@ __device__ __forceinline__ void gemm_load(GemmFrag<P, NCT>& f, const MrTask& d,
   const auto bidx = gptr(d.bidx);
   // the chunk's tiles (tile_off): a lane group past the chunk's last column
   // group reads the last one again (its B entries are zero: k >= kp >= K)
+  f.neg = 0;
+  f.bmask = 0;
   const int kc = min(KC, d.kp - k0);
   const int ng = kc / KG;
   const int gg = min(g, ng - 1);
@@ -223,13 +239,7 @@ __device__ __forceinline__ void gemm_load(GemmFrag<P, NCT>& f, const MrTask& d,
       f.a[rt][2] = u.z;
       f.a[rt][3] = u.w;
     } else {
-      const mr_u4 u = *((const __attribute__((address_space(1))) mr_u4*)(A + tb + ((int64_t)gg * rows + ii) * 8));
-      const uint32_t w[4] = {u.x, u.y, u.z, u.w};
-#pragma unroll
-      for (int t = 0; t < 4; t++) {
-        f.a[rt][2 * t] = (uint16_t)(w[t] & 0xFFFF);
-        f.a[rt][2 * t + 1] = (uint16_t)(w[t] >> 16);
-      }
+      f.au[rt] = *((const __attribute__((address_space(1))) mr_u4*)(A + tb + ((int64_t)gg * rows + ii) * 8));
     }
   }
 #pragma unroll
@@ -241,12 +251,83 @@ __device__ __forceinline__ void gemm_load(GemmFrag<P, NCT>& f, const MrTask& d,
     if constexpr (IM == 0) row = d.b0 + kc2;
     else if constexpr (IM == 1) row = sidx[kc2];
     else row = bidx[d.b0 + kc2];
+    f.bmask |= (uint32_t)ok << s;
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) {
-      const VT v = B[row * RP + 16 * ct + i];
-      if constexpr (P == PREC_BF16) f.bf[s][ct] = ok ? v : 0.0f;
-      else f.b[s][ct] = ok ? v : (VT)0;
+      if constexpr (P == PREC_BF16) f.bf[s][ct] = B[row * RP + 16 * ct + i];
+      else f.b[s][ct] = B[row * RP + 16 * ct + i];
     }
+  }
+}
+
+// The pair-stored M2L (FP64, FP32): lane group g of chunk k0 reads micro
+// column k0 / 4 + g, a forward or a transposed slice of one pair's block,
+// and the 4 partner rows that micro column multiplies (rows past its valid
+// columns read as zero).  Every lane issues 4 scalar loads per row tile, so
+// the two orientations cost the same and a wave never diverges on them.
+// IM 3: the descriptors staged in LDS (mic = the wave's copy, padded to
+// whole chunks); IM 4: read from global memory (mic = the task's list).
+// One descriptor from global memory (one 16-byte load).
+__device__ __forceinline__ MrMicro micro_at(const MrMicro* p, int idx) {
+  const mr_u4 v = *gptr(reinterpret_cast<const mr_u4*>(p) + idx);
+  MrMicro m;
+  m.a = (int64_t)(((uint64_t)v.y << 32) | (uint64_t)v.x);
+  m.brow = (int)v.z;
+  m.info = (int)v.w;
+  return m;
+}
+
+typedef __attribute__((address_space(3))) mr_u4 lds_u4;
+
+__device__ __forceinline__ MrMicro micro_unpack(const mr_u4 v) {
+  MrMicro m;
+  m.a = (int64_t)(((uint64_t)v.y << 32) | (uint64_t)v.x);
+  m.brow = (int)v.z;
+  m.info = (int)v.w;
+  return m;
+}
+
+template <int P, int NCT, int NRT, int IM>
+__device__ __forceinline__ void gemm_load_micro(GemmFrag<P, NCT>& f, const MrTask& d, int k0,
+                                                int i, int g, const MrMicro* mic) {
+  static_assert(P != PREC_BF16, "the pair-stored M2L is FP64 or FP32");
+  using T = MrT<P>;
+  using AT = typename T::A;
+  using VT = typename T::V;
+  constexpr int RP = 16 * NCT;
+  const auto A = gptr(static_cast<const AT*>(d.A));
+  const auto B = gptr(static_cast<const VT*>(d.B));
+  const int mi = k0 / 4 + g;
+  MrMicro u;
+  if constexpr (IM == 3) {
+    u = micro_unpack(((const lds_u4*)mic)[mi]);  // one 16-byte LDS read
+  } else {
+    const int nm = d.K / 4;
+    u = micro_at(mic, min(mi, nm - 1));
+    if (mi >= nm) u.info &= ~(7 << 25);
+  }
+  const int astr = u.info & 0xFFFF, amax = (u.info >> 16) & 0x1FF, nval = (u.info >> 25) & 7;
+  const bool fwd = (u.info >> 28) & 1;
+  // 4 loads per row tile in either orientation (forward: a micro row's 4
+  // contiguous values; transposed: a micro column's, 4 apart): the same
+  // number of loads on every path keeps the compiler's wait counts exact
+  const int step = fwd ? 1 : 4;
+  const int e0 = fwd ? (i & 3) * 4 : (i & 3);
+#pragma unroll
+  for (int rt = 0; rt < NRT; rt++) {
+    const int ai = min(4 * (d.rt0 + rt) + (i >> 2), amax);
+    const int64_t base = u.a + (int64_t)ai * astr + e0;
+#pragma unroll
+    for (int s = 0; s < 4; s++) f.a[rt][s] = A[base + s * step];
+  }
+  // sign -1 for an antisymmetric kernel's transposed block (exact)
+  f.neg = ((uint32_t)u.info >> 29 & 1u) << 31;
+  f.bmask = (1u << nval) - 1u;
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int64_t row = u.brow + (s < nval ? s : 0);
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) f.b[s][ct] = B[row * RP + 16 * ct + i];
   }
 }
 
@@ -259,14 +340,12 @@ __device__ __forceinline__ void gemm_mma(const GemmFrag<P, NCT>& f,
     for (int ct = 0; ct < NCT; ct++) {
       float x[8];
 #pragma unroll
-      for (int t = 0; t < 8; t++) x[t] = f.bf[t][ct];
+      for (int t = 0; t < 8; t++) x[t] = (f.bmask >> t) & 1 ? f.bf[t][ct] : 0.0f;
       bf16_split(x, bh[ct], bl[ct]);
     }
 #pragma unroll
     for (int rt = 0; rt < NRT; rt++) {
-      mr_bf8 a;
-#pragma unroll
-      for (int t = 0; t < 8; t++) a[t] = __builtin_bit_cast(__bf16, f.a[rt][t]);
+      const mr_bf8 a = __builtin_bit_cast(mr_bf8, f.au[rt]);  // element 2t: low half of word t
 #pragma unroll
       for (int ct = 0; ct < NCT; ct++) {
         acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[ct], acc[rt][ct], 0, 0, 0);
@@ -277,16 +356,23 @@ __device__ __forceinline__ void gemm_mma(const GemmFrag<P, NCT>& f,
 #pragma unroll
     for (int s = 0; s < 4; s++)
 #pragma unroll
-      for (int rt = 0; rt < NRT; rt++)
+      for (int rt = 0; rt < NRT; rt++) {
+        typename MrT<P>::A a = f.a[rt][s];
+        if constexpr (P == PREC_F64) {
+          uint64_t bits = __builtin_bit_cast(uint64_t, a) ^ ((uint64_t)f.neg << 32);
+          a = __builtin_bit_cast(double, bits);
+        } else {
+          a = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, a) ^ f.neg);
+        }
 #pragma unroll
-          for (int ct = 0; ct < NCT; ct++) {
-            if constexpr (P == PREC_F64)
-              acc[rt][ct] =
-                  __builtin_amdgcn_mfma_f64_16x16x4f64(f.a[rt][s], f.b[s][ct], acc[rt][ct], 0, 0, 0);
-            else
-              acc[rt][ct] =
-                  __builtin_amdgcn_mfma_f32_16x16x4f32(f.a[rt][s], f.b[s][ct], acc[rt][ct], 0, 0, 0);
-          }
+        for (int ct = 0; ct < NCT; ct++) {
+          const typename MrT<P>::V bv = (f.bmask >> s) & 1 ? f.b[s][ct] : (typename MrT<P>::V)0;
+          if constexpr (P == PREC_F64)
+            acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[rt][ct], 0, 0, 0);
+          else
+            acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc[rt][ct], 0, 0, 0);
+        }
+      }
   }
 }
 
@@ -304,7 +390,7 @@ __global__ __launch_bounds__(256) void k_mr_gemm(const MrTask* __restrict__ task
   using Acc = typename T::Acc;
   constexpr int RP = 16 * NCT;
   constexpr int KC = T::KC;
-  extern __shared__ int lds_idx[];
+  extern __shared__ __align__(16) int lds_idx[];
   const int l = threadIdx.x & 63, i = l & 15, g = l >> 4;
   const int blk = xcd_map ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   const int w = threadIdx.x >> 6;
@@ -313,9 +399,26 @@ __global__ __launch_bounds__(256) void k_mr_gemm(const MrTask* __restrict__ task
   const MrTask d = tasks[t];
   const int M = d.M, K = d.K;
   int* sidx = lds_idx + w * kmax;
+  // IM 3: kmax = micro descriptors per wave
+  MrMicro* smic = reinterpret_cast<MrMicro*>(lds_idx) + w * kmax;
+  const MrMicro* gmic = reinterpret_cast<const MrMicro*>(d.bidx) + d.b0;
   if constexpr (IM == 1) {
     const auto bidx = gptr(d.bidx);
     for (int k = l; k < K; k += 64) sidx[k] = bidx[d.b0 + k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if constexpr (IM == 3) {
+    // the task's descriptors, padded with a copy of the last one marked
+    // empty up to whole chunks (4 per chunk)
+    const int nm = K / 4, nmp = (K + KC - 1) / KC * 4;
+    lds_u4* sm = (lds_u4*)smic;
+    for (int u = l; u < nmp; u += 64) {
+      mr_u4 x = *gptr(reinterpret_cast<const mr_u4*>(gmic) + min(u, nm - 1));
+      if (u >= nm) x.w &= ~(7u << 25);
+      sm[u] = x;
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -331,17 +434,23 @@ __global__ __launch_bounds__(256) void k_mr_gemm(const MrTask* __restrict__ task
     // multiplied (static indices: the ring position is a compile-time
     // constant in the unrolled loop)
     GemmFrag<P, NCT> f[D];
+    // chunks past the last are loaded again as the last one (cache hits,
+    // never multiplied): every ring slot issues its loads unconditionally,
+    // so the compiler's wait before a chunk's products counts only that
+    // chunk's loads, not the ones just issued for later chunks
+    auto load = [&](GemmFrag<P, NCT>& fr, int ch) {
+      const int k0 = min(ch, nch - 1) * KC;
+      if constexpr (IM == 3) gemm_load_micro<P, NCT, NRT, 3>(fr, d, k0, i, g, smic);
+      else if constexpr (IM == 4) gemm_load_micro<P, NCT, NRT, 4>(fr, d, k0, i, g, gmic);
+      else gemm_load<P, NCT, NRT, IM>(fr, d, k0, i, g, sidx);
+    };
 #pragma unroll
-    for (int u = 0; u < D - 1; u++)
-      if (u < nch) gemm_load<P, NCT, NRT, IM>(f[u], d, u * KC, i, g, sidx);
+    for (int u = 0; u < D - 1; u++) load(f[u], u);
     for (int c = 0; c < nch; c += D) {
 #pragma unroll
       for (int u = 0; u < D; u++) {
-        if (c + u < nch) {
-          if (c + u + D - 1 < nch)
-            gemm_load<P, NCT, NRT, IM>(f[(u + D - 1) % D], d, (c + u + D - 1) * KC, i, g, sidx);
-          gemm_mma<P, NCT, NRT>(f[u], acc);
-        }
+        load(f[(u + D - 1) % D], c + u + D - 1);
+        if (c + u < nch) gemm_mma<P, NCT, NRT>(f[u], acc);
       }
     }
   }
@@ -872,6 +981,87 @@ __global__ __launch_bounds__(256) void k_mr_mprime(const MrBoxMP* __restrict__ b
   }
 }
 
+// Pair-stored M2L blocks (FP64, FP32): element (row, col) of a block with nb
+// micro columns (MrMicro).
+__host__ __device__ inline int64_t micro_off(int row, int col, int nb) {
+  return ((int64_t)(row >> 2) * nb + (col >> 2)) * 16 + (row & 3) * 4 + (col & 3);
+}
+
+struct MrPairB {
+  int64_t poff, s1, s2, r1off, r2off;
+  int r1, r2;
+};
+
+// FP64 block K(skeleton_first[row], skeleton_second[col]), the single-RHS
+// blocks' formula, zero in the padding to whole micro-tiles.
+template <int KID>
+__global__ __launch_bounds__(256) void k_mr_build_pairs(const MrPairB* __restrict__ pairs,
+                                                        const double* __restrict__ skx,
+                                                        const double* __restrict__ sky,
+                                                        double* __restrict__ out, KParams kp) {
+  const MrPairB d = pairs[blockIdx.x];
+  const int R1 = (d.r1 + 3) & ~3, R2 = (d.r2 + 3) & ~3, nb = R2 / 4;
+  const int64_t tot = (int64_t)R1 * R2;
+  for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int m = (int)(e >> 4), w = (int)(e & 15);
+    const int a = m / nb, bb = m - a * nb;
+    const int row = 4 * a + (w >> 2), col = 4 * bb + (w & 3);
+    double v = 0.0;
+    if (row < d.r1 && col < d.r2)
+      v = kernel_xy<KID>(skx[d.s1 + row], sky[d.s1 + row], skx[d.s2 + col], sky[d.s2 + col], kp);
+    out[d.poff + e] = v;
+  }
+}
+
+// The orthonormal basis' block R_first P R_second^T (FP64 arithmetic) of a
+// pair, written in precision P at the block's place in the same layout.
+template <int P>
+__global__ __launch_bounds__(256) void k_mr_pair_prime(const MrPairB* __restrict__ pairs,
+                                                       const double* __restrict__ P64,
+                                                       const double* __restrict__ Rbuf,
+                                                       typename MrT<P>::A* __restrict__ out) {
+  using AT = typename MrT<P>::A;
+  __shared__ double T[4096];
+  const MrPairB d = pairs[blockIdx.x];
+  const int r1 = d.r1, r2 = d.r2;
+  const int R1 = (r1 + 3) & ~3, R2 = (r2 + 3) & ~3, nb = R2 / 4;
+  const double* R1m = Rbuf + d.r1off;
+  const double* R2m = Rbuf + d.r2off;
+  const double* Pm = P64 + d.poff;
+  AT* O = out + d.poff;
+  auto put = [&](int64_t e, double v) {
+    if constexpr (P == PREC_F64) O[e] = v;
+    else if constexpr (P == PREC_F32) O[e] = (float)v;
+    else O[e] = __builtin_bit_cast(uint16_t, (__bf16)(float)v);
+  };
+  // the padding
+  for (int64_t e = threadIdx.x; e < (int64_t)R1 * R2; e += blockDim.x) {
+    const int m = (int)(e >> 4), w = (int)(e & 15);
+    const int a = m / nb, bb = m - a * nb;
+    if (4 * a + (w >> 2) >= r1 || 4 * bb + (w & 3) >= r2) put(e, 0.0);
+  }
+  const int RI = max(1, min(r1, 4096 / max(r2, 1)));  // rows of T per chunk
+  for (int i0 = 0; i0 < r1; i0 += RI) {
+    const int ni = min(RI, r1 - i0);
+    __syncthreads();
+    // T[ii][w] = sum_{p >= i} R_first[i][p] P[p][w]
+    for (int e = threadIdx.x; e < ni * r2; e += blockDim.x) {
+      const int ii = e / r2, wv = e - ii * r2, ix = i0 + ii;
+      double s = 0.0;
+      for (int p = ix; p < r1; p++) s = fma(R1m[(int64_t)ix * r1 + p], Pm[micro_off(p, wv, nb)], s);
+      T[ii * r2 + wv] = s;
+    }
+    __syncthreads();
+    // out[i][u] = sum_{w >= u} T[i][w] R_second[u][w]
+    for (int e = threadIdx.x; e < ni * r2; e += blockDim.x) {
+      const int ii = e / r2, u = e - ii * r2;
+      double s = 0.0;
+      for (int wv = u; wv < r2; wv++) s = fma(T[ii * r2 + wv], R2m[(int64_t)u * r2 + wv], s);
+      put(micro_off(i0 + ii, u, nb), s);
+    }
+  }
+}
+
 template <class T>
 void upload_vec(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
   d.alloc(sizeof(T) * std::max<size_t>(h.size(), 1));
@@ -893,28 +1083,158 @@ int slot_box(const Engine& e, int j, int k, int q) {
   return q < 16 ? e.inner[j][16 * (size_t)k + q] : e.outer[j][24 * (size_t)k + q - 16];
 }
 
-// Layout shared by all precisions: offsets, leading dimensions, the column
-// index of every M2L row block.
+// Layout shared by all precisions: the bases' offsets and leading
+// dimensions, and the pair-stored M2L of FP64 (blocks, per-box micro
+// descriptors).
 void mr_layout(Engine& e) {
   if (e.mr_layout) return;
   const int L = e.L;
   const int64_t nbox = e.lv[L].gbase + e.lv[L].nb;
   e.mr_voff.assign(nbox + 1, 0);
   e.mr_vtoff.assign(nbox + 1, 0);
-  e.mr_moff.assign(nbox + 1, 0);
-  e.mr_coloff.assign(nbox + 1, 0);
   e.mr_vld.assign(nbox, 8);
   e.mr_vtld.assign(nbox, 8);
-  e.mr_mld.assign(nbox, 8);
-  e.mr_K.assign(nbox, 0);
-  std::vector<int> colidx;
-  int64_t vo = 0, vto = 0, mo = 0;
-  e.mr_m_coeffs = e.mr_v_coeffs = 0;
+  int64_t vo = 0, vto = 0;
+  e.mr_v_coeffs = 0;
   for (int j = 2; j <= L; j++) {
     const LevelInfo& li = e.lv[j];
     for (int k = 0; k < li.nb; k++) {
       const int64_t g = li.gbase + k;
       const int r = li.rank[k], n = li.nin[k];
+      e.mr_vld[g] = round8(std::max(r, 1));
+      e.mr_vtld[g] = round8(std::max(n, 1));
+      e.mr_voff[g] = vo;
+      e.mr_vtoff[g] = vto;
+      vo += (int64_t)n * e.mr_vld[g];
+      vto += (int64_t)r * e.mr_vtld[g];
+      e.mr_v_coeffs += (int64_t)n * r;
+    }
+  }
+  e.mr_voff[nbox] = vo;
+  e.mr_vtoff[nbox] = vto;
+  e.mr_v_total = vo;
+  e.mr_vt_total = vto;
+  // pairs: (k, b) with k < b in the interaction list, first = k; blocks in
+  // level, box, slot order
+  e.mr_pairs.clear();
+  int64_t po = 0;
+  for (int j = 2; j <= L; j++) {
+    const LevelInfo& li = e.lv[j];
+    for (int k = 0; k < li.nb; k++) {
+      const int r1 = li.rank[k];
+      if (r1 <= 0) continue;
+      for (int q = 0; q < 40; q++) {
+        const int b = slot_box(e, j, k, q);
+        if (b <= k) continue;
+        const int r2 = li.rank[b];
+        if (r2 <= 0) continue;
+        Engine::PairBlk pb;
+        pb.poff = po;
+        pb.s1 = li.soff[k];
+        pb.s2 = li.soff[b];
+        pb.g1 = li.gbase + k;
+        pb.g2 = li.gbase + b;
+        pb.r1 = r1;
+        pb.r2 = r2;
+        e.mr_pairs.push_back(pb);
+        po += (int64_t)((r1 + 3) & ~3) * ((r2 + 3) & ~3);
+      }
+    }
+  }
+  e.mr_pm_total = po;
+  // the box-major layout (FP32, BF16) holds every block twice, unpadded
+  e.mr_m_coeffs = 0;
+  for (const Engine::PairBlk& pb : e.mr_pairs) e.mr_m_coeffs += 2 * (int64_t)pb.r1 * pb.r2;
+  // per box: its interaction slots in order, a micro column per 4 partner rows
+  std::vector<MrMicro> mic;
+  e.mr_uoff.assign(nbox + 1, 0);
+  size_t pi = 0;  // pairs were made in the same (level, box, slot) order
+  std::vector<std::vector<int64_t>> first_off(L + 1);
+  for (int j = 2; j <= L; j++) first_off[j].assign((size_t)e.lv[j].nb * 40, -1);
+  for (int j = 2; j <= L; j++) {
+    const LevelInfo& li = e.lv[j];
+    for (int k = 0; k < li.nb; k++) {
+      if (li.rank[k] <= 0) continue;
+      for (int q = 0; q < 40; q++) {
+        const int b = slot_box(e, j, k, q);
+        if (b <= k || li.rank[b] <= 0) continue;
+        first_off[j][(size_t)k * 40 + q] = e.mr_pairs[pi++].poff;
+      }
+    }
+  }
+  for (int j = 2; j <= L; j++) {
+    const LevelInfo& li = e.lv[j];
+    for (int k = 0; k < li.nb; k++) {
+      const int64_t g = li.gbase + k;
+      e.mr_uoff[g] = (int64_t)mic.size();
+      const int r = li.rank[k];
+      if (r <= 0) continue;
+      const int Rk = (r + 3) & ~3;
+      for (int q = 0; q < 40; q++) {
+        const int b = slot_box(e, j, k, q);
+        if (b < 0) continue;
+        const int rb = li.rank[b];
+        if (rb <= 0) continue;
+        const int Rb = (rb + 3) & ~3;
+        if (b > k) {
+          // k first: micro columns of its block, forward
+          const int64_t poff = first_off[j][(size_t)k * 40 + q];
+          for (int bb = 0; bb < Rb / 4; bb++) {
+            MrMicro m;
+            m.a = poff + 16 * (int64_t)bb;
+            m.brow = (int)(li.soff[b] + 4 * bb);
+            const int nval = std::min(4, rb - 4 * bb);
+            m.info = (Rb / 4 * 16) | ((Rk / 4 - 1) << 16) | (nval << 25) | (1 << 28);
+            mic.push_back(m);
+          }
+        } else {
+          // k second: b's block read transposed, a micro column per micro
+          // row of the block; find the slot of k in b's list
+          int64_t poff = -1;
+          for (int q2 = 0; q2 < 40 && poff < 0; q2++)
+            if (slot_box(e, j, b, q2) == k) poff = first_off[j][(size_t)b * 40 + q2];
+          if (poff < 0) throw Error(6, "interaction lists are not symmetric");
+          const int nbk = Rk / 4;  // micro columns of b's block (k's rows)
+          for (int aa = 0; aa < Rb / 4; aa++) {
+            MrMicro m;
+            m.a = poff + (int64_t)aa * nbk * 16;
+            m.brow = (int)(li.soff[b] + 4 * aa);
+            const int nval = std::min(4, rb - 4 * aa);
+            // an antisymmetric kernel's transposed block enters with sign -1
+            m.info = 16 | ((nbk - 1) << 16) | (nval << 25) | (e.kid == K_USER_ANTI ? 1 << 29 : 0);
+            mic.push_back(m);
+          }
+        }
+      }
+    }
+  }
+  e.mr_uoff[nbox] = (int64_t)mic.size();
+  for (const Engine::PairBlk& pb : e.mr_pairs)
+    if ((pb.r1 + 3) / 4 > 512 || (pb.r2 + 3) / 4 > 512 || ((pb.r2 + 3) & ~3) * 4 > 0xFFFF)
+      throw Error(6, "rank above the pair-stored M2L's range");
+  if (e.skel_total > INT32_MAX) throw Error(6, "skeleton count exceeds 2^31");
+  upload_vec(e.mr_micro, mic, e.s_aux);
+  e.mr_layout = true;
+}
+
+// The box-major M2L row blocks of FP32 and BF16: offsets, leading dimensions,
+// the column index of every row block (each block stored for both boxes).
+void mr_layout_boxmajor(Engine& e) {
+  if (e.mr_boxmajor_layout) return;
+  mr_layout(e);
+  const int L = e.L;
+  const int64_t nbox = e.lv[L].gbase + e.lv[L].nb;
+  e.mr_moff.assign(nbox + 1, 0);
+  e.mr_coloff.assign(nbox + 1, 0);
+  e.mr_mld.assign(nbox, 8);
+  e.mr_K.assign(nbox, 0);
+  std::vector<int> colidx;
+  int64_t mo = 0;
+  for (int j = 2; j <= L; j++) {
+    const LevelInfo& li = e.lv[j];
+    for (int k = 0; k < li.nb; k++) {
+      const int64_t g = li.gbase + k;
+      const int r = li.rank[k];
       int K = 0;
       e.mr_coloff[g] = (int64_t)colidx.size();
       for (int q = 0; q < 40; q++) {
@@ -925,30 +1245,17 @@ void mr_layout(Engine& e) {
         K += rb;
       }
       e.mr_K[g] = K;
-      e.mr_vld[g] = round8(std::max(r, 1));
-      e.mr_vtld[g] = round8(std::max(n, 1));
       e.mr_mld[g] = round8(std::max(K, 1));
-      e.mr_voff[g] = vo;
-      e.mr_vtoff[g] = vto;
       e.mr_moff[g] = mo;
-      vo += (int64_t)n * e.mr_vld[g];
-      vto += (int64_t)r * e.mr_vtld[g];
       mo += (int64_t)r * e.mr_mld[g];
-      e.mr_m_coeffs += (int64_t)r * K;
-      e.mr_v_coeffs += (int64_t)n * r;
     }
   }
-  e.mr_voff[nbox] = vo;
-  e.mr_vtoff[nbox] = vto;
   e.mr_moff[nbox] = mo;
   e.mr_coloff[nbox] = (int64_t)colidx.size();
-  e.mr_v_total = vo;
-  e.mr_vt_total = vto;
   e.mr_m_total = mo;
   if ((int64_t)colidx.size() > INT32_MAX) throw Error(6, "M2L column index exceeds 2^31 entries");
-  hipStream_t s = e.s_aux;
-  upload_vec(e.mr_colidx, colidx, s);
-  e.mr_layout = true;
+  upload_vec(e.mr_colidx, colidx, e.s_aux);
+  e.mr_boxmajor_layout = true;
 }
 
 // The row-major FP64 basis V (nin x rank, leading dimension vld) of every
@@ -1022,11 +1329,26 @@ void alloc_factors(Engine& e, int prec, MrFactors& f) {
   hipStream_t s = e.s_aux;
   f.V.alloc(fb * std::max<int64_t>(e.mr_v_total, 8));
   f.Vt.alloc(fb * std::max<int64_t>(e.mr_vt_total, 8));
-  f.M.alloc(fb * std::max<int64_t>(e.mr_m_total, 8));
+  f.M.alloc(fb * std::max<int64_t>(pair_m2l(prec) ? e.mr_pm_total : e.mr_m_total, 8));
   // boxes of rank 0 keep zero factors (their tasks read none)
   AFMM_HIP(hipMemsetAsync(f.V.p, 0, f.V.bytes, s));
   AFMM_HIP(hipMemsetAsync(f.Vt.p, 0, f.Vt.bytes, s));
   AFMM_HIP(hipMemsetAsync(f.M.p, 0, f.M.bytes, s));
+}
+
+std::vector<MrPairB> pair_records(const Engine& e) {
+  std::vector<MrPairB> v;
+  v.reserve(e.mr_pairs.size());
+  for (const Engine::PairBlk& pb : e.mr_pairs) {
+    MrPairB d{};
+    d.poff = pb.poff;
+    d.s1 = pb.s1;
+    d.s2 = pb.s2;
+    d.r1 = pb.r1;
+    d.r2 = pb.r2;
+    v.push_back(d);
+  }
+  return v;
 }
 
 void build_f64(Engine& e) {
@@ -1038,8 +1360,28 @@ void build_f64(Engine& e) {
     alloc_factors(e, PREC_F64, f);
     retile_basis(e, PREC_F64, Vrm, f);
   }
+  hipStream_t s = e.s_aux;
+  std::vector<MrPairB> pr = pair_records(e);
+  if (!pr.empty()) {
+    DevBuf dpr;
+    upload_vec(dpr, pr, s);
+    AFMM_DISPATCH_KERNEL(e.kid, KID,
+                         k_mr_build_pairs<KID><<<(unsigned)pr.size(), 256, 0, s>>>(
+                             dpr.as<MrPairB>(), e.skx.as<double>(), e.sky.as<double>(),
+                             f.M.as<double>(), e.kp));
+    AFMM_LAUNCH_CHECK();
+  }
+  AFMM_HIP(hipStreamSynchronize(s));
+  f.built = true;
+}
+
+// FP64 box-major M2L row blocks (the FP32 / BF16 transforms' source) into M64.
+void build_boxmajor64(Engine& e, DevBuf& M64) {
+  mr_layout_boxmajor(e);
   const int L = e.L;
   hipStream_t s = e.s_aux;
+  M64.alloc(sizeof(double) * std::max<int64_t>(e.mr_m_total, 8));
+  AFMM_HIP(hipMemsetAsync(M64.p, 0, M64.bytes, s));
   std::vector<MrBoxM> bm;
   for (int j = 2; j <= L; j++) {
     const LevelInfo& li = e.lv[j];
@@ -1056,18 +1398,18 @@ void build_f64(Engine& e) {
     AFMM_DISPATCH_KERNEL(e.kid, KID,
                          k_mr_build_m64<KID><<<(unsigned)bm.size(), 256, 0, s>>>(
                              dbm.as<MrBoxM>(), e.skx.as<double>(), e.sky.as<double>(),
-                             e.mr_colidx.as<int>(), f.M.as<double>(), e.kp));
+                             e.mr_colidx.as<int>(), M64.as<double>(), e.kp));
     AFMM_LAUNCH_CHECK();
   }
   AFMM_HIP(hipStreamSynchronize(s));
-  f.built = true;
 }
 
 // The orthonormal basis (R of every box) and the reduced-precision factors.
 void build_reduced(Engine& e, int prec) {
   MrFactors& f = e.mrf[prec];
   if (f.built) return;
-  build_f64(e);  // the FP64 M2L row blocks are the transform's source
+  build_f64(e);
+  if (!pair_m2l(prec)) mr_layout_boxmajor(e);
   DevBuf Vrm;
   build_vrm64(e, Vrm);
   const int L = e.L;
@@ -1130,6 +1472,29 @@ void build_reduced(Engine& e, int prec) {
   e.mr_ortho = true;
   alloc_factors(e, prec, f);
   retile_basis(e, prec, Q64, f);
+  if (pair_m2l(prec)) {
+    std::vector<MrPairB> pr = pair_records(e);
+    for (size_t u = 0; u < pr.size(); u++) {
+      pr[u].r1off = e.mr_roff[e.mr_pairs[u].g1];
+      pr[u].r2off = e.mr_roff[e.mr_pairs[u].g2];
+    }
+    if (!pr.empty()) {
+      DevBuf dpr;
+      upload_vec(dpr, pr, s);
+      if (prec == PREC_F32)
+        k_mr_pair_prime<PREC_F32><<<(unsigned)pr.size(), 256, 0, s>>>(
+            dpr.as<MrPairB>(), e.mrf[PREC_F64].M.as<double>(), e.mr_R.as<double>(),
+            f.M.as<float>());
+      else
+        throw Error(6, "pair-stored M2L in this precision");
+      AFMM_LAUNCH_CHECK();
+    }
+    AFMM_HIP(hipStreamSynchronize(s));
+    f.built = true;
+    return;
+  }
+  DevBuf M64;
+  build_boxmajor64(e, M64);
   std::vector<MrBoxMP> bm;
   for (int j = 2; j <= L; j++) {
     const LevelInfo& li = e.lv[j];
@@ -1165,10 +1530,10 @@ void build_reduced(Engine& e, int prec) {
     upload_vec(dbm, bm, s);
     const unsigned nb = (unsigned)bm.size();
     if (prec == PREC_F32)
-      k_mr_mprime<PREC_F32><<<nb, 256, 0, s>>>(dbm.as<MrBoxMP>(), e.mrf[PREC_F64].M.as<double>(),
+      k_mr_mprime<PREC_F32><<<nb, 256, 0, s>>>(dbm.as<MrBoxMP>(), M64.as<double>(),
                                               e.mr_R.as<double>(), f.M.as<float>());
     else
-      k_mr_mprime<PREC_BF16><<<nb, 256, 0, s>>>(dbm.as<MrBoxMP>(), e.mrf[PREC_F64].M.as<double>(),
+      k_mr_mprime<PREC_BF16><<<nb, 256, 0, s>>>(dbm.as<MrBoxMP>(), M64.as<double>(),
                                                e.mr_R.as<double>(), f.M.as<uint16_t>());
     AFMM_LAUNCH_CHECK();
   }
@@ -1237,14 +1602,16 @@ void add_row_groups(std::vector<MrTask>& out, const void* A, int kg, const void*
     t.mbox = M;
     t.rt0 = r0 / 16;
     t.beta = beta;
+    t.imode = bidx ? 1 : 0;
+    t.pad = 0;
     out.push_back(t);
   }
 }
 
 inline int task_k(const MrTask& t) { return t.K; }
 inline int task_k(const MrNearTask&) { return 0; }
-inline bool task_indexed(const MrTask& t) { return t.bidx != nullptr; }
-inline bool task_indexed(const MrNearTask&) { return false; }
+inline int task_mode(const MrTask& t) { return t.imode; }
+inline int task_mode(const MrNearTask&) { return 0; }
 
 // Orders a task list by row tiles and uploads it.
 template <class T>
@@ -1253,10 +1620,13 @@ void upload_tasks(MrTaskList& tl, std::vector<T>& tasks, hipStream_t s) {
                    [](const T& a, const T& b) { return (a.M + 15) / 16 < (b.M + 15) / 16; });
   tl.n = (int)tasks.size();
   for (int c = 0; c < 4; c++) tl.kmax[c] = 0;
-  tl.indexed = false;
+  tl.mode = 0;
   for (const T& t : tasks) {
     tl.kmax[(t.M + 15) / 16 - 1] = std::max(tl.kmax[(t.M + 15) / 16 - 1], task_k(t));
-    tl.indexed = tl.indexed || task_indexed(t);
+    if (task_mode(t) != 0) {
+      if (tl.mode != 0 && tl.mode != task_mode(t)) throw Error(6, "mixed task modes in one list");
+      tl.mode = task_mode(t);
+    }
   }
   for (int c = 0; c < 5; c++) tl.cls[c] = tl.n;
   for (int t = tl.n - 1; t >= 0; t--) tl.cls[(tasks[t].M + 15) / 16 - 1] = t;
@@ -1270,6 +1640,25 @@ void m2l_tasks(const Engine& e, const MrPlan& p, int rows, std::vector<MrTask>& 
   const MrFactors& f = e.mrf[p.prec];
   const size_t fb = factor_bytes(p.prec);
   const int kg = col_group(p.prec);
+  if (pair_m2l(p.prec)) {
+    // pair-stored blocks through the box's micro descriptors: K = 4 per
+    // descriptor
+    const MrMicro* mic = e.mr_micro.as<MrMicro>();
+    for (int j = 2; j <= e.L; j++) {
+      const LevelInfo& li = e.lv[j];
+      for (int k = 0; k < li.nb; k++) {
+        const int64_t g = li.gbase + k;
+        const int r = li.rank[k];
+        if (r <= 0) continue;
+        const int64_t u0 = e.mr_uoff[g], nu = e.mr_uoff[g + 1] - u0;
+        const size_t first = out.size();
+        add_row_groups(out, f.M.p, kg, p.outv.p, reinterpret_cast<const int*>(mic), u0, p.inv.p,
+                       li.soff[k], r, (int)(4 * nu), 0, rows);
+        for (size_t t = first; t < out.size(); t++) out[t].imode = 3;
+      }
+    }
+    return;
+  }
   for (int j = 2; j <= e.L; j++) {
     const LevelInfo& li = e.lv[j];
     for (int k = 0; k < li.nb; k++) {
@@ -1405,8 +1794,20 @@ void launch_gemm(const MrTaskList& tl, int xcd, int depth, hipStream_t s) {
     const int n = tl.cls[c + 1] - tl.cls[c];
     if (n <= 0) continue;
     const MrTask* t = tl.buf.as<MrTask>() + tl.cls[c];
-    if (!tl.indexed) {
+    if (tl.mode == 0) {
       launch_gemm_d<P, NCT, 0, 3>(t, n, c, xcd, 0, 0, s);
+    } else if (tl.mode == 3) {
+      if constexpr (P == PREC_BF16) {
+        throw Error(6, "pair-stored M2L tasks in BF16");
+      } else {
+        // descriptors per wave: the largest task's, padded to whole chunks
+        const int kmm = std::max(4, (tl.kmax[c] + 15) / 16 * 4);
+        const size_t shm = sizeof(MrMicro) * 4 * (size_t)kmm;
+        if (shm > kMrIdxLds) launch_gemm_d<P, NCT, 4, 3>(t, n, c, xcd, 0, 0, s);
+        else if (depth >= 6) launch_gemm_d<P, NCT, 3, 6>(t, n, c, xcd, kmm, shm, s);
+        else if (depth == 4) launch_gemm_d<P, NCT, 3, 4>(t, n, c, xcd, kmm, shm, s);
+        else launch_gemm_d<P, NCT, 3, 3>(t, n, c, xcd, kmm, shm, s);
+      }
     } else {
       const int kmax = std::max(1, tl.kmax[c]);
       const size_t shm = sizeof(int) * 4 * (size_t)kmax;

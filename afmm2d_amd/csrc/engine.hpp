@@ -163,6 +163,25 @@ struct MrTask {
   void* C;
   int64_t b0, c0;
   int M, K, kp, mbox, rt0, beta;
+  // 0: B rows b0 + k; 1: through the index list bidx; 3: A and B through the
+  // micro-column descriptors (MrMicro) at bidx + b0 (the pair-stored M2L)
+  int imode, pad;
+};
+
+// One micro column (4 columns) of a box's M2L in the FP64 blocked passes.  Every unordered interaction pair (first, second) stores its block
+// K(skeleton_first, skeleton_second) once, ceil4(r_first) x ceil4(r_second)
+// in 4 x 4 micro-tiles (row-major inside, micro (a, b) at 16 (a nb + b),
+// nb = ceil4(r_second) / 4), zero padded.  The first box reads a micro
+// column of it (row i of its task: micro row i / 4, 4 contiguous values);
+// the second box reads it transposed (its row i is the block's column i:
+// micro column i / 4 of a micro row, 4 values 4 apart).
+struct MrMicro {
+  int64_t a;  // element offset of micro row 0 of this micro column
+  int brow;   // the partner's skeleton row of the micro column's first column
+  int info;   // bits 0-15: elements between micro rows of the task; 16-24: the
+              // last micro row; 25-27: valid columns (1-4, 0 = padding);
+              // 28: forward (1) or transposed (0); 29: negated (the
+              // transposed block of an antisymmetric kernel)
 };
 
 // Near field of one group of up to 64 target points (KD-sorted indices ta ..
@@ -216,7 +235,7 @@ struct MrTaskList {
   int n = 0;
   int cls[5] = {0, 0, 0, 0, 0};
   int kmax[4] = {0, 0, 0, 0};  // largest K per class
-  bool indexed = false;        // B rows through an index list
+  int mode = 0;                // MrTask::imode of its tasks (one per list)
 };
 
 // Task lists and vectors of the blocked matvec for one (precision, column
@@ -416,6 +435,17 @@ struct Engine {
   int64_t mr_v_total = 0, mr_vt_total = 0, mr_m_total = 0;     // elements incl. padding
   int64_t mr_m_coeffs = 0, mr_v_coeffs = 0;                    // sum rank K, sum nin rank
   DevBuf mr_colidx, inv_perm_d;
+  // pair-stored M2L (FP64): per gid its micro descriptors; per pair
+  // its block (element offset, skeleton offsets, ranks)
+  bool mr_boxmajor_layout = false;  // mr_moff, mr_coloff, mr_K, mr_colidx (FP32, BF16) built
+  std::vector<int64_t> mr_uoff;  // per gid (+1): first descriptor
+  struct PairBlk {
+    int64_t poff, s1, s2, g1, g2;  // block offset, skeleton offsets, gids
+    int r1, r2;
+  };
+  std::vector<PairBlk> mr_pairs;
+  int64_t mr_pm_total = 0;  // pair-layout elements (zero padding included)
+  DevBuf mr_micro;
   MrFactors mrf[kPrecisions];
   std::vector<std::unique_ptr<MrPlan>> mr_plans;
   // orthonormal basis (FP32 / BF16): R of every box, rank x rank row-major

@@ -194,7 +194,7 @@ def batch_config(args, H, E):
     alg = 0
     for p in probs:
         st = p.stats()
-        alg += 8 * (st["blocked_m2l_coeffs"] + 2 * st["blocked_basis_coeffs"]) + 16 * p.N * R
+        alg += 8 * (st["m2l_bytes"] // 8 + 2 * st["blocked_basis_coeffs"]) + 16 * p.N * R
     ach = alg / (ms_graph * 1e-3) / 1e9
     B.close()
     for a in dB + dY:
@@ -487,9 +487,12 @@ def run_blocked(A, Q, steps, rows, ref_rows, H, cols=(16, 32)):
             Y = dY.download(np.empty((n, R), order="F"))
             err = float(np.linalg.norm(Y[rows, 0] - ref_rows) / np.linalg.norm(ref_rows))
             rp = 16 if R <= 16 else 32
-            # algorithmic bytes of the M2L launch: each box's M2L row block
-            # once, the multipoles read once, the locals written once
-            alg = st["blocked_m2l_coeffs"] * fb + 2 * st["skeleton_total"] * rp * vb
+            # algorithmic bytes of the M2L launch: every stored interaction
+            # block once (FP64: each block stored once, read by both of its
+            # boxes; FP32, BF16: stored for both boxes), the multipoles read
+            # once, the locals written once
+            coeffs = st["m2l_bytes"] // 8 if prec == "fp64" else st["blocked_m2l_coeffs_bf16"]
+            alg = coeffs * fb + 2 * st["skeleton_total"] * rp * vb
             ach = alg / (m2l_ms * 1e-3) / 1e9 if m2l_ms > 0 else 0.0
             out["%s_r%d" % (prec, R)] = {
                 "value": round(n * R / (ms * 1e-3), 1), "ms_per_pass": round(ms, 4),

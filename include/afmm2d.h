@@ -100,8 +100,8 @@ typedef struct {
   int64_t m2l_leaf_columns;  /* their partial columns (rank_k + rank_b per pair) */
   int64_t skeleton_leaf;     /* sum of the leaf level's ranks */
   int64_t near_waves;        /* near-field waves per SIMD used beside the M2L */
-  int64_t blocked_m2l_coeffs;   /* blocked passes: M2L row-block coefficients, sum rank_k K_k
-                                   (each interaction block stored for both of its boxes) */
+  int64_t blocked_m2l_coeffs;   /* blocked FP64 passes: stored M2L coefficients (each
+                                   interaction block once, padded to 4 x 4 tiles) */
   int64_t blocked_basis_coeffs; /* blocked passes: basis coefficients, sum nin_k rank_k */
   int64_t rank_boxes;           /* boxes with an ACA block (min(rows, columns) > 0) */
   int64_t rank_fallbacks;       /* of them, rerun at full capacity (learned capacities) */
@@ -111,6 +111,9 @@ typedef struct {
   int64_t m2l_factored_pairs;   /* interaction pairs stored as ACA factors U V^T */
   int64_t m2l_factored_bytes;   /* their factor bytes (included in m2l_bytes) */
   int64_t m2l_factored_dense_bytes; /* the bytes their dense blocks would take */
+  int64_t blocked_m2l_coeffs_bf16;  /* blocked FP32 and BF16 passes: M2L row-block
+                                       coefficients, sum rank_k K_k (each block stored for
+                                       both boxes) */
 } afmm2d_stats;
 
 /* Message of the calling thread's last failure ("" if none). */

@@ -57,7 +57,9 @@ def test_fp64_blocked_matches_single_rhs_and_oracle(engine, oracle_mod, n, leaf,
     for c in range(min(nrhs, 3)):
         assert rel(Yb[:, c], T.matvec(Q[:, c])) <= 1e-11
     st = A.stats()
-    assert st["blocked_m2l_coeffs"] == 2 * st["m2l_bytes"] // 8  # both directions stored
+    # FP64: every block once, padded to 4 x 4 tiles; FP32/BF16: both directions
+    assert st["m2l_bytes"] // 8 <= st["blocked_m2l_coeffs"] <= 1.5 * st["m2l_bytes"] // 8
+    assert st["blocked_m2l_coeffs_bf16"] == 2 * st["m2l_bytes"] // 8
     A.close()
     T.close()
 
