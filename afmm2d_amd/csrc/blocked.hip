@@ -524,96 +524,95 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
       sh[rt][ct] = Acc{};
       sl[rt][ct] = Acc{};
     }
-  for (int src = 0; src < nsrc; src++) {
+  // The sources as a sequence of steps (source src, chunk k0): the loads of
+  // the next step are issued before the current one is evaluated, into the
+  // other of two buffers (a step past the last reloads the last one and is
+  // not evaluated).  Loaded registers are not touched until their step runs:
+  // padding sources are masked at the product (bit s of okm).
+  constexpr int KS = P == PREC_BF16 ? 8 : 4;  // sources per lane group and step
+  struct Buf {
+    double sx[KS], sy[KS];
+    float qf[P == PREC_BF16 ? KS : 1][NCT];
+    VT q[P == PREC_BF16 ? 1 : KS][NCT];
+    uint32_t okm;
+  };
+  auto load = [&](Buf& b, int src, int k0) {
     const int st = __builtin_amdgcn_readfirstlane(d.ss[src]);
     const int cnt = __builtin_amdgcn_readfirstlane(d.sc[src]);
-    if constexpr (P != PREC_F64) {
+    b.okm = 0;
 #pragma unroll
-      for (int rt = 0; rt < 4; rt++)
+    for (int u = 0; u < KS; u++) {
+      const int k = k0 + KS * g + u;
+      const bool ok = k < cnt;
+      const int kc = st + (ok ? k : cnt - 1);
+      b.okm |= (uint32_t)ok << u;
+      b.sx[u] = X[kc];
+      b.sy[u] = Y[kc];
 #pragma unroll
-        for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = Acc{};
+      for (int ct = 0; ct < NCT; ct++) {
+        if constexpr (P == PREC_BF16) b.qf[u][ct] = Q[(int64_t)kc * RP + 16 * ct + i];
+        else b.q[u][ct] = Q[(int64_t)kc * RP + 16 * ct + i];
+      }
     }
-    for (int k0 = 0; k0 < cnt; k0 += KC) {
-      if constexpr (P == PREC_BF16) {
-        float bq[8][NCT];
-        double sx[8], sy[8];
+  };
+  auto compute = [&](const Buf& b) {
+    if constexpr (P == PREC_BF16) {
+      mr_bf8 bh[NCT], bl[NCT];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          const int k = k0 + 8 * g + u;
-          const bool ok = k < cnt;
-          const int kc = st + (ok ? k : cnt - 1);
-          sx[u] = X[kc];
-          sy[u] = Y[kc];
+      for (int ct = 0; ct < NCT; ct++) {
+        float x[8];
 #pragma unroll
-          for (int ct = 0; ct < NCT; ct++) {
-            const float v = Q[(int64_t)kc * RP + 16 * ct + i];
-            bq[u][ct] = ok ? v : 0.0f;
-          }
-        }
-        mr_bf8 bh[NCT], bl[NCT];
+        for (int u = 0; u < 8; u++) x[u] = (b.okm >> u) & 1 ? b.qf[u][ct] : 0.0f;
+        bf16_split(x, bh[ct], bl[ct]);
+      }
+#pragma unroll
+      for (int rt = 0; rt < NRT; rt++) {
+        float kv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          kv[u] = near_kernel_f32<KID>(xt[rt], yt[rt], b.sx[u], b.sy[u], kp, c0, c1);
+        const mr_bf8 a = bf16_pack(kv);
 #pragma unroll
         for (int ct = 0; ct < NCT; ct++) {
-          float x[8];
-#pragma unroll
-          for (int u = 0; u < 8; u++) x[u] = bq[u][ct];
-          bf16_split(x, bh[ct], bl[ct]);
+          acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[ct], acc[rt][ct], 0, 0, 0);
+          acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl[ct], acc[rt][ct], 0, 0, 0);
         }
+      }
+    } else {
+      VT bq[4][NCT];
 #pragma unroll
-        for (int rt = 0; rt < NRT; rt++) {
-          float kv[8];
+      for (int u = 0; u < 4; u++)
 #pragma unroll
-          for (int u = 0; u < 8; u++) kv[u] = near_kernel_f32<KID>(xt[rt], yt[rt], sx[u], sy[u], kp, c0, c1);
-          const mr_bf8 a = bf16_pack(kv);
+        for (int ct = 0; ct < NCT; ct++) bq[u][ct] = (b.okm >> u) & 1 ? b.q[u][ct] : (VT)0;
 #pragma unroll
-          for (int ct = 0; ct < NCT; ct++) {
-            acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[ct], acc[rt][ct], 0, 0, 0);
-            acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl[ct], acc[rt][ct], 0, 0, 0);
-          }
-        }
-      } else {
-        VT bq[4][NCT];
-        double sx[4], sy[4];
+      for (int rt = 0; rt < NRT; rt++) {
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-          const int k = k0 + 4 * g + s;
-          const bool ok = k < cnt;
-          const int kc = st + (ok ? k : cnt - 1);
-          sx[s] = X[kc];
-          sy[s] = Y[kc];
-#pragma unroll
-          for (int ct = 0; ct < NCT; ct++) {
-            const VT v = Q[(int64_t)kc * RP + 16 * ct + i];
-            bq[s][ct] = ok ? v : (VT)0;
-          }
-        }
-#pragma unroll
-        for (int rt = 0; rt < NRT; rt++) {
-          {
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-              if constexpr (P == PREC_F64) {
-                double kv;
-                if constexpr (KID >= K_USER) {
-                  kv = kernel_xy<KID>(xt[rt], yt[rt], sx[s], sy[s], kp);
-                } else {
-                  const double dx = xt[rt] - sx[s], dy = yt[rt] - sy[s];
-                  kv = scale * near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab,
-                                                ln2_hi, ln2_lo);
-                }
-#pragma unroll
-                for (int ct = 0; ct < NCT; ct++)
-                  acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv, bq[s][ct], acc[rt][ct], 0, 0, 0);
-              } else {
-                const float kv = near_kernel_f32<KID>(xt[rt], yt[rt], sx[s], sy[s], kp, c0, c1);
-#pragma unroll
-                for (int ct = 0; ct < NCT; ct++)
-                  acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(kv, bq[s][ct], acc[rt][ct], 0, 0, 0);
-              }
+        for (int u = 0; u < 4; u++) {
+          if constexpr (P == PREC_F64) {
+            double kv;
+            if constexpr (KID >= K_USER) {
+              kv = kernel_xy<KID>(xt[rt], yt[rt], b.sx[u], b.sy[u], kp);
+            } else {
+              const double dx = xt[rt] - b.sx[u], dy = yt[rt] - b.sy[u];
+              kv = scale * near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab, ln2_hi,
+                                            ln2_lo);
             }
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++)
+              acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv, bq[u][ct], acc[rt][ct], 0, 0, 0);
+          } else {
+            const float kv = near_kernel_f32<KID>(xt[rt], yt[rt], b.sx[u], b.sy[u], kp, c0, c1);
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++)
+              acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(kv, bq[u][ct], acc[rt][ct], 0, 0, 0);
           }
         }
       }
     }
+  };
+  // end of a source (FP32/BF16): its partial sum added to the compensated
+  // total, the partial restarted
+  auto close_source = [&]() {
     if constexpr (P != PREC_F64) {
 #pragma unroll
       for (int rt = 0; rt < NRT; rt++)
@@ -625,7 +624,41 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
             two_sum(s_, c_, acc[rt][ct][v]);
             sh[rt][ct][v] = s_;
             sl[rt][ct][v] = c_;
+            acc[rt][ct][v] = 0.0f;
           }
+    }
+  };
+  if (nsrc > 0) {
+    Buf b0, b1;
+    int src = 0, k0 = 0;  // the step to evaluate next
+    load(b0, 0, 0);
+    for (;;) {
+      // the step after (src, k0), or (src, k0) again past the end
+      int ns = src, nk = k0 + KC;
+      if (nk >= __builtin_amdgcn_readfirstlane(d.sc[src])) {
+        ns = src + 1;
+        nk = 0;
+      }
+      const bool more = ns < nsrc;
+      load(b1, more ? ns : src, more ? nk : k0);
+      compute(b0);
+      if (ns != src || !more) close_source();
+      if (!more) break;
+      src = ns;
+      k0 = nk;
+      ns = src;
+      nk = k0 + KC;
+      if (nk >= __builtin_amdgcn_readfirstlane(d.sc[src])) {
+        ns = src + 1;
+        nk = 0;
+      }
+      const bool more2 = ns < nsrc;
+      load(b0, more2 ? ns : src, more2 ? nk : k0);
+      compute(b1);
+      if (ns != src || !more2) close_source();
+      if (!more2) break;
+      src = ns;
+      k0 = nk;
     }
   }
   const auto out = gptr_w(pr.out);
