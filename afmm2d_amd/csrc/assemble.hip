@@ -499,14 +499,19 @@ bool pool_enabled() {
 struct Scratch {
   DevBuf own;  // when the pool is off
   int device;
-  double* get(size_t doubles, int64_t& peak) {
-    const size_t bytes = sizeof(double) * std::max<size_t>(doubles, 1);
+  // budget: the largest batch the caller makes.  A pooled buffer that must
+  // grow past half of it grows to all of it, so that the pool grows at most
+  // once more (a batch of a larger problem fills the budget to within a box,
+  // and growing by those few MB freed and re-allocated 16 GiB, which stalled)
+  double* get(size_t doubles, int64_t& peak, size_t budget) {
+    size_t bytes = sizeof(double) * std::max<size_t>(doubles, 1);
     if (!pool_enabled()) {
       own.alloc(bytes);
       return own.as<double>();
     }
     std::lock_guard<std::mutex> lk(g_pool_mu);
     DevBuf& b = g_pool[device];
+    if (bytes > b.bytes && 2 * bytes > budget) bytes = std::max(bytes, budget);
     b.alloc(bytes);  // grows only
     peak = std::max<int64_t>(peak, (int64_t)b.bytes);
     return b.as<double>();
@@ -648,7 +653,7 @@ void pair_aca(Engine& e, const std::vector<int>& first, const std::vector<int>& 
     i0 = i;
     upload(d_boxes, boxes, s);
     upload(d_scroff, soffs, s);
-    double* d_scr = scr.get((size_t)tot, peak_unused);
+    double* d_scr = scr.get((size_t)tot, peak_unused, budget);
     AcaArgs a{e.skx.as<double>(), e.sky.as<double>(), d_rstart.as<int>(), d_n1.as<int>(),
               d_n2.as<int>(), d_il.as<int>(), d_boxes.as<int>(), d_scroff.as<int64_t>(),
               d_scr, d_pivoff.as<int64_t>(), d_rank.as<int>(), d_prow.as<int>(),
@@ -859,7 +864,7 @@ void assemble_tree(Engine& e) {
         upload(d_scroff, soffs, s);
         if (capped) upload(d_cap, caps, s);
         int64_t held = 0;
-        double* d_scr = scr.get((size_t)tot, held);
+        double* d_scr = scr.get((size_t)tot, held, budget);
         e.aca_scratch_peak = std::max<int64_t>(e.aca_scratch_peak, 8 * tot);
         AcaArgs a{src_x, src_y, d_rstart.as<int>(), d_n1.as<int>(), d_n2.as<int>(), d_il.as<int>(),
                   d_boxes.as<int>(), d_scroff.as<int64_t>(), d_scr,
