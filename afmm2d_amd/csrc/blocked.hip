@@ -711,34 +711,83 @@ __global__ void k_mr_zero(int* __restrict__ v, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// Gather: qs[inv_perm[i]][c] = b[i + c ldb] (0 for c >= nrhs).  Lane c of a
-// row of RP threads, RP-column rows written whole, b read along columns.
-// Workgroup blk of a problem takes its caller points
-// [blk kMrIoPoints, (blk + 1) kMrIoPoints) of d.cnt.
+// Gather and output go through LDS, half a workgroup's kMrIoPoints caller
+// points at a time: b and y are read and written along their columns (a
+// wave's access is 64 consecutive caller points of one column), qs, pot and
+// near along their RP-column rows (one point's row whole), so neither side
+// touches a cache line for a few bytes of it.  Workgroup blk of a problem
+// takes its caller points [blk kMrIoPoints, (blk + 1) kMrIoPoints) of d.cnt.
+constexpr int kMrIoHalf = kMrIoPoints / 2;
+static_assert(kMrIoHalf == 128, "the column phase maps 256 threads to 128 points x 2 columns");
+
+// Gather: qs[inv_perm[i]][c] = b[i + c ldb] (0 for c >= nrhs).
 template <class VT, int NCT>
 __device__ __forceinline__ void mr_gather_block(const MrIoTask& d, int blk) {
-  constexpr int RP = 16 * NCT, PPI = 256 / RP;
-  const int c = threadIdx.x % RP, pl = threadIdx.x / RP;
-  const int i0 = blk * kMrIoPoints, cnt = min(kMrIoPoints, d.cnt - i0);
+  constexpr int RP = 16 * NCT, PPI = 256 / RP, LD = RP + 1;
+  __shared__ VT tile[kMrIoHalf * LD];
+  __shared__ int sperm[kMrIoHalf];
+  const int t = threadIdx.x, c = t % RP, pl = t / RP;
+  const int p = t % kMrIoHalf, c0 = t / kMrIoHalf;
   VT* qs = static_cast<VT*>(d.qs);
-  for (int p = pl; p < cnt; p += PPI) {
-    const int64_t i = i0 + p;
-    const double v = c < d.nrhs ? d.b[i + c * d.ldb] : 0.0;
-    qs[(int64_t)d.inv_perm[i] * RP + c] = (VT)v;
+  for (int h = 0; h < kMrIoPoints; h += kMrIoHalf) {
+    const int i0 = blk * kMrIoPoints + h;
+    const int cnt = min(kMrIoHalf, d.cnt - i0);
+    if (cnt <= 0) break;
+    // columns in: point p of columns c0, c0 + 2, ...
+    double v[RP / 2];
+#pragma unroll
+    for (int u = 0; u < RP / 2; u++) {
+      const int cc = c0 + 2 * u;
+      v[u] = p < cnt && cc < d.nrhs ? d.b[(int64_t)(i0 + p) + (int64_t)cc * d.ldb] : 0.0;
+    }
+    if (t < cnt) sperm[t] = d.inv_perm[i0 + t];
+#pragma unroll
+    for (int u = 0; u < RP / 2; u++) tile[p * LD + c0 + 2 * u] = (VT)v[u];
+    __syncthreads();
+    // rows out
+    for (int q = pl; q < cnt; q += PPI) qs[(int64_t)sperm[q] * RP + c] = tile[q * LD + c];
+    __syncthreads();
   }
 }
 
 // Output: y[i + c ldy] = pot[inv_perm[i]][c] + near[inv_perm[i]][c] in FP64.
 template <class VT, int NCT>
 __device__ __forceinline__ void mr_out_block(const MrIoTask& d, int blk) {
-  constexpr int RP = 16 * NCT, PPI = 256 / RP;
-  const int c = threadIdx.x % RP, pl = threadIdx.x / RP;
-  const int i0 = blk * kMrIoPoints, cnt = min(kMrIoPoints, d.cnt - i0);
+  constexpr int RP = 16 * NCT, PPI = 256 / RP, LD = RP + 1, NQ = kMrIoHalf / PPI;
+  __shared__ double tile[kMrIoHalf * LD];
+  __shared__ int sperm[kMrIoHalf];
+  const int t = threadIdx.x, c = t % RP, pl = t / RP;
+  const int p = t % kMrIoHalf, c0 = t / kMrIoHalf;
   const VT* pot = static_cast<const VT*>(d.pot);
-  for (int p = pl; p < cnt; p += PPI) {
-    const int64_t i = i0 + p;
-    const int64_t si = d.inv_perm[i];
-    if (c < d.nrhs) d.y[i + c * d.ldy] = (double)pot[si * RP + c] + d.nearv[si * RP + c];
+  for (int h = 0; h < kMrIoPoints; h += kMrIoHalf) {
+    const int i0 = blk * kMrIoPoints + h;
+    const int cnt = min(kMrIoHalf, d.cnt - i0);
+    if (cnt <= 0) break;
+    if (t < cnt) sperm[t] = d.inv_perm[i0 + t];
+    __syncthreads();
+    // rows in: every load of the thread issued before the first sum
+    VT a[NQ];
+    double b[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+      const int q = min(pl + k * PPI, cnt - 1);
+      const int64_t si = sperm[q];
+      a[k] = pot[si * RP + c];
+      b[k] = d.nearv[si * RP + c];
+    }
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+      const int q = pl + k * PPI;
+      if (q < cnt) tile[q * LD + c] = (double)a[k] + b[k];
+    }
+    __syncthreads();
+    // columns out: point p of columns c0, c0 + 2, ...
+#pragma unroll
+    for (int u = 0; u < RP / 2; u++) {
+      const int cc = c0 + 2 * u;
+      if (p < cnt && cc < d.nrhs) d.y[(int64_t)(i0 + p) + (int64_t)cc * d.ldy] = tile[p * LD + cc];
+    }
+    __syncthreads();
   }
 }
 
