@@ -52,6 +52,7 @@ EXPORTS = [
     "afmm2d_rank_model_set_params", "afmm2d_rank_model_predict", "afmm2d_rank_model_gradient",
     "afmm2d_rank_model_train", "afmm2d_set_rank_model", "afmm2d_get_rank_samples",
     "afmm2d_set_m2l_factoring", "afmm2d_get_m2l_storage", "afmm2d_check_symmetry",
+    "afmm2d_set_m2l_storage",
     "afmm2d_release_scratch",
 ]
 
@@ -86,6 +87,7 @@ class Stats(ctypes.Structure):
         ("aca_seconds", ctypes.c_double),
         ("m2l_factored_pairs", ctypes.c_int64), ("m2l_factored_bytes", ctypes.c_int64),
         ("m2l_factored_dense_bytes", ctypes.c_int64), ("blocked_m2l_coeffs_bf16", ctypes.c_int64),
+        ("m2l_storage", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -175,6 +177,7 @@ def lib():
         "afmm2d_set_rank_model": (i, [_vp, _vp, ctypes.c_float]),
         "afmm2d_get_rank_samples": (i, [_vp, _fp, _ip, ctypes.c_int64, _lp]),
         "afmm2d_set_m2l_factoring": (i, [_vp, i]),
+        "afmm2d_set_m2l_storage": (i, [_vp, i]),
         "afmm2d_get_m2l_storage": (i, [_vp, i, i, i, _ip, _ip]),
         "afmm2d_check_symmetry": (i, [_vp, _ip]),
         "afmm2d_release_scratch": (i, [i]),
@@ -449,6 +452,15 @@ class AFMM:
     def set_m2l_factoring(self, on=True):
         """Store compressible M2L blocks as ACA factors (before assemble)."""
         _check(lib().afmm2d_set_m2l_factoring(self._h, 1 if on else 0))
+
+    def set_m2l_storage(self, storage="fp64"):
+        """Dense M2L block storage (before assemble): "fp64", or "bfp48"
+        (48-bit block floating point: 6 bytes per entry, each within 2^-46
+        of its block's largest entry)."""
+        codes = {"fp64": 0, "bfp48": 1}
+        if storage not in codes:
+            raise ValueError("M2L storage must be one of %s" % sorted(codes))
+        _check(lib().afmm2d_set_m2l_storage(self._h, codes[storage]))
 
     def set_rank_model(self, model, margin=0.5):
         """Assemble with ACA capacities from model (None: the plain ACA)."""

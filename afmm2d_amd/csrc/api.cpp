@@ -886,6 +886,16 @@ int afmm2d_set_m2l_factoring(afmm2d_handle h, int on) {
   return AFMM2D_OK;
 }
 
+int afmm2d_set_m2l_storage(afmm2d_handle h, int storage) {
+  if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (storage != AFMM2D_M2L_FP64 && storage != AFMM2D_M2L_BFP48)
+    return fail(AFMM2D_ERR_INVALID_ARGUMENT, "unknown M2L storage");
+  if (h->assembled) return fail(AFMM2D_ERR_STATE, "M2L storage set after assembly");
+  for (Engine* e : {h->e.get(), h->e2.get()})
+    if (e) e->m2l_store = storage == AFMM2D_M2L_BFP48 ? M2L_STORE_B48 : M2L_STORE_F64;
+  return AFMM2D_OK;
+}
+
 int afmm2d_set_rank_model(afmm2d_handle h, afmm2d_rank_model m, float margin) {
   if (!h) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "NULL handle");
   if (!std::isfinite(margin)) return fail(AFMM2D_ERR_INVALID_ARGUMENT, "margin not finite");
@@ -1076,6 +1086,18 @@ int afmm2d_get_m2l(afmm2d_handle h, int level, int box, int slot, double* out, i
           for (int l = 0; l < fp; l++) acc += U[i + (size_t)l * rows_lo] * V[c + (size_t)l * cols_lo];
           blk[(size_t)c * ld + i] = acc;
         }
+    } else if (!blk.empty() && e.m2l_store == M2L_STORE_B48) {
+      // blk.size() is even (even leading dimension): whole row pair records
+      std::vector<B48Pair> rec(blk.size() / 2);
+      PairDesc d;
+      AFMM_HIP(hipMemcpy(rec.data(), e.m2l_b48.as<B48Pair>() + e.h_pair_off[p] / 2,
+                         sizeof(B48Pair) * rec.size(), hipMemcpyDeviceToHost));
+      AFMM_HIP(hipMemcpy(&d, e.pair_desc.as<PairDesc>() + p, sizeof(PairDesc),
+                         hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < rec.size(); i++) {
+        blk[2 * i] = b48_join(rec[i].hi0, (uint16_t)(rec[i].lo & 0xFFFF), d.sexp);
+        blk[2 * i + 1] = b48_join(rec[i].hi1, (uint16_t)(rec[i].lo >> 16), d.sexp);
+      }
     } else if (!blk.empty()) {
       AFMM_HIP(hipMemcpy(blk.data(), e.m2l.as<double>() + e.h_pair_off[p],
                          sizeof(double) * blk.size(), hipMemcpyDeviceToHost));
@@ -1235,7 +1257,10 @@ int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st) {
   st->skeleton_total = e.skel_total;
   st->il_pairs = e.il_pairs;
   st->near_pairs = e.near_pairs;
-  st->m2l_bytes = 8 * e.m2l_total;
+  // dense blocks at 8 or 6 bytes per entry, ACA factors at 8
+  const int64_t bpe = e.m2l_store == M2L_STORE_B48 ? 6 : 8;
+  const int64_t m2l_stored_bytes = bpe * (e.m2l_total - e.m2lf_total) + 8 * e.m2lf_total;
+  st->m2l_bytes = m2l_stored_bytes;
   st->m2l_columns = e.m2l_cols;
   st->l2p_bytes = 8 * e.l2p_total;  // stored rows (the skeleton rows are the identity)
   // One matvec reads q (gather) and the coordinates (near field), L2P twice
@@ -1249,12 +1274,15 @@ int afmm2d_get_stats(afmm2d_handle h, afmm2d_stats* st) {
   // vectors a few times.
   st->matvec_bytes = e.assembled
                          ? (int64_t)e.n * (8 + 4 + 8 + 8 + 16 + 8 + 8 + 4 + 8 + 2 * 4) +
-                               2 * 8 * e.l2p_total + 8 * e.m2l_total + 16 * e.m2l_cols +
+                               2 * 8 * e.l2p_total + m2l_stored_bytes + 16 * e.m2l_cols +
                                4 * e.il_pairs + (6 * 8 + 2 * 4) * e.skel_total
                          : 0;
   st->create_seconds = e.create_seconds;
   st->assemble_seconds = e.assemble_seconds;
-  st->m2l_leaf_bytes = e.assembled ? 8 * e.lv[e.L].m2l_coeffs : 0;
+  st->m2l_leaf_bytes = e.assembled ? bpe * (e.lv[e.L].m2l_coeffs - e.lv[e.L].m2l_fact_coeffs) +
+                                         8 * e.lv[e.L].m2l_fact_coeffs
+                                   : 0;
+  st->m2l_storage = e.m2l_store == M2L_STORE_B48 ? AFMM2D_M2L_BFP48 : AFMM2D_M2L_FP64;
   st->m2l_leaf_columns = e.assembled ? e.lv[e.L].m2l_cols : 0;
   st->skeleton_leaf = e.assembled ? e.lv[e.L].soff[e.lv[e.L].nb] - e.lv[e.L].soff[0] : 0;
   st->near_waves = e.p2p_waves_per_simd;

@@ -459,6 +459,36 @@ __global__ __launch_bounds__(64) void k_m2l_pair_build(
   }
 }
 
+// 48-bit block floating point of the dense M2L blocks (b48_split): one wave
+// per pair finds its block's largest magnitude, then writes the block's row
+// pair records and its exponent to the pair's descriptor.
+__global__ __launch_bounds__(64) void k_m2l_b48(const double* __restrict__ src,
+                                                const int* __restrict__ rank,
+                                                const int* __restrict__ pk,
+                                                const int* __restrict__ pb,
+                                                const int64_t* __restrict__ poff,
+                                                B48Pair* __restrict__ dst,
+                                                PairDesc* __restrict__ pd) {
+  const int64_t p = blockIdx.x;
+  const int64_t off = poff[p];
+  if (off < 0) return;  // stored as ACA factors
+  const int rk = rank[pk[p]], rb = rank[pb[p]];
+  const int tot = (rk + (rk & 1)) * rb;  // even: whole row pairs
+  double m = 0.0;
+  for (int i = threadIdx.x; i < tot; i += 64) m = fmax(m, fabs(src[off + i]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  const int sx = b48_exponent(m);
+  for (int i = threadIdx.x; i < tot / 2; i += 64) {
+    int32_t h0, h1;
+    uint16_t l0, l1;
+    b48_split(src[off + 2 * i], sx, h0, l0);
+    b48_split(src[off + 2 * i + 1], sx, h1, l1);
+    dst[(off >> 1) + i] = B48Pair{h0, h1, (uint32_t)l0 | ((uint32_t)l1 << 16)};
+  }
+  if (threadIdx.x == 0) pd[p].sexp = sx;
+}
+
 template <class T>
 void upload(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
   d.alloc(sizeof(T) * std::max<size_t>(h.size(), 1));
@@ -1056,7 +1086,7 @@ void assemble_tree(Engine& e) {
   std::vector<Cand> cand;
   for (int j = 2; j <= L; j++) {
     LevelInfo& li = e.lv[j];
-    li.m2l_coeffs = li.m2l_cols = 0;
+    li.m2l_coeffs = li.m2l_cols = li.m2l_fact_coeffs = 0;
     for (int k = 0; k < li.nb; k++) {
       for (int q = 0; q < 40; q++) {
         int b = slot_box(j, k, q);
@@ -1095,7 +1125,10 @@ void assemble_tree(Engine& e) {
       e.m2lf_pairs++;
       e.m2lf_dense += dense;
       e.m2l_total += stored - dense;
-      if (cand[i].group == 1) e.lv[L].m2l_coeffs += stored - dense;  // the leaf launch's bytes
+      if (cand[i].group == 1) {  // the leaf launch's bytes
+        e.lv[L].m2l_coeffs += stored - dense;
+        e.lv[L].m2l_fact_coeffs += stored;
+      }
     }
   }
   std::vector<int> order(cand.size());
@@ -1197,6 +1230,21 @@ void assemble_tree(Engine& e) {
                              e.soff_g.as<int64_t>(), e.pair_k.as<int>(), e.pair_b.as<int>(),
                              e.pair_off.as<int64_t>(), e.m2l.as<double>(), e.kp));
     AFMM_LAUNCH_CHECK();
+  }
+  e.m2l_b48.release();
+  if (e.m2l_store == M2L_STORE_B48) {
+    // the FP64 blocks become row pair records (the spare one zero)
+    const int64_t n = (off + 2) / 2;
+    e.m2l_b48.alloc(sizeof(B48Pair) * n);
+    AFMM_HIP(hipMemsetAsync(e.m2l_b48.p, 0, e.m2l_b48.bytes, s));
+    if (e.npairs > 0) {
+      k_m2l_b48<<<(unsigned)e.npairs, 64, 0, s>>>(
+          e.m2l.as<double>(), e.rank_g.as<int>(), e.pair_k.as<int>(), e.pair_b.as<int>(),
+          e.pair_off.as<int64_t>(), e.m2l_b48.as<B48Pair>(), e.pair_desc.as<PairDesc>());
+      AFMM_LAUNCH_CHECK();
+    }
+    AFMM_HIP(hipStreamSynchronize(s));
+    e.m2l.release();
   }
   AFMM_HIP(hipStreamSynchronize(s));
   e.outv.alloc(sizeof(double) * (sb + 1));

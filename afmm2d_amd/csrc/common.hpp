@@ -27,6 +27,30 @@ struct Error : std::runtime_error {
 
 #define AFMM_LAUNCH_CHECK() AFMM_HIP(hipGetLastError())
 
+// M2L block storage modes (Engine::m2l_store, afmm2d_set_m2l_storage).
+enum M2LStore { M2L_STORE_F64 = 0, M2L_STORE_B48 = 1 };
+
+// 48-bit block floating point: a block's entries v are stored as 48-bit
+// two's-complement integers m = rint(v 2^-s) with one exponent s per block,
+// the smallest for which every |m| <= 2^46 (s = e - 46 with max |v| < 2^e),
+// so |v - m 2^s| <= 2^(s-1) <= 2^-46 max |v|.  m is kept as its top 32 bits
+// (signed) and its low 16 bits; both convert to doubles exactly and
+// 2^16 hi + lo is m.
+__host__ __device__ __forceinline__ int b48_exponent(double maxabs) {
+  if (!(maxabs > 0.0)) return 0;
+  int e;
+  (void)frexp(maxabs, &e);  // maxabs = f 2^e, 1/2 <= f < 1
+  return e - 46;
+}
+__host__ __device__ __forceinline__ void b48_split(double v, int sexp, int32_t& hi, uint16_t& lo) {
+  const long long m = llrint(ldexp(v, -sexp));
+  hi = (int32_t)(m >> 16);
+  lo = (uint16_t)(m & 0xFFFF);
+}
+__host__ __device__ __forceinline__ double b48_join(int32_t hi, uint16_t lo, int sexp) {
+  return ldexp((double)(((long long)hi << 16) + (long long)lo), sexp);
+}
+
 // Kernel ids match afmm2d_kernel_id.
 // K_USER_SYM and K_USER_ANTI are the symmetric and antisymmetric parts of
 // the user kernel, (K(x, y) + K(y, x)) / 2 and (K(x, y) - K(y, x)) / 2: a

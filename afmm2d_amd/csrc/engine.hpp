@@ -93,6 +93,7 @@ struct LevelInfo {
   // this level's M2L coefficients (rank_k rank_b per pair) and partial
   // columns (rank_k + rank_b per pair)
   int64_t m2l_coeffs = 0, m2l_cols = 0;
+  int64_t m2l_fact_coeffs = 0;  // of m2l_coeffs: those stored as ACA factors
 };
 
 // Register-tile classes of the M2L pair kernel, cheapest first (VGPRs per
@@ -122,6 +123,12 @@ inline int m2l_tile_class(int ld, int cols) {
 }
 
 // Everything the M2L pair kernel needs about one pair, in one record.
+// One row pair of a block in the 48-bit storage (M2L_STORE_B48).
+struct B48Pair {
+  int32_t hi0, hi1;  // top 32 bits of the two mantissas
+  uint32_t lo;       // their low 16 bits: entry 2i in bits 0-15, 2i + 1 in 16-31
+};
+
 struct PairDesc {
   int rk, rb;      // ranks of the pair's boxes
   int part_k;      // where B out_b goes (the pair's partials)
@@ -129,7 +136,7 @@ struct PairDesc {
   int64_t off;     // element offset of the block
   int64_t sk, sb;  // skeleton offsets of the boxes
   int part_b;      // where B^T out_k goes: part_k + rk
-  int pad;
+  int sexp;        // 48-bit block storage: entry = mantissa * 2^sexp (else 0)
 };
 
 // Largest interaction list: children of the parent's 9-box neighbourhood
@@ -393,6 +400,17 @@ struct Engine {
   DevBuf m2lf;
   std::vector<int> h_pair_fact;  // per pair: p, or -1 for a dense block
   int64_t m2lf_pairs = 0, m2lf_total = 0, m2lf_dense = 0;  // pairs, factor / dense coefficients
+  // Dense block storage (m2l_store set before assembly):
+  //   M2L_STORE_F64: m2l, one double per entry;
+  //   M2L_STORE_B48: m2l_b48, 48-bit block floating point (b48_split): each
+  //   entry a 48-bit integer mantissa, the block's exponent in its
+  //   PairDesc; the entries 2i, 2i + 1 of a block (one row pair of a
+  //   column) form a 12-byte record at record index (off + 2i) / 2: their
+  //   top 32 bits, then their low 16 bits packed in one word (entry 2i in
+  //   the low half).  6 bytes per entry, error <= 2^-46 of the block's
+  //   largest entry.  Factored pairs stay FP64 in m2lf.
+  int m2l_store = 0;
+  DevBuf m2l_b48;
   std::vector<int> h_red_start;           // per box gid (nbox + 1): first entry of h_red_pair
   std::vector<int> h_red_pair;            // per present slot, slot order: pair index
   DevBuf pair_k, pair_b, pair_off, m2l;

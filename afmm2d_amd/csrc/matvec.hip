@@ -359,6 +359,42 @@ __device__ __forceinline__ double sum_lanes_stride8(double v) {
   return v;
 }
 
+// The stored M2L blocks as seen by the pair kernels: element i of a block
+// pair-indexed (entries 2 i and 2 i + 1 of the block, one row pair of a
+// column), and the factor the pair's products take at the output.
+// FP64: one 16-byte load, factor 1.  48-bit block floating point
+// (M2L_STORE_B48): one 12-byte load of the row pair's record, its mantissas
+// joined exactly into integer-valued doubles (2^16 hi + lo); the block's
+// power of two 2^sexp is the factor, so the sums over the block are those
+// of the FP64 products of the stored values.  Block offsets and leading
+// dimensions are even, so a row pair is one record.
+struct M2LBlocks {
+  const double* f64;
+  const B48Pair* b48;
+};
+template <int ST>
+struct BlockView;
+template <>
+struct BlockView<M2L_STORE_F64> {
+  const double2* p;
+  __device__ __forceinline__ BlockView(const M2LBlocks& b, const PairDesc& d)
+      : p(reinterpret_cast<const double2*>(b.f64 + d.off)) {}
+  __device__ __forceinline__ double2 operator[](int i) const { return p[i]; }
+  __device__ __forceinline__ double scale(const PairDesc&) const { return 1.0; }
+};
+template <>
+struct BlockView<M2L_STORE_B48> {
+  const B48Pair* p;
+  __device__ __forceinline__ BlockView(const M2LBlocks& b, const PairDesc& d)
+      : p(b.b48 + (d.off >> 1)) {}
+  __device__ __forceinline__ double2 operator[](int i) const {
+    const B48Pair a = p[i];
+    return make_double2(fma((double)a.hi0, 65536.0, (double)(a.lo & 0xFFFFu)),
+                        fma((double)a.hi1, 65536.0, (double)(a.lo >> 16)));
+  }
+  __device__ __forceinline__ double scale(const PairDesc& d) const { return ldexp(1.0, d.sexp); }
+};
+
 // M2L pair kernel: one wave per unordered pair, no LDS.  The block (ld x rb,
 // column-major, ld even) is read in tiles of 16 rows x 8 columns, one
 // 16-byte load per lane: lane 8 ci + rp holds rows 2 rp, 2 rp + 1 of column
@@ -368,8 +404,8 @@ __device__ __forceinline__ double sum_lanes_stride8(double v) {
 // rp are summed; B^T out_k: per column tile, the 8 lanes of a column are
 // summed.  Writes B out_b and B^T out_k to the pair's partials (part_k,
 // part_b).
-template <int MAXRT, int MAXCT>
-__device__ __forceinline__ void m2l_pair(const double* __restrict__ m2l, const PairDesc& d,
+template <int MAXRT, int MAXCT, int ST>
+__device__ __forceinline__ void m2l_pair(const M2LBlocks& m2l, const PairDesc& d,
                                          const double* __restrict__ outv,
                                          double* __restrict__ part, double tsign) {
   const int lane = threadIdx.x, rp = lane & 7, ci = lane >> 3;
@@ -378,7 +414,8 @@ __device__ __forceinline__ void m2l_pair(const double* __restrict__ m2l, const P
   double* tw = part + d.part_b;
   const double* xk = outv + d.sk;
   const double* xb = outv + d.sb;
-  const double2* B2 = reinterpret_cast<const double2*>(m2l + d.off);
+  const BlockView<ST> B2(m2l, d);
+  const double sc = B2.scale(d), tsc = tsign * sc;
   // staging registers as double arrays (arrays of double2 stay in scratch)
   double ax[MAXCT][MAXRT], ay[MAXCT][MAXRT], xbv[MAXCT], k0[MAXRT], k1[MAXRT];
 #pragma unroll
@@ -411,7 +448,7 @@ __device__ __forceinline__ void m2l_pair(const double* __restrict__ m2l, const P
     for (int t = 0; t < MAXRT; t++) s += ax[u][t] * k0[t] + ay[u][t] * k1[t];
     s = sum_lanes8(s);
     const int c = 8 * u + ci;
-    if (rp == 0 && c < rb) tw[c] = tsign * s;
+    if (rp == 0 && c < rb) tw[c] = tsc * s;
   }
 #pragma unroll
   for (int t = 0; t < MAXRT; t++) {
@@ -425,8 +462,8 @@ __device__ __forceinline__ void m2l_pair(const double* __restrict__ m2l, const P
     f1 = sum_lanes_stride8(f1);
     const int r = 16 * t + 2 * rp;
     if (ci == 0) {
-      if (r < rk) fw[r] = f0;
-      if (r + 1 < rk) fw[r + 1] = f1;
+      if (r < rk) fw[r] = sc * f0;
+      if (r + 1 < rk) fw[r + 1] = sc * f1;
     }
   }
 }
@@ -490,8 +527,9 @@ __global__ __launch_bounds__(256) void k_m2l_fact(const double* __restrict__ F,
   }
 }
 
-template <int MAXRT, int MAXCT>
+template <int MAXRT, int MAXCT, int ST>
 __global__ __launch_bounds__(64) void k_m2l_reg(const double* __restrict__ m2l,
+                                                const B48Pair* __restrict__ m2l_b48,
                                                 const PairDesc* __restrict__ pd,
                                                 const double* __restrict__ outv,
                                                 double* __restrict__ part, int npairs,
@@ -499,7 +537,7 @@ __global__ __launch_bounds__(64) void k_m2l_reg(const double* __restrict__ m2l,
   if (AFMM_MEM_PRIO) __builtin_amdgcn_s_setprio(AFMM_MEM_PRIO);
   const int p = blockIdx.x;
   if (p >= npairs) return;
-  m2l_pair<MAXRT, MAXCT>(m2l, pd[p], outv, part, tsign);
+  m2l_pair<MAXRT, MAXCT, ST>(M2LBlocks{m2l, m2l_b48}, pd[p], outv, part, tsign);
 }
 
 // M2L pair kernel for blocks taller or wider than the register tiles: one
@@ -510,8 +548,8 @@ __global__ __launch_bounds__(64) void k_m2l_reg(const double* __restrict__ m2l,
 // chunk is used.  Per chunk, B^T out_k of its columns is summed over the 8
 // lanes of each column and written; B out_b accumulates per lane (two rows
 // per row tile) and is summed over the column lanes at the end.
-template <int RT, int CT>
-__device__ __forceinline__ void m2l_load_chunk(const double2* __restrict__ B2, int ld, int rb,
+template <int RT, int CT, class View>
+__device__ __forceinline__ void m2l_load_chunk(const View& B2, int ld, int rb,
                                                int c0, int ci, const int (&rr)[RT],
                                                double (&ax)[CT][RT], double (&ay)[CT][RT],
                                                double (&xv)[CT], const double* __restrict__ xb) {
@@ -521,10 +559,10 @@ __device__ __forceinline__ void m2l_load_chunk(const double2* __restrict__ B2, i
     const int cc = min(c, rb - 1);
     xv[u] = xb[cc];
     if (c >= rb) xv[u] = 0.0;
-    const double2* col = B2 + ((cc * ld) >> 1);
+    const int col = (cc * ld) >> 1;
 #pragma unroll
     for (int t = 0; t < RT; t++) {
-      const double2 a = col[rr[t]];
+      const double2 a = B2[col + rr[t]];
       ax[u][t] = a.x;
       ay[u][t] = a.y;
     }
@@ -553,8 +591,9 @@ __device__ __forceinline__ void m2l_use_chunk(const double (&ax)[CT][RT], const 
   }
 }
 
-template <int RT, int CT>
+template <int RT, int CT, int ST>
 __global__ __launch_bounds__(64) void k_m2l_stream(const double* __restrict__ m2l,
+                                                   const B48Pair* __restrict__ m2l_b48,
                                                    const PairDesc* __restrict__ pd,
                                                    const double* __restrict__ outv,
                                                    double* __restrict__ part, int npairs,
@@ -569,7 +608,8 @@ __global__ __launch_bounds__(64) void k_m2l_stream(const double* __restrict__ m2
   double* tw = part + d.part_b;
   const double* xk = outv + d.sk;
   const double* xb = outv + d.sb;
-  const double2* B2 = reinterpret_cast<const double2*>(m2l + d.off);
+  const BlockView<ST> B2(M2LBlocks{m2l, m2l_b48}, d);
+  const double sc = B2.scale(d), tsc = tsign * sc;
   if (ld == 0 || rb == 0) {
     // an empty side: both products are zero
     for (int i = lane; i < rk; i += 64) fw[i] = 0.0;
@@ -600,10 +640,10 @@ __global__ __launch_bounds__(64) void k_m2l_stream(const double* __restrict__ m2
   }
   for (int c0 = 0; c0 < rb; c0 += 2 * W) {
     if (c0 + W < rb) m2l_load_chunk<RT, CT>(B2, ld, rb, c0 + W, ci, rr, bx, by, bv, xb);
-    m2l_use_chunk<RT, CT>(ax, ay, xv, k0, k1, f0, f1, c0, ci, rp, rb, tw, tsign);
+    m2l_use_chunk<RT, CT>(ax, ay, xv, k0, k1, f0, f1, c0, ci, rp, rb, tw, tsc);
     if (c0 + W >= rb) break;
     if (c0 + 2 * W < rb) m2l_load_chunk<RT, CT>(B2, ld, rb, c0 + 2 * W, ci, rr, ax, ay, xv, xb);
-    m2l_use_chunk<RT, CT>(bx, by, bv, k0, k1, f0, f1, c0 + W, ci, rp, rb, tw, tsign);
+    m2l_use_chunk<RT, CT>(bx, by, bv, k0, k1, f0, f1, c0 + W, ci, rp, rb, tw, tsc);
   }
 #pragma unroll
   for (int t = 0; t < RT; t++) {
@@ -611,8 +651,8 @@ __global__ __launch_bounds__(64) void k_m2l_stream(const double* __restrict__ m2
     const double g1 = sum_lanes_stride8(f1[t]);
     const int r = 16 * t + 2 * rp;
     if (ci == 0) {
-      if (r < rk) fw[r] = g0;
-      if (r + 1 < rk) fw[r + 1] = g1;
+      if (r < rk) fw[r] = sc * g0;
+      if (r + 1 < rk) fw[r + 1] = sc * g1;
     }
   }
 }
@@ -1497,11 +1537,13 @@ static_assert(kM2LAllClasses == 13 && kM2LStreamRows[0] == 4 && kM2LStreamRows[1
                   kM2LStreamRows[2] == 8 && kM2LStreamRows[3] == 10 &&
                   kM2LStreamRows[4] == 12 && kM2LStreamRows[5] == 16,
               "launch_m2l's cases follow the class tables");
-void launch_m2l(Engine& e, int g, hipStream_t s) {
+template <int ST>
+void launch_m2l_classes(Engine& e, int g, hipStream_t s) {
   const PairDesc* pd0 = e.pair_desc.as<PairDesc>();
   double* part = e.m2l_part.as<double>();
   const double* outv = e.outv.as<double>();
   const double* B = e.m2l.as<double>();
+  const B48Pair* Bh = e.m2l_b48.as<B48Pair>();
   for (int c = 0; c <= kM2LAllClasses; c++) {
     const int64_t p0 = e.m2l_cls_begin[g][c], p1 = e.m2l_cls_end[g][c];
     if (p1 <= p0) continue;
@@ -1509,24 +1551,34 @@ void launch_m2l(Engine& e, int g, hipStream_t s) {
     const int np = (int)(p1 - p0);
     // K(second, first) = ts K(first, second)^T: -1 for an antisymmetric kernel
     const double ts = e.kid == K_USER_ANTI ? -1.0 : 1.0;
+#define AFMM_M2L_ARGS <<<np, 64, 0, s>>>(B, Bh, pd, outv, part, np, ts)
     switch (c) {
-      case 0: k_m2l_reg<2, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
-      case 1: k_m2l_reg<2, 5><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
-      case 2: k_m2l_reg<2, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
-      case 3: k_m2l_reg<3, 4><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
-      case 4: k_m2l_reg<3, 5><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
-      case 5: k_m2l_reg<3, 6><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
-      case 6: k_m2l_reg<4, 8><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
-      case 7: k_m2l_stream<4, AFMM_SCT4><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
-      case 8: k_m2l_stream<6, AFMM_SCT6><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
-      case 9: k_m2l_stream<8, AFMM_SCT8><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
-      case 10: k_m2l_stream<10, AFMM_SCT10><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
-      case 11: k_m2l_stream<12, 1><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
-      case 12: k_m2l_stream<16, 1><<<np, 64, 0, s>>>(B, pd, outv, part, np, ts); break;
+      case 0: k_m2l_reg<2, 4, ST> AFMM_M2L_ARGS; break;
+      case 1: k_m2l_reg<2, 5, ST> AFMM_M2L_ARGS; break;
+      case 2: k_m2l_reg<2, 6, ST> AFMM_M2L_ARGS; break;
+      case 3: k_m2l_reg<3, 4, ST> AFMM_M2L_ARGS; break;
+      case 4: k_m2l_reg<3, 5, ST> AFMM_M2L_ARGS; break;
+      case 5: k_m2l_reg<3, 6, ST> AFMM_M2L_ARGS; break;
+      case 6: k_m2l_reg<4, 8, ST> AFMM_M2L_ARGS; break;
+      case 7: k_m2l_stream<4, AFMM_SCT4, ST> AFMM_M2L_ARGS; break;
+      case 8: k_m2l_stream<6, AFMM_SCT6, ST> AFMM_M2L_ARGS; break;
+      case 9: k_m2l_stream<8, AFMM_SCT8, ST> AFMM_M2L_ARGS; break;
+      case 10: k_m2l_stream<10, AFMM_SCT10, ST> AFMM_M2L_ARGS; break;
+      case 11: k_m2l_stream<12, 1, ST> AFMM_M2L_ARGS; break;
+      case 12: k_m2l_stream<16, 1, ST> AFMM_M2L_ARGS; break;
       default: throw Error(6, "M2L block taller than 256 rows");
     }
+#undef AFMM_M2L_ARGS
     AFMM_LAUNCH_CHECK();
   }
+}
+
+void launch_m2l(Engine& e, int g, hipStream_t s) {
+  const PairDesc* pd0 = e.pair_desc.as<PairDesc>();
+  double* part = e.m2l_part.as<double>();
+  const double* outv = e.outv.as<double>();
+  if (e.m2l_store == M2L_STORE_B48) launch_m2l_classes<M2L_STORE_B48>(e, g, s);
+  else launch_m2l_classes<M2L_STORE_F64>(e, g, s);
   const int64_t f0 = e.m2l_cls_begin[g][kM2LFactored], f1 = e.m2l_cls_end[g][kM2LFactored];
   if (f1 > f0) {
     const double ts = e.kid == K_USER_ANTI ? -1.0 : 1.0;

@@ -758,6 +758,35 @@ def main():
     A.close()
     del A
 
+    # the same operator with its dense M2L blocks in 48-bit block floating
+    # point (6 bytes per entry), products still in FP64
+    print("bench: 48-bit block floating-point M2L blocks", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    Ab48 = E.AFMM(args.n, args.leaf, args.tol_pow, xy, kernel=args.kernel)
+    Ab48.set_m2l_storage("bfp48")
+    Ab48.assemble()
+    t_asm_b = time.perf_counter() - t0
+    st_b48 = Ab48.stats()
+    ms_b48, _, kern_b48, y_b48 = run_gpu(Ab48, q, args.steps, args.warmup, H)
+    Ab48.close()
+    del Ab48
+    m2l_alg_b48 = (st_b48["m2l_leaf_bytes"] + 8 * st_b48["m2l_leaf_columns"] +
+                   8 * st_b48["skeleton_leaf"])
+    ach_b48 = m2l_alg_b48 / (kern_b48["m2l_timed_region"] * 1e-3) / 1e9
+    b48 = {"value": round(args.n / (ms_b48 * 1e-3), 1), "ms_per_step": round(ms_b48, 4),
+           "rel_err": float(np.linalg.norm(y_b48[rows] - ref_rows) / np.linalg.norm(ref_rows)),
+           "rel_diff_vs_fp64_blocks": float(np.linalg.norm(y_b48 - y) / np.linalg.norm(y)),
+           "create_and_assemble_s": round(t_asm_b, 3),
+           "m2l_bytes": st_b48["m2l_bytes"], "matvec_bytes": st_b48["matvec_bytes"],
+           "kernel_avg_ms": {k: round(v, 4) for k, v in kern_b48.items()},
+           "m2l_roofline": {"kernel": "m2l", "bound": "hbm", "achieved": round(ach_b48, 1),
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(ach_b48 / HBM_PEAK_GBS, 4),
+                            "algorithmic_bytes": int(m2l_alg_b48),
+                            "avg_ms": round(kern_b48["m2l_timed_region"], 4)},
+           "matvec_roofline_frac": round(st_b48["matvec_bytes"] / (ms_b48 * 1e-3) / 1e9 /
+                                         HBM_PEAK_GBS, 4)}
+
     # the same operator with ACA-factored M2L blocks where they are smaller
     print("bench: ACA-factored M2L", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -876,6 +905,7 @@ def main():
                          "avg_ms": round(p2p_ms, 4),
                          "note": "beside the M2L on the side stream"},
         "blocked": blocked,
+        "m2l_bfp48": b48,
         "m2l_factored": factored,
         "cpu_baseline": cpu,
     }

@@ -114,6 +114,7 @@ typedef struct {
   int64_t blocked_m2l_coeffs_bf16;  /* blocked FP32 and BF16 passes: M2L row-block
                                        coefficients, sum rank_k K_k (each block stored for
                                        both boxes) */
+  int64_t m2l_storage;              /* afmm2d_m2l_storage of the dense M2L blocks */
 } afmm2d_stats;
 
 /* Message of the calling thread's last failure ("" if none). */
@@ -242,6 +243,20 @@ int afmm2d_batch_destroy(afmm2d_batch b);
  * Symmetric kernels only (no effect on a non-symmetric user kernel's
  * antisymmetric part).  No reference counterpart. */
 int afmm2d_set_m2l_factoring(afmm2d_handle h, int on);
+
+/* Storage of the dense M2L blocks read by the single-RHS matvec (set before
+ * afmm2d_assemble; AFMM2D_M2L_FP64 by default).  The reference keeps each
+ * block K(skeleton_k, skeleton_kIL) as an FP64 Eigen matrix
+ * (AFMM2DTree.cpp:1070-1097).  AFMM2D_M2L_BFP48 keeps each block in 48-bit
+ * block floating point: one power-of-two exponent per block and a 48-bit
+ * integer mantissa per entry (6 bytes instead of 8), every entry within
+ * 2^-46 (1.4e-14) of the block's largest entry; the products are formed
+ * and summed in FP64 from the stored values.  The M2L pass, the matvec's
+ * largest HBM stream, then reads a quarter fewer bytes.  ACA-factored
+ * pairs stay FP64; the blocked passes build their own factors.
+ * afmm2d_get_m2l returns the stored values.  No reference counterpart. */
+typedef enum { AFMM2D_M2L_FP64 = 0, AFMM2D_M2L_BFP48 = 1 } afmm2d_m2l_storage;
+int afmm2d_set_m2l_storage(afmm2d_handle h, int storage);
 
 /* symmetric = 0 when K(i, j) != K(j, i) on one of 4096 sampled pairs of the
  * handle's points (a diagnostic for AFMM2D_KERNEL_USER_SYMMETRIC; 1 for the

@@ -50,3 +50,18 @@ def oracle_kernel(name):
     import afmm2d_amd
     kid = afmm2d_amd.KERNELS[name]
     return 4 if kid == 7 else kid
+
+
+def bfp48_round(block):
+    """A block as the 48-bit block floating-point M2L storage keeps it: one
+    exponent s for the block, the smallest with every |entry| / 2^s rounded
+    to an integer of magnitude <= 2^46 (s = e - 46 for max |entry| = f 2^e,
+    1/2 <= f < 1), each entry rounded to nearest (ties to even) on that
+    grid."""
+    import numpy as np
+    a = np.asarray(block, dtype=np.float64)
+    mx = float(np.max(np.abs(a))) if a.size else 0.0
+    if mx == 0.0:
+        return a.copy(), 0
+    s = int(np.frexp(mx)[1]) - 46
+    return np.ldexp(np.rint(np.ldexp(a, -s)), s), s
