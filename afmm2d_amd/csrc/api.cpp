@@ -1095,8 +1095,8 @@ int afmm2d_get_m2l(afmm2d_handle h, int level, int box, int slot, double* out, i
       AFMM_HIP(hipMemcpy(&d, e.pair_desc.as<PairDesc>() + p, sizeof(PairDesc),
                          hipMemcpyDeviceToHost));
       for (size_t i = 0; i < rec.size(); i++) {
-        blk[2 * i] = b48_join(rec[i].hi0, (uint16_t)(rec[i].lo & 0xFFFF), d.sexp);
-        blk[2 * i + 1] = b48_join(rec[i].hi1, (uint16_t)(rec[i].lo >> 16), d.sexp);
+        blk[2 * i] = b48_join(rec[i].lo0, (uint16_t)(rec[i].hi & 0xFFFF), d.sexp);
+        blk[2 * i + 1] = b48_join(rec[i].lo1, (uint16_t)(rec[i].hi >> 16), d.sexp);
       }
     } else if (!blk.empty()) {
       AFMM_HIP(hipMemcpy(blk.data(), e.m2l.as<double>() + e.h_pair_off[p],

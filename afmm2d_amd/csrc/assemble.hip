@@ -480,11 +480,11 @@ __global__ __launch_bounds__(64) void k_m2l_b48(const double* __restrict__ src,
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
   const int sx = b48_exponent(m);
   for (int i = threadIdx.x; i < tot / 2; i += 64) {
-    int32_t h0, h1;
-    uint16_t l0, l1;
-    b48_split(src[off + 2 * i], sx, h0, l0);
-    b48_split(src[off + 2 * i + 1], sx, h1, l1);
-    dst[(off >> 1) + i] = B48Pair{h0, h1, (uint32_t)l0 | ((uint32_t)l1 << 16)};
+    uint32_t l0, l1;
+    uint16_t h0, h1;
+    b48_split(src[off + 2 * i], sx, l0, h0);
+    b48_split(src[off + 2 * i + 1], sx, l1, h1);
+    dst[(off >> 1) + i] = B48Pair{l0, l1, (uint32_t)h0 | ((uint32_t)h1 << 16)};
   }
   if (threadIdx.x == 0) pd[p].sexp = sx;
 }

@@ -30,25 +30,31 @@ struct Error : std::runtime_error {
 // M2L block storage modes (Engine::m2l_store, afmm2d_set_m2l_storage).
 enum M2LStore { M2L_STORE_F64 = 0, M2L_STORE_B48 = 1 };
 
-// 48-bit block floating point: a block's entries v are stored as 48-bit
-// two's-complement integers m = rint(v 2^-s) with one exponent s per block,
-// the smallest for which every |m| <= 2^46 (s = e - 46 with max |v| < 2^e),
-// so |v - m 2^s| <= 2^(s-1) <= 2^-46 max |v|.  m is kept as its top 32 bits
-// (signed) and its low 16 bits; both convert to doubles exactly and
-// 2^16 hi + lo is m.
+// 48-bit block floating point: a block's entries v are stored as integers
+// m = rint(v 2^-s) with one exponent s per block, the smallest for which
+// every |m| <= 2^46 (s = e - 46 with max |v| < 2^e), so
+// |v - m 2^s| <= 2^(s-1) <= 2^-46 max |v|.  m is kept biased, as the
+// unsigned 48-bit u = m + 2^47: its low 32 bits and its top 16.  The double
+// whose exponent field is that of 2^52 and whose mantissa field is u is
+// exactly 2^52 + u, so one subtraction of kB48Bias = 2^52 + 2^47 returns m
+// (exactly: an integer of magnitude <= 2^46).
+constexpr double kB48Bias = 4503599627370496.0 + 140737488355328.0;  // 2^52 + 2^47
+constexpr uint32_t kB48ExpWord = 0x43300000u;  // top word of 2^52
 __host__ __device__ __forceinline__ int b48_exponent(double maxabs) {
   if (!(maxabs > 0.0)) return 0;
   int e;
   (void)frexp(maxabs, &e);  // maxabs = f 2^e, 1/2 <= f < 1
   return e - 46;
 }
-__host__ __device__ __forceinline__ void b48_split(double v, int sexp, int32_t& hi, uint16_t& lo) {
-  const long long m = llrint(ldexp(v, -sexp));
-  hi = (int32_t)(m >> 16);
-  lo = (uint16_t)(m & 0xFFFF);
+__host__ __device__ __forceinline__ void b48_split(double v, int sexp, uint32_t& lo32,
+                                                   uint16_t& hi16) {
+  const unsigned long long u = (unsigned long long)(llrint(ldexp(v, -sexp)) + (1ll << 47));
+  lo32 = (uint32_t)u;
+  hi16 = (uint16_t)(u >> 32);
 }
-__host__ __device__ __forceinline__ double b48_join(int32_t hi, uint16_t lo, int sexp) {
-  return ldexp((double)(((long long)hi << 16) + (long long)lo), sexp);
+__host__ __device__ __forceinline__ double b48_join(uint32_t lo32, uint16_t hi16, int sexp) {
+  const long long u = (long long)(((unsigned long long)hi16 << 32) | lo32);
+  return ldexp((double)(u - (1ll << 47)), sexp);
 }
 
 // Kernel ids match afmm2d_kernel_id.

@@ -123,10 +123,12 @@ inline int m2l_tile_class(int ld, int cols) {
 }
 
 // Everything the M2L pair kernel needs about one pair, in one record.
-// One row pair of a block in the 48-bit storage (M2L_STORE_B48).
+// One row pair of a block in the 48-bit storage (M2L_STORE_B48): the
+// biased mantissas' low 32 bits, then their top 16 bits packed in one word
+// (entry 2i in bits 0-15, 2i + 1 in 16-31).
 struct B48Pair {
-  int32_t hi0, hi1;  // top 32 bits of the two mantissas
-  uint32_t lo;       // their low 16 bits: entry 2i in bits 0-15, 2i + 1 in 16-31
+  uint32_t lo0, lo1;
+  uint32_t hi;
 };
 
 struct PairDesc {
@@ -405,9 +407,8 @@ struct Engine {
   //   M2L_STORE_B48: m2l_b48, 48-bit block floating point (b48_split): each
   //   entry a 48-bit integer mantissa, the block's exponent in its
   //   PairDesc; the entries 2i, 2i + 1 of a block (one row pair of a
-  //   column) form a 12-byte record at record index (off + 2i) / 2: their
-  //   top 32 bits, then their low 16 bits packed in one word (entry 2i in
-  //   the low half).  6 bytes per entry, error <= 2^-46 of the block's
+  //   column) form a 12-byte record (B48Pair) at record index
+  //   (off + 2i) / 2.  6 bytes per entry, error <= 2^-46 of the block's
   //   largest entry.  Factored pairs stay FP64 in m2lf.
   int m2l_store = 0;
   DevBuf m2l_b48;

@@ -363,8 +363,9 @@ __device__ __forceinline__ double sum_lanes_stride8(double v) {
 // pair-indexed (entries 2 i and 2 i + 1 of the block, one row pair of a
 // column), and the factor the pair's products take at the output.
 // FP64: one 16-byte load, factor 1.  48-bit block floating point
-// (M2L_STORE_B48): one 12-byte load of the row pair's record, its mantissas
-// joined exactly into integer-valued doubles (2^16 hi + lo); the block's
+// (M2L_STORE_B48): one 12-byte load of the row pair's record, its biased
+// mantissas turned exactly into integer-valued doubles (the bits of
+// 2^52 + u, minus 2^52 + 2^47: one bit operation and one subtraction); the block's
 // power of two 2^sexp is the factor, so the sums over the block are those
 // of the FP64 products of the stored values.  Block offsets and leading
 // dimensions are even, so a row pair is one record.
@@ -389,8 +390,9 @@ struct BlockView<M2L_STORE_B48> {
       : p(b.b48 + (d.off >> 1)) {}
   __device__ __forceinline__ double2 operator[](int i) const {
     const B48Pair a = p[i];
-    return make_double2(fma((double)a.hi0, 65536.0, (double)(a.lo & 0xFFFFu)),
-                        fma((double)a.hi1, 65536.0, (double)(a.lo >> 16)));
+    return make_double2(
+        __hiloint2double((int)(kB48ExpWord | (a.hi & 0xFFFFu)), (int)a.lo0) - kB48Bias,
+        __hiloint2double((int)(kB48ExpWord | (a.hi >> 16)), (int)a.lo1) - kB48Bias);
   }
   __device__ __forceinline__ double scale(const PairDesc& d) const { return ldexp(1.0, d.sexp); }
 };
