@@ -43,6 +43,13 @@ METRIC = "FMM matvec point\u00b7RHS/s at rel.err\u22641e-8, N=2^20; % HBM roofli
 UNIT = "point\u00b7RHS/s"
 
 
+STORAGE_NOTE = {
+    "fp64": "FP64",
+    "bfp48": "in 48-bit block floating point (one exponent per block, 48-bit integer "
+             "mantissas, entries within 2^-46 of the block's largest; FP64 products and sums)",
+}
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -54,6 +61,9 @@ def parse():
     p.add_argument("--kernel", default="log", choices=["log", "inv_r", "gaussian", "reg_inv_r"],
                    help="kernel of the operator (the headline metric is quoted on log)")
     p.add_argument("--config-tol-pow", type=int, default=8)
+    p.add_argument("--m2l-storage", default="bfp48", choices=["bfp48", "fp64"],
+                   help="dense M2L block storage of the headline operator (the other is "
+                        "reported beside it)")
     p.add_argument("--check-rows", type=int, default=4096)
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -61,6 +71,9 @@ def parse():
     p.add_argument("--cpu-reps", type=int, default=5)
     p.add_argument("--blocked-steps", type=int, default=10,
                    help="timed passes per blocked variant (0: skip the blocked variants)")
+    p.add_argument("--headline-only", action="store_true",
+                   help="the headline operator only (no blocked, alternative-storage, factored "
+                        "or configured-tolerance runs): for counter passes over its kernels")
     p.add_argument("--no-side-run", action="store_true",
                    help="skip the run at the configured tolerance (profiling runs)")
     p.add_argument("--calib-copy-mb", type=int, default=1024,
@@ -735,11 +748,15 @@ def main():
         assembly_repeat(args, H, E)
         return
     xy, q = inputs(args.n, args.seed)
+    if args.headline_only:
+        args.blocked_steps = 0
+        args.no_side_run = True
 
     # headline: smallest TOL_POW meeting the 1e-8 error bar
     print("bench: headline N=%d tol 1e-%d" % (args.n, args.tol_pow), file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     A = E.AFMM(args.n, args.leaf, args.tol_pow, xy, kernel=args.kernel)
+    A.set_m2l_storage(args.m2l_storage)
     t_create = time.perf_counter() - t0
     t0 = time.perf_counter()
     A.assemble()
@@ -758,55 +775,58 @@ def main():
     A.close()
     del A
 
-    # the same operator with its dense M2L blocks in 48-bit block floating
-    # point (6 bytes per entry), products still in FP64
-    print("bench: 48-bit block floating-point M2L blocks", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    Ab48 = E.AFMM(args.n, args.leaf, args.tol_pow, xy, kernel=args.kernel)
-    Ab48.set_m2l_storage("bfp48")
-    Ab48.assemble()
-    t_asm_b = time.perf_counter() - t0
-    st_b48 = Ab48.stats()
-    ms_b48, _, kern_b48, y_b48 = run_gpu(Ab48, q, args.steps, args.warmup, H)
-    Ab48.close()
-    del Ab48
-    m2l_alg_b48 = (st_b48["m2l_leaf_bytes"] + 8 * st_b48["m2l_leaf_columns"] +
-                   8 * st_b48["skeleton_leaf"])
-    ach_b48 = m2l_alg_b48 / (kern_b48["m2l_timed_region"] * 1e-3) / 1e9
-    b48 = {"value": round(args.n / (ms_b48 * 1e-3), 1), "ms_per_step": round(ms_b48, 4),
-           "rel_err": float(np.linalg.norm(y_b48[rows] - ref_rows) / np.linalg.norm(ref_rows)),
-           "rel_diff_vs_fp64_blocks": float(np.linalg.norm(y_b48 - y) / np.linalg.norm(y)),
-           "create_and_assemble_s": round(t_asm_b, 3),
-           "m2l_bytes": st_b48["m2l_bytes"], "matvec_bytes": st_b48["matvec_bytes"],
-           "kernel_avg_ms": {k: round(v, 4) for k, v in kern_b48.items()},
-           "m2l_roofline": {"kernel": "m2l", "bound": "hbm", "achieved": round(ach_b48, 1),
-                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(ach_b48 / HBM_PEAK_GBS, 4),
-                            "algorithmic_bytes": int(m2l_alg_b48),
-                            "avg_ms": round(kern_b48["m2l_timed_region"], 4)},
-           "matvec_roofline_frac": round(st_b48["matvec_bytes"] / (ms_b48 * 1e-3) / 1e9 /
-                                         HBM_PEAK_GBS, 4)}
+    alt = factored = None
+    if not args.headline_only:
+        # the same operator with the other dense M2L block storage
+        alt_storage = "fp64" if args.m2l_storage == "bfp48" else "bfp48"
+        print("bench: M2L blocks stored as %s" % alt_storage, file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        Aalt = E.AFMM(args.n, args.leaf, args.tol_pow, xy, kernel=args.kernel)
+        Aalt.set_m2l_storage(alt_storage)
+        Aalt.assemble()
+        t_asm_alt = time.perf_counter() - t0
+        st_alt = Aalt.stats()
+        ms_alt, _, kern_alt, y_alt = run_gpu(Aalt, q, args.steps, args.warmup, H)
+        Aalt.close()
+        del Aalt
+        m2l_alg_alt = (st_alt["m2l_leaf_bytes"] + 8 * st_alt["m2l_leaf_columns"] +
+                       8 * st_alt["skeleton_leaf"])
+        ach_alt = m2l_alg_alt / (kern_alt["m2l_timed_region"] * 1e-3) / 1e9
+        alt = {"m2l_storage": alt_storage, "value": round(args.n / (ms_alt * 1e-3), 1),
+               "ms_per_step": round(ms_alt, 4),
+               "rel_err": float(np.linalg.norm(y_alt[rows] - ref_rows) / np.linalg.norm(ref_rows)),
+               "rel_diff_vs_headline": float(np.linalg.norm(y_alt - y) / np.linalg.norm(y)),
+               "create_and_assemble_s": round(t_asm_alt, 3),
+               "m2l_bytes": st_alt["m2l_bytes"], "matvec_bytes": st_alt["matvec_bytes"],
+               "kernel_avg_ms": {k: round(v, 4) for k, v in kern_alt.items()},
+               "m2l_roofline": {"kernel": "m2l", "bound": "hbm", "achieved": round(ach_alt, 1),
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(ach_alt / HBM_PEAK_GBS, 4),
+                                "algorithmic_bytes": int(m2l_alg_alt),
+                                "avg_ms": round(kern_alt["m2l_timed_region"], 4)},
+               "matvec_roofline_frac": round(st_alt["matvec_bytes"] / (ms_alt * 1e-3) / 1e9 /
+                                             HBM_PEAK_GBS, 4)}
 
-    # the same operator with ACA-factored M2L blocks where they are smaller
-    print("bench: ACA-factored M2L", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    Af = E.AFMM(args.n, args.leaf, args.tol_pow, xy, kernel=args.kernel)
-    Af.set_m2l_factoring(True)
-    Af.assemble()
-    t_asm_f = time.perf_counter() - t0
-    st_f = Af.stats()
-    ms_f, _, kern_f, y_f = run_gpu(Af, q, args.steps, args.warmup, H)
-    factored = {"value": round(args.n / (ms_f * 1e-3), 1), "ms_per_step": round(ms_f, 4),
-                "rel_err": float(np.linalg.norm(y_f[rows] - ref_rows) / np.linalg.norm(ref_rows)),
-                "rel_diff_vs_dense": float(np.linalg.norm(y_f - y) / np.linalg.norm(y)),
-                "create_and_assemble_s": round(t_asm_f, 3),
-                "factored_pairs": st_f["m2l_factored_pairs"], "pairs": st_f["il_pairs"] // 2,
-                "m2l_bytes": st_f["m2l_bytes"],
-                "m2l_factored_dense_bytes": st_f["m2l_factored_dense_bytes"],
-                "m2l_factored_bytes": st_f["m2l_factored_bytes"],
-                "kernel_avg_ms": {k: round(v, 4) for k, v in kern_f.items()}}
-    Af.close()
-    del Af
+        # the same operator with ACA-factored M2L blocks where they are smaller
+        print("bench: ACA-factored M2L", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        Af = E.AFMM(args.n, args.leaf, args.tol_pow, xy, kernel=args.kernel)
+        Af.set_m2l_factoring(True)
+        Af.assemble()
+        t_asm_f = time.perf_counter() - t0
+        st_f = Af.stats()
+        ms_f, _, kern_f, y_f = run_gpu(Af, q, args.steps, args.warmup, H)
+        factored = {"value": round(args.n / (ms_f * 1e-3), 1), "ms_per_step": round(ms_f, 4),
+                    "rel_err": float(np.linalg.norm(y_f[rows] - ref_rows) / np.linalg.norm(ref_rows)),
+                    "rel_diff_vs_dense": float(np.linalg.norm(y_f - y) / np.linalg.norm(y)),
+                    "create_and_assemble_s": round(t_asm_f, 3),
+                    "factored_pairs": st_f["m2l_factored_pairs"], "pairs": st_f["il_pairs"] // 2,
+                    "m2l_bytes": st_f["m2l_bytes"],
+                    "m2l_factored_dense_bytes": st_f["m2l_factored_dense_bytes"],
+                    "m2l_factored_bytes": st_f["m2l_factored_bytes"],
+                    "kernel_avg_ms": {k: round(v, 4) for k, v in kern_f.items()}}
+        Af.close()
+        del Af
 
     # configured tolerance, reported beside the headline
     side = None
@@ -880,8 +900,10 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "N=%d uniform [-1,1]^2, %s kernel, leaf %d (L=%d), 1 RHS, "
-                        "ACA tol 1e-%d" % (args.n, args.kernel, args.leaf, st["levels"],
-                                           args.tol_pow),
+                        "ACA tol 1e-%d, M2L blocks %s" % (
+                            args.n, args.kernel, args.leaf, st["levels"], args.tol_pow,
+                            STORAGE_NOTE[args.m2l_storage]),
+            "m2l_storage": args.m2l_storage,
             "n": args.n, "leaf": args.leaf, "tol_pow": args.tol_pow, "kernel": args.kernel,
             "nrhs": 1, "levels": st["levels"], "max_rank": st["max_rank"],
             "rel_err": err, "rel_err_rows": int(len(rows)),
@@ -905,7 +927,7 @@ def main():
                          "avg_ms": round(p2p_ms, 4),
                          "note": "beside the M2L on the side stream"},
         "blocked": blocked,
-        "m2l_bfp48": b48,
+        "m2l_storage_alternative": alt,
         "m2l_factored": factored,
         "cpu_baseline": cpu,
     }
