@@ -127,6 +127,10 @@ class AFMM {
 
   void assemble() { check(afmm2d_assemble(h_)); }
 
+  // Dense M2L block storage, before assemble(): AFMM2D_M2L_FP64 (default) or
+  // AFMM2D_M2L_BFP48 (48-bit block floating point, FP64 products).
+  void setM2LStorage(int storage) { check(afmm2d_set_m2l_storage(h_, storage)); }
+
   // y = K b, both in the caller's point order.
   std::vector<double> computeMatVecProduct(const std::vector<double>& b) {
     if ((int)b.size() != N) throw Error(AFMM2D_ERR_INVALID_ARGUMENT, "b size");
