@@ -58,6 +58,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <unordered_map>
 #include <cstdlib>
 #include <cstring>
 
@@ -484,10 +485,15 @@ __device__ __forceinline__ void two_sum(float& s, float& c, float x) {
   s = t;
 }
 
+// Leading dimension (doubles) of a wave's FP64 kernel tile in LDS: 64
+// target rows by 16 sources, one pad column.
+constexpr int kNearTileLd = 17;
+
 template <int P, int NCT, int NRT, int KID>
 __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
                                              const MrNearProb* __restrict__ probs,
-                                             const double2* tab, double ln2_hi, double ln2_lo) {
+                                             const double2* tab, double ln2_hi, double ln2_lo,
+                                             double* ktile) {
   using T = MrT<P>;
   using VT = typename T::V;
   using Acc = typename T::Acc;
@@ -514,6 +520,24 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
                                     : (float)inv_a;
   const float c1 = (float)kp.a;
   const double scale = KID == K_LOG ? 0.5 : 1.0;
+  // FP64 pair sources: the task's own charges as the B operand of the
+  // transposed products (lane i + 16 g: target 16 rt + 4 tg + g, column
+  // 16 ct + i; zero past the task's rows)
+  const int npair = P == PREC_F64 ? __builtin_amdgcn_readfirstlane(d.npair) : 0;
+  VT qa[P == PREC_F64 ? NRT : 1][4][NCT];
+  if constexpr (P == PREC_F64) {
+#pragma unroll
+    for (int rt = 0; rt < NRT; rt++)
+#pragma unroll
+      for (int tg = 0; tg < 4; tg++) {
+        const int r = 16 * rt + 4 * tg + g;
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) {
+          qa[rt][tg][ct] = npair > 0 ? Q[(int64_t)(d.ta + min(r, M - 1)) * RP + 16 * ct + i] : 0.0;
+          if (r >= M) qa[rt][tg][ct] = 0.0;
+        }
+      }
+  }
   Acc acc[4][NCT];  // FP64: the sum; FP32/BF16: one source's partial
   Acc sh[4][NCT], sl[4][NCT];
 #pragma unroll
@@ -535,10 +559,14 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
     float qf[P == PREC_BF16 ? KS : 1][NCT];
     VT q[P == PREC_BF16 ? 1 : KS][NCT];
     uint32_t okm;
+    int src, k0, cnt;
   };
   auto load = [&](Buf& b, int src, int k0) {
     const int st = __builtin_amdgcn_readfirstlane(d.ss[src]);
     const int cnt = __builtin_amdgcn_readfirstlane(d.sc[src]);
+    b.src = src;
+    b.k0 = k0;
+    b.cnt = cnt;
     b.okm = 0;
 #pragma unroll
     for (int u = 0; u < KS; u++) {
@@ -584,6 +612,7 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
       for (int u = 0; u < 4; u++)
 #pragma unroll
         for (int ct = 0; ct < NCT; ct++) bq[u][ct] = (b.okm >> u) & 1 ? b.q[u][ct] : (VT)0;
+      const bool pair = P == PREC_F64 && b.src < npair;
 #pragma unroll
       for (int rt = 0; rt < NRT; rt++) {
 #pragma unroll
@@ -597,6 +626,9 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
               kv = scale * near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab, ln2_hi,
                                             ln2_lo);
             }
+            // the tile for the transposed products: row = target, column =
+            // source of the step
+            if (pair) ktile[(16 * rt + i) * kNearTileLd + 4 * g + u] = kv;
 #pragma unroll
             for (int ct = 0; ct < NCT; ct++)
               acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv, bq[u][ct], acc[rt][ct], 0, 0, 0);
@@ -606,6 +638,50 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
             for (int ct = 0; ct < NCT; ct++)
               acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(kv, bq[u][ct], acc[rt][ct], 0, 0, 0);
           }
+        }
+      }
+      if constexpr (P == PREC_F64) {
+        if (pair) {
+          // the source leaf's rows of this step: D' (16 sources x 16 columns)
+          // = K^T (sources x the task's targets, 4 per product, read
+          // transposed from the tile) x the task's charges, with the
+          // kernel's transpose sign (-1 for an antisymmetric part); stored to
+          // the pair's partial slot, rows past the source leaf's count
+          // dropped.  A wave's LDS operations run in order, so the tile reads
+          // see this step's writes and the next step's writes follow these
+          // reads; the wave barriers only keep the compiler from moving them
+          // (no wait on the next step's loads in flight).  Two accumulator
+          // chains (even and odd target groups) keep the products apart.
+          __builtin_amdgcn_wave_barrier();
+          Acc accT[2][NCT];
+#pragma unroll
+          for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) accT[h][ct] = Acc{};
+#pragma unroll
+          for (int rt = 0; rt < NRT; rt++)
+#pragma unroll
+            for (int tg = 0; tg < 4; tg++) {
+              const double a = ktile[(16 * rt + 4 * tg + g) * kNearTileLd + i];
+#pragma unroll
+              for (int ct = 0; ct < NCT; ct++)
+                accT[tg & 1][ct] =
+                    __builtin_amdgcn_mfma_f64_16x16x4f64(a, qa[rt][tg][ct], accT[tg & 1][ct], 0, 0, 0);
+            }
+          __builtin_amdgcn_wave_barrier();
+          const auto part = gptr_w(pr.part + d.pt[b.src]);
+#pragma unroll
+          for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+              int row, col;
+              acc_pos<P>(i, g, v, row, col);
+              const int k = b.k0 + row;
+              // K(source, target) = kTransposeSign K(target, source)^T
+              if (k < b.cnt)
+                part[(int64_t)k * RP + 16 * ct + col] =
+                    kTransposeSign<KID> * (accT[0][ct][v] + accT[1][ct][v]);
+            }
         }
       }
     }
@@ -691,17 +767,34 @@ __global__ __launch_bounds__(256) void k_mr_p2p(const MrNearTask* __restrict__ t
                                                 double ln2_hi, double ln2_lo,
                                                 int* __restrict__ ctr) {
   __shared__ double2 tab[128];
+  // FP64: each wave's kernel tile of its pair steps
+  __shared__ double ktile[P == PREC_F64 ? 4 * 16 * NRT * kNearTileLd : 1];
   if constexpr (P == PREC_F64 && KID == K_LOG) {
     for (int u = threadIdx.x; u < 128; u += blockDim.x) tab[u] = logtab[u];
     __syncthreads();
   }
   const int l = threadIdx.x & 63;
+  double* wtile = ktile + (P == PREC_F64 ? (threadIdx.x >> 6) * 16 * NRT * kNearTileLd : 0);
   for (;;) {
     int t = 0;
     if (l == 0) t = atomicAdd(ctr, 1);
     t = __builtin_amdgcn_readfirstlane(t);
     if (t >= ntasks) break;
-    mr_near_task<P, NCT, NRT, KID>(tasks[t], probs, tab, ln2_hi, ln2_lo);
+    mr_near_task<P, NCT, NRT, KID>(tasks[t], probs, tab, ln2_hi, ln2_lo, wtile);
+  }
+}
+
+// The pair partials of the near field added to the potentials, one
+// workgroup per leaf chunk, slots in order (deterministic).
+__global__ __launch_bounds__(256) void k_mr_near_red(const MrNearRed* __restrict__ tasks,
+                                                     int rp) {
+  const MrNearRed d = tasks[blockIdx.x];
+  const int tot = d.nrows * rp;
+  double* out = d.out + d.row0 * rp;
+  for (int e = threadIdx.x; e < tot; e += 256) {
+    double s = out[e];
+    for (int sl = 0; sl < d.nslots; sl++) s += d.part[d.base + sl * d.stride + e];
+    out[e] = s;
   }
 }
 
@@ -1629,9 +1722,14 @@ void build_reduced(Engine& e, int prec) {
 // Rows per task of the M2L and the other passes (AFMM2D_MR_ROWS_M2L,
 // AFMM2D_MR_ROWS: 16, 32, 48 or 64), the XCD mapping of the M2L launch
 // (AFMM2D_MR_XCD), the near field's waves per SIMD (AFMM2D_MR_NEAR_WAVES)
-// and the chunk ring depth of the M2L (AFMM2D_MR_DEPTH: 3, 4, 6), read once.
+// and target rows per task (AFMM2D_MR_NEAR_ROWS: 16, 32, 48 or 64), whether
+// FP64 near fields evaluate each leaf pair once (AFMM2D_MR_NEAR_PAIRS, on by
+// default; 0: from both sides) and the chunk ring depth of the M2L
+// (AFMM2D_MR_DEPTH: 3, 4, 6), read once.
 struct MrKnobs {
-  int rows_m2l = 64, rows = 16, xcd = 0, near_waves = 2, depth = 3;
+  int rows_m2l = 64, rows = 16, xcd = 0, near_waves = 2, depth = 3, near_rows = 64;
+  bool near_pairs = true;
+  bool near_after_up = false;  // fork the near field after the upward pass
   MrKnobs() {
     auto rd = [](const char* n, int d) {
       const char* v = std::getenv(n);
@@ -1642,6 +1740,9 @@ struct MrKnobs {
     xcd = rd("AFMM2D_MR_XCD", 0);
     near_waves = std::max(1, std::min(16, rd("AFMM2D_MR_NEAR_WAVES", 2)));
     depth = rd("AFMM2D_MR_DEPTH", 3);
+    near_rows = std::max(16, std::min(64, rd("AFMM2D_MR_NEAR_ROWS", 64))) / 16 * 16;
+    near_pairs = rd("AFMM2D_MR_NEAR_PAIRS", 1) != 0;
+    near_after_up = rd("AFMM2D_MR_NEAR_AFTER_UP", 0) != 0;
   }
 };
 const MrKnobs& knobs() {
@@ -1824,30 +1925,85 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
   upload_tasks(p.leaf, leaf, s);
   p.h_m2l = std::move(m2l);
   p.h_leaf = std::move(leaf);
-  // near field: every leaf's points against its neighbours N0..N7 and itself
+  // near field.  FP64 plans: each leaf pair once, by the task of the leaf
+  // whose E, NE, N or NW neighbour the other is (neighbour slots 3..6; the
+  // opposite slot is (q + 4) mod 8), which also stores the transposed
+  // product into a partial slot of the other leaf; every row block of a
+  // leaf's W, SW, S and SE neighbours owns one slot of its rows.  Other
+  // precisions: every leaf's points against its neighbours N0..N7 and
+  // itself.
   std::vector<MrNearTask> near;
   const int nleaf = 1 << (2 * L);
+  const int nr = knobs().near_rows;
+  const bool pairs = p.prec == PREC_F64 && knobs().near_pairs;
+  auto nbr_of = [&](int k, int q) { return e.nbr[L][8 * (size_t)k + q]; };
+  auto blocks_of = [&](int k) { return (e.leaf_count[k] + nr - 1) / nr; };
+  std::unordered_map<int64_t, int64_t> slot_at;  // (leaf, upper slot, row block) -> offset
+  auto slot_key = [](int k, int q, int rb) { return ((int64_t)k * 8 + q) * 4096 + rb; };
+  std::vector<MrNearRed> red;
+  int64_t ptot = 0;
+  if (pairs) {
+    for (int b = 0; b < nleaf; b++) {
+      const int cb = e.leaf_count[b];
+      if (cb == 0) continue;
+      const int64_t base = ptot;
+      int ns = 0;
+      for (int q : {7, 0, 1, 2}) {
+        const int a = nbr_of(b, q);
+        if (a < 0 || e.leaf_count[a] == 0) continue;
+        for (int rb = 0; rb < blocks_of(a); rb++) {
+          slot_at[slot_key(a, (q + 4) % 8, rb)] = ptot;
+          ptot += (int64_t)cb * RP;
+          ns++;
+        }
+      }
+      for (int r0 = 0; ns > 0 && r0 < cb; r0 += 64)
+        red.push_back(MrNearRed{p.nearv.as<double>(), nullptr, e.leaf_start[b] + r0,
+                                base + (int64_t)r0 * RP, (int64_t)cb * RP, std::min(64, cb - r0),
+                                ns});
+    }
+  }
+  p.nearp.alloc(sizeof(double) * (size_t)std::max<int64_t>(ptot, 1));
+  for (MrNearRed& r : red) r.part = p.nearp.as<double>();
   for (int k = 0; k < nleaf; k++) {
     const int st = e.leaf_start[k], n = e.leaf_count[k];
-    for (int r0 = 0; r0 < n; r0 += 64) {
+    for (int r0 = 0; r0 < n; r0 += nr) {
       MrNearTask t{};
       t.ta = st + r0;
-      t.M = std::min(64, n - r0);
+      t.M = std::min(nr, n - r0);
       t.out_row = st + r0;
-      for (int q = 0; q <= 8; q++) {
-        const int b = q < 8 ? e.nbr[L][8 * (size_t)k + q] : k;
-        if (b < 0 || e.leaf_count[b] == 0) continue;
-        t.ss[t.nsrc] = e.leaf_start[b];
-        t.sc[t.nsrc] = e.leaf_count[b];
+      if (pairs) {
+        for (int q : {3, 4, 5, 6}) {
+          const int b = nbr_of(k, q);
+          if (b < 0 || e.leaf_count[b] == 0) continue;
+          t.ss[t.nsrc] = e.leaf_start[b];
+          t.sc[t.nsrc] = e.leaf_count[b];
+          t.pt[t.nsrc] = slot_at.at(slot_key(k, q, r0 / nr));
+          t.nsrc++;
+        }
+        t.npair = t.nsrc;
+        t.ss[t.nsrc] = st;
+        t.sc[t.nsrc] = n;
         t.nsrc++;
+      } else {
+        for (int q = 0; q <= 8; q++) {
+          const int b = q < 8 ? nbr_of(k, q) : k;
+          if (b < 0 || e.leaf_count[b] == 0) continue;
+          t.ss[t.nsrc] = e.leaf_start[b];
+          t.sc[t.nsrc] = e.leaf_count[b];
+          t.nsrc++;
+        }
       }
       near.push_back(t);
     }
   }
+  upload_vec(p.near_red, red, s);
+  p.h_near_red = std::move(red);
   upload_tasks(p.near, near, s);
   p.h_near = std::move(near);
   p.near_ctr.alloc(sizeof(int) * 4 * kMrCtrStride);
-  MrNearProb pr{e.x.as<double>(), e.y.as<double>(), p.qs.p, p.nearv.as<double>(), e.kp};
+  MrNearProb pr{e.x.as<double>(), e.y.as<double>(), p.qs.p, p.nearv.as<double>(),
+                p.nearp.as<double>(), e.kp};
   upload_vec(p.probs, std::vector<MrNearProb>{pr}, s);
   AFMM_HIP(hipStreamSynchronize(s));
   e.mr_plans.push_back(std::move(pl));
@@ -1973,10 +2129,18 @@ void launch_near_p(int prec, int nct, const NearLaunch& nl, hipStream_t s) {
   }
 }
 
+// The pair partials' reduction after the near field (none without pairs).
+void launch_near_red(const DevBuf& red, size_t n, int nct, hipStream_t s) {
+  if (n == 0) return;
+  k_mr_near_red<<<(unsigned)n, 256, 0, s>>>(red.as<MrNearRed>(), 16 * nct);
+  AFMM_LAUNCH_CHECK();
+}
+
 void launch_near_plan(Engine& e, const MrPlan& p, int near_waves, hipStream_t s) {
   NearLaunch nl{&p.near, p.probs.as<MrNearProb>(), e.log_tab.as<double2>(), e.ln2_hi, e.ln2_lo,
                 p.near_ctr.as<int>(), e.kid, e.num_cus, near_waves};
   launch_near_p(p.prec, p.nct, nl, s);
+  launch_near_red(p.near_red, p.h_near_red.size(), p.nct, s);
 }
 
 template <class VT, int NCT>
@@ -2056,17 +2220,19 @@ void mr_matvec(Engine& e, const double* d_b, int64_t ldb, double* d_y, int64_t l
   e.prof_end(s, b);
   // the near field needs only the gathered charges: it runs beside the tree
   // (AFMM2D_NEAR_OVERLAP=0: after it, on the caller's stream)
-  if (e.near_overlap) {
+  auto fork_near = [&]() {
     AFMM_HIP(hipEventRecord(e.ev_fork, s));
     AFMM_HIP(hipStreamWaitEvent(e.s_aux, e.ev_fork, 0));
     e.prof_begin("mr_p2p", e.s_aux, &a, &b);
     launch_near_plan(e, p, tn.near_waves, e.s_aux);
     e.prof_end(e.s_aux, b);
     AFMM_HIP(hipEventRecord(e.ev_join, e.s_aux));
-  }
+  };
+  if (e.near_overlap && !knobs().near_after_up) fork_near();
   e.prof_begin("mr_m2m", s, &a, &b);
   for (int j = L; j >= 2; j--) launch_gemm_p(prec, nct, p.up[j], 0, 3, s);
   e.prof_end(s, b);
+  if (e.near_overlap && knobs().near_after_up) fork_near();
   e.prof_begin("mr_m2l", s, &a, &b);
   launch_gemm_p(prec, nct, p.m2l, tn.xcd, tn.depth, s);
   e.prof_end(s, b);
@@ -2112,6 +2278,8 @@ struct MrBatch {
   };
   std::vector<NearGroup> near;
   DevBuf ctr;                   // 4 counters per near group
+  DevBuf near_red;              // every problem's pair-partial reduction
+  size_t n_near_red = 0;
   DevBuf io;                    // MrIoTask per problem, i0 = first workgroup
   int nio = 0, io_blocks = 0;
   hipStream_t side = nullptr;   // near field, beside the tree passes
@@ -2201,18 +2369,23 @@ void batch_m2l(MrBatch& b, int rows, hipStream_t s) {
 void batch_enqueue(MrBatch& b, hipStream_t s) {
   const MrIoTask* io = b.io.as<MrIoTask>();
   launch_io_multi(b.prec, b.nct, true, io, b.nio, b.io_blocks, s);
-  AFMM_HIP(hipEventRecord(b.fork, s));
-  AFMM_HIP(hipStreamWaitEvent(b.side, b.fork, 0));
-  for (size_t g = 0; g < b.near.size(); g++) {
-    const MrBatch::NearGroup& ng = b.near[g];
-    NearLaunch nl{&ng.tl, ng.probs.as<MrNearProb>(), ng.first->log_tab.as<double2>(),
-                  ng.first->ln2_hi, ng.first->ln2_lo,
-                  b.ctr.as<int>() + 4 * kMrCtrStride * g, ng.kid, ng.first->num_cus,
-                  b.tune.near_waves};
-    launch_near_p(b.prec, b.nct, nl, b.side);
-  }
-  AFMM_HIP(hipEventRecord(b.join, b.side));
+  auto fork_near = [&]() {
+    AFMM_HIP(hipEventRecord(b.fork, s));
+    AFMM_HIP(hipStreamWaitEvent(b.side, b.fork, 0));
+    for (size_t g = 0; g < b.near.size(); g++) {
+      const MrBatch::NearGroup& ng = b.near[g];
+      NearLaunch nl{&ng.tl, ng.probs.as<MrNearProb>(), ng.first->log_tab.as<double2>(),
+                    ng.first->ln2_hi, ng.first->ln2_lo,
+                    b.ctr.as<int>() + 4 * kMrCtrStride * g, ng.kid, ng.first->num_cus,
+                    b.tune.near_waves};
+      launch_near_p(b.prec, b.nct, nl, b.side);
+    }
+    launch_near_red(b.near_red, b.n_near_red, b.nct, b.side);
+    AFMM_HIP(hipEventRecord(b.join, b.side));
+  };
+  if (!knobs().near_after_up) fork_near();
   for (const MrTaskList& tl : b.up) launch_gemm_p(b.prec, b.nct, tl, 0, 3, s);
+  if (knobs().near_after_up) fork_near();
   launch_gemm_p(b.prec, b.nct, b.m2l, b.tune.xcd, b.tune.depth, s);
   for (const MrTaskList& tl : b.down) launch_gemm_p(b.prec, b.nct, tl, 0, 3, s);
   AFMM_HIP(hipStreamWaitEvent(s, b.join, 0));
@@ -2290,7 +2463,7 @@ MrBatch* mr_batch_create(const std::vector<Engine*>& eng, const double* const* d
       const MrPlan& p = *b.plans[i];
       const int pi = (int)probs.size();
       probs.push_back(MrNearProb{e.x.as<double>(), e.y.as<double>(), p.qs.p, p.nearv.as<double>(),
-                                 e.kp});
+                                 p.nearp.as<double>(), e.kp});
       for (MrNearTask t : p.h_near) {
         t.prob = pi;
         tasks.push_back(t);
@@ -2305,6 +2478,12 @@ MrBatch* mr_batch_create(const std::vector<Engine*>& eng, const double* const* d
     upload_vec(ng.probs, probs, s);
   }
   b.ctr.alloc(sizeof(int) * 4 * kMrCtrStride * std::max<size_t>(b.near.size(), 1));
+  {
+    std::vector<MrNearRed> red;
+    for (const MrPlan* p : b.plans) red.insert(red.end(), p->h_near_red.begin(), p->h_near_red.end());
+    b.n_near_red = red.size();
+    upload_vec(b.near_red, red, s);
+  }
   // gather / output: problem i owns workgroups [i0, i0 + ceil(n / kMrIoPoints))
   std::vector<MrIoTask> io;
   int blocks = 0;

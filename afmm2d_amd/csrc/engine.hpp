@@ -196,10 +196,26 @@ struct MrMicro {
 // Near field of one group of up to 64 target points (KD-sorted indices ta ..
 // ta + M - 1) against up to 9 source ranges (the leaf's neighbours and
 // itself), written to rows out_row .. of the problem's near-field output.
+// Sources [0, npair) are pair sources (FP64 plans): the task evaluates
+// each such leaf pair once, for its own rows and, transposed, for the
+// source leaf's, whose product it stores to the problem's partials at pt
+// (the slot of this task's rows in the source leaf's list); the rest are
+// applied to the task's rows only.
 struct MrNearTask {
   int ta, M, out_row, nsrc;
   int ss[9], sc[9];
-  int prob, pad;
+  int prob, npair;
+  int64_t pt[4];
+};
+
+// The near-field partials of one leaf chunk added to its potentials, slot
+// after slot: out[(row0 + r) RP + c] += sum_s part[base + s stride + r RP + c]
+// for r < nrows.
+struct MrNearRed {
+  double* out;
+  const double* part;
+  int64_t row0, base, stride;
+  int nrows, nslots;
 };
 
 // Per-problem data of the near field (coordinates, charges, output).
@@ -208,6 +224,7 @@ struct MrNearProb {
   const double* y;
   const void* q;  // charges, rows of 16 NCT vector elements, KD order
   double* out;    // near-field potentials, rows of 16 NCT doubles
+  double* part;   // pair sources' transposed products (MrNearTask::pt)
   KParams kp;
 };
 
@@ -252,6 +269,9 @@ struct MrTaskList {
 struct MrPlan {
   int prec = -1, nct = 0;
   DevBuf qs, outv, inv, pot, nearv;
+  DevBuf nearp;                 // near-field pair partials (FP64 plans)
+  DevBuf near_red;              // their reduction, MrNearRed per leaf chunk
+  std::vector<MrNearRed> h_near_red;  // (empty without pairs)
   std::vector<MrTaskList> up, down;  // per level (index j)
   MrTaskList m2l, leaf, near;
   DevBuf probs;

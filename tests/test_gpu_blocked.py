@@ -241,3 +241,61 @@ def test_device_entry_on_a_stream(engine):
     hip.hipFree(dB)
     hip.hipFree(dY)
     A.close()
+
+
+@pytest.mark.parametrize("n,leaf,seed", [(30000, 64, 3), (8192, 32, 5)])
+def test_fp64_near_pairs_ragged_leaves(engine, n, leaf, seed):
+    """FP64 plans evaluate each neighbouring leaf pair once and add the
+    transposed products through per-slot partials: leaves of 117 points (two
+    row blocks, so two slots per lower neighbour) and of 32 (a partial row
+    tile), against the single-RHS passes (which evaluate the near field in
+    another kernel)."""
+    print_case("near pairs", n, leaf, seed)
+    xy, _ = uniform_points(n, seed)
+    rng = np.random.default_rng(seed)
+    Q = rng.uniform(-1, 1, size=(n, 6))
+    A = engine.AFMM(n, leaf, 10, xy, kernel="log")
+    A.assemble()
+    cnt = A.NumberOfParticlesInLeaves
+    print_case("  leaves %d, points per leaf %d..%d" % (len(cnt), cnt.min(), cnt.max()))
+    Yb = A.matvec(Q, "fp64", blocked=True)
+    Ys = A.matvec(Q, "fp64")
+    print_case("  blocked vs single-RHS %.2e" % col_rel(Yb, Ys))
+    assert col_rel(Yb, Ys) <= 1e-12
+    A.close()
+
+
+def test_fp64_near_pairs_match_both_sided_evaluation(engine, tmp_path):
+    """The same blocked FP64 matvec with the near field's leaf pairs
+    evaluated once (default) and from both sides (AFMM2D_MR_NEAR_PAIRS=0,
+    read once per process: run in a child process)."""
+    import os
+    import subprocess
+    import sys
+    n = 20000
+    script = tmp_path / "both_sides.py"
+    out = tmp_path / "y.npy"
+    script.write_text(
+        "import sys, numpy as np\n"
+        "sys.path.insert(0, %r)\n"
+        "from conftest import uniform_points\n"
+        "import afmm2d_amd\n"
+        "xy, _ = uniform_points(%d, 21)\n"
+        "Q = np.random.default_rng(22).uniform(-1, 1, size=(%d, 16))\n"
+        "A = afmm2d_amd.AFMM(%d, 100, 10, xy, kernel='inv_r')\n"
+        "A.assemble()\n"
+        "np.save(%r, A.matvec(Q, 'fp64', blocked=True))\n"
+        % (os.path.dirname(os.path.abspath(__file__)), n, n, n, str(out)))
+    env = dict(os.environ, AFMM2D_MR_NEAR_PAIRS="0")
+    r = subprocess.run([sys.executable, str(script)], env=env, capture_output=True, text=True,
+                       timeout=120, cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert r.returncode == 0, r.stderr[-2000:]
+    Y2 = np.load(out)
+    xy, _ = uniform_points(n, 21)
+    Q = np.random.default_rng(22).uniform(-1, 1, size=(n, 16))
+    A = engine.AFMM(n, 100, 10, xy, kernel="inv_r")
+    A.assemble()
+    Y1 = A.matvec(Q, "fp64", blocked=True)
+    print_case("near pairs vs both sides: %.2e" % col_rel(Y1, Y2))
+    assert col_rel(Y1, Y2) <= 1e-13
+    A.close()
