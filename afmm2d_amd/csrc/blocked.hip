@@ -785,15 +785,37 @@ __global__ __launch_bounds__(256) void k_mr_p2p(const MrNearTask* __restrict__ t
 }
 
 // The pair partials of the near field added to the potentials, one
-// workgroup per leaf chunk, slots in order (deterministic).
+// workgroup per leaf chunk, slots in order (deterministic).  16-byte
+// accesses (rows and slots are whole multiples of 16 doubles), four slots'
+// loads in flight before their sums.
 __global__ __launch_bounds__(256) void k_mr_near_red(const MrNearRed* __restrict__ tasks,
                                                      int rp) {
   const MrNearRed d = tasks[blockIdx.x];
-  const int tot = d.nrows * rp;
-  double* out = d.out + d.row0 * rp;
+  const int tot = d.nrows * rp / 2;
+  double2* out = reinterpret_cast<double2*>(d.out + d.row0 * rp);
+  const double2* part = reinterpret_cast<const double2*>(d.part + d.base);
+  const int64_t st = d.stride / 2;
+  const int ns = d.nslots;
   for (int e = threadIdx.x; e < tot; e += 256) {
-    double s = out[e];
-    for (int sl = 0; sl < d.nslots; sl++) s += d.part[d.base + sl * d.stride + e];
+    double2 s = out[e];
+    int sl = 0;
+    for (; sl + 4 <= ns; sl += 4) {
+      const double2 a0 = part[e + (sl + 0) * st], a1 = part[e + (sl + 1) * st];
+      const double2 a2 = part[e + (sl + 2) * st], a3 = part[e + (sl + 3) * st];
+      s.x += a0.x;
+      s.y += a0.y;
+      s.x += a1.x;
+      s.y += a1.y;
+      s.x += a2.x;
+      s.y += a2.y;
+      s.x += a3.x;
+      s.y += a3.y;
+    }
+    for (; sl < ns; sl++) {
+      const double2 a = part[e + sl * st];
+      s.x += a.x;
+      s.y += a.y;
+    }
     out[e] = s;
   }
 }
@@ -1727,7 +1749,7 @@ void build_reduced(Engine& e, int prec) {
 // default; 0: from both sides) and the chunk ring depth of the M2L
 // (AFMM2D_MR_DEPTH: 3, 4, 6), read once.
 struct MrKnobs {
-  int rows_m2l = 64, rows = 16, xcd = 0, near_waves = 2, depth = 3, near_rows = 64;
+  int rows_m2l = 64, rows = 16, xcd = 0, near_waves = 1, depth = 3, near_rows = 64;
   bool near_pairs = true;
   bool near_after_up = false;  // fork the near field after the upward pass
   MrKnobs() {
@@ -1738,7 +1760,9 @@ struct MrKnobs {
     rows_m2l = std::max(16, std::min(64, rd("AFMM2D_MR_ROWS_M2L", 64))) / 16 * 16;
     rows = std::max(16, std::min(64, rd("AFMM2D_MR_ROWS", 16))) / 16 * 16;
     xcd = rd("AFMM2D_MR_XCD", 0);
-    near_waves = std::max(1, std::min(16, rd("AFMM2D_MR_NEAR_WAVES", 2)));
+    // one near-field wave per SIMD leaves the upward pass room beside it
+    // (N = 2^20, 16 RHS: FP64 2.49 against 2.69 ms per pass with 2 waves)
+    near_waves = std::max(1, std::min(16, rd("AFMM2D_MR_NEAR_WAVES", 1)));
     depth = rd("AFMM2D_MR_DEPTH", 3);
     near_rows = std::max(16, std::min(64, rd("AFMM2D_MR_NEAR_ROWS", 64))) / 16 * 16;
     near_pairs = rd("AFMM2D_MR_NEAR_PAIRS", 1) != 0;

@@ -288,7 +288,7 @@ struct MrPlan {
 // environment (AFMM2D_MR_DEPTH, AFMM2D_MR_XCD, AFMM2D_MR_ROWS_M2L,
 // AFMM2D_MR_NEAR_WAVES).
 struct MrTune {
-  int depth = 3, xcd = 0, near_waves = 2, rows_m2l = 64;
+  int depth = 3, xcd = 0, near_waves = 1, rows_m2l = 64;
 };
 MrTune mr_default_tune();
 

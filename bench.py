@@ -71,6 +71,9 @@ def parse():
     p.add_argument("--cpu-reps", type=int, default=5)
     p.add_argument("--blocked-steps", type=int, default=10,
                    help="timed passes per blocked variant (0: skip the blocked variants)")
+    p.add_argument("--blocked-only", default="",
+                   help="comma-separated blocked variants (fp64_r16, fp32_r32, ...): the headline "
+                        "operator and only these variants, for counter passes over their kernels")
     p.add_argument("--headline-only", action="store_true",
                    help="the headline operator only (no blocked, alternative-storage, factored "
                         "or configured-tolerance runs): for counter passes over its kernels")
@@ -464,7 +467,7 @@ def run_gpu(A, q, steps, warmup, H):
     return ms, wall, kern, y
 
 
-def run_blocked(A, Q, steps, rows, ref_rows, H, cols=(16, 32)):
+def run_blocked(A, Q, steps, rows, ref_rows, H, cols=(16, 32), only=None):
     """The blocked passes on the same operator: 16 and 32 columns of Q per
     pass in FP64, FP32 and BF16.  For each: ms per pass, point.RHS/s, the
     relative error of column 0 on the sampled rows, the M2L launch's time
@@ -475,11 +478,15 @@ def run_blocked(A, Q, steps, rows, ref_rows, H, cols=(16, 32)):
     dY = H.DeviceArray(Q.nbytes)
     out = {}
     for prec, fb, vb in (("fp64", 8, 8), ("fp32", 4, 4), ("bf16", 2, 4)):
+        if only and not any(v.startswith(prec + "_") for v in only):
+            continue
         t0 = time.perf_counter()
         A.prepare_precision(prec)
         t_prep = time.perf_counter() - t0
         st = A.stats()
         for R in cols:
+            if only and "%s_r%d" % (prec, R) not in only:
+                continue
             fn = lambda: A.matvec_device_ex(dB.ptr, n, dY.ptr, n, R, prec)
             A.reset_kernel_times()
             A.set_profiling(True, m2l_only=True)
@@ -748,9 +755,14 @@ def main():
         assembly_repeat(args, H, E)
         return
     xy, q = inputs(args.n, args.seed)
+    only = [v for v in args.blocked_only.split(",") if v]
     if args.headline_only:
         args.blocked_steps = 0
         args.no_side_run = True
+    if only:
+        args.headline_only = True  # no alternative-storage, factored or side runs
+        args.no_side_run = True
+        args.blocked_steps = args.blocked_steps or 10
 
     # headline: smallest TOL_POW meeting the 1e-8 error bar
     print("bench: headline N=%d tol 1e-%d" % (args.n, args.tol_pow), file=sys.stderr, flush=True)
@@ -770,7 +782,7 @@ def main():
         print("bench: blocked variants", file=sys.stderr, flush=True)
         Qb = np.random.default_rng(args.seed + 1).uniform(-1.0, 1.0, size=(args.n, 32))
         Qb[:, 0] = q
-        blocked = run_blocked(A, Qb, args.blocked_steps, rows, ref_rows, H)
+        blocked = run_blocked(A, Qb, args.blocked_steps, rows, ref_rows, H, only=only or None)
     near_evals = near_field_evaluations(A)
     A.close()
     del A
