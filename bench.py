@@ -511,7 +511,10 @@ def run_blocked(A, Q, steps, rows, ref_rows, H, cols=(16, 32), only=None):
             # block once (FP64: each block stored once, read by both of its
             # boxes; FP32, BF16: stored for both boxes), the multipoles read
             # once, the locals written once
-            coeffs = st["m2l_bytes"] // 8 if prec == "fp64" else st["blocked_m2l_coeffs_bf16"]
+            # (the dense blocks' entry count from the single-RHS storage: 6 or
+            # 8 bytes each; the blocked passes keep their own FP64 factors)
+            bpe = 6 if st["m2l_storage"] == 1 else 8
+            coeffs = st["m2l_bytes"] // bpe if prec == "fp64" else st["blocked_m2l_coeffs_bf16"]
             alg = coeffs * fb + 2 * st["skeleton_total"] * rp * vb
             ach = alg / (m2l_ms * 1e-3) / 1e9 if m2l_ms > 0 else 0.0
             out["%s_r%d" % (prec, R)] = {
