@@ -678,7 +678,7 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
               acc_pos<P>(i, g, v, row, col);
               const int k = b.k0 + row;
               // K(source, target) = kTransposeSign K(target, source)^T
-              if (k < b.cnt)
+              if (k < b.cnt && 16 * ct + col < pr.pw)
                 part[(int64_t)k * RP + 16 * ct + col] =
                     kTransposeSign<KID> * (accT[0][ct][v] + accT[1][ct][v]);
             }
@@ -785,23 +785,26 @@ __global__ __launch_bounds__(256) void k_mr_p2p(const MrNearTask* __restrict__ t
 }
 
 // The pair partials of the near field added to the potentials, one
-// workgroup per leaf chunk, slots in order (deterministic).  16-byte
-// accesses (rows and slots are whole multiples of 16 doubles), four slots'
-// loads in flight before their sums.
+// workgroup per leaf chunk, slots in order (deterministic).  The first pw
+// columns of each row (pw even, rows and slots whole multiples of 16
+// doubles): 16-byte accesses, four slots' loads in flight before their sums.
 __global__ __launch_bounds__(256) void k_mr_near_red(const MrNearRed* __restrict__ tasks,
                                                      int rp) {
   const MrNearRed d = tasks[blockIdx.x];
-  const int tot = d.nrows * rp / 2;
+  const int hw = d.pw / 2, hr = rp / 2;
+  const int tot = d.nrows * hw;
   double2* out = reinterpret_cast<double2*>(d.out + d.row0 * rp);
   const double2* part = reinterpret_cast<const double2*>(d.part + d.base);
   const int64_t st = d.stride / 2;
   const int ns = d.nslots;
   for (int e = threadIdx.x; e < tot; e += 256) {
-    double2 s = out[e];
+    const int r = e / hw;
+    const int o = r * hr + (e - r * hw);
+    double2 s = out[o];
     int sl = 0;
     for (; sl + 4 <= ns; sl += 4) {
-      const double2 a0 = part[e + (sl + 0) * st], a1 = part[e + (sl + 1) * st];
-      const double2 a2 = part[e + (sl + 2) * st], a3 = part[e + (sl + 3) * st];
+      const double2 a0 = part[o + (sl + 0) * st], a1 = part[o + (sl + 1) * st];
+      const double2 a2 = part[o + (sl + 2) * st], a3 = part[o + (sl + 3) * st];
       s.x += a0.x;
       s.y += a0.y;
       s.x += a1.x;
@@ -812,11 +815,11 @@ __global__ __launch_bounds__(256) void k_mr_near_red(const MrNearRed* __restrict
       s.y += a3.y;
     }
     for (; sl < ns; sl++) {
-      const double2 a = part[e + sl * st];
+      const double2 a = part[o + sl * st];
       s.x += a.x;
       s.y += a.y;
     }
-    out[e] = s;
+    out[o] = s;
   }
 }
 
@@ -1984,7 +1987,7 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
       for (int r0 = 0; ns > 0 && r0 < cb; r0 += 64)
         red.push_back(MrNearRed{p.nearv.as<double>(), nullptr, e.leaf_start[b] + r0,
                                 base + (int64_t)r0 * RP, (int64_t)cb * RP, std::min(64, cb - r0),
-                                ns});
+                                ns, RP, 0});
     }
   }
   p.nearp.alloc(sizeof(double) * (size_t)std::max<int64_t>(ptot, 1));
@@ -2027,7 +2030,7 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
   p.h_near = std::move(near);
   p.near_ctr.alloc(sizeof(int) * 4 * kMrCtrStride);
   MrNearProb pr{e.x.as<double>(), e.y.as<double>(), p.qs.p, p.nearv.as<double>(),
-                p.nearp.as<double>(), e.kp};
+                p.nearp.as<double>(), e.kp, RP, 0};
   upload_vec(p.probs, std::vector<MrNearProb>{pr}, s);
   AFMM_HIP(hipStreamSynchronize(s));
   e.mr_plans.push_back(std::move(pl));
@@ -2486,8 +2489,9 @@ MrBatch* mr_batch_create(const std::vector<Engine*>& eng, const double* const* d
       if (!first) first = &e;
       const MrPlan& p = *b.plans[i];
       const int pi = (int)probs.size();
+      // the batch's columns: the partials' unused columns are never written
       probs.push_back(MrNearProb{e.x.as<double>(), e.y.as<double>(), p.qs.p, p.nearv.as<double>(),
-                                 p.nearp.as<double>(), e.kp});
+                                 p.nearp.as<double>(), e.kp, (nrhs + 1) & ~1, 0});
       for (MrNearTask t : p.h_near) {
         t.prob = pi;
         tasks.push_back(t);
@@ -2505,6 +2509,7 @@ MrBatch* mr_batch_create(const std::vector<Engine*>& eng, const double* const* d
   {
     std::vector<MrNearRed> red;
     for (const MrPlan* p : b.plans) red.insert(red.end(), p->h_near_red.begin(), p->h_near_red.end());
+    for (MrNearRed& r : red) r.pw = (nrhs + 1) & ~1;
     b.n_near_red = red.size();
     upload_vec(b.near_red, red, s);
   }

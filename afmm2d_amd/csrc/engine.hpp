@@ -210,12 +210,13 @@ struct MrNearTask {
 
 // The near-field partials of one leaf chunk added to its potentials, slot
 // after slot: out[(row0 + r) RP + c] += sum_s part[base + s stride + r RP + c]
-// for r < nrows.
+// for r < nrows and c < pw.
 struct MrNearRed {
   double* out;
   const double* part;
   int64_t row0, base, stride;
   int nrows, nslots;
+  int pw, pad;  // columns used in each row (even; the rest never written)
 };
 
 // Per-problem data of the near field (coordinates, charges, output).
@@ -226,6 +227,7 @@ struct MrNearProb {
   double* out;    // near-field potentials, rows of 16 NCT doubles
   double* part;   // pair sources' transposed products (MrNearTask::pt)
   KParams kp;
+  int pw, pad;    // partial columns written (the right-hand sides, rounded up to even)
 };
 
 // Gather (caller order -> KD order) or final sum and scatter of up to
