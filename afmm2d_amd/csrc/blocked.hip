@@ -784,6 +784,204 @@ __global__ __launch_bounds__(256) void k_mr_p2p(const MrNearTask* __restrict__ t
   }
 }
 
+// Near field with the sources staged in LDS: a workgroup takes one task
+// (up to 64 target rows of a leaf, one 16-row tile per wave) and walks its
+// source leaves in stages of kNearStage points.  The whole workgroup loads a
+// stage's coordinates and charges (whole rows, coalesced) into registers
+// while the previous stage is evaluated, then into the other of two LDS
+// buffers; every wave evaluates its 16 targets against the stage (the
+// kernel on the fly into the MFMA A operand, the charges from LDS as B), so
+// the sources are read from memory once per task instead of once per wave.
+// Sources past a leaf's count read as the leaf's last point with charge 0.
+// Tasks are taken largest first (the list is ordered by row tiles).
+constexpr int kNearStage = 64;
+
+template <int P, int NCT, int KID>
+__global__ __launch_bounds__(256) void k_mr_p2p_lds(const MrNearTask* __restrict__ tasks,
+                                                    int ntasks,
+                                                    const MrNearProb* __restrict__ probs,
+                                                    const double2* __restrict__ logtab,
+                                                    double ln2_hi, double ln2_lo,
+                                                    int* __restrict__ ctr) {
+  using T = MrT<P>;
+  using VT = typename T::V;
+  using Acc = typename T::Acc;
+  constexpr int RP = 16 * NCT;
+  constexpr int KS = P == PREC_BF16 ? 8 : 4;  // sources per lane group and product
+  constexpr int KC = 4 * KS;                   // sources per product
+  constexpr int SB = kNearStage;
+  constexpr int NLD = (SB * RP + 255) / 256;  // charges per thread and stage
+  __shared__ double2 tab[128];
+  __shared__ double2 sxy[2][SB];
+  __shared__ VT sq[2][SB * RP];
+  __shared__ int s_task;
+  if constexpr (P == PREC_F64 && KID == K_LOG) {
+    for (int u = threadIdx.x; u < 128; u += 256) tab[u] = logtab[u];
+  }
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, i = l & 15, g = l >> 4;
+  for (;;) {
+    __syncthreads();  // the previous task's LDS reads are done
+    if (tid == 0) s_task = atomicAdd(ctr, 1);
+    __syncthreads();
+    const int tq = s_task;
+    if (tq >= ntasks) break;  // the same for every thread
+    const MrNearTask& d = tasks[ntasks - 1 - tq];
+    const MrNearProb& pr = probs[d.prob];
+    const KParams kp = pr.kp;
+    const auto X = gptr(pr.x);
+    const auto Y = gptr(pr.y);
+    const auto Q = gptr(static_cast<const VT*>(pr.q));
+    const int M = d.M, nsrc = d.nsrc;
+    const bool act = 16 * w < M;  // the wave has target rows
+    const int tr = min(16 * w + i, M - 1);
+    const double xt = X[d.ta + tr], yt = Y[d.ta + tr];
+    const double ninv_p0 = -1.0 / kp.p0, inv_a = 1.0 / kp.a;
+    const float c0 = KID == K_LOG ? 0.34657359f
+                     : KID == K_GAUSS ? (float)(-1.4426950408889634 / kp.p0)
+                                      : (float)inv_a;
+    const float c1 = (float)kp.a;
+    const double scale = KID == K_LOG ? 0.5 : 1.0;
+    Acc acc[NCT], sh[NCT], sl[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      acc[ct] = Acc{};
+      sh[ct] = Acc{};
+      sl[ct] = Acc{};
+    }
+    double2 pxy = make_double2(0.0, 0.0);
+    VT pq[NLD];
+    auto fetch = [&](int src, int k0) {
+      const int st = __builtin_amdgcn_readfirstlane(d.ss[src]);
+      const int nv = min(SB, __builtin_amdgcn_readfirstlane(d.sc[src]) - k0);
+      if (tid < SB) {
+        const int k = st + k0 + min(tid, nv - 1);
+        pxy = make_double2(X[k], Y[k]);
+      }
+#pragma unroll
+      for (int u = 0; u < NLD; u++) {
+        const int e = tid + 256 * u, row = e / RP, col = e - row * RP;
+        pq[u] = (VT)0;
+        if (e < SB * RP && row < nv) pq[u] = Q[(int64_t)(st + k0 + row) * RP + col];
+      }
+    };
+    auto commit = [&](int b) {
+      if (tid < SB) sxy[b][tid] = pxy;
+#pragma unroll
+      for (int u = 0; u < NLD; u++) {
+        const int e = tid + 256 * u;
+        if (e < SB * RP) sq[b][e] = pq[u];
+      }
+    };
+    auto compute = [&](int b, int nv) {
+      if (!act) return;
+      const int nch = (nv + KC - 1) / KC;
+      for (int ch = 0; ch < nch; ch++) {
+        const int kb = ch * KC + KS * g;  // this lane group's first source of the product
+        if constexpr (P == PREC_BF16) {
+          float kv[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const double2 sp = sxy[b][kb + u];
+            kv[u] = near_kernel_f32<KID>(xt, yt, sp.x, sp.y, kp, c0, c1);
+          }
+          const mr_bf8 a = bf16_pack(kv);
+#pragma unroll
+          for (int ct = 0; ct < NCT; ct++) {
+            float x[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) x[u] = sq[b][(kb + u) * RP + 16 * ct + i];
+            mr_bf8 bh, bl;
+            bf16_split(x, bh, bl);
+            acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh, acc[ct], 0, 0, 0);
+            acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl, acc[ct], 0, 0, 0);
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const double2 sp = sxy[b][kb + u];
+            typename T::A kv;
+            if constexpr (P == PREC_F64) {
+              if constexpr (KID >= K_USER) {
+                kv = kernel_xy<KID>(xt, yt, sp.x, sp.y, kp);
+              } else {
+                const double dx = xt - sp.x, dy = yt - sp.y;
+                kv = scale * near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab,
+                                              ln2_hi, ln2_lo);
+              }
+            } else {
+              kv = near_kernel_f32<KID>(xt, yt, sp.x, sp.y, kp, c0, c1);
+            }
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) {
+              const VT bq = sq[b][(kb + u) * RP + 16 * ct + i];
+              if constexpr (P == PREC_F64)
+                acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv, bq, acc[ct], 0, 0, 0);
+              else
+                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(kv, bq, acc[ct], 0, 0, 0);
+            }
+          }
+        }
+      }
+    };
+    // end of a source leaf (FP32/BF16): its partial sum added to the
+    // compensated total, the partial restarted
+    auto close_source = [&]() {
+      if constexpr (P != PREC_F64) {
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            float s_ = sh[ct][v], c_ = sl[ct][v];
+            two_sum(s_, c_, acc[ct][v]);
+            sh[ct][v] = s_;
+            sl[ct][v] = c_;
+            acc[ct][v] = 0.0f;
+          }
+      }
+    };
+    int src = 0, k0 = 0, b = 0;
+    fetch(0, 0);
+    commit(0);
+    __syncthreads();
+    for (;;) {
+      const int cnt = __builtin_amdgcn_readfirstlane(d.sc[src]);
+      const int nv = min(SB, cnt - k0);
+      int ns = src, nk = k0 + SB;
+      if (nk >= cnt) {
+        ns = src + 1;
+        nk = 0;
+      }
+      const bool more = ns < nsrc;
+      if (more) fetch(ns, nk);  // in flight while this stage is evaluated
+      compute(b, nv);
+      if (!more || ns != src) close_source();
+      if (more) commit(b ^ 1);
+      __syncthreads();
+      if (!more) break;
+      src = ns;
+      k0 = nk;
+      b ^= 1;
+    }
+    if (act) {
+      const auto out = gptr_w(pr.out);
+#pragma unroll
+      for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          int row, col;
+          acc_pos<P>(i, g, v, row, col);
+          const int r = 16 * w + row;
+          if (r < M) {
+            double val;
+            if constexpr (P == PREC_F64) val = acc[ct][v];
+            else val = (double)sh[ct][v] + (double)sl[ct][v];
+            out[(int64_t)(d.out_row + r) * RP + 16 * ct + col] = val;
+          }
+        }
+    }
+  }
+}
+
 // The pair partials of the near field added to the potentials, one
 // workgroup per leaf chunk, slots in order (deterministic).  The first pw
 // columns of each row (pw even, rows and slots whole multiples of 16
@@ -1755,6 +1953,11 @@ struct MrKnobs {
   int rows_m2l = 64, rows = 16, xcd = 0, near_waves = 1, depth = 3, near_rows = 64;
   bool near_pairs = true;
   bool near_after_up = false;  // fork the near field after the upward pass
+  // k_mr_p2p_lds instead of k_mr_p2p: 1 always, 0 never, -1 (default) for
+  // FP64 with 32 columns only (N = 2^20: 3.49 against 3.90 ms per pass; for
+  // 16 columns and FP32/BF16 equal or slower, the faster near field taking
+  // as much from the tree passes beside it as it gains)
+  int near_lds = -1;
   MrKnobs() {
     auto rd = [](const char* n, int d) {
       const char* v = std::getenv(n);
@@ -1770,11 +1973,18 @@ struct MrKnobs {
     near_rows = std::max(16, std::min(64, rd("AFMM2D_MR_NEAR_ROWS", 64))) / 16 * 16;
     near_pairs = rd("AFMM2D_MR_NEAR_PAIRS", 1) != 0;
     near_after_up = rd("AFMM2D_MR_NEAR_AFTER_UP", 0) != 0;
+    near_lds = rd("AFMM2D_MR_NEAR_LDS", -1);
   }
 };
 const MrKnobs& knobs() {
   static MrKnobs k;
   return k;
+}
+
+// Whether the near field of a plan runs k_mr_p2p_lds (see near_lds).
+bool near_uses_lds(int prec, int nct) {
+  const int v = knobs().near_lds;
+  return v < 0 ? prec == PREC_F64 && nct == 2 : v != 0;
 }
 
 }  // namespace
@@ -1962,7 +2172,7 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
   std::vector<MrNearTask> near;
   const int nleaf = 1 << (2 * L);
   const int nr = knobs().near_rows;
-  const bool pairs = p.prec == PREC_F64 && knobs().near_pairs;
+  const bool pairs = p.prec == PREC_F64 && knobs().near_pairs && !near_uses_lds(p.prec, nct);
   auto nbr_of = [&](int k, int q) { return e.nbr[L][8 * (size_t)k + q]; };
   auto blocks_of = [&](int k) { return (e.leaf_count[k] + nr - 1) / nr; };
   std::unordered_map<int64_t, int64_t> slot_at;  // (leaf, upper slot, row block) -> offset
@@ -2117,6 +2327,19 @@ void launch_near_t(const NearLaunch& nl, hipStream_t s) {
   int* ctr = nl.ctr;
   k_mr_zero<<<1, 4 * kMrCtrStride, 0, s>>>(ctr, 4 * kMrCtrStride);
   AFMM_LAUNCH_CHECK();
+  if (near_uses_lds(P, NCT)) {
+    // one persistent launch over every task: near_waves (at least 2)
+    // workgroups of 4 waves per CU
+    if (tl.n <= 0) return;
+    const MrNearTask* t = tl.buf.as<MrNearTask>();
+    const unsigned grid = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>(tl.n, (int64_t)nl.num_cus * std::max(2, nl.near_waves)));
+    AFMM_DISPATCH_KERNEL(nl.kid, KID,
+                         k_mr_p2p_lds<P, NCT, KID><<<grid, 256, 0, s>>>(
+                             t, tl.n, nl.probs, nl.logtab, nl.ln2_hi, nl.ln2_lo, ctr));
+    AFMM_LAUNCH_CHECK();
+    return;
+  }
   for (int c = 0; c < 4; c++) {
     const int n = tl.cls[c + 1] - tl.cls[c];
     if (n <= 0) continue;
