@@ -867,8 +867,12 @@ void assemble_tree(Engine& e) {
     // buffer reused by every batch and level: a level's whole scratch at once
     // (24-55 GB at N = 1e6) made assembly take 4-5 s instead of ~1 s in some
     // runs; 4 GiB batches left too few boxes per launch at the leaf level.
-    const size_t budget = std::min<size_t>((size_t)16 << 30,
-                                           std::max<size_t>((size_t)1 << 30, free_device_bytes() / 4));
+    // AFMM2D_ACA_BUDGET_MB (at least 16) sets it instead: a process that
+    // leaves the engine little device memory
+    size_t budget = std::min<size_t>((size_t)16 << 30,
+                                     std::max<size_t>((size_t)1 << 30, free_device_bytes() / 4));
+    if (const char* v = std::getenv("AFMM2D_ACA_BUDGET_MB"))
+      budget = (size_t)std::max(16L, std::atol(v)) << 20;
     // boxes: the order to run; capped: hold cap[] (else full capacity)
     auto run_aca = [&](const std::vector<int>& boxes_in, bool capped) {
       size_t b0 = 0;

@@ -101,6 +101,9 @@ def parse():
                         "N = 2^23 (log kernel, ACA tol 1e-8) with and without it and time "
                         "16-RHS matvecs of both (one JSON line)")
     p.add_argument("--train-trees", type=int, default=200)
+    p.add_argument("--aca-budget-mb", default="1024,256",
+                   help="with --rank-model: also assemble both arms with these ACA scratch "
+                        "budgets (MiB, comma-separated; empty: none)")
     p.add_argument("--assembly-repeat", type=int, default=0,
                    help="instead of the headline, create/assemble/matvec/destroy an N = 2^20 "
                         "handle (log kernel, tol 1e-12) this many times, with an N = 2^22 "
@@ -317,6 +320,33 @@ def rank_model_ab(args, H, E):
         A.close()
         del A
     same = all(np.array_equal(a, b) for a, b in zip(ranks["plain"], ranks["learned"]))
+    # the same assemblies with a smaller ACA scratch budget (a process that
+    # leaves the engine little device memory): capacities sized by the model
+    # let more boxes share each launch
+    budgets = {}
+    for mb in [int(v) for v in args.aca_budget_mb.split(",") if v]:
+        os.environ["AFMM2D_ACA_BUDGET_MB"] = str(mb)
+        row = {}
+        for arm in ("plain", "learned"):
+            print("bench: N=%d assembly (%s), ACA budget %d MiB" % (n, arm, mb), file=sys.stderr,
+                  flush=True)
+            A = E.AFMM(n, 64, 8, xy)
+            if arm == "learned":
+                A.set_rank_model(m, margin)
+            t0 = time.perf_counter()
+            A.assemble()
+            t_asm = time.perf_counter() - t0
+            st = A.stats()
+            same = same and all(np.array_equal(A.ranks(j), ranks["plain"][j - 2])
+                                for j in range(2, A.nLevels + 1))
+            row[arm] = {"assemble_s": round(t_asm, 3), "aca_s": round(st["aca_seconds"], 3),
+                        "aca_launches": st["aca_launches"],
+                        "aca_scratch_peak_bytes": st["aca_scratch_peak"],
+                        "fallbacks": st["rank_fallbacks"]}
+            A.close()
+            del A
+        budgets["%d_MiB" % mb] = row
+    os.environ.pop("AFMM2D_ACA_BUDGET_MB", None)
     dB.free()
     dY.free()
     m.close()
@@ -330,6 +360,7 @@ def rank_model_ab(args, H, E):
                    "collect_s": round(t_collect, 2), "train_s": round(t_train, 2),
                    "margin_log2": margin, "ranks_identical": bool(same)},
         "plain": out["plain"], "learned": out["learned"],
+        "aca_budget_ab": budgets,
     }))
 
 

@@ -172,3 +172,35 @@ def test_untrained_model_falls_back(engine):
     A.close()
     B.close()
     m.close()
+
+
+def test_small_aca_budget(engine, trained, monkeypatch):
+    """A 16 MiB ACA scratch budget (AFMM2D_ACA_BUDGET_MB) splits every level
+    into many launches; plain and learned capacities still give the default
+    assembly's ranks and matvec bit for bit, and the learned capacities need
+    fewer launches."""
+    m, _, _ = trained
+    n = 1 << 16
+    xy, q = uniform_points(n, 79)
+    A = engine.AFMM(n, 64, 8, xy)
+    A.assemble()
+    ya = A.computeMatVecProduct(q)
+    launches = {}
+    monkeypatch.setenv("AFMM2D_ACA_BUDGET_MB", "16")
+    for arm in ("plain", "learned"):
+        B = engine.AFMM(n, 64, 8, xy)
+        if arm == "learned":
+            B.set_rank_model(m, 0.5)
+        B.assemble()
+        st = B.stats()
+        launches[arm] = st["aca_launches"]
+        assert st["aca_scratch_peak"] <= 16 << 20
+        for j in range(2, A.nLevels + 1):
+            np.testing.assert_array_equal(A.ranks(j), B.ranks(j))
+        np.testing.assert_array_equal(ya, B.computeMatVecProduct(q))
+        B.close()
+    print_case("16 MiB ACA budget: launches plain %d, learned %d (default budget %d)"
+               % (launches["plain"], launches["learned"], A.stats()["aca_launches"]))
+    assert launches["learned"] < launches["plain"]
+    assert launches["plain"] > A.stats()["aca_launches"]
+    A.close()
