@@ -288,10 +288,17 @@ __device__ __forceinline__ MrMicro micro_unpack(const mr_u4 v) {
   return m;
 }
 
+// IM 5 and 6: the FP64 blocks in 48-bit block floating point, both
+// orientations stored (every read forward): descriptors as IM 3 and 4, the
+// micro column's first tile in the low 48 bits of a and the pair's exponent
+// in its top 16; a lane reads its micro row as the four entries' low words
+// (16 bytes) and top 16 bits (8 bytes) and turns each into the exact double
+// (2^52 + u) 2^s - (2^52 + 2^47) 2^s = m 2^s with one FMA.
 template <int P, int NCT, int NRT, int IM>
 __device__ __forceinline__ void gemm_load_micro(GemmFrag<P, NCT>& f, const MrTask& d, int k0,
                                                 int i, int g, const MrMicro* mic) {
   static_assert(P != PREC_BF16, "the pair-stored M2L is FP64 or FP32");
+  static_assert(IM < 5 || P == PREC_F64, "48-bit pair storage is FP64");
   using T = MrT<P>;
   using AT = typename T::A;
   using VT = typename T::V;
@@ -300,7 +307,7 @@ __device__ __forceinline__ void gemm_load_micro(GemmFrag<P, NCT>& f, const MrTas
   const auto B = gptr(static_cast<const VT*>(d.B));
   const int mi = k0 / 4 + g;
   MrMicro u;
-  if constexpr (IM == 3) {
+  if constexpr (IM == 3 || IM == 5) {
     u = micro_unpack(((const lds_u4*)mic)[mi]);  // one 16-byte LDS read
   } else {
     const int nm = d.K / 4;
@@ -308,6 +315,33 @@ __device__ __forceinline__ void gemm_load_micro(GemmFrag<P, NCT>& f, const MrTas
     if (mi >= nm) u.info &= ~(7 << 25);
   }
   const int astr = u.info & 0xFFFF, amax = (u.info >> 16) & 0x1FF, nval = (u.info >> 25) & 7;
+  if constexpr (IM >= 5) {
+    const uint64_t ua = (uint64_t)u.a;
+    const int64_t t0 = (int64_t)(ua & 0xFFFFFFFFFFFFull);
+    const double sc = ldexp(1.0, (int)(int16_t)(ua >> 48)), nbias = -kB48Bias * sc;
+    const char* base = static_cast<const char*>(d.A);
+#pragma unroll
+    for (int rt = 0; rt < NRT; rt++) {
+      const int ai = min(4 * (d.rt0 + rt) + (i >> 2), amax);
+      const char* tp = base + (t0 + (int64_t)ai * astr) * 96;
+      const mr_u4 lo = *gptr(reinterpret_cast<const mr_u4*>(tp + (i & 3) * 16));
+      const auto hp = gptr(reinterpret_cast<const uint32_t*>(tp + 64 + (i & 3) * 8));
+      const uint32_t h0 = hp[0], h1 = hp[1];
+      f.a[rt][0] = fma(__hiloint2double((int)(kB48ExpWord | (h0 & 0xFFFFu)), (int)lo.x), sc, nbias);
+      f.a[rt][1] = fma(__hiloint2double((int)(kB48ExpWord | (h0 >> 16)), (int)lo.y), sc, nbias);
+      f.a[rt][2] = fma(__hiloint2double((int)(kB48ExpWord | (h1 & 0xFFFFu)), (int)lo.z), sc, nbias);
+      f.a[rt][3] = fma(__hiloint2double((int)(kB48ExpWord | (h1 >> 16)), (int)lo.w), sc, nbias);
+    }
+    f.neg = 0;  // the transposed copies carry their sign
+    f.bmask = (1u << nval) - 1u;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int64_t row = u.brow + (s < nval ? s : 0);
+#pragma unroll
+      for (int ct = 0; ct < NCT; ct++) f.b[s][ct] = B[row * RP + 16 * ct + i];
+    }
+    return;
+  }
   const bool fwd = (u.info >> 28) & 1;
   // 4 loads per row tile in either orientation (forward: a micro row's 4
   // contiguous values; transposed: a micro column's, 4 apart): the same
@@ -410,7 +444,7 @@ __global__ __launch_bounds__(256) void k_mr_gemm(const MrTask* __restrict__ task
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  if constexpr (IM == 3) {
+  if constexpr (IM == 3 || IM == 5) {
     // the task's descriptors, padded with a copy of the last one marked
     // empty up to whole chunks (4 per chunk)
     const int nm = K / 4, nmp = (K + KC - 1) / KC * 4;
@@ -441,8 +475,8 @@ __global__ __launch_bounds__(256) void k_mr_gemm(const MrTask* __restrict__ task
     // chunk's loads, not the ones just issued for later chunks
     auto load = [&](GemmFrag<P, NCT>& fr, int ch) {
       const int k0 = min(ch, nch - 1) * KC;
-      if constexpr (IM == 3) gemm_load_micro<P, NCT, NRT, 3>(fr, d, k0, i, g, smic);
-      else if constexpr (IM == 4) gemm_load_micro<P, NCT, NRT, 4>(fr, d, k0, i, g, gmic);
+      if constexpr (IM == 3 || IM == 5) gemm_load_micro<P, NCT, NRT, IM>(fr, d, k0, i, g, smic);
+      else if constexpr (IM == 4 || IM == 6) gemm_load_micro<P, NCT, NRT, IM>(fr, d, k0, i, g, gmic);
       else gemm_load<P, NCT, NRT, IM>(fr, d, k0, i, g, sidx);
     };
 #pragma unroll
@@ -1411,6 +1445,49 @@ __global__ __launch_bounds__(256) void k_mr_build_pairs(const MrPairB* __restric
   }
 }
 
+// 48-bit block floating point (b48_split) of the pair-stored FP64 blocks for
+// the blocked passes: per pair its block F (the first box's rows) and F's
+// transpose times the kernel's transpose sign (the second box's rows), each
+// as micro tiles in the FP64 layout's order at tile offsets 2 poff / 16 and
+// after F; a tile holds its 16 entries' low 32 bits, then their top 16 bits
+// (96 bytes).  One exponent per pair, from the block's largest magnitude.
+template <int KID>
+__global__ __launch_bounds__(256) void k_mr_b48_pairs(const MrPairB* __restrict__ pairs,
+                                                      const double* __restrict__ M64,
+                                                      char* __restrict__ out,
+                                                      int* __restrict__ sexp) {
+  __shared__ double wmax[4];
+  const MrPairB d = pairs[blockIdx.x];
+  const int R1 = (d.r1 + 3) & ~3, R2 = (d.r2 + 3) & ~3, nb = R2 / 4, nbt = R1 / 4;
+  const int64_t tot = (int64_t)R1 * R2;
+  const double* F = M64 + d.poff;
+  double m = 0.0;
+  for (int64_t e = threadIdx.x; e < tot; e += 256) m = fmax(m, fabs(F[e]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  m = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+  const int sx = b48_exponent(m);
+  char* Fo = out + (d.poff / 16) * 2 * 96;
+  char* To = Fo + (tot / 16) * 96;
+  for (int64_t e = threadIdx.x; e < tot; e += 256) {
+    const int t = (int)(e >> 4), w = (int)(e & 15);
+    const int a = t / nb, bb = t - a * nb;
+    uint32_t lo;
+    uint16_t hi;
+    b48_split(F[e], sx, lo, hi);
+    reinterpret_cast<uint32_t*>(Fo + (int64_t)t * 96)[w] = lo;
+    reinterpret_cast<uint16_t*>(Fo + (int64_t)t * 96 + 64)[w] = hi;
+    // entry (4 a + w / 4, 4 bb + w % 4) of F is entry (4 bb + w % 4, 4 a + w / 4) of T
+    const int tt = bb * nbt + a, wt = (w & 3) * 4 + (w >> 2);
+    b48_split(kTransposeSign<KID> * F[e], sx, lo, hi);
+    reinterpret_cast<uint32_t*>(To + (int64_t)tt * 96)[wt] = lo;
+    reinterpret_cast<uint16_t*>(To + (int64_t)tt * 96 + 64)[wt] = hi;
+  }
+  if (threadIdx.x == 0) sexp[blockIdx.x] = sx;
+}
+
 // The orthonormal basis' block R_first P R_second^T (FP64 arithmetic) of a
 // pair, written in precision P at the block's place in the same layout.
 template <int P>
@@ -1773,6 +1850,84 @@ void build_f64(Engine& e) {
   f.built = true;
 }
 
+// The FP64 blocks in 48-bit block floating point for the blocked passes
+// (Engine::mr_b48): both orientations of every pair (k_mr_b48_pairs) and a
+// descriptor per micro column with the same box order, slot order and
+// micro columns as mr_micro, every one forward.
+void build_b48(Engine& e) {
+  if (e.mr_b48) return;
+  build_f64(e);
+  hipStream_t s = e.s_aux;
+  std::vector<MrPairB> pr = pair_records(e);
+  e.mr_m48.alloc(96 * (size_t)std::max<int64_t>(2 * (e.mr_pm_total / 16), 1));
+  std::vector<int> sexp(pr.size(), 0);
+  if (!pr.empty()) {
+    DevBuf dpr, dsx;
+    upload_vec(dpr, pr, s);
+    dsx.alloc(sizeof(int) * pr.size());
+    AFMM_DISPATCH_KERNEL(e.kid, KID,
+                         k_mr_b48_pairs<KID><<<(unsigned)pr.size(), 256, 0, s>>>(
+                             dpr.as<MrPairB>(), e.mrf[PREC_F64].M.as<double>(),
+                             e.mr_m48.as<char>(), dsx.as<int>()));
+    AFMM_LAUNCH_CHECK();
+    AFMM_HIP(hipMemcpyAsync(sexp.data(), dsx.p, sizeof(int) * pr.size(), hipMemcpyDeviceToHost, s));
+    AFMM_HIP(hipStreamSynchronize(s));
+  }
+  std::unordered_map<int64_t, int> pair_of;  // (first gid, second gid) -> pair
+  for (size_t u = 0; u < e.mr_pairs.size(); u++)
+    pair_of[((int64_t)e.mr_pairs[u].g1 << 32) | e.mr_pairs[u].g2] = (int)u;
+  const int L = e.L;
+  std::vector<MrMicro> mic;
+  auto tagged = [&](int64_t tile, int pair) {
+    return (int64_t)((uint64_t)tile | ((uint64_t)(uint16_t)(int16_t)sexp[pair] << 48));
+  };
+  for (int j = 2; j <= L; j++) {
+    const LevelInfo& li = e.lv[j];
+    for (int k = 0; k < li.nb; k++) {
+      const int r = li.rank[k];
+      if (r <= 0) continue;
+      const int Rk = (r + 3) & ~3;
+      for (int q = 0; q < 40; q++) {
+        const int b = slot_box(e, j, k, q);
+        if (b < 0) continue;
+        const int rb = li.rank[b];
+        if (rb <= 0) continue;
+        const int Rb = (rb + 3) & ~3;
+        // the pair's first box is the lower one; its FP64 block offset
+        const int lo = std::min(k, b), hi = std::max(k, b);
+        const int pi = pair_of.at(((li.gbase + lo) << 32) | (li.gbase + hi));
+        const int64_t f_off = e.mr_pairs[pi].poff;
+        const int64_t t0 = 2 * (f_off / 16);
+        const int Rlo = lo == k ? Rk : Rb, Rhi = lo == k ? Rb : Rk;
+        if (k == lo) {
+          // F: k's rows, b's columns; micro column bb is tile t0 + bb
+          for (int bb = 0; bb < Rb / 4; bb++) {
+            MrMicro m;
+            m.a = tagged(t0 + bb, pi);
+            m.brow = (int)(li.soff[b] + 4 * bb);
+            const int nval = std::min(4, rb - 4 * bb);
+            m.info = (Rb / 4) | ((Rk / 4 - 1) << 16) | (nval << 25) | (1 << 28);
+            mic.push_back(m);
+          }
+        } else {
+          // T: k's rows, b's columns (b first); after F's Rlo Rhi / 16 tiles
+          const int64_t tt0 = t0 + (int64_t)(Rlo / 4) * (Rhi / 4);
+          for (int aa = 0; aa < Rb / 4; aa++) {
+            MrMicro m;
+            m.a = tagged(tt0 + aa, pi);
+            m.brow = (int)(li.soff[b] + 4 * aa);
+            const int nval = std::min(4, rb - 4 * aa);
+            m.info = (Rb / 4) | ((Rk / 4 - 1) << 16) | (nval << 25) | (1 << 28);
+            mic.push_back(m);
+          }
+        }
+      }
+    }
+  }
+  upload_vec(e.mr_micro48, mic, s);
+  e.mr_b48 = true;
+}
+
 // FP64 box-major M2L row blocks (the FP32 / BF16 transforms' source) into M64.
 void build_boxmajor64(Engine& e, DevBuf& M64) {
   mr_layout_boxmajor(e);
@@ -2062,8 +2217,10 @@ void m2l_tasks(const Engine& e, const MrPlan& p, int rows, std::vector<MrTask>& 
   const int kg = col_group(p.prec);
   if (pair_m2l(p.prec)) {
     // pair-stored blocks through the box's micro descriptors: K = 4 per
-    // descriptor
-    const MrMicro* mic = e.mr_micro.as<MrMicro>();
+    // descriptor (48-bit storage: both orientations, mode 5)
+    const bool b48 = e.mr_b48 && e.m2l_store == M2L_STORE_B48;
+    const MrMicro* mic = (b48 ? e.mr_micro48 : e.mr_micro).as<MrMicro>();
+    const void* Ap = b48 ? e.mr_m48.p : f.M.p;
     for (int j = 2; j <= e.L; j++) {
       const LevelInfo& li = e.lv[j];
       for (int k = 0; k < li.nb; k++) {
@@ -2072,9 +2229,9 @@ void m2l_tasks(const Engine& e, const MrPlan& p, int rows, std::vector<MrTask>& 
         if (r <= 0) continue;
         const int64_t u0 = e.mr_uoff[g], nu = e.mr_uoff[g + 1] - u0;
         const size_t first = out.size();
-        add_row_groups(out, f.M.p, kg, p.outv.p, reinterpret_cast<const int*>(mic), u0, p.inv.p,
+        add_row_groups(out, Ap, kg, p.outv.p, reinterpret_cast<const int*>(mic), u0, p.inv.p,
                        li.soff[k], r, (int)(4 * nu), 0, rows);
-        for (size_t t = first; t < out.size(); t++) out[t].imode = 3;
+        for (size_t t = first; t < out.size(); t++) out[t].imode = b48 ? 5 : 3;
       }
     }
     return;
@@ -2095,8 +2252,12 @@ void m2l_tasks(const Engine& e, const MrPlan& p, int rows, std::vector<MrTask>& 
 MrPlan& get_plan(Engine& e, int prec, int nct) {
   for (auto& p : e.mr_plans)
     if (p->prec == prec && p->nct == nct) return *p;
-  if (prec == PREC_F64) build_f64(e);
-  else build_reduced(e, prec);
+  if (prec == PREC_F64) {
+    if (e.m2l_store == M2L_STORE_B48) build_b48(e);
+    else build_f64(e);
+  } else {
+    build_reduced(e, prec);
+  }
   auto pl = std::make_unique<MrPlan>();
   MrPlan& p = *pl;
   p.prec = prec;
@@ -2283,6 +2444,17 @@ void launch_gemm(const MrTaskList& tl, int xcd, int depth, hipStream_t s) {
         else if (depth == 4) launch_gemm_d<P, NCT, 3, 4>(t, n, c, xcd, kmm, shm, s);
         else launch_gemm_d<P, NCT, 3, 3>(t, n, c, xcd, kmm, shm, s);
       }
+    } else if (tl.mode == 5) {
+      if constexpr (P != PREC_F64) {
+        throw Error(6, "48-bit pair-stored M2L tasks outside FP64");
+      } else {
+        const int kmm = std::max(4, (tl.kmax[c] + 15) / 16 * 4);
+        const size_t shm = sizeof(MrMicro) * 4 * (size_t)kmm;
+        if (shm > kMrIdxLds) launch_gemm_d<P, NCT, 6, 3>(t, n, c, xcd, 0, 0, s);
+        else if (depth >= 6) launch_gemm_d<P, NCT, 5, 6>(t, n, c, xcd, kmm, shm, s);
+        else if (depth == 4) launch_gemm_d<P, NCT, 5, 4>(t, n, c, xcd, kmm, shm, s);
+        else launch_gemm_d<P, NCT, 5, 3>(t, n, c, xcd, kmm, shm, s);
+      }
     } else {
       const int kmax = std::max(1, tl.kmax[c]);
       const size_t shm = sizeof(int) * 4 * (size_t)kmax;
@@ -2441,8 +2613,12 @@ int list_launches(const MrTaskList& tl) {
 }  // namespace
 
 void mr_prepare(Engine& e, int prec) {
-  if (prec == PREC_F64) build_f64(e);
-  else build_reduced(e, prec);
+  if (prec == PREC_F64) {
+    if (e.m2l_store == M2L_STORE_B48) build_b48(e);
+    else build_f64(e);
+  } else {
+    build_reduced(e, prec);
+  }
 }
 
 void mr_matvec(Engine& e, const double* d_b, int64_t ldb, double* d_y, int64_t ldy, int nrhs,

@@ -487,6 +487,12 @@ struct Engine {
   std::vector<PairBlk> mr_pairs;
   int64_t mr_pm_total = 0;  // pair-layout elements (zero padding included)
   DevBuf mr_micro;
+  // the same blocks for the FP64 blocked passes when m2l_store is
+  // M2L_STORE_B48: every pair's block and its transpose in 48-bit block
+  // floating point (96-byte micro tiles, one exponent per pair), read
+  // forward through their own descriptors (tile offset | exponent << 48)
+  bool mr_b48 = false;
+  DevBuf mr_m48, mr_micro48;
   MrFactors mrf[kPrecisions];
   std::vector<std::unique_ptr<MrPlan>> mr_plans;
   // orthonormal basis (FP32 / BF16): R of every box, rank x rank row-major

@@ -546,7 +546,9 @@ def run_blocked(A, Q, steps, rows, ref_rows, H, cols=(16, 32), only=None):
             # 8 bytes each; the blocked passes keep their own FP64 factors)
             bpe = 6 if st["m2l_storage"] == 1 else 8
             coeffs = st["m2l_bytes"] // bpe if prec == "fp64" else st["blocked_m2l_coeffs_bf16"]
-            alg = coeffs * fb + 2 * st["skeleton_total"] * rp * vb
+            # FP64 with 48-bit storage: both orientations stored, each read once
+            fbe = 12 if (prec == "fp64" and bpe == 6) else fb
+            alg = coeffs * fbe + 2 * st["skeleton_total"] * rp * vb
             ach = alg / (m2l_ms * 1e-3) / 1e9 if m2l_ms > 0 else 0.0
             out["%s_r%d" % (prec, R)] = {
                 "value": round(n * R / (ms * 1e-3), 1), "ms_per_pass": round(ms, 4),

@@ -144,3 +144,49 @@ def test_storage_is_set_before_assembly(engine):
         A.set_m2l_storage("bfp48")
     assert ex.value.status == engine.ERR_STATE
     A.close()
+
+
+@pytest.mark.parametrize("n,leaf,tol_pow,kernel,nrhs", [
+    (20000, 64, 12, "log", 16), (20000, 64, 12, "log", 32), (8192, 64, 10, "inv_r", 5),
+    # ranks above 64 (long interaction-list stacks: descriptors past the LDS budget)
+    (10000, 100, 12, "reg_inv_r", 16), (5000, 100, 14, "reg_inv_r", 3),
+    (8192, 64, 10, "gaussian", 8)])
+def test_blocked_passes_use_the_same_blocks(engine, oracle_mod, n, leaf, tol_pow, kernel, nrhs):
+    """On a handle with 48-bit M2L storage the blocked FP64 passes read every
+    pair's block and its transpose in the same 48-bit form: the same stored
+    values as the single-RHS passes, so the two agree like the FP64 forms do
+    (1e-12), and both stay within 1e-11 of the oracle."""
+    print("bfp48 blocked", n, leaf, tol_pow, kernel, nrhs, flush=True)
+    xy, _ = uniform_points(n, 31 + n)
+    kp = [0.01, 0, 0, 0] if kernel == "gaussian" else None
+    A = engine.AFMM(n, leaf, tol_pow, xy, kernel=kernel, kparams=kp)
+    A.set_m2l_storage("bfp48")
+    A.assemble()
+    Q = np.random.default_rng(n).uniform(-1, 1, size=(n, nrhs))
+    Yb = A.matvec(Q, "fp64", blocked=True)
+    Ys = A.matvec(Q, "fp64")
+    d = max(rel(Yb[:, c], Ys[:, c]) for c in range(nrhs))
+    T = oracle_mod.Tree(xy, leaf, tol_pow, oracle_kernel(kernel), kparams=kp)
+    T.assemble()
+    do = max(rel(Yb[:, c], T.matvec(Q[:, c])) for c in range(min(nrhs, 2)))
+    print("  blocked vs single-RHS %.2e, vs oracle %.2e" % (d, do), flush=True)
+    assert d <= 1e-12
+    assert do <= 1e-11
+    T.close()
+    A.close()
+
+
+def test_blocked_nonsymmetric_user_kernel(engine):
+    kp = [0.0, 0.25, 0.0, 0.0]
+    n = 12000
+    xy, _ = uniform_points(n, 96)
+    A = engine.AFMM(n, 64, 10, xy, kernel="user", kparams=kp)
+    A.set_m2l_storage("bfp48")
+    A.assemble()
+    Q = np.random.default_rng(97).uniform(-1, 1, size=(n, 6))
+    Yb = A.matvec(Q, "fp64", blocked=True)
+    Ys = A.matvec(Q, "fp64")
+    d = max(rel(Yb[:, c], Ys[:, c]) for c in range(6))
+    print("bfp48 blocked, non-symmetric user kernel: vs single-RHS %.2e" % d, flush=True)
+    assert d <= 1e-12
+    A.close()
