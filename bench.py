@@ -135,6 +135,7 @@ def config4(args, H, E):
     print("bench: config 4 N=%d" % n, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     A = E.AFMM(n, 64, 8, xy, kernel="inv_r")
+    A.set_m2l_storage(args.m2l_storage)
     A.assemble()
     t_asm = time.perf_counter() - t0
     st = A.stats()
@@ -153,7 +154,8 @@ def config4(args, H, E):
         "config": {"workload": "N=%d, 16 clipped Gaussians (sigma 0.05), 1/r kernel, leaf 64 "
                                "(L=%d), ACA tol 1e-8, %d RHS" % (n, st["levels"], R),
                    "max_rank": st["max_rank"], "assemble_s": round(t_asm, 3),
-                   "m2l_bytes_fp64_once": st["m2l_bytes"], "rel_err_rows": int(len(rows))},
+                   "m2l_storage": args.m2l_storage, "m2l_bytes": st["m2l_bytes"],
+                   "rel_err_rows": int(len(rows))},
         "by_precision": res,
     }))
 
@@ -174,6 +176,7 @@ def batch_config(args, H, E):
         name, kp = kernels[int(rng.integers(0, 3))]
         xy = rng.uniform(-1.0, 1.0, size=(n, 2))
         probs.append(E.AFMM(n, 64, tol_pow, xy, kernel=name, kparams=kp))
+        probs[-1].set_m2l_storage(args.m2l_storage)
         names.append(name)
     E.batch_assemble(probs)
     t_asm = time.perf_counter() - t0
@@ -213,7 +216,10 @@ def batch_config(args, H, E):
     alg = 0
     for p in probs:
         st = p.stats()
-        alg += 8 * (st["m2l_bytes"] // 8 + 2 * st["blocked_basis_coeffs"]) + 16 * p.N * R
+        # (48-bit storage: both orientations of each block, 12 bytes per entry)
+        bpe = 6 if st["m2l_storage"] == 1 else 8
+        alg += ((st["m2l_bytes"] // bpe) * (12 if bpe == 6 else 8) +
+                16 * st["blocked_basis_coeffs"] + 16 * p.N * R)
     ach = alg / (ms_graph * 1e-3) / 1e9
     B.close()
     for a in dB + dY:
@@ -227,7 +233,8 @@ def batch_config(args, H, E):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "256 problems, N log-uniform in [2048, 65536] (total %d), "
                                "uniform [-1,1]^2, kernels log r / 1/r / exp(-r^2/0.01), leaf 64, "
-                               "ACA tol 1e-%d, %d RHS ~ N(0,1), FP64" % (total, tol_pow, R),
+                               "ACA tol 1e-%d, %d RHS ~ N(0,1), FP64, M2L blocks %s"
+                               % (total, tol_pow, R, args.m2l_storage),
                    "total_points": total, "assemble_s": round(t_asm, 2),
                    "tune_s": round(t_tune, 2), "tuned_ms": round(tune_ms, 4), "batch": info},
         "ms_per_run": {"graph_tuned": round(ms_graph, 4), "eager_tuned": round(ms_eager, 4),
