@@ -2844,6 +2844,9 @@ MrBatch* mr_batch_create(const std::vector<Engine*>& eng, const double* const* d
     const Engine& e = *eng[i];
     if (e.device != b.device) throw Error(1, "batch problems on different devices");
     if (ldb[i] < e.n || ldy[i] < e.n) throw Error(1, "leading dimension below N");
+    // FP64 batches read every problem's blocks through one M2L task list
+    if (prec == PREC_F64 && e.m2l_store != eng[0]->m2l_store)
+      throw Error(1, "FP64 batch of problems with different M2L storage");
     maxL = std::max(maxL, e.L);
   }
   AFMM_HIP(hipSetDevice(b.device));

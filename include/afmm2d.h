@@ -206,7 +206,8 @@ int afmm2d_batch_assemble(const afmm2d_handle* handles, int count);
 /* A batch over count assembled handles.  d_b[i], d_y[i]: device arrays of
  * problem i (caller order, column-major, leading dimensions ldb[i], ldy[i]
  * >= N_i, nrhs <= 32 columns), fixed for the batch's life.  precision: an
- * afmm2d_precision.  Parameters come from the autotuner's cache for this
+ * afmm2d_precision; an FP64 batch needs every handle on the same M2L
+ * storage (afmm2d_set_m2l_storage).  Parameters come from the autotuner's cache for this
  * GPU and shape class when it holds them (AFMM2D_TUNE_CACHE names a file
  * that keeps it across processes), else from the defaults. */
 int afmm2d_batch_create(const afmm2d_handle* handles, int count, const double* const* d_b,

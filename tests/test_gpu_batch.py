@@ -184,3 +184,48 @@ def test_batch_leading_dimensions_and_refusals(engine):
         d.free()
     for p in probs:
         p.close()
+
+
+def test_batch_on_48_bit_storage(engine):
+    """Problems whose M2L blocks are in 48-bit storage: the batch reads both
+    orientations of every block like each problem's own blocked matvec (bit
+    for bit) and the single-RHS passes on the same stored values (1e-12);
+    mixing storages in one batch is refused."""
+    from afmm2d_amd import hip
+    nrhs = 8
+    rng0 = np.random.default_rng(41)
+    probs = []
+    for i in range(12):
+        n = int(round(np.exp(rng0.uniform(np.log(2048), np.log(40000)))))
+        name, kp = KERNEL_SET[i % 3]
+        xy, _ = uniform_points(n, 4100 + i)
+        p = engine.AFMM(n, 64, 8, xy, kernel=name, kparams=kp)
+        p.set_m2l_storage("bfp48")
+        probs.append(p)
+    engine.batch_assemble(probs)
+    Q, dB, dY = device_io(hip, probs, rng0, nrhs)
+    B = engine.Batch(probs, [d.ptr for d in dB], [d.ptr for d in dY], nrhs, "fp64")
+    B.capture(True)
+    for _ in range(2):
+        B.matvec()
+        hip.synchronize()
+        Yb = download(dY, probs, nrhs)
+        for i, p in enumerate(probs):
+            np.testing.assert_array_equal(Yb[i], p.matvec(Q[i], "fp64", blocked=True))
+    for i in (0, 5, 11):
+        Y1 = probs[i].matvec(Q[i], "fp64")
+        assert max(rel(Yb[i][:, c], Y1[:, c]) for c in range(nrhs)) <= 1e-12
+    B.close()
+    # one problem with FP64 blocks among 48-bit ones
+    xy, _ = uniform_points(3000, 4200)
+    f = engine.AFMM(3000, 64, 8, xy)
+    f.assemble()
+    qf = np.asfortranarray(rng0.standard_normal((3000, nrhs)))
+    db, dy = hip.DeviceArray.from_numpy(qf), hip.DeviceArray(qf.nbytes)
+    with pytest.raises(engine.AFMM2DError):
+        engine.Batch(probs[:2] + [f], [dB[0].ptr, dB[1].ptr, db.ptr],
+                     [dY[0].ptr, dY[1].ptr, dy.ptr], nrhs, "fp64")
+    for a in dB + dY + [db, dy]:
+        a.free()
+    for p in probs + [f]:
+        p.close()
