@@ -713,7 +713,7 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
               const int k = b.k0 + row;
               // K(source, target) = kTransposeSign K(target, source)^T
               if (k < b.cnt && 16 * ct + col < pr.pw)
-                part[(int64_t)k * RP + 16 * ct + col] =
+                part[(int64_t)k * pr.prs + 16 * ct + col] =
                     kTransposeSign<KID> * (accT[0][ct][v] + accT[1][ct][v]);
             }
         }
@@ -1023,20 +1023,20 @@ __global__ __launch_bounds__(256) void k_mr_p2p_lds(const MrNearTask* __restrict
 __global__ __launch_bounds__(256) void k_mr_near_red(const MrNearRed* __restrict__ tasks,
                                                      int rp) {
   const MrNearRed d = tasks[blockIdx.x];
-  const int hw = d.pw / 2, hr = rp / 2;
+  const int hw = d.pw / 2, hr = rp / 2, hs = d.prs / 2;
   const int tot = d.nrows * hw;
   double2* out = reinterpret_cast<double2*>(d.out + d.row0 * rp);
   const double2* part = reinterpret_cast<const double2*>(d.part + d.base);
   const int64_t st = d.stride / 2;
   const int ns = d.nslots;
   for (int e = threadIdx.x; e < tot; e += 256) {
-    const int r = e / hw;
-    const int o = r * hr + (e - r * hw);
+    const int r = e / hw, c = e - r * hw;
+    const int o = r * hr + c, po = (d.lr0 + r) * hs + c;
     double2 s = out[o];
     int sl = 0;
     for (; sl + 4 <= ns; sl += 4) {
-      const double2 a0 = part[o + (sl + 0) * st], a1 = part[o + (sl + 1) * st];
-      const double2 a2 = part[o + (sl + 2) * st], a3 = part[o + (sl + 3) * st];
+      const double2 a0 = part[po + (sl + 0) * st], a1 = part[po + (sl + 1) * st];
+      const double2 a2 = part[po + (sl + 2) * st], a3 = part[po + (sl + 3) * st];
       s.x += a0.x;
       s.y += a0.y;
       s.x += a1.x;
@@ -1047,7 +1047,7 @@ __global__ __launch_bounds__(256) void k_mr_near_red(const MrNearRed* __restrict
       s.y += a3.y;
     }
     for (; sl < ns; sl++) {
-      const double2 a = part[o + sl * st];
+      const double2 a = part[po + sl * st];
       s.x += a.x;
       s.y += a.y;
     }
@@ -2356,9 +2356,8 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
         }
       }
       for (int r0 = 0; ns > 0 && r0 < cb; r0 += 64)
-        red.push_back(MrNearRed{p.nearv.as<double>(), nullptr, e.leaf_start[b] + r0,
-                                base + (int64_t)r0 * RP, (int64_t)cb * RP, std::min(64, cb - r0),
-                                ns, RP, 0});
+        red.push_back(MrNearRed{p.nearv.as<double>(), nullptr, e.leaf_start[b] + r0, base,
+                                (int64_t)cb * RP, std::min(64, cb - r0), ns, RP, RP, r0, 0});
     }
   }
   p.nearp.alloc(sizeof(double) * (size_t)std::max<int64_t>(ptot, 1));
@@ -2401,7 +2400,7 @@ MrPlan& get_plan(Engine& e, int prec, int nct) {
   p.h_near = std::move(near);
   p.near_ctr.alloc(sizeof(int) * 4 * kMrCtrStride);
   MrNearProb pr{e.x.as<double>(), e.y.as<double>(), p.qs.p, p.nearv.as<double>(),
-                p.nearp.as<double>(), e.kp, RP, 0};
+                p.nearp.as<double>(), e.kp, RP, RP};
   upload_vec(p.probs, std::vector<MrNearProb>{pr}, s);
   AFMM_HIP(hipStreamSynchronize(s));
   e.mr_plans.push_back(std::move(pl));
@@ -2893,7 +2892,7 @@ MrBatch* mr_batch_create(const std::vector<Engine*>& eng, const double* const* d
       const int pi = (int)probs.size();
       // the batch's columns: the partials' unused columns are never written
       probs.push_back(MrNearProb{e.x.as<double>(), e.y.as<double>(), p.qs.p, p.nearv.as<double>(),
-                                 p.nearp.as<double>(), e.kp, (nrhs + 1) & ~1, 0});
+                                 p.nearp.as<double>(), e.kp, (nrhs + 1) & ~1, (nrhs + 1) & ~1});
       for (MrNearTask t : p.h_near) {
         t.prob = pi;
         tasks.push_back(t);
@@ -2911,7 +2910,8 @@ MrBatch* mr_batch_create(const std::vector<Engine*>& eng, const double* const* d
   {
     std::vector<MrNearRed> red;
     for (const MrPlan* p : b.plans) red.insert(red.end(), p->h_near_red.begin(), p->h_near_red.end());
-    for (MrNearRed& r : red) r.pw = (nrhs + 1) & ~1;
+    // the batch's columns only, the partials' rows packed at that width
+    for (MrNearRed& r : red) r.pw = r.prs = (nrhs + 1) & ~1;
     b.n_near_red = red.size();
     upload_vec(b.near_red, red, s);
   }

@@ -209,14 +209,18 @@ struct MrNearTask {
 };
 
 // The near-field partials of one leaf chunk added to its potentials, slot
-// after slot: out[(row0 + r) RP + c] += sum_s part[base + s stride + r RP + c]
-// for r < nrows and c < pw.
+// after slot: out[(row0 + r) RP + c] += sum_s part[base + s stride +
+// (lr0 + r) prs + c] for r < nrows and c < pw (slots keep their places
+// whatever the row stride).
 struct MrNearRed {
   double* out;
   const double* part;
-  int64_t row0, base, stride;
+  int64_t row0, base, stride;  // base: the leaf's slot 0, its first row
   int nrows, nslots;
-  int pw, pad;  // columns used in each row (even; the rest never written)
+  int pw;   // columns used in each row (even)
+  int prs;  // row stride of the partials (pw or the plan's row width)
+  int lr0;  // the chunk's first row within the leaf
+  int pad;
 };
 
 // Per-problem data of the near field (coordinates, charges, output).
@@ -227,7 +231,8 @@ struct MrNearProb {
   double* out;    // near-field potentials, rows of 16 NCT doubles
   double* part;   // pair sources' transposed products (MrNearTask::pt)
   KParams kp;
-  int pw, pad;    // partial columns written (the right-hand sides, rounded up to even)
+  int pw;         // partial columns written (the right-hand sides, rounded up to even)
+  int prs;        // the partials' row stride: pw, or the plan's row width
 };
 
 // Gather (caller order -> KD order) or final sum and scatter of up to
