@@ -3053,6 +3053,7 @@ MrBatchInfo mr_batch_info(const MrBatch* b) {
   for (const MrTaskList& tl : b->down) n += list_launches(tl);
   n += list_launches(b->m2l);
   for (const MrBatch::NearGroup& g : b->near) n += 1 + list_launches(g.tl);
+  if (b->n_near_red > 0) n++;  // the pair partials' reduction
   in.launches = n;
   in.depth = b->tune.depth;
   in.xcd = b->tune.xcd;
