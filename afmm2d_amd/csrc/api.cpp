@@ -529,20 +529,16 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
                               hipMemcpyHostToDevice, e->s_aux));
       AFMM_HIP(hipStreamSynchronize(e->s_aux));
     }
-    // Table of the near-field log: (RN(1/c_j), -log RN(1/c_j)) for
-    // c_j = 1 + (j + 1/2)/128, evaluated in extended precision; ln2 split so
-    // that e * ln2_hi is exact.
+    // Table of the near-field log (log2_near): (RN(1/c_j), -log2 RN(1/c_j) -
+    // 1) for c_0 = 1 and c_j = 1 + (j + 1/2)/256, in extended precision.
     {
-      std::vector<double> tab(256);
-      for (int j = 0; j < 128; j++) {
-        long double c = 1.0L + ((long double)j + 0.5L) / 128.0L;
-        double invc = (double)(1.0L / c);
+      std::vector<double> tab(2 * kNearLogTab);
+      for (int j = 0; j < kNearLogTab; j++) {
+        const long double c = j == 0 ? 1.0L : 1.0L + ((long double)j + 0.5L) / kNearLogTab;
+        const double invc = (double)(1.0L / c);
         tab[2 * j] = invc;
-        tab[2 * j + 1] = (double)(-logl((long double)invc));
+        tab[2 * j + 1] = (double)(-log2l((long double)invc) - 1.0L);
       }
-      const long double ln2 = logl(2.0L);
-      e->ln2_hi = (double)(float)ln2;
-      e->ln2_lo = (double)(ln2 - (long double)e->ln2_hi);
       e->log_tab.alloc(sizeof(double) * tab.size());
       AFMM_HIP(hipMemcpyAsync(e->log_tab.p, tab.data(), sizeof(double) * tab.size(),
                               hipMemcpyHostToDevice, e->s_aux));

@@ -526,8 +526,7 @@ constexpr int kNearTileLd = 17;
 template <int P, int NCT, int NRT, int KID>
 __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
                                              const MrNearProb* __restrict__ probs,
-                                             const double2* tab, double ln2_hi, double ln2_lo,
-                                             double* ktile) {
+                                             const double2* tab, double* ktile) {
   using T = MrT<P>;
   using VT = typename T::V;
   using Acc = typename T::Acc;
@@ -553,7 +552,7 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
                    : KID == K_GAUSS ? (float)(-1.4426950408889634 / kp.p0)
                                     : (float)inv_a;
   const float c1 = (float)kp.a;
-  const double scale = KID == K_LOG ? 0.5 : 1.0;
+  constexpr double scale = near_scale<KID>;  // of the FP64 sums
   // FP64 pair sources: the task's own charges as the B operand of the
   // transposed products (lane i + 16 g: target 16 rt + 4 tg + g, column
   // 16 ct + i; zero past the task's rows)
@@ -657,8 +656,7 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
               kv = kernel_xy<KID>(xt[rt], yt[rt], b.sx[u], b.sy[u], kp);
             } else {
               const double dx = xt[rt] - b.sx[u], dy = yt[rt] - b.sy[u];
-              kv = scale * near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab, ln2_hi,
-                                            ln2_lo);
+              kv = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab);
             }
             // the tile for the transposed products: row = target, column =
             // source of the step
@@ -714,7 +712,7 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
               // K(source, target) = kTransposeSign K(target, source)^T
               if (k < b.cnt && 16 * ct + col < pr.pw)
                 part[(int64_t)k * pr.prs + 16 * ct + col] =
-                    kTransposeSign<KID> * (accT[0][ct][v] + accT[1][ct][v]);
+                    (kTransposeSign<KID> * scale) * (accT[0][ct][v] + accT[1][ct][v]);
             }
         }
       }
@@ -783,7 +781,7 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
         row += 16 * rt;
         if (row < M) {
           double val;
-          if constexpr (P == PREC_F64) val = acc[rt][ct][v];
+          if constexpr (P == PREC_F64) val = scale * acc[rt][ct][v];
           else val = (double)sh[rt][ct][v] + (double)sl[rt][ct][v];
           out[(int64_t)(d.out_row + row) * RP + 16 * ct + col] = val;
         }
@@ -795,16 +793,16 @@ __device__ __forceinline__ void mr_near_task(const MrNearTask& d,
 // memory-bound tree passes run beside it) taking tasks from a counter, one
 // atomic per wave and task.
 template <int P, int NCT, int NRT, int KID>
-__global__ __launch_bounds__(256) void k_mr_p2p(const MrNearTask* __restrict__ tasks, int ntasks,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == PREC_F64 && NCT == 1 ? 2 : 1)))
+void k_mr_p2p(const MrNearTask* __restrict__ tasks, int ntasks,
                                                 const MrNearProb* __restrict__ probs,
                                                 const double2* __restrict__ logtab,
-                                                double ln2_hi, double ln2_lo,
                                                 int* __restrict__ ctr) {
-  __shared__ double2 tab[128];
+  __shared__ double2 tab[kNearLogTab];
   // FP64: each wave's kernel tile of its pair steps
   __shared__ double ktile[P == PREC_F64 ? 4 * 16 * NRT * kNearTileLd : 1];
   if constexpr (P == PREC_F64 && KID == K_LOG) {
-    for (int u = threadIdx.x; u < 128; u += blockDim.x) tab[u] = logtab[u];
+    for (int u = threadIdx.x; u < kNearLogTab; u += blockDim.x) tab[u] = logtab[u];
     __syncthreads();
   }
   const int l = threadIdx.x & 63;
@@ -814,7 +812,7 @@ __global__ __launch_bounds__(256) void k_mr_p2p(const MrNearTask* __restrict__ t
     if (l == 0) t = atomicAdd(ctr, 1);
     t = __builtin_amdgcn_readfirstlane(t);
     if (t >= ntasks) break;
-    mr_near_task<P, NCT, NRT, KID>(tasks[t], probs, tab, ln2_hi, ln2_lo, wtile);
+    mr_near_task<P, NCT, NRT, KID>(tasks[t], probs, tab, wtile);
   }
 }
 
@@ -835,7 +833,6 @@ __global__ __launch_bounds__(256) void k_mr_p2p_lds(const MrNearTask* __restrict
                                                     int ntasks,
                                                     const MrNearProb* __restrict__ probs,
                                                     const double2* __restrict__ logtab,
-                                                    double ln2_hi, double ln2_lo,
                                                     int* __restrict__ ctr) {
   using T = MrT<P>;
   using VT = typename T::V;
@@ -845,12 +842,12 @@ __global__ __launch_bounds__(256) void k_mr_p2p_lds(const MrNearTask* __restrict
   constexpr int KC = 4 * KS;                   // sources per product
   constexpr int SB = kNearStage;
   constexpr int NLD = (SB * RP + 255) / 256;  // charges per thread and stage
-  __shared__ double2 tab[128];
+  __shared__ double2 tab[kNearLogTab];
   __shared__ double2 sxy[2][SB];
   __shared__ VT sq[2][SB * RP];
   __shared__ int s_task;
   if constexpr (P == PREC_F64 && KID == K_LOG) {
-    for (int u = threadIdx.x; u < 128; u += 256) tab[u] = logtab[u];
+    for (int u = threadIdx.x; u < kNearLogTab; u += 256) tab[u] = logtab[u];
   }
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, i = l & 15, g = l >> 4;
   for (;;) {
@@ -874,7 +871,7 @@ __global__ __launch_bounds__(256) void k_mr_p2p_lds(const MrNearTask* __restrict
                      : KID == K_GAUSS ? (float)(-1.4426950408889634 / kp.p0)
                                       : (float)inv_a;
     const float c1 = (float)kp.a;
-    const double scale = KID == K_LOG ? 0.5 : 1.0;
+    constexpr double scale = near_scale<KID>;  // of the FP64 sums
     Acc acc[NCT], sh[NCT], sl[NCT];
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) {
@@ -939,8 +936,7 @@ __global__ __launch_bounds__(256) void k_mr_p2p_lds(const MrNearTask* __restrict
                 kv = kernel_xy<KID>(xt, yt, sp.x, sp.y, kp);
               } else {
                 const double dx = xt - sp.x, dy = yt - sp.y;
-                kv = scale * near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab,
-                                              ln2_hi, ln2_lo);
+                kv = near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab);
               }
             } else {
               kv = near_kernel_f32<KID>(xt, yt, sp.x, sp.y, kp, c0, c1);
@@ -1007,7 +1003,7 @@ __global__ __launch_bounds__(256) void k_mr_p2p_lds(const MrNearTask* __restrict
           const int r = 16 * w + row;
           if (r < M) {
             double val;
-            if constexpr (P == PREC_F64) val = acc[ct][v];
+            if constexpr (P == PREC_F64) val = scale * acc[ct][v];
             else val = (double)sh[ct][v] + (double)sl[ct][v];
             out[(int64_t)(d.out_row + r) * RP + 16 * ct + col] = val;
           }
@@ -2487,7 +2483,6 @@ struct NearLaunch {
   const MrTaskList* tl;
   const MrNearProb* probs;
   const double2* logtab;
-  double ln2_hi, ln2_lo;
   int* ctr;
   int kid, num_cus, near_waves;
 };
@@ -2507,7 +2502,7 @@ void launch_near_t(const NearLaunch& nl, hipStream_t s) {
         1, std::min<int64_t>(tl.n, (int64_t)nl.num_cus * std::max(2, nl.near_waves)));
     AFMM_DISPATCH_KERNEL(nl.kid, KID,
                          k_mr_p2p_lds<P, NCT, KID><<<grid, 256, 0, s>>>(
-                             t, tl.n, nl.probs, nl.logtab, nl.ln2_hi, nl.ln2_lo, ctr));
+                             t, tl.n, nl.probs, nl.logtab, ctr));
     AFMM_LAUNCH_CHECK();
     return;
   }
@@ -2522,7 +2517,7 @@ void launch_near_t(const NearLaunch& nl, hipStream_t s) {
 #define AFMM_MR_NEAR(NRT)                                                                        \
   AFMM_DISPATCH_KERNEL(nl.kid, KID,                                                              \
                        k_mr_p2p<P, NCT, NRT, KID><<<grid, 256, 0, s>>>(t, n, nl.probs, nl.logtab, \
-                                                                       nl.ln2_hi, nl.ln2_lo, cc))
+                                                                       cc))
     switch (c) {
       case 0: AFMM_MR_NEAR(1); break;
       case 1: AFMM_MR_NEAR(2); break;
@@ -2558,7 +2553,7 @@ void launch_near_red(const DevBuf& red, size_t n, int nct, hipStream_t s) {
 }
 
 void launch_near_plan(Engine& e, const MrPlan& p, int near_waves, hipStream_t s) {
-  NearLaunch nl{&p.near, p.probs.as<MrNearProb>(), e.log_tab.as<double2>(), e.ln2_hi, e.ln2_lo,
+  NearLaunch nl{&p.near, p.probs.as<MrNearProb>(), e.log_tab.as<double2>(),
                 p.near_ctr.as<int>(), e.kid, e.num_cus, near_waves};
   launch_near_p(p.prec, p.nct, nl, s);
   launch_near_red(p.near_red, p.h_near_red.size(), p.nct, s);
@@ -2800,7 +2795,6 @@ void batch_enqueue(MrBatch& b, hipStream_t s) {
     for (size_t g = 0; g < b.near.size(); g++) {
       const MrBatch::NearGroup& ng = b.near[g];
       NearLaunch nl{&ng.tl, ng.probs.as<MrNearProb>(), ng.first->log_tab.as<double2>(),
-                    ng.first->ln2_hi, ng.first->ln2_lo,
                     b.ctr.as<int>() + 4 * kMrCtrStride * g, ng.kid, ng.first->num_cus,
                     b.tune.near_waves};
       launch_near_p(b.prec, b.nct, nl, b.side);

@@ -78,6 +78,11 @@ constexpr int kNumKernels = 7;
 template <int KID>
 constexpr double kTransposeSign = KID == K_USER_ANTI ? -1.0 : 1.0;
 
+// Entries of the near-field log table (log2_near): (RN(1/c_j), -log2
+// RN(1/c_j) - 1) for c_0 = 1 and c_j = 1 + (j + 1/2)/256, j = 1 .. 255,
+// evaluated in extended precision on the host.
+constexpr int kNearLogTab = 256;
+
 // Parameters every kernel needs: p0 = Gaussian width (s^2), a = the
 // regularisation radius of K_REG_INV_R (0.5^L), u = the kparams as given
 // (zeros if none), for the user kernel.

@@ -462,9 +462,8 @@ struct Engine {
   int p2p_waves_per_simd = 4;   // set at assembly from the problem's sizes
   bool p2p_waves_fixed = false;  // AFMM2D_P2P_WAVES given
   bool near_tune = true;         // AFMM2D_NEAR_TUNE=0: the estimate, untimed
-  // FP64 log table for the near field: (1/c_j, -log(1/c_j)), c_j = 1 + (j + 1/2)/128
+  // the near field's FP64 log table, kNearLogTab double2 entries
   DevBuf log_tab;
-  double ln2_hi = 0.0, ln2_lo = 0.0;
   double create_seconds = 0.0, assemble_seconds = 0.0;
 
   // matvec state

@@ -953,9 +953,8 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
                                              const int* __restrict__ pair_part,
                                              double* __restrict__ part, int npairs, KParams kp,
                                              const double2* __restrict__ logtab,
-                                             double ln2_hi, double ln2_lo,
                                              int* __restrict__ queue) {
-  __shared__ double2 tab[128];
+  __shared__ double2 tab[kNearLogTab];
   // one address per step for each array, 8-byte lane stride: no bank
   // conflicts.  The staged chunk is stored twice (entries t and t + 64), so
   // source t + s of lane t needs no wrap: each lane's addresses are its base
@@ -967,14 +966,14 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
   __shared__ int grab;
   const int t = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if constexpr (KID == K_LOG) {
-    for (int u = threadIdx.x; u < 128; u += blockDim.x) tab[u] = logtab[u];
+    for (int u = threadIdx.x; u < kNearLogTab; u += blockDim.x) tab[u] = logtab[u];
   }
   // padding: zero charge, far away for the built-in kernels (K stays finite
   // there); the user kernel is only known to be finite at real points, so
   // its padding repeats the leaf's first point
   const double ninv_p0 = KID == K_GAUSS ? -1.0 / kp.p0 : 0.0;
   const double inv_a = KID == K_REG_INV_R ? 1.0 / kp.a : 0.0;
-  const double scale = KID == K_LOG ? 0.5 : 1.0;
+  const double scale = near_scale<KID>;
   // work queue: the pairs are cut into kNearSlices slices, each with its own
   // chunk counter (one cache line apart); a workgroup drains the slice of
   // its own index first and then steals from the others, so a workgroup that
@@ -1060,8 +1059,7 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
             const double sx = bx[j], sy = by[j], sq = bq[j];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-              const double v = pair_value<KID>(xq[u], yq[u], sx, sy, kp, ninv_p0, inv_a, tab,
-                                               ln2_hi, ln2_lo);
+              const double v = pair_value<KID>(xq[u], yq[u], sx, sy, kp, ninv_p0, inv_a, tab);
               aq[u] = fma(v, sq, aq[u]);
               bs[u] = dpp_move<kDppWaveRol1>(fma(kTransposeSign<KID> * v, qq[u], bs[u]));
             }
@@ -1092,10 +1090,8 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
           for (int s = 0; s < 64; s++) {
             const int j = t + s;
             const double sx = bx[j], sy = by[j], sq = bq[j];
-            const double v0 = pair_value<KID>(x0, y0, sx, sy, kp, ninv_p0, inv_a, tab, ln2_hi,
-                                              ln2_lo);
-            const double v1 = pair_value<KID>(x1, y1, sx, sy, kp, ninv_p0, inv_a, tab, ln2_hi,
-                                              ln2_lo);
+            const double v0 = pair_value<KID>(x0, y0, sx, sy, kp, ninv_p0, inv_a, tab);
+            const double v1 = pair_value<KID>(x1, y1, sx, sy, kp, ninv_p0, inv_a, tab);
             acc0 = fma(v0, sq, acc0);
             acc1 = fma(v1, sq, acc1);
             bs0 = dpp_move<kDppWaveRol1>(fma(kTransposeSign<KID> * v0, q0, bs0));
@@ -1117,8 +1113,7 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
           if (diag) {
             // step 0: the point with itself (R2 = 0)
             acc += (KID >= K_USER ? kernel_xy<KID>(xi, yi, xi, yi, kp)
-                                  : near_kernel<KID>(0.0, kp, ninv_p0, inv_a, tab, ln2_hi,
-                                                     ln2_lo)) *
+                                  : near_kernel<KID>(0.0, kp, ninv_p0, inv_a, tab)) *
                    qi;
             s = 1;
           }
@@ -1128,8 +1123,7 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
 #pragma unroll 2
             for (; s < 64; s++) {
               const int j = t + s;
-              const double v = pair_value<KID>(xi, yi, bx[j], by[j], kp, ninv_p0, inv_a, tab,
-                                               ln2_hi, ln2_lo);
+              const double v = pair_value<KID>(xi, yi, bx[j], by[j], kp, ninv_p0, inv_a, tab);
               acc = fma(v, bq[j], acc);
               bs = dpp_move<kDppWaveRol1>(fma(kTransposeSign<KID> * v, qi, bs));
             }
@@ -1139,15 +1133,13 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
 #pragma unroll 2
           for (; s < s_end; s++) {
             const int j = t + s;
-            const double v = pair_value<KID>(xi, yi, bx[j], by[j], kp, ninv_p0, inv_a, tab,
-                                             ln2_hi, ln2_lo);
+            const double v = pair_value<KID>(xi, yi, bx[j], by[j], kp, ninv_p0, inv_a, tab);
             acc = fma(v, bq[j], acc);
             atomicAdd(&bc[ca][j], kTransposeSign<KID> * v * qi);
           }
           if (diag && t < 32) {
             const int j = t + 32;
-            const double v = pair_value<KID>(xi, yi, bx[j], by[j], kp, ninv_p0, inv_a, tab,
-                                             ln2_hi, ln2_lo);
+            const double v = pair_value<KID>(xi, yi, bx[j], by[j], kp, ninv_p0, inv_a, tab);
             acc = fma(v, bq[j], acc);
             atomicAdd(&bc[ca][j], kTransposeSign<KID> * v * qi);
           }
@@ -1639,8 +1631,8 @@ void launch_near(Engine& e, hipStream_t s) {
   k_p2p<KID, NCH><<<grid, 64 * nw, 0, s>>>(                                                \
       e.x.as<double>(), e.y.as<double>(), e.qs.as<double>(), e.d_leaf_start.as<int>(),     \
       e.d_leaf_count.as<int>(), e.near_pair.as<int>(), e.near_pair_part.as<int>(),         \
-      e.near_part.as<double>(), (int)e.near_upairs, e.kp, e.log_tab.as<double2>(), e.ln2_hi, \
-      e.ln2_lo, e.near_queue.as<int>())
+      e.near_part.as<double>(), (int)e.near_upairs, e.kp, e.log_tab.as<double2>(),       \
+      e.near_queue.as<int>())
   // A's chunks in registers: arrays for the largest leaf (up to 512 points)
   AFMM_DISPATCH_KERNEL(e.kid, KID,
                        if (nch <= 1) AFMM_NEAR_LAUNCH(KID, 1);

@@ -7,28 +7,35 @@
 
 namespace afmm {
 
-// log(x) for normal x > 0 by Tang's table method (ACM TOMS 16(4), 1990):
-// x = 2^e m, m in [1,2); j = the top 7 bits of m's fraction;
-// c_j = 1 + (j + 1/2)/128, t = (RN(1/c_j), -log RN(1/c_j));
-// log x = e ln2 + t.y + log1p(m t.x - 1), |m t.x - 1| < 2^-8, log1p by its
-// Taylor polynomial to degree 6 (truncation < 4e-18).
-__device__ __forceinline__ double log_tang(double x, const double2* __restrict__ tab,
-                                           double ln2_hi, double ln2_lo) {
-  const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
-  const int ex = (int)(bits >> 52) - 1023;
-  const int j = (int)((bits >> 45) & 127);
-  const double m = __longlong_as_double((long long)((bits & 0x000FFFFFFFFFFFFFull) |
-                                                    0x3FF0000000000000ull));
-  const double2 t = tab[j];
+// The log kernel's near field works in log2 units: log2_near gives
+// log2(R^2), and every sum of such values is multiplied once by
+// kHalfLn2 = ln(2)/2 (log R = ln(2)/2 log2 R^2).
+constexpr double kHalfLn2 = 0.34657359027997264;
+
+// log2(r2) for r2 >= 0 by a table method after Tang (ACM TOMS 16(4), 1990):
+// r2 = 2^e m, m in [1, 2); j = the top 8 bits of m's fraction; r = m RN(1/c_j)
+// - 1, |r| <= 2^-9 (j >= 1) or in [0, 2^-8) (j = 0, where c_0 = 1);
+// log2 r2 = (e + 1) + (-log2 RN(1/c_j) - 1) + log2(1 + r), the last by its
+// Taylor polynomial to degree 5 (truncation below 1.3e-17, and 8.5e-16 in
+// bin 0).  e + 1 is the exponent frexp returns.  r2 below 2^-1022 (zero:
+// the diagonal and coincident points) is read as 1 and gives 0 exactly (j =
+// 0, m = 1, r = 0).  About one ulp of the result; no branch.
+__device__ __forceinline__ double log2_near(double r2, const double2* __restrict__ tab) {
+  int lo = __double2loint(r2);
+  int hi = __double2hiint(r2);
+  const bool tiny = hi < 0x00100000;
+  hi = tiny ? 0x3FF00000 : hi;
+  lo = tiny ? 0 : lo;
+  const int fe = __builtin_amdgcn_frexp_exp(__hiloint2double(hi, lo));
+  const double m = __hiloint2double((hi & 0x000FFFFF) | 0x3FF00000, lo);
+  const double2 t = tab[(hi >> 12) & (kNearLogTab - 1)];
   const double r = fma(m, t.x, -1.0);
-  double p = fma(r, -1.0 / 6.0, 1.0 / 5.0);
-  p = fma(r, p, -0.25);
-  p = fma(r, p, 1.0 / 3.0);
-  p = fma(r, p, -0.5);
-  p = r * p;
-  p = fma(r, p, r);  // log1p(r)
-  const double e = (double)ex;
-  return fma(e, ln2_hi, t.y) + fma(e, ln2_lo, p);
+  // (-1)^(k+1) / (k ln 2), k = 5 .. 1
+  double q = fma(r, 0.28853900817779266, -0.36067376022224085);
+  q = fma(r, q, 0.48089834696298778);
+  q = fma(r, q, -0.72134752044448169);
+  q = fma(r, q, 1.4426950408889634);
+  return (double)fe + fma(r, q, t.y);
 }
 
 // 1/sqrt(x) for normal x > 0: the hardware estimate y0 = (1 + d)/sqrt(x)
@@ -44,20 +51,15 @@ __device__ __forceinline__ double rsqrt_refined(double x) {
 }
 
 // The kernel from the squared distance for the near field.  Same values as
-// kernel_r2 to a few ulp, without its branches and correctly rounded
-// divisions and square roots: the log by Tang's table, the 1/R kernels
-// through rsqrt_refined (one estimate, no division), the Gaussian with the
-// reciprocal of its width taken once (ninv_p0 = -1/p0).
+// kernel_r2 to a few ulp (times near_scale<KID>), without its branches and
+// correctly rounded divisions and square roots: the log as log2_near, the
+// 1/R kernels through rsqrt_refined (one estimate, no division), the
+// Gaussian with the reciprocal of its width taken once (ninv_p0 = -1/p0).
 template <int KID>
 __device__ __forceinline__ double near_kernel(double r2, const KParams& kp, double ninv_p0,
-                                              double inv_a, const double2* tab, double ln2_hi,
-                                              double ln2_lo) {
+                                              double inv_a, const double2* tab) {
   if constexpr (KID == K_LOG) {
-    // 2 x the log kernel; the factor 1/2 is applied to the sums.  The log is
-    // taken of max(r2, tiny) and the diagonal value selected after, so no
-    // branch splits the wave
-    const double lg = log_tang(fmax(r2, 1e-300), tab, ln2_hi, ln2_lo);
-    return r2 == 0.0 ? 0.0 : lg;
+    return log2_near(r2, tab);
   } else if constexpr (KID == K_INV_R) {
     const double rs = rsqrt_refined(r2);
     return r2 == 0.0 ? 0.0 : rs;
@@ -72,19 +74,23 @@ __device__ __forceinline__ double near_kernel(double r2, const KParams& kp, doub
   }
 }
 
-// K(target, source) for the near field: the user kernel takes the points,
-// the built-in ones the squared distance.
+// K(target, source) for the near field (times near_scale<KID>): the user
+// kernel takes the points, the built-in ones the squared distance.
 template <int KID>
 __device__ __forceinline__ double pair_value(double xi, double yi, double sx, double sy,
                                              const KParams& kp, double ninv_p0, double inv_a,
-                                             const double2* tab, double ln2_hi, double ln2_lo) {
+                                             const double2* tab) {
   if constexpr (KID >= K_USER) {
     return kernel_xy<KID>(xi, yi, sx, sy, kp);
   } else {
     const double dx = xi - sx, dy = yi - sy;
-    return near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab, ln2_hi, ln2_lo);
+    return near_kernel<KID>(fma(dx, dx, dy * dy), kp, ninv_p0, inv_a, tab);
   }
 }
+
+// The factor of the near field's FP64 sums: near_kernel<K_LOG> is log2 R^2.
+template <int KID>
+constexpr double near_scale = KID == K_LOG ? kHalfLn2 : 1.0;
 
 // FP32 kernel value for the reduced-precision near field: the difference of
 // the FP64 coordinates rounded to FP32, the squared distance in FP32, then

@@ -128,6 +128,30 @@ def test_near_field_matches_oracle(engine, oracle_mod, n, leaf, kernel, quant):
     assert float(np.max(np.abs(got - ref))) <= 1e-12 * float(np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("scale,quant", [(1e-7, 0), (1e3, 0), (1.0, 16), (1.0, 4096)])
+def test_log_near_field_over_scales(engine, oracle_mod, scale, quant):
+    """The log kernel's table evaluation (log2 R^2 from a 256-entry table and
+    a degree-5 polynomial, ln(2)/2 applied to the sums) against the oracle's
+    0.5 log(R^2) over squared distances from ~1e-16 to ~1e7; duplicated
+    points (quant) give R = 0 off the diagonal, which must add exactly
+    nothing."""
+    n = 6000
+    xy, q = uniform_points(n, 91)
+    if quant:
+        xy = np.round(xy * quant) / quant
+    xy = xy * scale
+    T = oracle_mod.Tree(xy, 64, 8, oracle_kernel("log"))
+    ref = T.near_field(q)
+    a = engine.AFMM(n, 64, 8, xy, kernel="log")
+    got = a.near_field(q)
+    a.close()
+    T.close()
+    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    print("log near field, scale %g, quant %d: rel %.2e" % (scale, quant, err), flush=True)
+    assert err <= 1e-13, err
+    assert float(np.max(np.abs(got - ref))) <= 1e-12 * float(np.max(np.abs(ref)))
+
+
 @pytest.mark.parametrize("n,leaf,kernel,chunks", [
     (14336, 200, "log", 4),        # 224 points per leaf: four 64-point chunks
     (14336, 200, "reg_inv_r", 4),
