@@ -798,7 +798,7 @@ void k_mr_p2p(const MrNearTask* __restrict__ tasks, int ntasks,
                                                 const MrNearProb* __restrict__ probs,
                                                 const double2* __restrict__ logtab,
                                                 int* __restrict__ ctr) {
-  __shared__ double2 tab[kNearLogTab];
+  __shared__ double2 tab[P == PREC_F64 && KID == K_LOG ? kNearLogTab : 1];
   // FP64: each wave's kernel tile of its pair steps
   __shared__ double ktile[P == PREC_F64 ? 4 * 16 * NRT * kNearTileLd : 1];
   if constexpr (P == PREC_F64 && KID == K_LOG) {
@@ -842,7 +842,7 @@ __global__ __launch_bounds__(256) void k_mr_p2p_lds(const MrNearTask* __restrict
   constexpr int KC = 4 * KS;                   // sources per product
   constexpr int SB = kNearStage;
   constexpr int NLD = (SB * RP + 255) / 256;  // charges per thread and stage
-  __shared__ double2 tab[kNearLogTab];
+  __shared__ double2 tab[P == PREC_F64 && KID == K_LOG ? kNearLogTab : 1];
   __shared__ double2 sxy[2][SB];
   __shared__ VT sq[2][SB * RP];
   __shared__ int s_task;

@@ -954,7 +954,7 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
                                              double* __restrict__ part, int npairs, KParams kp,
                                              const double2* __restrict__ logtab,
                                              int* __restrict__ queue) {
-  __shared__ double2 tab[kNearLogTab];
+  __shared__ double2 tab[KID == K_LOG ? kNearLogTab : 1];
   // one address per step for each array, 8-byte lane stride: no bank
   // conflicts.  The staged chunk is stored twice (entries t and t + 64), so
   // source t + s of lane t needs no wrap: each lane's addresses are its base

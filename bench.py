@@ -36,6 +36,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# The log-kernel near field (k_p2p<0, 1> at N = 2^20, leaf 64), per evaluation:
+# FP64 FLOP (64 x SQ_INSTS_VALU_FLOPS_FP64) and VALU lane instructions (64 x
+# SQ_INSTS_VALU), counted over a launch of 299,376,640 evaluations
+P2P_FLOP_PER_EVAL = 21.97
+P2P_LANE_INSTS_PER_EVAL = 26.52
+VALU_LANE_SLOTS_PER_S = 256 * 4 * 16 * 2.4e9  # FP64 lanes: one FMA each per cycle
 
 
 # the headline metric, as the benchmark's definition words it
@@ -917,12 +923,16 @@ def main():
             "algorithmic_bytes": int(m2l_alg), "avg_ms": round(m2l_ms, 4)}
     if traffic is not None:
         roof["traffic_source"] = traffic_src
-    # the whole matvec against HBM, and the near field against the FP64 VALU
-    # (26.9 FP64 FLOP per log-kernel evaluation, from the SQ_INSTS_VALU_FLOPS
+    # the whole matvec against HBM, and the near field against the FP64 VALU:
+    # its FP64 FLOP rate against 78.6 TFLOP/s, and its VALU instruction issue
+    # against the VALU's 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane slots
+    # (P2P_FLOP_PER_EVAL and P2P_LANE_INSTS_PER_EVAL from the SQ_INSTS_VALU*
     # counters of profiles/p2p_near_field_alone_fp64_valu.md)
     whole = st["matvec_bytes"] / (ms * 1e-3) / 1e9
     p2p_ms = kern["p2p"]
-    p2p_tf = near_evals * 26.9 / (p2p_ms * 1e-3) / 1e12 if p2p_ms > 0 else 0.0
+    p2p_tf = near_evals * P2P_FLOP_PER_EVAL / (p2p_ms * 1e-3) / 1e12 if p2p_ms > 0 else 0.0
+    p2p_issue = (near_evals * P2P_LANE_INSTS_PER_EVAL / (p2p_ms * 1e-3) / VALU_LANE_SLOTS_PER_S
+                 if p2p_ms > 0 else 0.0)
 
     cpu = None
     if not args.no_cpu:
@@ -978,7 +988,9 @@ def main():
                             "algorithmic_bytes": st["matvec_bytes"]},
         "roofline_p2p": {"kernel": "p2p", "bound": "valu_fp64", "achieved": round(p2p_tf, 2),
                          "peak": 78.6, "unit": "TFLOP/s", "frac": round(p2p_tf / 78.6, 4),
-                         "evaluations": near_evals, "flop_per_evaluation": 26.9,
+                         "evaluations": near_evals, "flop_per_evaluation": P2P_FLOP_PER_EVAL,
+                         "valu_lane_insts_per_evaluation": P2P_LANE_INSTS_PER_EVAL,
+                         "valu_issue_frac": round(p2p_issue, 4),
                          "avg_ms": round(p2p_ms, 4),
                          "note": "beside the M2L on the side stream"},
         "blocked": blocked,
