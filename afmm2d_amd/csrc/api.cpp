@@ -529,18 +529,21 @@ int afmm2d_create(int n, int leaf, int tol_pow, const double* loc, int layout, i
                               hipMemcpyHostToDevice, e->s_aux));
       AFMM_HIP(hipStreamSynchronize(e->s_aux));
     }
-    // Table of the near-field log (log2_near): (RN(1/c_j), -log2 RN(1/c_j) -
-    // 1) for c_0 = 1 and c_j = 1 + (j + 1/2)/256, in extended precision.
+    // The near field's tables, in extended precision: log2_near's (RN(1/c_j),
+    // -log2 RN(1/c_j) - 1) for c_0 = 1 and c_j = 1 + (j + 1/2)/256, then
+    // exp_near's RN(2^(j/64)).
     {
-      std::vector<double> tab(2 * kNearLogTab);
+      std::vector<double> tab(2 * kNearLogTab + kNearExpTab);
       for (int j = 0; j < kNearLogTab; j++) {
         const long double c = j == 0 ? 1.0L : 1.0L + ((long double)j + 0.5L) / kNearLogTab;
         const double invc = (double)(1.0L / c);
         tab[2 * j] = invc;
         tab[2 * j + 1] = (double)(-log2l((long double)invc) - 1.0L);
       }
-      e->log_tab.alloc(sizeof(double) * tab.size());
-      AFMM_HIP(hipMemcpyAsync(e->log_tab.p, tab.data(), sizeof(double) * tab.size(),
+      for (int j = 0; j < kNearExpTab; j++)
+        tab[2 * kNearLogTab + j] = (double)exp2l((long double)j / kNearExpTab);
+      e->near_tab.alloc(sizeof(double) * tab.size());
+      AFMM_HIP(hipMemcpyAsync(e->near_tab.p, tab.data(), sizeof(double) * tab.size(),
                               hipMemcpyHostToDevice, e->s_aux));
       AFMM_HIP(hipStreamSynchronize(e->s_aux));
     }

@@ -796,13 +796,13 @@ template <int P, int NCT, int NRT, int KID>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == PREC_F64 && NCT == 1 ? 2 : 1)))
 void k_mr_p2p(const MrNearTask* __restrict__ tasks, int ntasks,
                                                 const MrNearProb* __restrict__ probs,
-                                                const double2* __restrict__ logtab,
+                                                const double2* __restrict__ ntab,
                                                 int* __restrict__ ctr) {
-  __shared__ double2 tab[P == PREC_F64 && KID == K_LOG ? kNearLogTab : 1];
+  __shared__ double2 tab[P == PREC_F64 ? kNearTabLen<KID> : 1];
   // FP64: each wave's kernel tile of its pair steps
   __shared__ double ktile[P == PREC_F64 ? 4 * 16 * NRT * kNearTileLd : 1];
-  if constexpr (P == PREC_F64 && KID == K_LOG) {
-    for (int u = threadIdx.x; u < kNearLogTab; u += blockDim.x) tab[u] = logtab[u];
+  if constexpr (P == PREC_F64) {
+    load_near_tab<KID>(tab, ntab, threadIdx.x, blockDim.x);
     __syncthreads();
   }
   const int l = threadIdx.x & 63;
@@ -832,7 +832,7 @@ template <int P, int NCT, int KID>
 __global__ __launch_bounds__(256) void k_mr_p2p_lds(const MrNearTask* __restrict__ tasks,
                                                     int ntasks,
                                                     const MrNearProb* __restrict__ probs,
-                                                    const double2* __restrict__ logtab,
+                                                    const double2* __restrict__ ntab,
                                                     int* __restrict__ ctr) {
   using T = MrT<P>;
   using VT = typename T::V;
@@ -842,13 +842,11 @@ __global__ __launch_bounds__(256) void k_mr_p2p_lds(const MrNearTask* __restrict
   constexpr int KC = 4 * KS;                   // sources per product
   constexpr int SB = kNearStage;
   constexpr int NLD = (SB * RP + 255) / 256;  // charges per thread and stage
-  __shared__ double2 tab[P == PREC_F64 && KID == K_LOG ? kNearLogTab : 1];
+  __shared__ double2 tab[P == PREC_F64 ? kNearTabLen<KID> : 1];
   __shared__ double2 sxy[2][SB];
   __shared__ VT sq[2][SB * RP];
   __shared__ int s_task;
-  if constexpr (P == PREC_F64 && KID == K_LOG) {
-    for (int u = threadIdx.x; u < kNearLogTab; u += 256) tab[u] = logtab[u];
-  }
+  if constexpr (P == PREC_F64) load_near_tab<KID>(tab, ntab, threadIdx.x, 256);
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, i = l & 15, g = l >> 4;
   for (;;) {
     __syncthreads();  // the previous task's LDS reads are done
@@ -2482,7 +2480,7 @@ void launch_gemm_p(int prec, int nct, const MrTaskList& tl, int xcd, int depth, 
 struct NearLaunch {
   const MrTaskList* tl;
   const MrNearProb* probs;
-  const double2* logtab;
+  const double2* ntab;  // the engine's near-field tables
   int* ctr;
   int kid, num_cus, near_waves;
 };
@@ -2502,7 +2500,7 @@ void launch_near_t(const NearLaunch& nl, hipStream_t s) {
         1, std::min<int64_t>(tl.n, (int64_t)nl.num_cus * std::max(2, nl.near_waves)));
     AFMM_DISPATCH_KERNEL(nl.kid, KID,
                          k_mr_p2p_lds<P, NCT, KID><<<grid, 256, 0, s>>>(
-                             t, tl.n, nl.probs, nl.logtab, ctr));
+                             t, tl.n, nl.probs, nl.ntab, ctr));
     AFMM_LAUNCH_CHECK();
     return;
   }
@@ -2516,7 +2514,7 @@ void launch_near_t(const NearLaunch& nl, hipStream_t s) {
     int* cc = ctr + c * kMrCtrStride;
 #define AFMM_MR_NEAR(NRT)                                                                        \
   AFMM_DISPATCH_KERNEL(nl.kid, KID,                                                              \
-                       k_mr_p2p<P, NCT, NRT, KID><<<grid, 256, 0, s>>>(t, n, nl.probs, nl.logtab, \
+                       k_mr_p2p<P, NCT, NRT, KID><<<grid, 256, 0, s>>>(t, n, nl.probs, nl.ntab,   \
                                                                        cc))
     switch (c) {
       case 0: AFMM_MR_NEAR(1); break;
@@ -2553,7 +2551,7 @@ void launch_near_red(const DevBuf& red, size_t n, int nct, hipStream_t s) {
 }
 
 void launch_near_plan(Engine& e, const MrPlan& p, int near_waves, hipStream_t s) {
-  NearLaunch nl{&p.near, p.probs.as<MrNearProb>(), e.log_tab.as<double2>(),
+  NearLaunch nl{&p.near, p.probs.as<MrNearProb>(), e.near_tab.as<double2>(),
                 p.near_ctr.as<int>(), e.kid, e.num_cus, near_waves};
   launch_near_p(p.prec, p.nct, nl, s);
   launch_near_red(p.near_red, p.h_near_red.size(), p.nct, s);
@@ -2794,7 +2792,7 @@ void batch_enqueue(MrBatch& b, hipStream_t s) {
     AFMM_HIP(hipStreamWaitEvent(b.side, b.fork, 0));
     for (size_t g = 0; g < b.near.size(); g++) {
       const MrBatch::NearGroup& ng = b.near[g];
-      NearLaunch nl{&ng.tl, ng.probs.as<MrNearProb>(), ng.first->log_tab.as<double2>(),
+      NearLaunch nl{&ng.tl, ng.probs.as<MrNearProb>(), ng.first->near_tab.as<double2>(),
                     b.ctr.as<int>() + 4 * kMrCtrStride * g, ng.kid, ng.first->num_cus,
                     b.tune.near_waves};
       launch_near_p(b.prec, b.nct, nl, b.side);

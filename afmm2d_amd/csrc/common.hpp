@@ -82,6 +82,9 @@ constexpr double kTransposeSign = KID == K_USER_ANTI ? -1.0 : 1.0;
 // RN(1/c_j) - 1) for c_0 = 1 and c_j = 1 + (j + 1/2)/256, j = 1 .. 255,
 // evaluated in extended precision on the host.
 constexpr int kNearLogTab = 256;
+// Entries of the near-field exp table (exp_near), after the log table in the
+// same buffer: RN(2^(j/64)), j = 0 .. 63.
+constexpr int kNearExpTab = 64;
 
 // Parameters every kernel needs: p0 = Gaussian width (s^2), a = the
 // regularisation radius of K_REG_INV_R (0.5^L), u = the kparams as given

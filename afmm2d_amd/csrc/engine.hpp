@@ -462,8 +462,9 @@ struct Engine {
   int p2p_waves_per_simd = 4;   // set at assembly from the problem's sizes
   bool p2p_waves_fixed = false;  // AFMM2D_P2P_WAVES given
   bool near_tune = true;         // AFMM2D_NEAR_TUNE=0: the estimate, untimed
-  // the near field's FP64 log table, kNearLogTab double2 entries
-  DevBuf log_tab;
+  // the near field's FP64 tables: kNearLogTab double2 entries of the log
+  // table, then kNearExpTab doubles of the exp table
+  DevBuf near_tab;
   double create_seconds = 0.0, assemble_seconds = 0.0;
 
   // matvec state

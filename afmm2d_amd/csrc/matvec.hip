@@ -952,9 +952,9 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
                                              const int* __restrict__ pairs,
                                              const int* __restrict__ pair_part,
                                              double* __restrict__ part, int npairs, KParams kp,
-                                             const double2* __restrict__ logtab,
+                                             const double2* __restrict__ ntab,
                                              int* __restrict__ queue) {
-  __shared__ double2 tab[KID == K_LOG ? kNearLogTab : 1];
+  __shared__ double2 tab[kNearTabLen<KID>];
   // one address per step for each array, 8-byte lane stride: no bank
   // conflicts.  The staged chunk is stored twice (entries t and t + 64), so
   // source t + s of lane t needs no wrap: each lane's addresses are its base
@@ -965,9 +965,7 @@ __global__ __launch_bounds__(256) void k_p2p(const double* __restrict__ x,
   __shared__ double bx[128], by[128], bq[128], bc[NCH][128];
   __shared__ int grab;
   const int t = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  if constexpr (KID == K_LOG) {
-    for (int u = threadIdx.x; u < kNearLogTab; u += blockDim.x) tab[u] = logtab[u];
-  }
+  load_near_tab<KID>(tab, ntab, threadIdx.x, blockDim.x);
   // padding: zero charge, far away for the built-in kernels (K stays finite
   // there); the user kernel is only known to be finite at real points, so
   // its padding repeats the leaf's first point
@@ -1631,7 +1629,7 @@ void launch_near(Engine& e, hipStream_t s) {
   k_p2p<KID, NCH><<<grid, 64 * nw, 0, s>>>(                                                \
       e.x.as<double>(), e.y.as<double>(), e.qs.as<double>(), e.d_leaf_start.as<int>(),     \
       e.d_leaf_count.as<int>(), e.near_pair.as<int>(), e.near_pair_part.as<int>(),         \
-      e.near_part.as<double>(), (int)e.near_upairs, e.kp, e.log_tab.as<double2>(),       \
+      e.near_part.as<double>(), (int)e.near_upairs, e.kp, e.near_tab.as<double2>(),       \
       e.near_queue.as<int>())
   // A's chunks in registers: arrays for the largest leaf (up to 512 points)
   AFMM_DISPATCH_KERNEL(e.kid, KID,

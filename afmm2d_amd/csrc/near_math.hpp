@@ -38,6 +38,43 @@ __device__ __forceinline__ double log2_near(double r2, const double2* __restrict
   return (double)fe + fma(r, q, t.y);
 }
 
+// exp(x) for x <= 0 by a table method after Tang (ACM TOMS 15(2), 1989):
+// x = (64 k + j) ln(2)/64 + r with n = 64 k + j = rint(64 x / ln 2), |r| <=
+// ln(2)/128; exp x = 2^k 2^(j/64) (1 + p(r)), p the Taylor polynomial of
+// e^r - 1 to degree 5 (truncation below 3.5e-17 relative).  ln(2)/64 is
+// split in two so that n times its leading part is exact; x below -746 is
+// raised to -746, where the result underflows to 0 either way.  About one
+// ulp; exp(0) = 1 exactly.  tab: RN(2^(j/64)).
+__device__ __forceinline__ double exp_near(double x, const double* __restrict__ tab) {
+  x = fmax(x, -746.0);
+  const double n = rint(x * 92.332482616893657);  // 64 / ln 2
+  double r = fma(-n, 0.010830424726009369, x);   // ln(2)/64 to 24 bits
+  r = fma(-n, -2.9760223436661086e-11, r);       // and the rest
+  double q = fma(r, 1.0 / 120.0, 1.0 / 24.0);
+  q = fma(r, q, 1.0 / 6.0);
+  q = fma(r, q, 0.5);
+  const double p = fma(r * r, q, r);
+  const int ni = (int)n;
+  const double t = tab[ni & (kNearExpTab - 1)];
+  return ldexp(fma(t, p, t), ni >> 6);
+}
+
+// double2 entries of a kernel's near-field table in LDS: the log table, the
+// Gaussian's exp table (two values per entry), or none.
+template <int KID>
+constexpr int kNearTabLen = KID == K_LOG ? kNearLogTab : KID == K_GAUSS ? kNearExpTab / 2 : 1;
+
+// A kernel's table from the engine's buffer (the log table, then the exp
+// table) into LDS; the caller synchronises before the first use.
+template <int KID>
+__device__ __forceinline__ void load_near_tab(double2* tab, const double2* __restrict__ src,
+                                              int tid, int nthreads) {
+  if constexpr (KID == K_LOG || KID == K_GAUSS) {
+    const double2* s = src + (KID == K_GAUSS ? kNearLogTab : 0);
+    for (int u = tid; u < kNearTabLen<KID>; u += nthreads) tab[u] = s[u];
+  }
+}
+
 // 1/sqrt(x) for normal x > 0: the hardware estimate y0 = (1 + d)/sqrt(x)
 // (v_rsq_f64) refined by one second-order Newton step.  With
 // e = 1 - x y0^2 = -2d - d^2, sqrt(x) y0 (1 + e/2 + 3e^2/8) = 1 + O(d^3);
@@ -54,7 +91,8 @@ __device__ __forceinline__ double rsqrt_refined(double x) {
 // kernel_r2 to a few ulp (times near_scale<KID>), without its branches and
 // correctly rounded divisions and square roots: the log as log2_near, the
 // 1/R kernels through rsqrt_refined (one estimate, no division), the
-// Gaussian with the reciprocal of its width taken once (ninv_p0 = -1/p0).
+// Gaussian as exp_near with the reciprocal of its width taken once
+// (ninv_p0 = -1/p0).  tab: the kernel's table (kNearTabLen<KID>).
 template <int KID>
 __device__ __forceinline__ double near_kernel(double r2, const KParams& kp, double ninv_p0,
                                               double inv_a, const double2* tab) {
@@ -64,7 +102,7 @@ __device__ __forceinline__ double near_kernel(double r2, const KParams& kp, doub
     const double rs = rsqrt_refined(r2);
     return r2 == 0.0 ? 0.0 : rs;
   } else if constexpr (KID == K_GAUSS) {
-    return exp(r2 * ninv_p0);
+    return exp_near(r2 * ninv_p0, reinterpret_cast<const double*>(tab));
   } else {
     // R < a ? R/a : a/R = min(R/a, a/R) with R = r2 / sqrt(r2); r2 is kept
     // off zero inside the square root only, so R = 0 gives R/a = 0 exactly

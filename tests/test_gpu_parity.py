@@ -152,6 +152,26 @@ def test_log_near_field_over_scales(engine, oracle_mod, scale, quant):
     assert float(np.max(np.abs(got - ref))) <= 1e-12 * float(np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("p0", [1e-6, 1e-2, 1e2])
+def test_gaussian_near_field_over_widths(engine, oracle_mod, p0):
+    """The Gaussian's table evaluation (exp by a 64-entry 2^(j/64) table and
+    a degree-5 polynomial) against the oracle's exp(-R^2/p0), from widths
+    where most values underflow to widths where all are near 1."""
+    n = 20000
+    xy, q = uniform_points(n, 93)
+    kp = [p0, 0.0, 0.0, 0.0]
+    T = oracle_mod.Tree(xy, 64, 8, oracle_kernel("gaussian"), kparams=kp)
+    ref = T.near_field(q)
+    a = engine.AFMM(n, 64, 8, xy, kernel="gaussian", kparams=kp)
+    got = a.near_field(q)
+    a.close()
+    T.close()
+    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    print("gaussian near field, p0 %g: rel %.2e" % (p0, err), flush=True)
+    assert err <= 1e-13, err
+    assert float(np.max(np.abs(got - ref))) <= 1e-12 * float(np.max(np.abs(ref)))
+
+
 @pytest.mark.parametrize("n,leaf,kernel,chunks", [
     (14336, 200, "log", 4),        # 224 points per leaf: four 64-point chunks
     (14336, 200, "reg_inv_r", 4),
