@@ -419,6 +419,10 @@ def inputs(n, seed):
     return xy, q
 
 
+# the machine lines of the leaf-level M2L's HBM-byte summaries, by block storage
+TRAFFIC_KEY = {"fp64": "k_m2l_reg", "bfp48": "k_m2l_reg_bfp48"}
+
+
 def measured_traffic(kernel, alg_bytes):
     """HBM bytes per launch of `kernel` as measured by two rocprofv3 --pmc
     passes (FETCH_SIZE, WRITE_SIZE, calibrated against a copy of known size)
@@ -863,6 +867,8 @@ def main():
                "m2l_roofline": {"kernel": "m2l", "bound": "hbm", "achieved": round(ach_alt, 1),
                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": round(ach_alt / HBM_PEAK_GBS, 4),
+                                "traffic": measured_traffic(TRAFFIC_KEY[alt_storage],
+                                                            int(m2l_alg_alt))[0],
                                 "algorithmic_bytes": int(m2l_alg_alt),
                                 "avg_ms": round(kern_alt["m2l_timed_region"], 4)},
                "matvec_roofline_frac": round(st_alt["matvec_bytes"] / (ms_alt * 1e-3) / 1e9 /
@@ -917,7 +923,7 @@ def main():
     m2l_alg = st["m2l_leaf_bytes"] + 8 * st["m2l_leaf_columns"] + 8 * st["skeleton_leaf"]
     m2l_ms = kern["m2l_timed_region"]
     ach = m2l_alg / (m2l_ms * 1e-3) / 1e9 if m2l_ms > 0 else 0.0
-    traffic, traffic_src = measured_traffic("k_m2l_reg", int(m2l_alg))
+    traffic, traffic_src = measured_traffic(TRAFFIC_KEY[args.m2l_storage], int(m2l_alg))
     roof = {"kernel": "m2l", "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes": int(m2l_alg), "avg_ms": round(m2l_ms, 4)}
