@@ -74,6 +74,7 @@ typedef float mr_f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 mr_bf8 __attribute__((ext_vector_type(8)));
 typedef double mr_d2 __attribute__((ext_vector_type(2)));
 typedef uint32_t mr_u4 __attribute__((ext_vector_type(4)));
+typedef uint32_t mr_u2 __attribute__((ext_vector_type(2)));
 
 template <int P>
 struct MrT;
@@ -292,7 +293,8 @@ __device__ __forceinline__ MrMicro micro_unpack(const mr_u4 v) {
 // orientations stored (every read forward): descriptors as IM 3 and 4, the
 // micro column's first tile in the low 48 bits of a and the pair's exponent
 // in its top 16; a lane reads its micro row as the four entries' low words
-// (16 bytes) and top 16 bits (8 bytes) and turns each into the exact double
+// (one 16-byte load) and top 16 bits (one 8-byte load) and turns each into
+// the exact double
 // (2^52 + u) 2^s - (2^52 + 2^47) 2^s = m 2^s with one FMA.
 template <int P, int NCT, int NRT, int IM>
 __device__ __forceinline__ void gemm_load_micro(GemmFrag<P, NCT>& f, const MrTask& d, int k0,
@@ -325,8 +327,8 @@ __device__ __forceinline__ void gemm_load_micro(GemmFrag<P, NCT>& f, const MrTas
       const int ai = min(4 * (d.rt0 + rt) + (i >> 2), amax);
       const char* tp = base + (t0 + (int64_t)ai * astr) * 96;
       const mr_u4 lo = *gptr(reinterpret_cast<const mr_u4*>(tp + (i & 3) * 16));
-      const auto hp = gptr(reinterpret_cast<const uint32_t*>(tp + 64 + (i & 3) * 8));
-      const uint32_t h0 = hp[0], h1 = hp[1];
+      const mr_u2 hw = *((const __attribute__((address_space(1))) mr_u2*)(tp + 64 + (i & 3) * 8));
+      const uint32_t h0 = hw.x, h1 = hw.y;
       f.a[rt][0] = fma(__hiloint2double((int)(kB48ExpWord | (h0 & 0xFFFFu)), (int)lo.x), sc, nbias);
       f.a[rt][1] = fma(__hiloint2double((int)(kB48ExpWord | (h0 >> 16)), (int)lo.y), sc, nbias);
       f.a[rt][2] = fma(__hiloint2double((int)(kB48ExpWord | (h1 & 0xFFFFu)), (int)lo.z), sc, nbias);
