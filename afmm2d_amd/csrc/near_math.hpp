@@ -104,11 +104,11 @@ __device__ __forceinline__ double near_kernel(double r2, const KParams& kp, doub
   } else if constexpr (KID == K_GAUSS) {
     return exp_near(r2 * ninv_p0, reinterpret_cast<const double*>(tab));
   } else {
-    // R < a ? R/a : a/R = min(R/a, a/R) with R = r2 / sqrt(r2); r2 is kept
-    // off zero inside the square root only, so R = 0 gives R/a = 0 exactly
+    // R < a ? R/a : a/R = min(R/a, a/R) = (1/R) min(R^2/a, a); r2 is kept
+    // off zero inside the square root only, so R = 0 gives 1e150 x 0 = 0
+    // exactly
     const double rs = rsqrt_refined(fmax(r2, 1e-300));
-    const double r = r2 * rs;
-    return fmin(r * inv_a, kp.a * rs);
+    return rs * fmin(r2 * inv_a, kp.a);
   }
 }
 
@@ -153,7 +153,7 @@ __device__ __forceinline__ float near_kernel_f32(double xi, double yi, double sx
       return __builtin_amdgcn_exp2f(r2 * c0);
     } else {
       const float rs = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
-      return fminf(r2 * rs * c0, c1 * rs);
+      return rs * fminf(r2 * c0, c1);
     }
   }
 }
