@@ -2105,9 +2105,12 @@ struct MrKnobs {
   bool near_pairs = true;
   bool near_after_up = false;  // fork the near field after the upward pass
   // k_mr_p2p_lds instead of k_mr_p2p: 1 always, 0 never, -1 (default) for
-  // FP64 with 32 columns only (N = 2^20: 3.49 against 3.90 ms per pass; for
-  // 16 columns and FP32/BF16 equal or slower, the faster near field taking
-  // as much from the tree passes beside it as it gains)
+  // FP64 with 32 columns (N = 2^20: 3.49 against 3.90 ms per pass) and FP32
+  // (N = 2^19, 16 columns: 0.75 against 0.83 ms; 2^20: 1.39 against 1.40 and,
+  // 32 columns, 1.93 against 1.96; N = 2^22 mixture, 32 columns: 9.34 against
+  // 9.62).  FP64 with 16 columns keeps the leaf pairs evaluated once (2^20:
+  // 2.36 against 2.48 ms; the staged form wins only below 2^17), BF16 the
+  // per-wave form (2^20: 1.07 against 1.09 ms)
   int near_lds = -1;
   MrKnobs() {
     auto rd = [](const char* n, int d) {
@@ -2135,7 +2138,7 @@ const MrKnobs& knobs() {
 // Whether the near field of a plan runs k_mr_p2p_lds (see near_lds).
 bool near_uses_lds(int prec, int nct) {
   const int v = knobs().near_lds;
-  return v < 0 ? prec == PREC_F64 && nct == 2 : v != 0;
+  return v < 0 ? (prec == PREC_F64 && nct == 2) || prec == PREC_F32 : v != 0;
 }
 
 }  // namespace
