@@ -2695,7 +2695,7 @@ struct MrBatch {
   int m2l_rows = 0;
   struct NearGroup {
     int kid = 0;
-    const Engine* first = nullptr;  // log table owner
+    const Engine* first = nullptr;  // owner of the near-field tables
     MrTaskList tl;
     DevBuf probs;
   };

@@ -112,7 +112,7 @@ def test_direct_sum_matches_oracle(engine, oracle_mod):
 ])
 def test_near_field_matches_oracle(engine, oracle_mod, n, leaf, kernel, quant):
     """The near-field kernel alone (AFMM2DTree.cpp:1199-1213) against the
-    oracle's: the rsqrt-based 1/R kernels and the table log agree to a few
+    oracle's: the rsqrt-based 1/R kernels, the table log and exp agree to a few
     ulp per entry, so the sums agree to 1e-13 relative."""
     kid = oracle_kernel(kernel)
     xy, q = uniform_points(n, 40 + n % 97)
