@@ -161,7 +161,12 @@ def config4(args, H, E):
                                "(L=%d), ACA tol 1e-8, %d RHS" % (n, st["levels"], R),
                    "max_rank": st["max_rank"], "assemble_s": round(t_asm, 3),
                    "m2l_storage": args.m2l_storage, "m2l_bytes": st["m2l_bytes"],
-                   "rel_err_rows": int(len(rows))},
+                   "rel_err_rows": int(len(rows)),
+                   "note": "rel_err_col0 is dominated by the method's own error on these clustered "
+                           "points (the reference algorithm's CPU restatement errs 1e-4 .. 1e-1 at "
+                           "N = 2^16 .. 2^19 and 0.25 .. 3.1 at 2^22 on this input; the GPU "
+                           "operator reproduces it, tests/test_gpu_config4.py); rel_diff_vs_fp64 "
+                           "is each reduced precision against the FP64 pass, all 32 columns"},
         "by_precision": res,
     }))
 
@@ -519,12 +524,14 @@ def run_blocked(A, Q, steps, rows, ref_rows, H, cols=(16, 32), only=None):
     """The blocked passes on the same operator: 16 and 32 columns of Q per
     pass in FP64, FP32 and BF16.  For each: ms per pass, point.RHS/s, the
     relative error of column 0 on the sampled rows, the M2L launch's time
-    (HIP events on its stream) and roofline."""
+    (HIP events on its stream) and roofline; FP32 and BF16 also their
+    difference from the FP64 pass over every column (when that ran)."""
     n = A.N
     st = A.stats()
     dB = H.DeviceArray.from_numpy(np.asfortranarray(Q))
     dY = H.DeviceArray(Q.nbytes)
     out = {}
+    y64 = {}  # the FP64 pass's output by column count
     for prec, fb, vb in (("fp64", 8, 8), ("fp32", 4, 4), ("bf16", 2, 4)):
         if only and not any(v.startswith(prec + "_") for v in only):
             continue
@@ -554,6 +561,11 @@ def run_blocked(A, Q, steps, rows, ref_rows, H, cols=(16, 32), only=None):
                 phases[name[3:]] = round(tot / cnt, 4) if cnt else 0.0
             Y = dY.download(np.empty((n, R), order="F"))
             err = float(np.linalg.norm(Y[rows, 0] - ref_rows) / np.linalg.norm(ref_rows))
+            diff64 = None
+            if prec == "fp64":
+                y64[R] = Y
+            elif R in y64:
+                diff64 = float(np.linalg.norm(Y - y64[R]) / np.linalg.norm(y64[R]))
             rp = 16 if R <= 16 else 32
             # algorithmic bytes of the M2L launch: every stored interaction
             # block once (FP64: each block stored once, read by both of its
@@ -567,7 +579,8 @@ def run_blocked(A, Q, steps, rows, ref_rows, H, cols=(16, 32), only=None):
             fbe = 12 if (prec == "fp64" and bpe == 6) else fb
             alg = coeffs * fbe + 2 * st["skeleton_total"] * rp * vb
             ach = alg / (m2l_ms * 1e-3) / 1e9 if m2l_ms > 0 else 0.0
-            out["%s_r%d" % (prec, R)] = {
+            key = "%s_r%d" % (prec, R)
+            out[key] = {
                 "value": round(n * R / (ms * 1e-3), 1), "ms_per_pass": round(ms, 4),
                 "rel_err_col0": err, "prepare_s": round(t_prep, 3),
                 "phase_ms": phases,
@@ -575,6 +588,8 @@ def run_blocked(A, Q, steps, rows, ref_rows, H, cols=(16, 32), only=None):
                                  "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                                  "algorithmic_bytes": int(alg), "avg_ms": round(m2l_ms, 4)},
             }
+            if diff64 is not None:
+                out[key]["rel_diff_vs_fp64"] = diff64
     dB.free()
     dY.free()
     return out
