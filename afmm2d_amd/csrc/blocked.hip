@@ -2103,7 +2103,12 @@ void build_reduced(Engine& e, int prec) {
 struct MrKnobs {
   int rows_m2l = 64, rows = 16, xcd = 0, near_waves = 1, depth = 3, near_rows = 64;
   bool near_pairs = true;
-  bool near_after_up = false;  // fork the near field after the upward pass
+  // fork the near field after the upward pass: 1 always, 0 never, -1
+  // (default) in a plan's own matvec when its near field runs the LDS-staged
+  // kernel on 16 columns (N = 2^20, FP32: 1.35 against 1.38 ms per pass; 32
+  // columns and batches keep the early fork; the results do not depend on
+  // the fork point)
+  int near_after_up = -1;
   // k_mr_p2p_lds instead of k_mr_p2p: 1 always, 0 never, -1 (default) for
   // FP64 with 32 columns (N = 2^20: 3.49 against 3.90 ms per pass) and FP32
   // (N = 2^19, 16 columns: 0.75 against 0.83 ms; 2^20: 1.39 against 1.40 and,
@@ -2126,7 +2131,7 @@ struct MrKnobs {
     depth = rd("AFMM2D_MR_DEPTH", 3);
     near_rows = std::max(16, std::min(64, rd("AFMM2D_MR_NEAR_ROWS", 64))) / 16 * 16;
     near_pairs = rd("AFMM2D_MR_NEAR_PAIRS", 1) != 0;
-    near_after_up = rd("AFMM2D_MR_NEAR_AFTER_UP", 0) != 0;
+    near_after_up = rd("AFMM2D_MR_NEAR_AFTER_UP", -1);
     near_lds = rd("AFMM2D_MR_NEAR_LDS", -1);
   }
 };
@@ -2651,11 +2656,13 @@ void mr_matvec(Engine& e, const double* d_b, int64_t ldb, double* d_y, int64_t l
     e.prof_end(e.s_aux, b);
     AFMM_HIP(hipEventRecord(e.ev_join, e.s_aux));
   };
-  if (e.near_overlap && !knobs().near_after_up) fork_near();
+  const int au = knobs().near_after_up;
+  const bool late = au > 0 || (au < 0 && nct == 1 && near_uses_lds(prec, nct));
+  if (e.near_overlap && !late) fork_near();
   e.prof_begin("mr_m2m", s, &a, &b);
   for (int j = L; j >= 2; j--) launch_gemm_p(prec, nct, p.up[j], 0, 3, s);
   e.prof_end(s, b);
-  if (e.near_overlap && knobs().near_after_up) fork_near();
+  if (e.near_overlap && late) fork_near();
   e.prof_begin("mr_m2l", s, &a, &b);
   launch_gemm_p(prec, nct, p.m2l, tn.xcd, tn.depth, s);
   e.prof_end(s, b);
@@ -2805,9 +2812,10 @@ void batch_enqueue(MrBatch& b, hipStream_t s) {
     launch_near_red(b.near_red, b.n_near_red, b.nct, b.side);
     AFMM_HIP(hipEventRecord(b.join, b.side));
   };
-  if (!knobs().near_after_up) fork_near();
+  const bool late = knobs().near_after_up > 0;
+  if (!late) fork_near();
   for (const MrTaskList& tl : b.up) launch_gemm_p(b.prec, b.nct, tl, 0, 3, s);
-  if (knobs().near_after_up) fork_near();
+  if (late) fork_near();
   launch_gemm_p(b.prec, b.nct, b.m2l, b.tune.xcd, b.tune.depth, s);
   for (const MrTaskList& tl : b.down) launch_gemm_p(b.prec, b.nct, tl, 0, 3, s);
   AFMM_HIP(hipStreamWaitEvent(s, b.join, 0));
