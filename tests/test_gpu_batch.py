@@ -117,8 +117,11 @@ def test_batch_256_problems_graph_and_tuner(engine):
         p.close()
 
 
-@pytest.mark.parametrize("precision,nrhs", [("fp32", 16), ("bf16", 20), ("fp64", 32)])
+@pytest.mark.parametrize("precision,nrhs", [("fp32", 16), ("bf16", 20), ("fp64", 32),
+                                             ("fp64", 5), ("fp64", 1)])
 def test_batch_precisions_and_column_counts(engine, precision, nrhs):
+    # FP64 with 5 and 1 columns: the near field's pair partials packed at 6
+    # and 2 columns per row (each problem's own matvec keeps 16)
     from afmm2d_amd import hip
     print_case("batch", precision, nrhs)
     probs, rng = make_problems(engine, 12, seed=7, nmin=3000, nmax=40000, tol_pow=10)
